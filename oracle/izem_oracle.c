@@ -1,0 +1,1174 @@
+/*
+ * izem_oracle.c -- CPU restatement of the IZEM reacting-flow path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see izem_oracle.h).  Build with
+ * -ffp-contract=off: each expression below keeps the reference's
+ * left-to-right evaluation order so that a HIP build with the same order
+ * agrees bit-for-bit on the flow kernels.
+ *
+ * Reference: saras13015/Phd, devel-izem/src/ (cited as file:line).
+ */
+#include "izem_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NRV ORA_NRV
+#define NIV ORA_NIV
+#define NV ORA_NV
+
+static inline long NXg(const ora_grid *g) { return g->nx + 2 * g->ng; }
+static inline long NYg(const ora_grid *g) { return g->ny + 2 * g->ng; }
+static inline long NZg(const ora_grid *g) { return g->nz + 2 * g->ng; }
+static inline long ncell(const ora_grid *g) { return NXg(g) * NYg(g) * NZg(g); }
+static inline long cidx(const ora_grid *g, int i, int j, int k) {
+  return (long)(i + g->ng) + NXg(g) * ((long)(j + g->ng) + NYg(g) * (long)(k + g->ng));
+}
+
+/* domain_select: src/module_parallel.f90:1936-2009.  0 = interior,
+ * -1/+1 x ghost slabs, -2/+2 y, -3/+3 z (faces only). */
+static void domain_select(const ora_grid *g, int d, int *i0, int *i1, int *j0, int *j1,
+                          int *k0, int *k1) {
+  int nx = g->nx, ny = g->ny, nz = g->nz, ng = g->ng;
+  *i0 = 0; *i1 = nx - 1; *j0 = 0; *j1 = ny - 1; *k0 = 0; *k1 = nz - 1;
+  switch (d) {
+    case -1: *i0 = -ng; *i1 = -1; break;
+    case 1: *i0 = nx; *i1 = nx + ng - 1; break;
+    case -2: *j0 = -ng; *j1 = -1; break;
+    case 2: *j0 = ny; *j1 = ny + ng - 1; break;
+    case -3: *k0 = -ng; *k1 = -1; break;
+    case 3: *k0 = nz; *k1 = nz + ng - 1; break;
+    default: break;
+  }
+}
+
+/* ---------------------------------------------------------------- thermo */
+
+/* Wmix_i_scalar: src/module_thermodynamics.f90:739-756 */
+double ora_wmix_i_scalar(const ora_thd *thd, const double *Y) {
+  double w = 0.0;
+  for (int l = 0; l < NRV; l++) w = w + Y[l] * thd->Wc_i[l];
+  return w;
+}
+
+/* cp_scalar: src/module_thermodynamics.f90:767-862 (linear extrapolation of
+ * cp outside [Tpolymin, Tpolymax]) */
+double ora_cp_scalar(const ora_thd *thd, double temp, const double *Y) {
+  double cp = 0.0;
+  for (int is = 0; is < NRV; is++) {
+    double tk1 = temp * thd->Tinf, tk2 = tk1 * tk1, tk3 = tk2 * tk1, tk4 = tk3 * tk1;
+    const double *hl = thd->HeatCap[is][0], *hh = thd->HeatCap[is][1];
+    double fac = Y[is] * thd->gam2 * thd->Wc_i[is];
+    if (tk1 < thd->Tpolymin[is]) {
+      tk1 = thd->Tpolymin[is]; tk2 = tk1 * tk1; tk3 = tk2 * tk1; tk4 = tk3 * tk1;
+      double cp0 = hl[0] + hl[1] * tk1 + hl[2] * tk2 + hl[3] * tk3 + hl[4] * tk4;
+      double dcp0 = hl[1] + hl[2] * tk1 * 2.0 + hl[3] * tk2 * 3.0 + hl[4] * tk3 * 4.0;
+      cp = cp + fac * (dcp0 * (temp * thd->Tinf - thd->Tpolymin[is]) + cp0);
+    } else if (tk1 < thd->Tpolylim[is]) {
+      cp = cp + fac * (hl[0] + hl[1] * tk1 + hl[2] * tk2 + hl[3] * tk3 + hl[4] * tk4);
+    } else if (tk1 < thd->Tpolymax[is]) {
+      cp = cp + fac * (hh[0] + hh[1] * tk1 + hh[2] * tk2 + hh[3] * tk3 + hh[4] * tk4);
+    } else {
+      tk1 = thd->Tpolymax[is]; tk2 = tk1 * tk1; tk3 = tk2 * tk1; tk4 = tk3 * tk1;
+      double cp0 = hh[0] + hh[1] * tk1 + hh[2] * tk2 + hh[3] * tk3 + hh[4] * tk4;
+      double dcp0 = hh[1] + hh[2] * tk1 * 2.0 + hh[3] * tk2 * 3.0 + hh[4] * tk3 * 4.0;
+      cp = cp + fac * (dcp0 * (temp * thd->Tinf - thd->Tpolymax[is]) + cp0);
+    }
+  }
+  return cp;
+}
+
+static inline double h_poly(const double *h, double t1, double t2, double t3, double t4,
+                            double t5, double onethird) {
+  return h[0] * t1 + h[1] * t2 * 0.50 + h[2] * t3 * onethird + h[3] * t4 * 0.25 +
+         h[4] * t5 * 0.20 + h[5];
+}
+static inline double dh_poly(const double *h, double t1, double t2, double t3, double t4) {
+  return h[0] + h[1] * t1 + h[2] * t2 + h[3] * t3 + h[4] * t4;
+}
+
+/* ha_scalar: src/module_thermodynamics.f90:873-978 */
+void ora_ha_scalar(const ora_thd *thd, double temp, double *ha) {
+  double Tinf_i = 1.0 / thd->Tinf, onethird = 1.0 / 3.0;
+  for (int is = 0; is < NRV; is++) {
+    double t1 = temp * thd->Tinf, t2 = t1 * t1, t3 = t2 * t1, t4 = t3 * t1, t5 = t4 * t1;
+    const double *hl = thd->HeatCap[is][0], *hh = thd->HeatCap[is][1];
+    double fac = thd->Wc_i[is] * Tinf_i;
+    double h = 0.0;
+    if (t1 < thd->Tpolymin[is]) {
+      t1 = thd->Tpolymin[is]; t2 = t1 * t1; t3 = t2 * t1; t4 = t3 * t1; t5 = t4 * t1;
+      double ha0 = h_poly(hl, t1, t2, t3, t4, t5, onethird);
+      double dha0 = dh_poly(hl, t1, t2, t3, t4);
+      h = h + fac * (dha0 * (temp * thd->Tinf - thd->Tpolymin[is]) + ha0);
+    } else if (t1 < thd->Tpolylim[is]) {
+      h = h + fac * h_poly(hl, t1, t2, t3, t4, t5, onethird);
+    } else if (t1 < thd->Tpolymax[is]) {
+      h = h + fac * h_poly(hh, t1, t2, t3, t4, t5, onethird);
+    } else {
+      t1 = thd->Tpolymax[is]; t2 = t1 * t1; t3 = t2 * t1; t4 = t3 * t1; t5 = t4 * t1;
+      double ha0 = h_poly(hh, t1, t2, t3, t4, t5, onethird);
+      double dha0 = dh_poly(hh, t1, t2, t3, t4);
+      h = h + fac * (dha0 * (temp * thd->Tinf - thd->Tpolymax[is]) + ha0);
+    }
+    ha[is] = h;
+  }
+}
+
+/* ------------------------------------------------------------------ WENO */
+
+/* wenopar (ng=3): src/module_weno.f90:114-140, 176-190 */
+static const double Z1 = 13.0 / 12.0, Z2 = 1.0 / 4.0, ZA = -2.0, ZB = -4.0, ZC = 3.0, ZD = -1.0;
+static const double EPS_WENO = 1.0e-10; /* src/module_weno.f90:31 */
+
+static void cweno_tab(double c[4][3]) {
+  c[0][0] = 11.0 / 6.0; c[0][1] = -7.0 / 6.0; c[0][2] = 1.0 / 3.0;   /* r = -1 */
+  c[1][0] = 1.0 / 3.0; c[1][1] = 5.0 / 6.0; c[1][2] = -1.0 / 6.0;    /* r = 0 */
+  c[2][0] = -1.0 / 6.0; c[2][1] = 5.0 / 6.0; c[2][2] = 1.0 / 3.0;    /* r = 1 */
+  c[3][0] = 1.0 / 3.0; c[3][1] = -7.0 / 6.0; c[3][2] = 11.0 / 6.0;   /* r = 2 */
+}
+
+static inline void betas(const double *q, int i, double *beta) {
+  /* q is 1-based as in the reference (q[1..6]) */
+  double a0 = q[i] + ZA * q[i + 1] + q[i + 2], b0 = ZC * q[i] + ZB * q[i + 1] + q[i + 2];
+  beta[0] = Z1 * a0 * a0 + Z2 * b0 * b0;
+  double a1 = q[i - 1] + ZA * q[i] + q[i + 1], b1 = q[i - 1] + ZD * q[i + 1];
+  beta[1] = Z1 * a1 * a1 + Z2 * b1 * b1;
+  double a2 = q[i - 2] + ZA * q[i - 1] + q[i], b2 = q[i - 2] + ZB * q[i - 1] + ZC * q[i];
+  beta[2] = Z1 * a2 * a2 + Z2 * b2 * b2;
+}
+
+/* wenorec (JS weights): src/module_weno.f90:286-527, ng = 3 branch.
+ * vp, vm hold the 6 stencil values (0-based here, reference is 1-based). */
+void ora_wenorec(const double *vp0, const double *vm0, double *vminus, double *vplus) {
+  double c[4][3];
+  cweno_tab(c);
+  const double d[3] = {3.0 / 10.0, 3.0 / 5.0, 1.0 / 10.0};
+  const double dt[3] = {d[2], d[1], d[0]};
+  const double *vp = vp0 - 1, *vm = vm0 - 1; /* 1-based views */
+  double vpr[3], beta[3], alfa[3], omega[3], sum;
+  int i = 3;
+  for (int r = 0; r < 3; r++) {
+    vpr[r] = 0.0;
+    for (int j = 0; j < 3; j++) vpr[r] = vpr[r] + c[r + 1][j] * vp[i - r + j];
+  }
+  betas(vp, i, beta);
+  sum = 0.0;
+  for (int r = 0; r < 3; r++) {
+    alfa[r] = d[r] / ((EPS_WENO + beta[r]) * (EPS_WENO + beta[r]));
+    sum = sum + alfa[r];
+  }
+  sum = 1.0 / sum;
+  for (int r = 0; r < 3; r++) omega[r] = alfa[r] * sum;
+  double vmn = 0.0;
+  for (int r = 0; r < 3; r++) vmn = vmn + omega[r] * vpr[r];
+  *vminus = vmn;
+
+  i = 4;
+  double vmr[3];
+  for (int r = 0; r < 3; r++) {
+    vmr[r] = 0.0;
+    for (int j = 0; j < 3; j++) vmr[r] = vmr[r] + c[r][j] * vm[i - r + j];
+  }
+  betas(vm, i, beta);
+  sum = 0.0;
+  for (int r = 0; r < 3; r++) {
+    alfa[r] = dt[r] / ((EPS_WENO + beta[r]) * (EPS_WENO + beta[r]));
+    sum = sum + alfa[r];
+  }
+  sum = 1.0 / sum;
+  for (int r = 0; r < 3; r++) omega[r] = alfa[r] * sum;
+  double vpl = 0.0;
+  for (int r = 0; r < 3; r++) vpl = vpl + omega[r] * vmr[r];
+  *vplus = vpl;
+}
+
+/* wenorec_nw, ng = 3: src/module_weno.f90:258-266.  Kept exactly as the
+ * reference writes it (its coefficient order reconstructs at i-1/2, not
+ * i+1/2; the JS path above is the one parity runs use). */
+void ora_wenorec_nw(const double *vp0, const double *vm0, double *vminus, double *vplus) {
+  const double *vp = vp0 - 1, *vm = vm0 - 1;
+  double denom = 1.0 / 60.0;
+  *vminus = (-3.0 * vp[1] + 27.0 * vp[2] + 47.0 * vp[3] - 13.0 * vp[4] + 2.0 * vp[5]) * denom;
+  *vplus = (-3.0 * vm[6] + 27.0 * vm[5] + 47.0 * vm[4] - 13.0 * vm[3] + 2.0 * vm[2]) * denom;
+}
+
+/* --------------------------------------------------------- Euler fluxes */
+
+/* euler_LLF_x / _y / _z: src/module_weno.f90:537-1044, 1055-1584, 1595-2118.
+ * dir 0/1/2.  x assigns fl, y and z accumulate (src/module_weno.f90:1002-1018
+ * vs the += form of the y/z routines). */
+int ora_euler_llf(int dir, const ora_grid *g, const ora_thd *thd, const double *d_i,
+                  const double *T, const double *W_i, const double *cp, const double *ha,
+                  const uint8_t *sensor, int weno_avg, const double *v, double *fl) {
+  const int ng = g->ng;
+  const long nc = ncell(g);
+  int n, na, nb;
+  long stride;
+  if (dir == 0) { n = g->nx; na = g->ny; nb = g->nz; stride = 1; }
+  else if (dir == 1) { n = g->ny; na = g->nx; nb = g->nz; stride = NXg(g); }
+  else { n = g->nz; na = g->nx; nb = g->ny; stride = NXg(g) * NYg(g); }
+  const int cn = 1 + dir;                       /* normal momentum column */
+  const int t1 = (dir == 0) ? 2 : 1;            /* first tangential column */
+  const int t2 = (dir == 2) ? 2 : 3;            /* second tangential column */
+  const int L = n + 2 * ng;
+  double *rho_i = malloc(sizeof(double) * L), *P = malloc(sizeof(double) * L),
+         *gam = malloc(sizeof(double) * L), *cs = malloc(sizeof(double) * L),
+         *vel = malloc(sizeof(double) * 3 * L), *ht = malloc(sizeof(double) * L),
+         *et = malloc(sizeof(double) * L), *Ya = malloc(sizeof(double) * NRV * L),
+         *fhat = malloc(sizeof(double) * NV * (n + 1));
+  int status = 0;
+  for (int b = 0; b < nb; b++)
+    for (int a = 0; a < na; a++) {
+      long base;
+      if (dir == 0) base = cidx(g, 0, a, b);
+      else if (dir == 1) base = cidx(g, a, 0, b);
+      else base = cidx(g, a, b, 0);
+#define C(p) (base + (long)(p) * stride)
+#define V(l, p) v[(long)(l) * nc + C(p)]
+      /* presolve 1/2: src/module_weno.f90:634-650 */
+      for (int p = -ng; p < n + ng; p++) {
+        int q = p + ng;
+        rho_i[q] = 1.0 / V(0, p);
+        vel[cn - 1 + 3 * q] = V(cn, p) * rho_i[q];
+        P[q] = V(0, p) * T[C(p)] * W_i[C(p)];
+        if (P[q] <= 0.0 && status == 0) status = -1;
+        gam[q] = thd->gam2 * W_i[C(p)] / cp[C(p)];
+        gam[q] = 1.0 / (1.0 - gam[q]);
+        cs[q] = sqrt(gam[q] * P[q] * rho_i[q]);
+      }
+      /* presolve 2/2: src/module_weno.f90:654-667 */
+      for (int p = -1; p <= n; p++) {
+        int q = p + ng;
+        vel[t1 - 1 + 3 * q] = V(t1, p) * rho_i[q];
+        vel[t2 - 1 + 3 * q] = V(t2, p) * rho_i[q];
+        for (int l = 0; l < NRV; l++) Ya[l + NRV * q] = V(NIV + l, p) * rho_i[q];
+        double h = 0.0;
+        for (int l = 0; l < NRV; l++) h = h + ha[(long)l * nc + C(p)] * Ya[l + NRV * q];
+        double ux = vel[3 * q], vy = vel[3 * q + 1], wz = vel[3 * q + 2];
+        ht[q] = h + 0.5 * (ux * ux + vy * vy + wz * wz);
+        et[q] = V(4, p) * rho_i[q];
+      }
+      for (int il = -1; il < n; il++) {
+        int ir = il + 1, ql = il + ng, qr = ir + ng;
+        double fs[6][NV], Ls[6][3];
+        for (int st = 0; st < 6; st++) {
+          int is = il + st + 1 - ng, q = is + ng;
+          double un = vel[cn - 1 + 3 * q];
+          fs[st][0] = V(cn, is);
+          for (int c = 1; c <= 3; c++) {
+            if (c == cn) fs[st][c] = V(c, is) * un + P[q];
+            else fs[st][c] = V(c, is) * un;
+          }
+          fs[st][4] = un * (V(4, is) + P[q]);
+          for (int l = NIV; l < NV; l++) fs[st][l] = V(l, is) * un;
+          Ls[st][0] = fabs(un - cs[q]);
+          Ls[st][1] = fabs(un);
+          Ls[st][2] = fabs(un + cs[q]);
+        }
+        /* Roe average: src/module_weno.f90:726-735 */
+        double r = sqrt(V(0, ir) * rho_i[ql]);
+        double r1 = 1.0 / (r + 1.0);
+        double uc[3];
+        for (int c = 0; c < 3; c++) uc[c] = (r * vel[c + 3 * qr] + vel[c + 3 * ql]) * r1;
+        double Yc[NRV];
+        for (int l = 0; l < NRV; l++) Yc[l] = (r * (Ya[l + NRV * qr]) + Ya[l + NRV * ql]) * r1;
+        double ht_c = (r * ht[qr] + ht[ql]) * r1;
+        double et_c = (r * et[qr] + et[ql]) * r1;
+        /* EOS-compatible average state: src/module_weno.f90:750-761 */
+        double W_i_c = ora_wmix_i_scalar(thd, Yc);
+        double T_c = (ht_c - et_c) / W_i_c;
+        double cp_c = ora_cp_scalar(thd, T_c, Yc);
+        double gam_c = thd->gam2 * W_i_c / cp_c;
+        gam_c = 1.0 / (1.0 - gam_c);
+        double cs_c = sqrt(gam_c * T_c * W_i_c);
+        double ha_c[NRV], psi_c[NRV];
+        ora_ha_scalar(thd, T_c, ha_c);
+        double wrk = (gam_c / (gam_c - 1.0)) * T_c;
+        for (int l = 0; l < NRV; l++) {
+          psi_c[l] = -(r * ha_c[l] + ha_c[l]) * r1;
+          psi_c[l] = psi_c[l] + wrk * thd->Wc_i[l];
+        }
+        double cs_c_i = 1.0 / cs_c, cs_c_2 = cs_c * cs_c;
+        double ux_c = uc[0], vy_c = uc[1], wz_c = uc[2];
+        double q = 0.5 * (ux_c * ux_c + vy_c * vy_c + wz_c * wz_c);
+        double b2 = (gam_c - 1.0) / cs_c_2;
+        double b1 = b2 * q;
+        double xi = b1 - b2 * ht_c;
+        double un_c = uc[cn - 1];
+        /* left eigenvectors: src/module_weno.f90:775-874 (and y/z rotations) */
+        double el[NV][NV], er[NV][NV];
+        memset(el, 0, sizeof el);
+        memset(er, 0, sizeof er);
+        el[0][0] = 0.5 * (b1 + un_c * cs_c_i);
+        el[2][0] = 0.5 * (b1 - un_c * cs_c_i);
+        for (int c = 1; c <= 3; c++) {
+          if (c == cn) {
+            el[0][c] = -0.5 * (b2 * uc[c - 1] + cs_c_i);
+            el[2][c] = -0.5 * (b2 * uc[c - 1] - cs_c_i);
+          } else {
+            el[0][c] = -0.5 * (b2 * uc[c - 1]);
+            el[2][c] = -0.5 * (b2 * uc[c - 1]);
+          }
+        }
+        el[0][4] = 0.5 * b2;
+        el[2][4] = 0.5 * b2;
+        el[1][0] = 1.0 - b1;
+        el[1][1] = b2 * ux_c; el[1][2] = b2 * vy_c; el[1][3] = b2 * wz_c;
+        el[1][4] = -b2;
+        el[3][0] = -uc[t1 - 1]; el[3][t1] = 1.0;
+        el[4][0] = -uc[t2 - 1]; el[4][t2] = 1.0;
+        wrk = 0.5 * b2;
+        for (int l = NIV; l < NV; l++) el[0][l] = wrk * psi_c[l - NIV];
+        for (int l = NIV; l < NV; l++) el[1][l] = -el[0][l] - el[0][l];
+        for (int l = NIV; l < NV; l++) el[2][l] = el[0][l];
+        el[5][0] = (1 + xi) * q;
+        el[5][1] = -(1 + xi) * ux_c;
+        el[5][2] = -(1 + xi) * vy_c;
+        el[5][3] = -(1 + xi) * wz_c;
+        el[5][4] = (1 + xi);
+        for (int l = NIV; l < NV; l++) el[5][l] = xi * psi_c[l - NIV];
+        for (int l = 6; l < NV; l++) {
+          double y = Yc[l - 6];
+          el[l][0] = -b1 * y;
+          el[l][1] = b2 * ux_c * y;
+          el[l][2] = b2 * vy_c * y;
+          el[l][3] = b2 * wz_c * y;
+          el[l][4] = -b2 * y;
+        }
+        for (int s1 = 0; s1 < NRV - 1; s1++)
+          for (int s2 = 0; s2 < NRV; s2++) el[s1 + 6][s2 + 5] = -b2 * Yc[s1] * psi_c[s2];
+        for (int l = 0; l < NRV - 1; l++) el[l + 6][l + 5] = el[l + 6][l + 5] + 1.0;
+        /* right eigenvectors: src/module_weno.f90:881-951 */
+        er[0][0] = 1.0; er[0][1] = 1.0; er[0][2] = 1.0;
+        for (int c = 1; c <= 3; c++) {
+          if (c == cn) {
+            er[c][0] = uc[c - 1] - cs_c;
+            er[c][1] = uc[c - 1];
+            er[c][2] = uc[c - 1] + cs_c;
+          } else {
+            er[c][0] = uc[c - 1]; er[c][1] = uc[c - 1]; er[c][2] = uc[c - 1];
+          }
+        }
+        er[t1][3] = 1.0;
+        er[t2][4] = 1.0;
+        er[4][0] = ht_c - un_c * cs_c;
+        er[4][1] = q;
+        er[4][2] = ht_c + un_c * cs_c;
+        er[4][3] = uc[t1 - 1];
+        er[4][4] = uc[t2 - 1];
+        for (int l = NIV; l < NV; l++) { er[l][0] = Yc[l - NIV]; er[l][2] = Yc[l - NIV]; }
+        for (int l = NIV; l < NV; l++) er[l - 1][l] = 1.0;
+        er[NV - 1][5] = -1.0 / psi_c[NRV - 1];
+        wrk = -1.0 / psi_c[NRV - 1];
+        for (int l = 0; l < NRV - 1; l++) er[NV - 1][l + NIV + 1] = wrk * psi_c[l];
+        /* characteristic LLF + WENO per field: src/module_weno.f90:955-989 */
+        int shk = 0;
+        for (int p = il - ng + 1; p <= il + ng; p++) {
+          long cc = C(p);
+          if (sensor[cc]) shk = 1;
+        }
+        double ghat[NV];
+        for (int m = 0; m < NV; m++) {
+          int lam = (m == 0) ? 0 : (m == 2 ? 2 : 1);
+          double eigenmax = -1.0;
+          for (int st = 0; st < 6; st++) eigenmax = fmax(Ls[st][lam], eigenmax);
+          double gp[6], gm[6];
+          for (int st = 0; st < 6; st++) {
+            int is = il + st + 1 - ng;
+            double wc = 0.0, gc = 0.0;
+            for (int mm = 0; mm < NV; mm++) {
+              wc = wc + el[m][mm] * V(mm, is);
+              gc = gc + el[m][mm] * fs[st][mm];
+            }
+            gp[st] = 0.5 * (gc + eigenmax * wc);
+            gm[st] = gc - gp[st];
+          }
+          double gl, gr;
+          if (shk && weno_avg) ora_wenorec(gp, gm, &gl, &gr);
+          else ora_wenorec_nw(gp, gm, &gl, &gr);
+          ghat[m] = gl + gr;
+        }
+        for (int m = 0; m < NV; m++) {
+          double s = 0.0;
+          for (int mm = 0; mm < NV; mm++) s = s + er[m][mm] * ghat[mm];
+          fhat[m * (n + 1) + (il + 1)] = s;
+        }
+      }
+      /* flux difference: src/module_weno.f90:998-1020 */
+      for (int m = 0; m < NV; m++)
+        for (int ic = 0; ic < n; ic++) {
+          double df = (fhat[m * (n + 1) + ic + 1] - fhat[m * (n + 1) + ic]) * d_i[ic + ng];
+          long cc = (long)m * nc + C(ic);
+          if (dir == 0) fl[cc] = df * 1.0;
+          else fl[cc] = fl[cc] + df * 1.0;
+        }
+#undef V
+#undef C
+    }
+  free(rho_i); free(P); free(gam); free(cs); free(vel); free(ht); free(et); free(Ya);
+  free(fhat);
+  return status;
+}
+
+/* ------------------------------------------------- primitive variables */
+
+/* Wmix_i: src/module_thermodynamics.f90:700-728 */
+void ora_wmix_i(int domain, const ora_grid *g, const ora_thd *thd, const double *v, double *W_i) {
+  int i0, i1, j0, j1, k0, k1;
+  long nc = ncell(g);
+  domain_select(g, domain, &i0, &i1, &j0, &j1, &k0, &k1);
+  for (int k = k0; k <= k1; k++)
+    for (int j = j0; j <= j1; j++)
+      for (int i = i0; i <= i1; i++) {
+        long c = cidx(g, i, j, k);
+        double w = 0.0;
+        for (int l = 0; l < NRV; l++) w = w + (v[(NIV + l) * nc + c] / v[c]) * thd->Wc_i[l];
+        W_i[c] = w;
+      }
+}
+
+/* force_sumY (npv = 0 branch): src/module_thermodynamics.f90:1709-1760 */
+void ora_force_sumY(int domain, const ora_grid *g, double *v) {
+  int i0, i1, j0, j1, k0, k1;
+  long nc = ncell(g);
+  domain_select(g, domain, &i0, &i1, &j0, &j1, &k0, &k1);
+  for (int k = k0; k <= k1; k++)
+    for (int j = j0; j <= j1; j++)
+      for (int i = i0; i <= i1; i++) {
+        long c = cidx(g, i, j, k);
+        double rho = v[c], rho_i = 1.0 / rho, sum = 0.0, Y;
+        for (int l = 0; l < NRV - 1; l++) {
+          Y = v[(NIV + l) * nc + c] * rho_i;
+          Y = fmin(fmax(Y, 0.0), 1.0);
+          v[(NIV + l) * nc + c] = rho * Y;
+          sum = sum + Y;
+        }
+        Y = 1.0 - sum;
+        Y = fmin(fmax(Y, 0.0), 1.0);
+        v[(NIV + NRV - 1) * nc + c] = rho * Y;
+      }
+}
+
+/* per-cell body of newton: src/module_thermodynamics.f90:271-508.  Writes
+ * cp and ha at the current T and returns DF = F / (sum Y Wc_i - cp/gam2). */
+static double newton_cell(const ora_thd *thd, const double *vc, double T, double *cp_out,
+                          double *ha_out) {
+  double Tinf_i = 1.0 / thd->Tinf, onethird = 1.0 / 3.0, gam2_i = 1.0 / thd->gam2;
+  double rho_i = 1.0 / vc[0];
+  double F = -vc[4] * rho_i + 0.5 * (vc[1] * vc[1] + vc[2] * vc[2] + vc[3] * vc[3]) *
+                                  (rho_i * rho_i);
+  double Ya[NRV];
+  for (int is = 0; is < NRV; is++) Ya[is] = vc[NIV + is] * rho_i;
+  double cp = 0.0;
+  for (int is = 0; is < NRV; is++) {
+    double t1 = T * thd->Tinf, t2 = t1 * t1, t3 = t2 * t1, t4 = t3 * t1, t5 = t4 * t1;
+    double wrk1 = -T;
+    const double *hl = thd->HeatCap[is][0], *hh = thd->HeatCap[is][1];
+    double fh = thd->Wc_i[is] * Tinf_i;
+    double fc = Ya[is] * thd->gam2 * thd->Wc_i[is];
+    double h = 0.0;
+    if (t1 < thd->Tpolymin[is]) {
+      t1 = thd->Tpolymin[is]; t2 = t1 * t1; t3 = t2 * t1; t4 = t3 * t1; t5 = t4 * t1;
+      double ha0 = h_poly(hl, t1, t2, t3, t4, t5, onethird);
+      double dha0 = dh_poly(hl, t1, t2, t3, t4);
+      h = h + fh * (dha0 * (T * thd->Tinf - thd->Tpolymin[is]) + ha0);
+      double cp0 = dha0;
+      double dcp0 = hl[1] + hl[2] * t1 * 2.0 + hl[3] * t2 * 3.0 + hl[4] * t3 * 4.0;
+      cp = cp + fc * (dcp0 * (T * thd->Tinf - thd->Tpolymin[is]) + cp0);
+    } else if (t1 < thd->Tpolylim[is]) {
+      h = h + fh * h_poly(hl, t1, t2, t3, t4, t5, onethird);
+      cp = cp + fc * dh_poly(hl, t1, t2, t3, t4);
+    } else if (t1 < thd->Tpolymax[is]) {
+      h = h + fh * h_poly(hh, t1, t2, t3, t4, t5, onethird);
+      cp = cp + fc * dh_poly(hh, t1, t2, t3, t4);
+    } else {
+      t1 = thd->Tpolymax[is]; t2 = t1 * t1; t3 = t2 * t1; t4 = t3 * t1; t5 = t4 * t1;
+      double ha0 = h_poly(hh, t1, t2, t3, t4, t5, onethird);
+      double dha0 = dh_poly(hh, t1, t2, t3, t4);
+      h = h + fh * (dha0 * (T * thd->Tinf - thd->Tpolymax[is]) + ha0);
+      double cp0 = dha0;
+      double dcp0 = hh[1] + hh[2] * t1 * 2.0 + hh[3] * t2 * 3.0 + hh[4] * t3 * 4.0;
+      cp = cp + fc * (dcp0 * (T * thd->Tinf - thd->Tpolymax[is]) + cp0);
+    }
+    ha_out[is] = h;
+    F = F + Ya[is] * (h + wrk1 * thd->Wc_i[is]);
+  }
+  double wrk = 0.0;
+  for (int is = 0; is < NRV; is++) wrk = wrk + Ya[is] * thd->Wc_i[is];
+  *cp_out = cp;
+  return F / (wrk - cp * gam2_i);
+}
+
+/* prim_inv_var: src/module_thermodynamics.f90:79-186.  All cells of the
+ * domain iterate together until sqrt(max DF^2) < 1e-5 or 200 iterations;
+ * cp and ha keep the values of the last evaluation (pre-update T).  The
+ * negative-temperature repair branch (:125-157) is reported as status -2
+ * instead of being applied. */
+int ora_prim_inv_var(int domain, const ora_grid *g, const ora_thd *thd, double *v, double *W_i,
+                     double *T, double *cp, double *ha, int *iters) {
+  const int itmax = 200;
+  const double eps_temp = 1.0e-5;
+  int i0, i1, j0, j1, k0, k1;
+  long nc = ncell(g);
+  domain_select(g, domain, &i0, &i1, &j0, &j1, &k0, &k1);
+  ora_wmix_i(domain, g, thd, v, W_i);
+  int it = 0, status = 0;
+  double error = 0.0;
+  for (;;) {
+    it = it + 1;
+    if (it > itmax) break;
+    double maxdf = -INFINITY;
+    int nan_seen = 0;
+    for (int k = k0; k <= k1; k++)
+      for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+          long c = cidx(g, i, j, k);
+          double vc[NV], hc[NRV], cpc;
+          for (int l = 0; l < NV; l++) vc[l] = v[l * nc + c];
+          double DF = newton_cell(thd, vc, T[c], &cpc, hc);
+          cp[c] = cpc;
+          for (int l = 0; l < NRV; l++) ha[l * nc + c] = hc[l];
+          T[c] = T[c] + DF;
+          DF = DF * DF;
+          if (T[c] < 0.0) status = -2;
+          if (DF != DF) nan_seen = 1;
+          maxdf = fmax(maxdf, DF);
+        }
+    error = nan_seen ? NAN : sqrt(maxdf);
+    if (error < eps_temp) break;
+    if (status) break;
+  }
+  if (iters) *iters = it;
+  if (error != error) status = -3;
+  return status;
+}
+
+/* upd_prim_var_woT: src/module_thermodynamics.f90:522-689 */
+void ora_upd_prim_var_woT(int domain, const ora_grid *g, const ora_thd *thd, const double *v,
+                          const double *T, double *W_i, double *cp, double *ha) {
+  int i0, i1, j0, j1, k0, k1;
+  long nc = ncell(g);
+  domain_select(g, domain, &i0, &i1, &j0, &j1, &k0, &k1);
+  ora_wmix_i(domain, g, thd, v, W_i);
+  for (int k = k0; k <= k1; k++)
+    for (int j = j0; j <= j1; j++)
+      for (int i = i0; i <= i1; i++) {
+        long c = cidx(g, i, j, k);
+        double vc[NV], hc[NRV], cpc;
+        for (int l = 0; l < NV; l++) vc[l] = v[l * nc + c];
+        /* the same per-species branches as newton; the F part is unused */
+        (void)newton_cell(thd, vc, T[c], &cpc, hc);
+        cp[c] = cpc;
+        for (int l = 0; l < NRV; l++) ha[l * nc + c] = hc[l];
+      }
+}
+
+/* comm_cons on one rank with periodic faces: src/module_parallel.f90:1715-1777
+ * (self-exchange through the periodic Cartesian communicator; faces only,
+ * edges and corners are not written). */
+void ora_comm_cons(const ora_grid *g, double *v) {
+  int nx = g->nx, ny = g->ny, nz = g->nz, ng = g->ng;
+  long nc = ncell(g);
+  for (int l = 0; l < NV; l++) {
+    double *a = v + l * nc;
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int s = 1; s <= ng; s++) {
+          a[cidx(g, -s, j, k)] = a[cidx(g, nx - s, j, k)];
+          a[cidx(g, nx - 1 + s, j, k)] = a[cidx(g, s - 1, j, k)];
+        }
+    if (g->ndim >= 2)
+      for (int k = 0; k < nz; k++)
+        for (int i = 0; i < nx; i++)
+          for (int s = 1; s <= ng; s++) {
+            a[cidx(g, i, -s, k)] = a[cidx(g, i, ny - s, k)];
+            a[cidx(g, i, ny - 1 + s, k)] = a[cidx(g, i, s - 1, k)];
+          }
+    if (g->ndim == 3)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++)
+          for (int s = 1; s <= ng; s++) {
+            a[cidx(g, i, j, -s)] = a[cidx(g, i, j, nz - s)];
+            a[cidx(g, i, j, nz - 1 + s)] = a[cidx(g, i, j, s - 1)];
+          }
+  }
+}
+
+/* upd_prim_var_ghost: src/module_BCs.f90:254-284 (faces -1,1,-2,2,-3,3) */
+int ora_upd_prim_var_ghost(const ora_grid *g, const ora_thd *thd, int forcesum, double *v,
+                           double *W_i, double *T, double *cp, double *ha) {
+  static const int face[6] = {-1, 1, -2, 2, -3, 3};
+  int st = 0;
+  for (int l = 0; l < 2 * g->ndim; l++) {
+    if (forcesum) ora_force_sumY(face[l], g, v);
+    int s = ora_prim_inv_var(face[l], g, thd, v, W_i, T, cp, ha, NULL);
+    if (s && !st) st = s;
+  }
+  return st;
+}
+
+/* shock_sensor_modified_adams_shariff: src/module_tools.f90:2733-2814.
+ * Sticky: flags are only ever set. */
+void ora_sensor_adams_shariff(const ora_grid *g, double max_rel_weight, const double *v,
+                              const double *T, const double *W_i, uint8_t *sensor) {
+  int nx = g->nx, ny = g->ny, nz = g->nz;
+  long nc = ncell(g);
+  (void)nc;
+  for (int d = 0; d < g->ndim; d++) {
+    int n = (d == 0) ? nx : (d == 1 ? ny : nz);
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          int p = (d == 0) ? i : (d == 1 ? j : k);
+          if (p >= n - 1) continue;
+          int i2 = i + (d == 0), j2 = j + (d == 1), k2 = k + (d == 2);
+          long c = cidx(g, i, j, k), c2 = cidx(g, i2, j2, k2);
+          double ri = 1.0 / v[c], ri2 = 1.0 / v[c2];
+          double P = v[c] * T[c] * W_i[c], P2 = v[c2] * T[c2] * W_i[c2];
+          double drho = fabs(v[c2] - v[c]) * fmin(ri, ri2);
+          double dpres = fabs(P2 - P) / fmin(P, P2);
+          if (drho > max_rel_weight && dpres > max_rel_weight) sensor[c] = 1;
+        }
+  }
+}
+
+/* --------------------------------------------------------- transport */
+
+/* prim_vis_vars for one cell: src/module_thermodynamics.f90:989-1230 */
+void ora_transport_cell(const ora_thd *thd, double rho, double T, double W_i, const double *Xa,
+                        double *dm, double *mu, double *ct) {
+  const double eps = 1.0e-12; /* epsm12_asp, src/thd/param_thd.f90:19 */
+  double Kl = 0.5 * 1.0e-5 / thd->laminf;
+  double Ku = 1.0e-1 / thd->muinf;
+  double lt[4];
+  lt[0] = 1.0;
+  lt[1] = log(T * thd->Tinf);
+  lt[2] = lt[1] * lt[1];
+  lt[3] = lt[2] * lt[1];
+  double P = rho * T * W_i;
+  for (int m = 0; m < NRV; m++) {
+    double Kd = 1.0e-4 * thd->p0 / (thd->pinf * thd->Dinf);
+    double wrk1 = 0.0, d = 0.0;
+    for (int l = 0; l < NRV; l++) {
+      if (m == l) continue;
+      double dkj = 0.0;
+      for (int n = 0; n < 4; n++) dkj = dkj + thd->CofD[m][l][n] * lt[n];
+      wrk1 = wrk1 + (Xa[l] + eps) * thd->Wc[l] * W_i;
+      d = d + (Xa[l] + eps) * (1.0 / exp(dkj));
+    }
+    dm[m] = wrk1 * Kd / (d * P);
+  }
+  double w1 = 0.0, c = 0.0;
+  for (int l = 0; l < NRV; l++) {
+    double s = 0.0;
+    for (int n = 0; n < 4; n++) s = s + thd->CofLam[l][n] * lt[n];
+    double c1 = exp(s), c2 = Xa[l];
+    c = c + c2 / c1;
+    w1 = w1 + c2 * c1;
+  }
+  *ct = Kl * (1.0 / c + w1);
+  double eta[NRV];
+  for (int l = 0; l < NRV; l++) {
+    double s = 0.0;
+    for (int n = 0; n < 4; n++) s = s + thd->CofEta[l][n] * lt[n];
+    eta[l] = exp(s);
+  }
+  double mus = 0.0;
+  for (int m = 0; m < NRV; m++) {
+    double c4 = 0.0;
+    for (int l = 0; l < NRV; l++) {
+      double c1 = thd->wilke_c1[m][l];
+      double c2 = thd->wilke_c2[m][l];
+      double c3 = 1.0 + c1 * sqrt(eta[m] / eta[l]);
+      c2 = c3 * c3 * c2;
+      c4 = c4 + c2 * Xa[l];
+    }
+    mus = mus + Ku * (eta[m] * Xa[m] / c4);
+  }
+  *mu = mus;
+}
+
+/* 6th-order central first derivative, periodic/MPI branch (alpha6):
+ * src/module_deriv.f90:37-40, 212-469 (dx), 926-1090 (dx_fixed1),
+ * 1644-1720 (dx_fixed2).  The z operators of the reference always use the
+ * 8-point alph8 stencil (src/module_deriv.f90:753-763), which reads k+-4 out
+ * of an ng = 3 halo; this restatement uses alpha6 in z as in x and y. */
+static const double ALPHA6[7] = {-1.0 / 60.0, 9.0 / 60.0, -45.0 / 60.0, 0.0,
+                                 45.0 / 60.0, -9.0 / 60.0, 1.0 / 60.0};
+
+static inline double deriv7(const double *a, long c, long stride, double dinv) {
+  double dd = 0.0;
+  for (int l = -3; l <= 3; l++) dd = dd + ALPHA6[l + 3] * a[c + l * stride];
+  return dd * dinv;
+}
+
+/* viscflux_xy / viscflux_xyz: src/module_viscflux.f90:721-1135, 1660-2189.
+ * Periodic faces only (no wall, symmetry or injection branches). */
+int ora_viscflux(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const double *dx_i,
+                 const double *dy_i, const double *dz_i, const double *T, const double *W_i,
+                 const double *ha, const double *v, double *fl) {
+  const int ndim = g->ndim, ng = g->ng;
+  const long nc = ncell(g);
+  const int nfluid = ndim * (ndim + 1);
+  const int nderiv = nfluid + ndim * NRV;
+  const double twothirds = 2.0 / 3.0;
+  const double rvd0 = adi->sqgmr / adi->Sc;
+  const double rlam0 = adi->sqgmr * adi->ggmopr;
+  const long sx = 1, sy = NXg(g), sz = NXg(g) * NYg(g);
+  double *vel = calloc((size_t)3 * nc, sizeof(double));
+  double *Xa = calloc((size_t)NRV * nc, sizeof(double));
+  double *fd = calloc((size_t)nderiv * nc, sizeof(double));
+  int nflux = (ndim == 3) ? 9 : 5; /* sx_x..sz_z / q_x.. */
+  double *sf = calloc((size_t)nflux * nc, sizeof(double));
+  int i0, i1, j0, j1, k0, k1;
+  /* molefrac + velocities on interior and face ghosts (:1738-1760) */
+  for (int d = -ndim; d <= ndim; d++) {
+    domain_select(g, d, &i0, &i1, &j0, &j1, &k0, &k1);
+    for (int k = k0; k <= k1; k++)
+      for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+          long c = cidx(g, i, j, k);
+          for (int l = 0; l < NRV; l++)
+            Xa[l * nc + c] = v[(NIV + l) * nc + c] / (v[c] * W_i[c] * thd->Wc[l]);
+          double rho_i = 1.0 / v[c];
+          for (int q = 0; q < ndim; q++) vel[q * nc + c] = v[(1 + q) * nc + c] * rho_i;
+        }
+  }
+  /* first derivatives on the interior (d?_fixed1) */
+  const double *dinv[3] = {dx_i, dy_i, dz_i};
+  const long strd[3] = {sx, sy, sz};
+  for (int k = 0; k < g->nz; k++)
+    for (int j = 0; j < g->ny; j++)
+      for (int i = 0; i < g->nx; i++) {
+        long c = cidx(g, i, j, k);
+        int idx3[3] = {i + ng, j + ng, k + ng};
+        int slot = 0;
+        for (int q = 0; q < ndim; q++)          /* du/dx_j, dv/dx_j, dw/dx_j */
+          for (int dd = 0; dd < ndim; dd++)
+            fd[(slot++) * nc + c] = deriv7(vel + q * nc, c, strd[dd], dinv[dd][idx3[dd]]);
+        for (int dd = 0; dd < ndim; dd++)       /* dT/dx_j */
+          fd[(slot++) * nc + c] = deriv7(T, c, strd[dd], dinv[dd][idx3[dd]]);
+        for (int l = 0; l < NRV; l++)           /* dX_l/dx_j */
+          for (int dd = 0; dd < ndim; dd++)
+            fd[(slot++) * nc + c] = deriv7(Xa + l * nc, c, strd[dd], dinv[dd][idx3[dd]]);
+      }
+  /* comm_deriv: periodic self-exchange of fd on the faces (:1923) */
+  {
+    ora_grid gd = *g;
+    for (int f = 0; f < nderiv; f += NV) {
+      /* reuse comm_cons on blocks of NV fields */
+      int nb = (nderiv - f < NV) ? nderiv - f : NV;
+      if (nb == NV) ora_comm_cons(&gd, fd + (long)f * nc);
+      else {
+        double *tmp = calloc((size_t)NV * nc, sizeof(double));
+        memcpy(tmp, fd + (long)f * nc, sizeof(double) * nb * nc);
+        ora_comm_cons(&gd, tmp);
+        memcpy(fd + (long)f * nc, tmp, sizeof(double) * nb * nc);
+        free(tmp);
+      }
+    }
+  }
+  /* fluxes on interior and face ghosts (:1930-2041) */
+  for (int d = -ndim; d <= ndim; d++) {
+    domain_select(g, d, &i0, &i1, &j0, &j1, &k0, &k1);
+    for (int k = k0; k <= k1; k++)
+      for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+          long c = cidx(g, i, j, k);
+          double X[NRV], dm[NRV], mu, ct;
+          for (int l = 0; l < NRV; l++) X[l] = Xa[l * nc + c];
+          ora_transport_cell(thd, v[c], T[c], W_i[c], X, dm, &mu, &ct);
+          double rho_i = 1.0 / v[c];
+          double uxs = v[nc + c] * rho_i, vys = v[2 * nc + c] * rho_i, wzs = v[3 * nc + c] * rho_i;
+          double rmu = -adi->sqgmr * mu;
+          double rvd = rvd0 * v[c];
+          double rlam = -rlam0 * ct;
+#define FD(s) fd[(long)((s) - 1) * nc + c]
+          if (ndim == 3) {
+            double sxx = twothirds * rmu * (FD(1) + FD(1) - (FD(5) + FD(9)));
+            double sxy = rmu * (FD(4) + FD(2));
+            double sxz = rmu * (FD(7) + FD(3));
+            double syy = twothirds * rmu * (FD(5) + FD(5) - (FD(1) + FD(9)));
+            double syz = rmu * (FD(8) + FD(6));
+            double szz = twothirds * rmu * (FD(9) + FD(9) - (FD(1) + FD(5)));
+            double vdcx = 0.0, vdcy = 0.0, vdcz = 0.0;
+            for (int l = 1; l <= NRV; l++) {
+              double wrk = dm[l - 1] * thd->Wc[l - 1] * W_i[c];
+              vdcx = vdcx + wrk * FD(nfluid + ndim * l - 2);
+              vdcy = vdcy + wrk * FD(nfluid + ndim * l - 1);
+              vdcz = vdcz + wrk * FD(nfluid + ndim * l);
+            }
+            double vdhx = 0.0, vdhy = 0.0, vdhz = 0.0;
+            for (int l = 1; l <= NRV; l++) {
+              double Ya = v[(NIV + l - 1) * nc + c] * rho_i;
+              double vdtx = vdcx * Ya, vdty = vdcy * Ya, vdtz = vdcz * Ya;
+              double wrk = -dm[l - 1] * thd->Wc[l - 1] * W_i[c];
+              vdtx = vdtx + wrk * FD(nfluid + ndim * l - 2);
+              vdty = vdty + wrk * FD(nfluid + ndim * l - 1);
+              vdtz = vdtz + wrk * FD(nfluid + ndim * l);
+              FD(nfluid + ndim * l - 2) = rvd * vdtx;
+              FD(nfluid + ndim * l - 1) = rvd * vdty;
+              FD(nfluid + ndim * l) = rvd * vdtz;
+              double h = ha[(long)(l - 1) * nc + c];
+              vdhx = vdhx + h * FD(nfluid + ndim * l - 2);
+              vdhy = vdhy + h * FD(nfluid + ndim * l - 1);
+              vdhz = vdhz + h * FD(nfluid + ndim * l);
+            }
+            sf[0 * nc + c] = sxx; sf[1 * nc + c] = sxy; sf[2 * nc + c] = sxz;
+            sf[3 * nc + c] = syy; sf[4 * nc + c] = syz; sf[5 * nc + c] = szz;
+            sf[6 * nc + c] = uxs * sxx + vys * sxy + wzs * sxz + rlam * FD(10) + vdhx;
+            sf[7 * nc + c] = uxs * sxy + vys * syy + wzs * syz + rlam * FD(11) + vdhy;
+            sf[8 * nc + c] = uxs * sxz + vys * syz + wzs * szz + rlam * FD(12) + vdhz;
+          } else {
+            double sxx = twothirds * rmu * (FD(1) + FD(1) - FD(4));
+            double sxy = rmu * (FD(3) + FD(2));
+            double syy = twothirds * rmu * (FD(4) + FD(4) - FD(1));
+            double vdcx = 0.0, vdcy = 0.0;
+            for (int l = 1; l <= NRV; l++) {
+              double wrk = dm[l - 1] * thd->Wc[l - 1] * W_i[c];
+              vdcx = vdcx + wrk * FD(nfluid + ndim * l - 1);
+              vdcy = vdcy + wrk * FD(nfluid + ndim * l);
+            }
+            double vdhx = 0.0, vdhy = 0.0;
+            for (int l = 1; l <= NRV; l++) {
+              double Ya = v[(NIV + l - 1) * nc + c] * rho_i;
+              double vdtx = vdcx * Ya, vdty = vdcy * Ya;
+              double wrk = -dm[l - 1] * thd->Wc[l - 1] * W_i[c];
+              vdtx = vdtx + wrk * FD(nfluid + ndim * l - 1);
+              vdty = vdty + wrk * FD(nfluid + ndim * l);
+              FD(nfluid + ndim * l - 1) = rvd * vdtx;
+              FD(nfluid + ndim * l) = rvd * vdty;
+              double h = ha[(long)(l - 1) * nc + c];
+              vdhx = vdhx + h * FD(nfluid + ndim * l - 1);
+              vdhy = vdhy + h * FD(nfluid + ndim * l);
+            }
+            sf[0 * nc + c] = sxx; sf[1 * nc + c] = sxy; sf[2 * nc + c] = syy;
+            sf[3 * nc + c] = uxs * sxx + vys * sxy + rlam * FD(5) + vdhx;
+            sf[4 * nc + c] = uxs * sxy + vys * syy + rlam * FD(6) + vdhy;
+          }
+#undef FD
+        }
+  }
+  /* divergence (dx/dy/dz, d?_fixed2) accumulated into fl (:2087-2183) */
+  for (int k = 0; k < g->nz; k++)
+    for (int j = 0; j < g->ny; j++)
+      for (int i = 0; i < g->nx; i++) {
+        long c = cidx(g, i, j, k);
+        double dxi = dx_i[i + ng], dyi = dy_i[j + ng], dzi = dz_i[k + ng];
+        if (ndim == 3) {
+          double wx, wy, wz;
+          wx = deriv7(sf + 0 * nc, c, sx, dxi); wy = deriv7(sf + 1 * nc, c, sy, dyi);
+          wz = deriv7(sf + 2 * nc, c, sz, dzi);
+          fl[1 * nc + c] = fl[1 * nc + c] + (wx + wy + wz);
+          wx = deriv7(sf + 1 * nc, c, sx, dxi); wy = deriv7(sf + 3 * nc, c, sy, dyi);
+          wz = deriv7(sf + 4 * nc, c, sz, dzi);
+          fl[2 * nc + c] = fl[2 * nc + c] + (wx + wy + wz);
+          wx = deriv7(sf + 2 * nc, c, sx, dxi); wy = deriv7(sf + 4 * nc, c, sy, dyi);
+          wz = deriv7(sf + 5 * nc, c, sz, dzi);
+          fl[3 * nc + c] = fl[3 * nc + c] + (wx + wy + wz);
+          wx = deriv7(sf + 6 * nc, c, sx, dxi); wy = deriv7(sf + 7 * nc, c, sy, dyi);
+          wz = deriv7(sf + 8 * nc, c, sz, dzi);
+          fl[4 * nc + c] = fl[4 * nc + c] + (wx + wy + wz);
+          for (int l = 1; l <= NRV; l++) {
+            wx = deriv7(fd + (long)(nfluid + ndim * l - 3) * nc, c, sx, dxi);
+            wy = deriv7(fd + (long)(nfluid + ndim * l - 2) * nc, c, sy, dyi);
+            wz = deriv7(fd + (long)(nfluid + ndim * l - 1) * nc, c, sz, dzi);
+            fl[(NIV + l - 1) * nc + c] = fl[(NIV + l - 1) * nc + c] + (wx + wy + wz);
+          }
+        } else {
+          double wx, wy;
+          wx = deriv7(sf + 0 * nc, c, sx, dxi); wy = deriv7(sf + 1 * nc, c, sy, dyi);
+          fl[1 * nc + c] = fl[1 * nc + c] + (wx + wy);
+          wx = deriv7(sf + 1 * nc, c, sx, dxi); wy = deriv7(sf + 2 * nc, c, sy, dyi);
+          fl[2 * nc + c] = fl[2 * nc + c] + (wx + wy);
+          wx = deriv7(sf + 3 * nc, c, sx, dxi); wy = deriv7(sf + 4 * nc, c, sy, dyi);
+          fl[4 * nc + c] = fl[4 * nc + c] + (wx + wy);
+          for (int l = 1; l <= NRV; l++) {
+            wx = deriv7(fd + (long)(nfluid + ndim * l - 2) * nc, c, sx, dxi);
+            wy = deriv7(fd + (long)(nfluid + ndim * l - 1) * nc, c, sy, dyi);
+            fl[(NIV + l - 1) * nc + c] = fl[(NIV + l - 1) * nc + c] + (wx + wy);
+          }
+        }
+      }
+  free(vel); free(Xa); free(fd); free(sf);
+  return 0;
+}
+
+/* ------------------------------------------------------------ SSP-RK3 */
+
+/* rk3: src/module_solver.f90:694-856 (periodic faces, no LES/EGLIB/NSCBC,
+ * Adams-Shariff sensor).  f_out (optional, ncell*NV) receives the RHS of
+ * the first stage. */
+int ora_rk3(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const ora_opts *op,
+            double dt, const double *dx_i, const double *dy_i, const double *dz_i, double *T,
+            double *W_i, double *cp, double *ha, uint8_t *sensor, double *v, double *f_out) {
+  static const double bk3[3] = {0.00, 0.75, 1.00 / 3.00};
+  static const double ck3[3] = {1.00, 0.25, 2.00 / 3.00};
+  const long nc = ncell(g);
+  double *v0 = malloc(sizeof(double) * NV * nc);
+  double *f = calloc((size_t)NV * nc, sizeof(double));
+  memcpy(v0, v, sizeof(double) * NV * nc);
+  int status = 0;
+  for (int ik = 0; ik < 3; ik++) {
+    ora_sensor_adams_shariff(g, op->max_rel_weight, v, T, W_i, sensor);
+    int s;
+    const double *dd[3] = {dx_i, dy_i, dz_i};
+    for (int d = 0; d < g->ndim; d++) {
+      s = ora_euler_llf(d, g, thd, dd[d], T, W_i, cp, ha, sensor, op->weno_avg, v, f);
+      if (s && !status) status = s;
+    }
+    if (op->vis) ora_viscflux(g, thd, adi, dx_i, dy_i, dz_i, T, W_i, ha, v, f);
+    if (ik == 0 && f_out) memcpy(f_out, f, sizeof(double) * NV * nc);
+    for (int l = 0; l < NV; l++)
+      for (int k = 0; k < g->nz; k++)
+        for (int j = 0; j < g->ny; j++)
+          for (int i = 0; i < g->nx; i++) {
+            long c = (long)l * nc + cidx(g, i, j, k);
+            v[c] = v0[c] * bk3[ik] + ck3[ik] * (v[c] - dt * f[c]);
+          }
+    if (op->forcesum) ora_force_sumY(0, g, v);
+    s = ora_prim_inv_var(0, g, thd, v, W_i, T, cp, ha, NULL);
+    if (s && !status) status = s;
+    ora_comm_cons(g, v);
+    s = ora_upd_prim_var_ghost(g, thd, op->forcesum, v, W_i, T, cp, ha);
+    if (s && !status) status = s;
+  }
+  free(v0); free(f);
+  return status;
+}
+
+/* timestep: src/module_solver.f90:202-408 (no LES: mu_SGS = 0; FIXEDDT off) */
+void ora_timestep(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const ora_opts *op,
+                  const double *dx_i, const double *dy_i, const double *dz_i, const double *cp,
+                  const double *W_i, const double *T, const double *v, double DTmax,
+                  double *dt_4, double *dtmin) {
+  const long nc = ncell(g);
+  const int ng = g->ng;
+  double rvd0_i = adi->Sc / adi->sqgmr, rlamcp0_i = adi->Pr / adi->sqgmr;
+  double dtc = 1.0;
+  for (int q = 0; q < 4; q++) dt_4[q] = 1.0;
+  for (int k = 0; k < g->nz; k++)
+    for (int j = 0; j < g->ny; j++)
+      for (int i = 0; i < g->nx; i++) {
+        long c = cidx(g, i, j, k);
+        double rho_i = 1.0 / v[c];
+        double ux = v[nc + c] * rho_i, vy = v[2 * nc + c] * rho_i, wz = v[3 * nc + c] * rho_i;
+        double gamma = thd->gam2 * W_i[c] / cp[c];
+        gamma = 1.0 / (1.0 - gamma);
+        double P = v[c] * T[c] * W_i[c];
+        double csound = sqrt(gamma * P * rho_i);
+        double dtcx_i = (fabs(ux) + csound) * dx_i[i + ng];
+        double dtcy_i = (fabs(vy) + csound) * dy_i[j + ng];
+        double dtcz_i = (fabs(wz) + csound) * dz_i[k + ng];
+        dtc = fmax(fmax(fmax(dtc, dtcx_i), dtcy_i), dtcz_i);
+      }
+  dt_4[0] = op->CFL / dtc;
+  if (op->vis && op->vis_dt) {
+    /* molefrac + prim_vis_vars_wo_mu + filter_width (:297-374) */
+    int ldmax = 0;
+    double diffmax = -INFINITY;
+    double *dmall = malloc(sizeof(double) * NRV * g->nx * g->ny * g->nz);
+    double *ctall = malloc(sizeof(double) * g->nx * g->ny * g->nz);
+    long q = 0;
+    for (int k = 0; k < g->nz; k++)
+      for (int j = 0; j < g->ny; j++)
+        for (int i = 0; i < g->nx; i++, q++) {
+          long c = cidx(g, i, j, k);
+          double X[NRV], dm[NRV], mu, ct;
+          for (int l = 0; l < NRV; l++) X[l] = v[(NIV + l) * nc + c] / (v[c] * W_i[c] * thd->Wc[l]);
+          ora_transport_cell(thd, v[c], T[c], W_i[c], X, dm, &mu, &ct);
+          for (int l = 0; l < NRV; l++) dmall[l * g->nx * g->ny * g->nz + q] = dm[l];
+          ctall[q] = ct;
+        }
+    long ni = (long)g->nx * g->ny * g->nz;
+    diffmax = dmall[0];
+    for (int l = 0; l < NRV; l++)
+      for (long p = 0; p < ni; p++)
+        if (diffmax < dmall[l * ni + p]) { ldmax = l; diffmax = dmall[l * ni + p]; }
+    double m2 = INFINITY, m3 = INFINITY;
+    q = 0;
+    for (int k = 0; k < g->nz; k++)
+      for (int j = 0; j < g->ny; j++)
+        for (int i = 0; i < g->nx; i++, q++) {
+          long c = cidx(g, i, j, k);
+          double delta;
+          if (g->ndim == 3)
+            delta = 1.0 / pow(dx_i[i + ng] * dy_i[j + ng] * dz_i[k + ng], 1.0 / 3.0);
+          else if (g->ndim == 2)
+            delta = 1.0 / sqrt(dx_i[i + ng] * dy_i[j + ng]);
+          else
+            delta = 1.0 / dx_i[i + ng];
+          double d2 = delta * delta;
+          double a = d2 / (dmall[ldmax * ni + q] + adi->Sc * 0.0 / v[c]);
+          double b = d2 * v[c] / (ctall[q] / cp[c] + adi->Pr * 0.0);
+          m2 = fmin(m2, a);
+          m3 = fmin(m3, b);
+        }
+    dt_4[1] = rvd0_i * op->Fo * m2;
+    dt_4[2] = rlamcp0_i * op->Fo * m3;
+    free(dmall); free(ctall);
+  }
+  if (op->reaction) {
+    dt_4[3] = *dtmin * op->CPR / DTmax;
+    dt_4[3] = fmin(dt_4[3], 1.0);
+    dt_4[3] = fmax(dt_4[3], dt_4[0] * op->CFLmin / op->CFL);
+  }
+  double mn = dt_4[0];
+  for (int q2 = 1; q2 < 4; q2++) mn = fmin(mn, dt_4[q2]);
+  *dtmin = mn;
+}
+
+/* ------------------------------------------------------------ kinetics */
+
+/* CKSMH: src/chemkin/vode.f:1474-1535 */
+static void cksmh(const ora_mech *m, double T, double *smh) {
+  double tn[5];
+  tn[0] = log(T) - 1.0;
+  for (int n = 2; n <= 5; n++) tn[n - 1] = pow(T, n - 1) / ((n - 1) * n);
+  for (int k = 0; k < m->kk; k++) {
+    int L = (T > m->tmid[k]) ? 1 : 0;
+    const double *a = m->a[k][L];
+    double sum = 0.0;
+    for (int n = 0; n < 5; n++) sum = sum + tn[n] * a[n];
+    smh[k] = sum + a[6] - a[5] / T;
+  }
+}
+
+static inline double ipow(double x, int n) {
+  double r = 1.0;
+  for (int q = 0; q < n; q++) r *= x;
+  return r;
+}
+
+/* CKWYR = CKRATT + CKYTCR + CKRATX + stoichiometric sum:
+ * src/chemkin/vode.f:1797-1904, 935-1126, 3171-3221, 1130-1314 */
+void ora_ckwyr(const ora_mech *m, double rho, double T, const double *Y, double *wdot) {
+  int ii = m->ii, kk = m->kk;
+  double rkft[ORA_IMAX], rkrt[ORA_IMAX], eqk[ORA_IMAX], smh[ORA_KMAX], c[ORA_KMAX];
+  double alogt = log(T);
+  for (int i = 0; i < ii; i++) rkft[i] = m->A[i] * exp(m->b[i] * alogt - m->EoR[i] / T);
+  cksmh(m, T, smh);
+  for (int i = 0; i < ii; i++) {
+    double sumsmh = 0.0;
+    for (int n = 0; n < 6; n++)
+      if (m->nunk[i][n] != 0) sumsmh = sumsmh + (double)m->nu[i][n] * smh[m->nunk[i][n] - 1];
+    if (sumsmh != 0.0) eqk[i] = exp(fmin(sumsmh, m->exparg));
+    else eqk[i] = 0.0; /* left unset in the reference; unused when the sum is 0 */
+  }
+  double pfac = m->patm / (m->ru * T);
+  for (int i = 0; i < ii; i++) {
+    int nusumk = 0;
+    for (int n = 0; n < 6; n++) nusumk += m->nu[i][n];
+    eqk[i] = eqk[i] * (nusumk >= 0 ? ipow(pfac, nusumk) : 1.0 / ipow(pfac, -nusumk));
+  }
+  for (int i = 0; i < ii; i++) {
+    rkrt[i] = 0.0;
+    if (m->reversible[i]) rkrt[i] = rkft[i] / fmax(eqk[i], m->small_);
+  }
+  for (int k = 0; k < kk; k++) c[k] = rho * Y[k] / m->wt[k];
+  double ctb[ORA_IMAX];
+  for (int i = 0; i < ii; i++) ctb[i] = 1.0;
+  double ctot = 0.0;
+  for (int k = 0; k < kk; k++) ctot = ctot + c[k];
+  for (int i = 0; i < ii; i++) {
+    if (m->ntb[i] < 0) continue;
+    ctb[i] = ctot;
+    for (int l = 0; l < m->ntb[i]; l++)
+      ctb[i] = ctb[i] + (m->tb_eff[i][l] - 1.0) * c[m->tb_k[i][l] - 1];
+  }
+  for (int i = 0; i < ii; i++) {
+    if (!m->fal[i]) continue;
+    const double *fp = m->fpar[i];
+    double rklow = fp[0] * exp(fp[1] * alogt - fp[2] / T);
+    double pr = rklow * ctb[i] / rkft[i];
+    ctb[i] = 1.0;
+    double pcor = pr / (1.0 + pr);
+    if (m->fal[i] > 1) {
+      double prlog = log10(fmax(pr, m->small_));
+      double fcent = (1.0 - fp[3]) * exp(-T / fp[4]) + fp[3] * exp(-T / fp[5]);
+      if (m->fal[i] == 4) fcent = fcent + exp(-fp[6] / T);
+      double fclog = log10(fmax(fcent, m->small_));
+      double xn = 0.75 - 1.27 * fclog;
+      double cprlog = prlog - (0.4 + 0.67 * fclog);
+      double flog = fclog / (1.0 + (cprlog / (xn - 0.14 * cprlog)) * (cprlog / (xn - 0.14 * cprlog)));
+      double fc = pow(10.0, flog);
+      pcor = fc * pcor;
+    }
+    rkft[i] = rkft[i] * pcor;
+    rkrt[i] = rkrt[i] * pcor;
+  }
+  double rkf[ORA_IMAX], rkr[ORA_IMAX];
+  for (int i = 0; i < ii; i++) {
+    rkft[i] = rkft[i] * ctb[i] * 1.0;
+    rkrt[i] = rkrt[i] * ctb[i] * 1.0;
+    rkf[i] = 0.0; rkr[i] = 0.0;
+    if (m->nu[i][0] != 0) {
+      rkf[i] = rkft[i] * ipow(c[m->nunk[i][0] - 1], abs(m->nu[i][0]));
+      rkr[i] = rkrt[i] * ipow(c[m->nunk[i][3] - 1], m->nu[i][3]);
+      if (m->nunk[i][1] != 0) {
+        rkf[i] = rkf[i] * ipow(c[m->nunk[i][1] - 1], abs(m->nu[i][1]));
+        if (m->nunk[i][2] != 0) rkf[i] = rkf[i] * ipow(c[m->nunk[i][2] - 1], abs(m->nu[i][2]));
+      }
+      if (m->nunk[i][4] != 0) {
+        rkr[i] = rkr[i] * ipow(c[m->nunk[i][4] - 1], m->nu[i][4]);
+        if (m->nunk[i][5] != 0) rkr[i] = rkr[i] * ipow(c[m->nunk[i][5] - 1], m->nu[i][5]);
+      }
+    }
+  }
+  for (int k = 0; k < kk; k++) wdot[k] = 0.0;
+  for (int n = 0; n < 6; n++)
+    for (int i = 0; i < ii; i++) {
+      int k = m->nunk[i][n];
+      if (k != 0) wdot[k - 1] = wdot[k - 1] + (rkf[i] - rkr[i]) * (double)m->nu[i][n];
+    }
+}
+
+/* CKCVBS via CKCVMS/CKCPMS: src/chemkin/vode.f:64-226 */
+double ora_ckcvbs(const ora_mech *m, double T, const double *Y) {
+  double tn[5];
+  tn[0] = 1.0;
+  for (int n = 2; n <= 5; n++) tn[n - 1] = pow(T, n - 1);
+  double cvb = 0.0;
+  for (int k = 0; k < m->kk; k++) {
+    int L = (T > m->tmid[k]) ? 1 : 0;
+    const double *a = m->a[k][L];
+    double sum = 0.0;
+    for (int n = 0; n < 5; n++) sum = sum + tn[n] * a[n];
+    double cpms = m->ru * sum / m->wt[k];
+    double cvms = cpms - m->ru / m->wt[k];
+    cvb = cvb + Y[k] * cvms;
+  }
+  return cvb;
+}
+
+/* CKHMS: src/chemkin/vode.f:228-285 */
+void ora_ckhms(const ora_mech *m, double T, double *hms) {
+  double rut = T * m->ru, tn[5];
+  tn[0] = 1.0;
+  for (int n = 2; n <= 5; n++) tn[n - 1] = pow(T, n - 1) / n;
+  for (int k = 0; k < m->kk; k++) {
+    int L = (T > m->tmid[k]) ? 1 : 0;
+    const double *a = m->a[k][L];
+    double sum = 0.0;
+    for (int n = 0; n < 5; n++) sum = sum + tn[n] * a[n];
+    hms[k] = rut * (sum + a[5] / T) / m->wt[k];
+  }
+}
+
+/* FUN, constant-volume adiabatic reactor (OMS = 1): src/chemkin/reaction_v.f:69-114.
+ * z = (T, Y_1..Y_KK), zp = dz/dt. */
+void ora_reactor_rhs(const ora_mech *m, double rho, const double *z, double *zp) {
+  double wdot[ORA_KMAX], h[ORA_KMAX];
+  double cvb = ora_ckcvbs(m, z[0], z + 1);
+  ora_ckwyr(m, rho, z[0], z + 1, wdot);
+  ora_ckhms(m, z[0], h);
+  double rho_i = 1.0 / rho * 1.0;
+  double mru = -m->ru;
+  double sum = 0.0;
+  for (int k = 0; k < m->kk; k++) {
+    zp[k + 1] = wdot[k] * m->wt[k] * rho_i;
+    sum = sum + h[k] * wdot[k] * m->wt[k] + mru * z[0] * wdot[k];
+  }
+  zp[0] = -sum / (rho * cvb) * 1.0;
+}

@@ -1,0 +1,136 @@
+/*
+ * izem_oracle.h -- CPU restatement of the IZEM reacting-flow right-hand side,
+ * time step and CHEMKIN-form kinetics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it.  The product library never links or calls it.
+ *
+ * Every function follows the reference Fortran operation by operation (same
+ * loop order, same left-to-right expression trees, no FMA contraction: build
+ * with -ffp-contract=off) so that results are comparable bit-for-bit with a
+ * HIP build that keeps the same order.  Citations are to
+ * devel-izem/src/<file>:<line> of saras13015/Phd.
+ *
+ * Array layout (identical to the reference allocations
+ * v(sx-ng:ex+ng, sy-ng:ey+ng, sz-ng:ez+ng, 1:nv), Fortran column-major):
+ *   cell (i,j,k), i in [-ng, nx+ng), ... -> (i+ng) + NX*((j+ng) + NY*(k+ng))
+ *   field l of a multi-field array at  l*ncell + cell
+ * Interior index i = 0..nx-1 corresponds to Fortran sx..ex.
+ * Ghost z-planes exist even in 2-D (the reference allocates them too).
+ */
+#ifndef IZEM_ORACLE_H
+#define IZEM_ORACLE_H
+
+#include <stdint.h>
+
+#define ORA_NRV 9
+#define ORA_NIV 5
+#define ORA_NV (ORA_NIV + ORA_NRV)
+
+typedef struct {
+  int nx, ny, nz, ng, ndim;
+} ora_grid;
+
+/* thd_type subset used by the flow kernels (src/thd/type_thd.f90:10-91) */
+typedef struct {
+  double W[ORA_NRV];        /* kg/mol */
+  double Wc[ORA_NRV];       /* W / Winf */
+  double Wc_i[ORA_NRV];     /* 1 / Wc */
+  double HeatCap[ORA_NRV][2][7]; /* [species][range 0=low 1=high][coef] */
+  double Tpolymin[ORA_NRV], Tpolylim[ORA_NRV], Tpolymax[ORA_NRV];
+  double CofD[ORA_NRV][ORA_NRV][4]; /* CofD(n,l,m) stored [l][m][n] */
+  double CofLam[ORA_NRV][4];        /* CofLam(n,l) stored [l][n] */
+  double CofEta[ORA_NRV][4];
+  double wilke_c1[ORA_NRV][ORA_NRV]; /* (W(l)/W(m))**0.25 stored [m][l] */
+  double wilke_c2[ORA_NRV][ORA_NRV]; /* 1/sqrt((1+W(m)/W(l))*8) stored [m][l] */
+  double gam2, Tinf, p0, pinf, Dinf, laminf, muinf;
+} ora_thd;
+
+/* adi_type subset (src/module_adim.f90, values from src/module_tools.f90:453-562) */
+typedef struct {
+  double sqgmr, Sc, Pr, ggmopr, time_ref, T_ref, rho_ref, L_ref;
+} ora_adi;
+
+/* run-time switches of the path (src/module_parameters.f90:53-54) */
+typedef struct {
+  int weno_avg;       /* WENO%WEIGHTS */
+  double max_rel_weight; /* WENO%PERCENT_WEIGHT */
+  int vis, vis_dt, reaction, forcesum;
+  double CFL, CFLmin, Fo, CPR;
+} ora_opts;
+
+/* CHEMKIN-II mechanism tables (the content of the linking file, see
+ * src/chemkin/vode.f:391 CKINIT for the layout it replaces). */
+#define ORA_KMAX 16
+#define ORA_IMAX 48
+#define ORA_TBMAX 12
+typedef struct {
+  int kk, ii;
+  double wt[ORA_KMAX];                 /* g/mol */
+  double a[ORA_KMAX][2][7];            /* NASA-7, range 0 = T <= Tmid */
+  double tmid[ORA_KMAX];
+  double A[ORA_IMAX], b[ORA_IMAX], EoR[ORA_IMAX];
+  int nu[ORA_IMAX][6], nunk[ORA_IMAX][6]; /* 1-based species; slots 0..2 reactants (nu<0), 3..5 products */
+  int reversible[ORA_IMAX];
+  int ntb[ORA_IMAX];                   /* -1: no third body; else number of efficiencies */
+  int tb_k[ORA_IMAX][ORA_TBMAX];       /* 1-based species */
+  double tb_eff[ORA_IMAX][ORA_TBMAX];
+  int fal[ORA_IMAX];                   /* 0 none, 1 Lindemann, 3 Troe-3, 4 Troe-4 */
+  double fpar[ORA_IMAX][8];            /* A0, b0, E0/R, alpha, T***, T*, T** */
+  double ru, patm;                     /* erg/(mol K), dyn/cm^2 */
+  double small_, big_, exparg;
+} ora_mech;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* thermo scalars */
+double ora_wmix_i_scalar(const ora_thd *thd, const double *Y);
+double ora_cp_scalar(const ora_thd *thd, double temp, const double *Y);
+void ora_ha_scalar(const ora_thd *thd, double temp, double *ha);
+
+/* WENO */
+void ora_wenorec(const double *vp, const double *vm, double *vminus, double *vplus);
+void ora_wenorec_nw(const double *vp, const double *vm, double *vminus, double *vplus);
+
+/* flow-solver kernels; all return 0 or a negative status */
+int ora_euler_llf(int dir, const ora_grid *g, const ora_thd *thd, const double *d_i,
+                  const double *T, const double *W_i, const double *cp, const double *ha,
+                  const uint8_t *sensor, int weno_avg, const double *v, double *fl);
+void ora_wmix_i(int domain, const ora_grid *g, const ora_thd *thd, const double *v, double *W_i);
+void ora_force_sumY(int domain, const ora_grid *g, double *v);
+int ora_prim_inv_var(int domain, const ora_grid *g, const ora_thd *thd, double *v,
+                     double *W_i, double *T, double *cp, double *ha, int *iters);
+void ora_upd_prim_var_woT(int domain, const ora_grid *g, const ora_thd *thd, const double *v,
+                          const double *T, double *W_i, double *cp, double *ha);
+void ora_comm_cons(const ora_grid *g, double *v);
+int ora_upd_prim_var_ghost(const ora_grid *g, const ora_thd *thd, int forcesum, double *v,
+                           double *W_i, double *T, double *cp, double *ha);
+void ora_sensor_adams_shariff(const ora_grid *g, double max_rel_weight, const double *v,
+                              const double *T, const double *W_i, uint8_t *sensor);
+void ora_transport_cell(const ora_thd *thd, double rho, double T, double W_i, const double *Xa,
+                        double *dm, double *mu, double *ct);
+int ora_viscflux(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const double *dx_i,
+                 const double *dy_i, const double *dz_i, const double *T, const double *W_i,
+                 const double *ha, const double *v, double *fl);
+int ora_rk3(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const ora_opts *op,
+            double dt, const double *dx_i, const double *dy_i, const double *dz_i,
+            double *T, double *W_i, double *cp, double *ha, uint8_t *sensor, double *v,
+            double *f_out);
+void ora_timestep(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const ora_opts *op,
+                  const double *dx_i, const double *dy_i, const double *dz_i, const double *cp,
+                  const double *W_i, const double *T, const double *v, double DTmax,
+                  double *dt_4, double *dtmin);
+
+/* kinetics (cgs units, as CHEMKIN) */
+void ora_ckwyr(const ora_mech *m, double rho, double T, const double *Y, double *wdot);
+double ora_ckcvbs(const ora_mech *m, double T, const double *Y);
+void ora_ckhms(const ora_mech *m, double T, double *hms);
+void ora_reactor_rhs(const ora_mech *m, double rho, const double *z, double *zp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
