@@ -1,0 +1,166 @@
+/*
+ * izem_hip.h -- C ABI of the MI355X (gfx950) implementation of the IZEM
+ * reacting compressible Navier-Stokes right-hand side, SSP-RK3 time step
+ * and operator-split stiff chemistry.
+ *
+ * Every array argument is a DEVICE pointer to FP64 data in the layout of
+ * the reference's Fortran allocations (devel-izem/src/module_ICs.f90:2364):
+ *     a(sx-ng:ex+ng, sy-ng:ey+ng, sz-ng:ez+ng[, 1:nf])
+ * i.e. column-major, i fastest, ghost cells included, one field after the
+ * other for multi-field arrays (v: nf = 14 = 5 + 9 species; ha: nf = 9).
+ * Ghost z-planes exist in 2-D too.  The 1/dx arrays have n+2*ng entries.
+ * A Fortran host passes c_loc(v(sx-ng,sy-ng,sz-ng,1)) of device-resident
+ * arrays (see the ISO_C_BINDING module phd_amd/fortran/izem_hip_mod.f90).
+ *
+ * All functions enqueue on the context's HIP stream and return 0 on
+ * success or a negative IZEM_E_* code; physical failures that the
+ * reference aborts on (negative pressure, negative temperature, NaN,
+ * Newton or VODE non-convergence) are accumulated in a status word read by
+ * izem_status().  A context is not thread-safe; one context per host
+ * thread.  Citations are to saras13015/Phd devel-izem/src/<file>:<line>.
+ */
+#ifndef IZEM_HIP_H
+#define IZEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IZEM_OK 0
+#define IZEM_E_ARG -1
+#define IZEM_E_HIP -2
+#define IZEM_E_NOMEM -3
+#define IZEM_E_STATE -4
+
+/* status bits (izem_status) */
+#define IZEM_ST_NEG_PRESSURE 1  /* module_weno.f90:1027-1036 */
+#define IZEM_ST_NEG_TEMPERATURE 2 /* module_thermodynamics.f90:125-157 */
+#define IZEM_ST_NAN 4           /* module_thermodynamics.f90:180 */
+#define IZEM_ST_NEWTON_ITMAX 8  /* module_thermodynamics.f90:177 */
+#define IZEM_ST_VODE_FAIL 16    /* chemkin/reaction_v.f:59-62 */
+
+typedef struct izem_ctx izem_ctx;
+
+/* local box (sx..ex etc. of module_parameters), ng = 3 (WENO5) */
+typedef struct {
+  int nx, ny, nz, ng, ndim;
+} izem_grid;
+
+/* run-time switches of the path (module_parameters.f90:53-54,
+ * module_input.f90:266-399) */
+typedef struct {
+  int weno_avg;          /* WENO%WEIGHTS */
+  double max_rel_weight; /* WENO%PERCENT_WEIGHT */
+  int vis, vis_dt, reaction, forcesum;
+  double CFL, CFLmin, Fo, CPR;
+  /* nondimensional constants (adi_type, module_tools.f90:453-562) */
+  double sqgmr, Sc, Pr, ggmopr, time_ref, T_ref, rho_ref, L_ref;
+} izem_opts;
+
+/* ---- context ---------------------------------------------------------- */
+/* thd: flat image of the thermodynamic/transport tables (thd_init,
+ * thd/thd_tools_perfectgas.f90:12-248), see phd_amd.chemdata.Thd.flat() */
+int izem_create(izem_ctx **out, const izem_grid *grid, const double *thd, long nthd,
+                const izem_opts *opts);
+int izem_destroy(izem_ctx *ctx);
+int izem_synchronize(izem_ctx *ctx);
+/* run on a caller-owned stream (hipStream_t); NULL restores the context's */
+int izem_set_stream(izem_ctx *ctx, void *stream);
+/* OR of all status bits raised since the last call (synchronises) */
+int izem_status(izem_ctx *ctx, int *bits);
+/* device pointer to 6 doubles: dt_4(1:4), dtmin, DTmax [K] of the last
+ * izem_timestep / izem_reaction_dvode (keeps dt resident on the device) */
+double *izem_time_state(izem_ctx *ctx);
+/* copy the 6 time-state doubles to host memory (synchronises) */
+int izem_get_time_state(izem_ctx *ctx, double *host6);
+/* overwrite the time state from host memory (dtmin seeds the chemical
+ * limit of the next izem_timestep) */
+int izem_set_time_state(izem_ctx *ctx, const double *host6);
+
+/* ---- module_weno ------------------------------------------------------ */
+/* wenorec (js = 1, module_weno.f90:286) or wenorec_nw (js = 0, :236) on n
+ * stencils of 6 values; out = (vminus, vplus) pairs */
+int izem_wenorec(izem_ctx *ctx, int n, int js, const double *vp, const double *vm,
+                 double *out);
+/* euler_LLF_x / _y / _z (module_weno.f90:537, 1055, 1595): x assigns fl,
+ * y and z add to it; sensor: one byte per cell (logical shock_sensor) */
+int izem_euler_llf_x(izem_ctx *ctx, const double *dx_i, const double *T, const double *W_i,
+                     const double *cp, const double *ha, const uint8_t *sensor,
+                     const double *v, double *fl);
+int izem_euler_llf_y(izem_ctx *ctx, const double *dy_i, const double *T, const double *W_i,
+                     const double *cp, const double *ha, const uint8_t *sensor,
+                     const double *v, double *fl);
+int izem_euler_llf_z(izem_ctx *ctx, const double *dz_i, const double *T, const double *W_i,
+                     const double *cp, const double *ha, const uint8_t *sensor,
+                     const double *v, double *fl);
+
+/* ---- module_viscflux -------------------------------------------------- */
+/* viscflux_xy (ndim 2) / viscflux_xyz (ndim 3) (module_viscflux.f90:721,
+ * 1660), periodic faces; adds to fl */
+int izem_viscflux(izem_ctx *ctx, const double *dx_i, const double *dy_i, const double *dz_i,
+                  const double *T, const double *W_i, const double *ha, const double *v,
+                  double *fl);
+
+/* ---- module_thermodynamics / module_BCs / module_parallel ------------- */
+/* domain: 0 interior, -1/1 x ghost slabs, -2/2 y, -3/3 z (domain_select,
+ * module_parallel.f90:1936) */
+int izem_force_sumY(izem_ctx *ctx, int domain, double *v);
+/* prim_inv_var (module_thermodynamics.f90:79): Newton with the reference's
+ * global exit test; iters (host, may be NULL) receives the iteration count */
+int izem_prim_inv_var(izem_ctx *ctx, int domain, double *v, double *W_i, double *T,
+                      double *cp, double *ha, int *iters);
+/* upd_prim_var_woT (module_thermodynamics.f90:522) */
+int izem_upd_prim_var_woT(izem_ctx *ctx, int domain, const double *v, const double *T,
+                          double *W_i, double *cp, double *ha);
+/* comm_cons on one rank with periodic faces (module_parallel.f90:1715) */
+int izem_comm_cons(izem_ctx *ctx, double *v);
+/* upd_prim_var_ghost (module_BCs.f90:254) */
+int izem_upd_prim_var_ghost(izem_ctx *ctx, double *T, double *W_i, double *cp, double *ha,
+                            double *v);
+/* shock_sensor_modified_adams_shariff (module_tools.f90:2733), sticky */
+int izem_shock_sensor(izem_ctx *ctx, const double *v, const double *T, const double *W_i,
+                      uint8_t *sensor);
+
+/* ---- module_solver ---------------------------------------------------- */
+/* rk3 (module_solver.f90:694): one SSP-RK3 step with the time step read
+ * from the device pointer dt (nondimensional) */
+int izem_rk3(izem_ctx *ctx, const double *dt, const double *dx_i, const double *dy_i,
+             const double *dz_i, double *T, double *W_i, double *cp, double *ha,
+             uint8_t *sensor, double *v);
+/* right-hand side f of one stage (sensor, Euler x/y/z, viscous) without
+ * the update; f has the layout of v */
+int izem_rhs(izem_ctx *ctx, const double *dx_i, const double *dy_i, const double *dz_i,
+             const double *T, const double *W_i, const double *cp, const double *ha,
+             uint8_t *sensor, const double *v, double *f);
+/* timestep (module_solver.f90:202): results in izem_time_state(ctx);
+ * DTmax is taken from the time state (set by izem_reaction_dvode) and the
+ * previous dtmin from slot 4 */
+int izem_timestep(izem_ctx *ctx, const double *dx_i, const double *dy_i, const double *dz_i,
+                  const double *cp, const double *W_i, const double *T, const double *v);
+
+/* ---- module_reaction / chemkin ---------------------------------------- */
+/* mechanism tables (the linking-file content read by CKINIT,
+ * chemkin/vode.f:391), flat image of phd_amd.chemdata.Mechanism */
+int izem_chem_setup(izem_ctx *ctx, const double *mech, long nmech, double rtol, double atol,
+                    int trim_box);
+/* ckwyr (chemkin/vode.f:1797) on n cells: rho [g/cm3], T [K], Y[n][kk] ->
+ * wdot[n][kk] [mol/cm3/s] */
+int izem_ckwyr(izem_ctx *ctx, int n, const double *rho, const double *T, const double *Y,
+               double *wdot);
+/* PSR_CHEMKIN constant-volume reactor (chemkin/psr_chemkin.f:1, reaction_v.f:1)
+ * on n independent cells: T [K] in/out, rho [g/cm3], Y[n][kk] in/out, dt [s];
+ * nsteps (may be NULL) receives the BDF step count of each cell */
+int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho, double *Y, double dt,
+                   int *nsteps);
+/* reaction_selec with reac_selec = DVODE (module_reaction.f90:47, 102):
+ * integrates every cell over scale*dt (Strang half step: scale = 0.5),
+ * refreshes W_i/cp/ha, fills ghosts, ghost Newton; DTmax into the time state */
+int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale, double *T, double *W_i,
+                        double *cp, double *ha, double *v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1405 @@
+// flow.hip -- FP64 HIP kernels of the IZEM reacting-flow right-hand side and
+// SSP-RK3 time step for CDNA4 (gfx950).
+//
+// Compiled with -ffp-contract=off: every kernel evaluates the reference's
+// expressions in the reference's order, so its results are the reference
+// algorithm's own roundings (flux parity is a bit-level property, not a
+// tolerance).  Reference: saras13015/Phd devel-izem/src (file:line cited
+// per kernel).
+#include "flow_device.h"
+#include "izem_ctx.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+__constant__ IzThd c_thd;
+
+// ===================================================================
+// Euler fluxes: one thread per cell face.  The face value F^ of all 14
+// equations is written to a face array; the flux difference is applied by
+// a separate (or fused) pass.  euler_LLF_x/_y/_z: module_weno.f90:537-1044,
+// 1055-1584, 1595-2118.
+// ===================================================================
+template <int DIR>
+__global__ void __launch_bounds__(64)
+k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
+              const double *__restrict__ W_i, const double *__restrict__ cp,
+              const double *__restrict__ ha, const uint8_t *__restrict__ sensor, int weno_avg,
+              double *__restrict__ F, long nfaces, int *__restrict__ status) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nfaces) return;
+  const long nc = d.nc;
+  int a0, a1, a2;  // (i, j, k) of the left cell of the face; the face index runs along DIR
+  long stride;
+  if (DIR == 0) {
+    int nf = d.nx + 1;
+    a0 = (int)(tid % nf) - 1; long r = tid / nf; a1 = (int)(r % d.ny); a2 = (int)(r / d.ny);
+    stride = 1;
+  } else if (DIR == 1) {
+    a0 = (int)(tid % d.nx); long r = tid / d.nx; int nf = d.ny + 1;
+    a1 = (int)(r % nf) - 1; a2 = (int)(r / nf);
+    stride = d.NX;
+  } else {
+    a0 = (int)(tid % d.nx); long r = tid / d.nx; a1 = (int)(r % d.ny); a2 = (int)(r / d.ny) - 1;
+    stride = d.NX * d.NY;
+  }
+  constexpr int cn = 1 + DIR;               // normal momentum column
+  constexpr int t1 = (DIR == 0) ? 2 : 1;    // tangential columns (el rows 4, 5)
+  constexpr int t2 = (DIR == 2) ? 2 : 3;
+  const long cl = iz_cidx(d, a0, a1, a2);   // left cell of the face (i_l)
+  const long c0 = cl - 2 * stride;          // first stencil cell (i_l - ng + 1)
+
+  // stencil presolve (module_weno.f90:634-650)
+  double rho_i[6], un[6], P[6];
+  double lmax0 = -1.0, lmax1 = -1.0, lmax2 = -1.0;
+  int neg = 0;
+#pragma unroll
+  for (int st = 0; st < 6; st++) {
+    long c = c0 + st * stride;
+    double r0 = v[c];
+    rho_i[st] = 1.0 / r0;
+    un[st] = v[cn * nc + c] * rho_i[st];
+    P[st] = r0 * T[c] * W_i[c];
+    if (P[st] <= 0.0) neg = 1;
+    double gam = c_thd.gam2 * W_i[c] / cp[c];
+    gam = 1.0 / (1.0 - gam);
+    double cs = sqrt(gam * P[st] * rho_i[st]);
+    lmax0 = fmax(fabs(un[st] - cs), lmax0);
+    lmax1 = fmax(fabs(un[st]), lmax1);
+    lmax2 = fmax(fabs(un[st] + cs), lmax2);
+  }
+  if (neg) atomicOr(status, IZ_ST_NEG_PRESSURE);
+
+  // cell quantities of i_l (st = 2) and i_r (st = 3) for the Roe average
+  // (module_weno.f90:654-667, 726-735)
+  double uc[3], Yc[IZ_NRV], ht_c, et_c, r, r1;
+  {
+    const long cL = cl, cR = cl + stride;
+    const double riL = rho_i[2], riR = rho_i[3];
+    double uL[3], uR[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) { uL[q] = v[(1 + q) * nc + cL] * riL; uR[q] = v[(1 + q) * nc + cR] * riR; }
+    double htL = 0.0, htR = 0.0;
+    r = sqrt(v[cR] * riL);
+    r1 = 1.0 / (r + 1.0);
+#pragma unroll
+    for (int l = 0; l < IZ_NRV; l++) {
+      double yL = v[(IZ_NIV + l) * nc + cL] * riL, yR = v[(IZ_NIV + l) * nc + cR] * riR;
+      htL = htL + ha[l * nc + cL] * yL;
+      htR = htR + ha[l * nc + cR] * yR;
+      Yc[l] = (r * (yR) + yL) * r1;
+    }
+    htL = htL + 0.5 * (uL[0] * uL[0] + uL[1] * uL[1] + uL[2] * uL[2]);
+    htR = htR + 0.5 * (uR[0] * uR[0] + uR[1] * uR[1] + uR[2] * uR[2]);
+    double etL = v[4 * nc + cL] * riL, etR = v[4 * nc + cR] * riR;
+#pragma unroll
+    for (int q = 0; q < 3; q++) uc[q] = (r * uR[q] + uL[q]) * r1;
+    ht_c = (r * htR + htL) * r1;
+    et_c = (r * etR + etL) * r1;
+  }
+  // EOS-compatible average state (module_weno.f90:750-761)
+  double W_i_c = d_wmix_scalar(Yc);
+  double T_c = (ht_c - et_c) / W_i_c;
+  double cp_c = d_cp_scalar(T_c, Yc);
+  double gam_c = c_thd.gam2 * W_i_c / cp_c;
+  gam_c = 1.0 / (1.0 - gam_c);
+  double cs_c = sqrt(gam_c * T_c * W_i_c);
+  double psi[IZ_NRV];
+  {
+    double ha_c[IZ_NRV];
+    d_ha_scalar(T_c, ha_c);
+    double wrk = (gam_c / (gam_c - 1.0)) * T_c;
+#pragma unroll
+    for (int l = 0; l < IZ_NRV; l++) {
+      psi[l] = -(r * ha_c[l] + ha_c[l]) * r1;
+      psi[l] = psi[l] + wrk * c_thd.Wc_i[l];
+    }
+  }
+  const double cs_c_i = 1.0 / cs_c, cs_c_2 = cs_c * cs_c;
+  const double ux_c = uc[0], vy_c = uc[1], wz_c = uc[2];
+  const double q = 0.5 * (ux_c * ux_c + vy_c * vy_c + wz_c * wz_c);
+  const double b2 = (gam_c - 1.0) / cs_c_2;
+  const double b1 = b2 * q;
+  const double xi = b1 - b2 * ht_c;
+  const double un_c = uc[cn - 1];
+  const double hb2 = 0.5 * b2;
+
+  // shock-sensor switch for this face (module_weno.f90:973)
+  int shk = 0;
+#pragma unroll
+  for (int st = 0; st < 6; st++) shk |= sensor[c0 + st * stride];
+  const bool js = shk && weno_avg;
+
+  double ghat[IZ_NV];
+#pragma unroll 1
+  for (int m = 0; m < IZ_NV; m++) {
+    // row m of the left eigenvector matrix (module_weno.f90:775-874)
+    double e[IZ_NV];
+    double lam;
+    if (m == 0) {
+      e[0] = 0.5 * (b1 + un_c * cs_c_i);
+#pragma unroll
+      for (int c = 1; c <= 3; c++)
+        e[c] = (c == cn) ? -0.5 * (b2 * uc[c - 1] + cs_c_i) : -0.5 * (b2 * uc[c - 1]);
+      e[4] = 0.5 * b2;
+#pragma unroll
+      for (int l = 0; l < IZ_NRV; l++) e[5 + l] = hb2 * psi[l];
+      lam = lmax0;
+    } else if (m == 1) {
+      e[0] = 1.0 - b1;
+      e[1] = b2 * ux_c; e[2] = b2 * vy_c; e[3] = b2 * wz_c;
+      e[4] = -b2;
+#pragma unroll
+      for (int l = 0; l < IZ_NRV; l++) { double a = hb2 * psi[l]; e[5 + l] = -a - a; }
+      lam = lmax1;
+    } else if (m == 2) {
+      e[0] = 0.5 * (b1 - un_c * cs_c_i);
+#pragma unroll
+      for (int c = 1; c <= 3; c++)
+        e[c] = (c == cn) ? -0.5 * (b2 * uc[c - 1] - cs_c_i) : -0.5 * (b2 * uc[c - 1]);
+      e[4] = 0.5 * b2;
+#pragma unroll
+      for (int l = 0; l < IZ_NRV; l++) e[5 + l] = hb2 * psi[l];
+      lam = lmax2;
+    } else if (m == 5) {
+      double opx = (1 + xi);
+      e[0] = opx * q;
+      e[1] = -opx * ux_c; e[2] = -opx * vy_c; e[3] = -opx * wz_c;
+      e[4] = opx;
+#pragma unroll
+      for (int l = 0; l < IZ_NRV; l++) e[5 + l] = xi * psi[l];
+      lam = lmax1;
+    } else if (m >= 6) {
+      const int s1 = m - 6;
+      const double y = Yc[s1];
+      e[0] = -b1 * y;
+      e[1] = b2 * ux_c * y; e[2] = b2 * vy_c * y; e[3] = b2 * wz_c * y;
+      e[4] = -b2 * y;
+      const double by = b2 * y;
+#pragma unroll
+      for (int l = 0; l < IZ_NRV; l++) {
+        double x = -(by * psi[l]);
+        e[5 + l] = (l == s1) ? x + 1.0 : x;
+      }
+      lam = lmax1;
+    } else {
+      lam = lmax1;  // m = 3, 4: two nonzeros, handled below
+    }
+    double gp[6], gm[6];
+#pragma unroll
+    for (int st = 0; st < 6; st++) {
+      const long c = c0 + st * stride;
+      const double u = un[st], p = P[st];
+      double wc, gc;
+      if (m == 3 || m == 4) {
+        const int tc = (m == 3) ? t1 : t2;
+        const double et0 = -uc[tc - 1];
+        const double v0 = v[c], vt = v[tc * nc + c];
+        const double f0 = v[cn * nc + c];
+        const double ft = (tc == cn) ? vt * u + p : vt * u;
+        wc = et0 * v0 + vt;
+        gc = et0 * f0 + ft;
+      } else {
+        const double v0 = v[c];
+        wc = e[0] * v0;
+        gc = e[0] * v[cn * nc + c];
+#pragma unroll
+        for (int cc = 1; cc <= 3; cc++) {
+          const double vc = v[cc * nc + c];
+          wc = wc + e[cc] * vc;
+          gc = gc + e[cc] * ((cc == cn) ? vc * u + p : vc * u);
+        }
+        const double v4 = v[4 * nc + c];
+        wc = wc + e[4] * v4;
+        gc = gc + e[4] * (u * (v4 + p));
+#pragma unroll
+        for (int l = IZ_NIV; l < IZ_NV; l++) {
+          const double vl = v[l * nc + c];
+          wc = wc + e[l] * vl;
+          gc = gc + e[l] * (vl * u);
+        }
+      }
+      gp[st] = 0.5 * (gc + lam * wc);
+      gm[st] = gc - gp[st];
+    }
+    double gl, gr;
+    if (js) d_wenorec(gp, gm, gl, gr);
+    else d_wenorec_nw(gp, gm, gl, gr);
+    ghat[m] = gl + gr;
+  }
+
+  // F^ = er . ghat with the nonzero pattern of er (module_weno.f90:881-989)
+  const long nfc = nfaces;
+  double fh;
+  fh = ghat[0] + ghat[1];
+  fh = fh + ghat[2];
+  F[0 * nfc + tid] = fh;
+#pragma unroll
+  for (int c = 1; c <= 3; c++) {
+    const double u = uc[c - 1];
+    double e0 = (c == cn) ? u - cs_c : u;
+    double e2 = (c == cn) ? u + cs_c : u;
+    fh = e0 * ghat[0] + u * ghat[1];
+    fh = fh + e2 * ghat[2];
+    if (c == t1) fh = fh + ghat[3];
+    if (c == t2) fh = fh + ghat[4];
+    F[(long)c * nfc + tid] = fh;
+  }
+  fh = (ht_c - un_c * cs_c) * ghat[0] + q * ghat[1];
+  fh = fh + (ht_c + un_c * cs_c) * ghat[2];
+  fh = fh + uc[t1 - 1] * ghat[3];
+  fh = fh + uc[t2 - 1] * ghat[4];
+  fh = fh + ghat[5];
+  F[4 * nfc + tid] = fh;
+#pragma unroll
+  for (int l = 0; l < IZ_NRV - 1; l++) {
+    fh = Yc[l] * ghat[0] + Yc[l] * ghat[2];
+    fh = fh + ghat[6 + l];
+    F[(long)(IZ_NIV + l) * nfc + tid] = fh;
+  }
+  {
+    const double wrk = -1.0 / psi[IZ_NRV - 1];
+    fh = Yc[IZ_NRV - 1] * ghat[0] + Yc[IZ_NRV - 1] * ghat[2];
+    fh = fh + wrk * ghat[5];
+#pragma unroll
+    for (int l = 0; l < IZ_NRV - 1; l++) fh = fh + (wrk * psi[l]) * ghat[6 + l];
+    F[(long)(IZ_NV - 1) * nfc + tid] = fh;
+  }
+}
+
+// fl = (F^_r - F^_l) * d_i  (x assigns, y/z accumulate): module_weno.f90:998-1020
+template <int DIR>
+__global__ void k_euler_apply(IzDims d, const double *__restrict__ F, long nfaces,
+                              const double *__restrict__ d_i, int accumulate,
+                              double *__restrict__ fl) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ncin = (long)d.nx * d.ny * d.nz;
+  if (tid >= ncin) return;
+  int i = (int)(tid % d.nx);
+  long r = tid / d.nx;
+  int j = (int)(r % d.ny), k = (int)(r / d.ny);
+  long fr, fl_;
+  double di;
+  if (DIR == 0) { fl_ = i + (long)(d.nx + 1) * (j + (long)d.ny * k); fr = fl_ + 1; di = d_i[i + d.ng]; }
+  else if (DIR == 1) { fl_ = i + (long)d.nx * (j + (long)(d.ny + 1) * k); fr = fl_ + d.nx; di = d_i[j + d.ng]; }
+  else { fl_ = i + (long)d.nx * (j + (long)d.ny * k); fr = fl_ + (long)d.nx * d.ny; di = d_i[k + d.ng]; }
+  long c = iz_cidx(d, i, j, k);
+#pragma unroll
+  for (int m = 0; m < IZ_NV; m++) {
+    double df = (F[m * nfaces + fr] - F[m * nfaces + fl_]) * di;
+    if (accumulate) fl[m * d.nc + c] = fl[m * d.nc + c] + df * 1.0;
+    else fl[m * d.nc + c] = df * 1.0;
+  }
+}
+
+// ===================================================================
+// Primitive variables: force_sumY, Newton with the reference's global exit.
+// ===================================================================
+struct IzBox { int i0, i1, j0, j1, k0, k1; };
+
+__device__ inline bool box_cell(const IzBox &b, long tid, int &i, int &j, int &k) {
+  int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
+  if (tid >= (long)ni * nj * nk) return false;
+  i = b.i0 + (int)(tid % ni);
+  long r = tid / ni;
+  j = b.j0 + (int)(r % nj);
+  k = b.k0 + (int)(r / nj);
+  return true;
+}
+
+// force_sumY (npv = 0): module_thermodynamics.f90:1709-1760
+__device__ inline void d_force_sumY(double *vc) {
+  double rho = vc[0], rho_i = 1.0 / rho, sum = 0.0, Y;
+#pragma unroll
+  for (int l = 0; l < IZ_NRV - 1; l++) {
+    Y = vc[IZ_NIV + l] * rho_i;
+    Y = fmin(fmax(Y, 0.0), 1.0);
+    vc[IZ_NIV + l] = rho * Y;
+    sum = sum + Y;
+  }
+  Y = 1.0 - sum;
+  Y = fmin(fmax(Y, 0.0), 1.0);
+  vc[IZ_NIV + IZ_NRV - 1] = rho * Y;
+}
+
+// Newton probe: iterate one cell and record in bit (it-1) whether its
+// increment blocks the global exit test sqrt(max DF^2) < eps at iteration it
+// (module_thermodynamics.f90:111-150).  Cells stop once their increment has
+// fallen far below eps: Newton's quadratic contraction keeps all later
+// increments below it.
+__device__ inline unsigned long long d_newton_probe(const double *vc, double T, double eps) {
+  unsigned long long mask = 0ull;
+  for (int it = 1; it <= 64; it++) {
+    double cpv, h[IZ_NRV];
+    double DF = d_newton_cell(vc, T, cpv, h);
+    T = T + DF;
+    double e = sqrt(DF * DF);
+    if (!(e < eps)) mask |= 1ull << (it - 1);
+    else if (e < eps * 1.0e-3) break;
+  }
+  return mask;
+}
+
+__device__ inline void block_or_u64(unsigned long long m, unsigned long long *dst) {
+  // wave-level OR then one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off, 64);
+  if ((threadIdx.x & 63) == 0 && m) atomicOr(dst, m);
+}
+
+__global__ void k_force_probe(IzDims d, IzBox b, int forcesum, double *__restrict__ v,
+                              const double *__restrict__ T, double eps,
+                              unsigned long long *__restrict__ mask) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  unsigned long long m = 0ull;
+  if (box_cell(b, tid, i, j, k)) {
+    long c = iz_cidx(d, i, j, k);
+    double vc[IZ_NV];
+#pragma unroll
+    for (int l = 0; l < IZ_NV; l++) vc[l] = v[l * d.nc + c];
+    if (forcesum) {
+      d_force_sumY(vc);
+#pragma unroll
+      for (int l = IZ_NIV; l < IZ_NV; l++) v[l * d.nc + c] = vc[l];
+    }
+    if (eps > 0.0) m = d_newton_probe(vc, T[c], eps);
+  }
+  block_or_u64(m, mask);
+}
+
+// number of global iterations from the OR of all probes: the first
+// iteration with no blocking cell; itmax when every probed iteration blocked
+__device__ inline int d_newton_count(unsigned long long mask, int itmax) {
+  if (~mask == 0ull) return itmax;
+  return __builtin_ctzll(~mask) + 1;
+}
+
+// Apply exactly N iterations, write T, W_i, cp (last evaluation, i.e. at the
+// pre-update T), ha (module_thermodynamics.f90:79-186, Wmix_i :700-728).
+__global__ void k_newton_apply(IzDims d, IzBox b, const double *__restrict__ v,
+                               double *__restrict__ T, double *__restrict__ W_i,
+                               double *__restrict__ cp, double *__restrict__ ha,
+                               const unsigned long long *__restrict__ mask, int itmax,
+                               int *__restrict__ status, int *__restrict__ iters) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(b, tid, i, j, k)) return;
+  const int N = d_newton_count(*mask, itmax);
+  if (tid == 0 && iters) *iters = N;
+  long c = iz_cidx(d, i, j, k);
+  double vc[IZ_NV];
+#pragma unroll
+  for (int l = 0; l < IZ_NV; l++) vc[l] = v[l * d.nc + c];
+  double w = 0.0;
+#pragma unroll
+  for (int l = 0; l < IZ_NRV; l++) w = w + (vc[IZ_NIV + l] / vc[0]) * c_thd.Wc_i[l];
+  W_i[c] = w;
+  double Tc = T[c], cpv = 0.0, h[IZ_NRV];
+  for (int it = 0; it < N; it++) {
+    double DF = d_newton_cell(vc, Tc, cpv, h);
+    Tc = Tc + DF;
+  }
+  T[c] = Tc;
+  cp[c] = cpv;
+#pragma unroll
+  for (int l = 0; l < IZ_NRV; l++) ha[l * d.nc + c] = h[l];
+  int st = 0;
+  if (Tc < 0.0) st |= IZ_ST_NEG_TEMPERATURE;
+  if (Tc != Tc) st |= IZ_ST_NAN;
+  if (N >= itmax) st |= IZ_ST_NEWTON_ITMAX;
+  if (st) atomicOr(status, st);
+}
+
+// upd_prim_var_woT: module_thermodynamics.f90:522-689
+__global__ void k_upd_prim_woT(IzDims d, IzBox b, const double *__restrict__ v,
+                               const double *__restrict__ T, double *__restrict__ W_i,
+                               double *__restrict__ cp, double *__restrict__ ha) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(b, tid, i, j, k)) return;
+  long c = iz_cidx(d, i, j, k);
+  double vc[IZ_NV];
+#pragma unroll
+  for (int l = 0; l < IZ_NV; l++) vc[l] = v[l * d.nc + c];
+  double w = 0.0;
+#pragma unroll
+  for (int l = 0; l < IZ_NRV; l++) w = w + (vc[IZ_NIV + l] / vc[0]) * c_thd.Wc_i[l];
+  W_i[c] = w;
+  double cpv, h[IZ_NRV];
+  (void)d_newton_cell(vc, T[c], cpv, h);
+  cp[c] = cpv;
+#pragma unroll
+  for (int l = 0; l < IZ_NRV; l++) ha[l * d.nc + c] = h[l];
+}
+
+// Periodic face-ghost fill of nf fields (comm_cons / comm_deriv on one rank,
+// module_parallel.f90:1715-1777).  grid.y = direction, threads cover the
+// 2*ng slabs of that direction.
+__global__ void k_ghost_fill(IzDims d, double *__restrict__ a, int nf, int dir) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int n0, n1, n;  // transverse sizes and the direction length
+  if (dir == 0) { n0 = d.ny; n1 = d.nz; n = d.nx; }
+  else if (dir == 1) { n0 = d.nx; n1 = d.nz; n = d.ny; }
+  else { n0 = d.nx; n1 = d.ny; n = d.nz; }
+  long per = (long)n0 * n1 * 2 * d.ng;
+  if (tid >= per * nf) return;
+  int f = (int)(tid / per);
+  long r = tid % per;
+  int s = (int)(r % (2 * d.ng));
+  r /= (2 * d.ng);
+  int p0 = (int)(r % n0), p1 = (int)(r / n0);
+  int dst, src;
+  if (s < d.ng) { dst = -1 - s; src = n - 1 - s; }
+  else { int q = s - d.ng; dst = n + q; src = q; }
+  long cd, cs;
+  if (dir == 0) { cd = iz_cidx(d, dst, p0, p1); cs = iz_cidx(d, src, p0, p1); }
+  else if (dir == 1) { cd = iz_cidx(d, p0, dst, p1); cs = iz_cidx(d, p0, src, p1); }
+  else { cd = iz_cidx(d, p0, p1, dst); cs = iz_cidx(d, p0, p1, src); }
+  a[f * d.nc + cd] = a[f * d.nc + cs];
+}
+
+// shock_sensor_modified_adams_shariff (sticky): module_tools.f90:2733-2814
+__global__ void k_sensor(IzDims d, double max_rel_weight, const double *__restrict__ v,
+                         const double *__restrict__ T, const double *__restrict__ W_i,
+                         uint8_t *__restrict__ sensor) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ncin = (long)d.nx * d.ny * d.nz;
+  if (tid >= ncin) return;
+  int i = (int)(tid % d.nx);
+  long r = tid / d.nx;
+  int j = (int)(r % d.ny), k = (int)(r / d.ny);
+  long c = iz_cidx(d, i, j, k);
+  double ri = 1.0 / v[c];
+  double P = v[c] * T[c] * W_i[c];
+  bool flag = false;
+  for (int dd = 0; dd < d.ndim; dd++) {
+    int p = (dd == 0) ? i : (dd == 1 ? j : k);
+    int n = (dd == 0) ? d.nx : (dd == 1 ? d.ny : d.nz);
+    if (p >= n - 1) continue;
+    long c2 = c + ((dd == 0) ? 1 : (dd == 1 ? d.NX : d.NX * d.NY));
+    double ri2 = 1.0 / v[c2];
+    double P2 = v[c2] * T[c2] * W_i[c2];
+    double drho = fabs(v[c2] - v[c]) * fmin(ri, ri2);
+    double dpres = fabs(P2 - P) / fmin(P, P2);
+    if (drho > max_rel_weight && dpres > max_rel_weight) flag = true;
+  }
+  if (flag) sensor[c] = 1;
+}
+
+// ===================================================================
+// Viscous and species-diffusion fluxes: viscflux_xy / viscflux_xyz
+// (module_viscflux.f90:721-1135, 1660-2189), periodic faces.
+// ===================================================================
+
+// velocities and mole fractions on a domain (molefrac :1669-1698)
+__global__ void k_visc_prim(IzDims d, IzBox b, const double *__restrict__ v,
+                            const double *__restrict__ W_i, double *__restrict__ prim) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(b, tid, i, j, k)) return;
+  long c = iz_cidx(d, i, j, k);
+  double r0 = v[c];
+  for (int l = 0; l < IZ_NRV; l++)
+    prim[(3 + l) * d.nc + c] = v[(IZ_NIV + l) * d.nc + c] / (r0 * W_i[c] * c_thd.Wc[l]);
+  double rho_i = 1.0 / r0;
+  for (int q = 0; q < d.ndim; q++) prim[q * d.nc + c] = v[(1 + q) * d.nc + c] * rho_i;
+}
+
+// first derivatives on the interior (d?_fixed1): fd slots in reference order
+__global__ void k_visc_grad(IzDims d, const double *__restrict__ prim,
+                            const double *__restrict__ T, const double *__restrict__ dx_i,
+                            const double *__restrict__ dy_i, const double *__restrict__ dz_i,
+                            double *__restrict__ fd) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ncin = (long)d.nx * d.ny * d.nz;
+  if (tid >= ncin) return;
+  int i = (int)(tid % d.nx);
+  long r = tid / d.nx;
+  int j = (int)(r % d.ny), k = (int)(r / d.ny);
+  long c = iz_cidx(d, i, j, k);
+  const long strd[3] = {1, d.NX, d.NX * d.NY};
+  const double dinv[3] = {dx_i[i + d.ng], dy_i[j + d.ng], dz_i[k + d.ng]};
+  int slot = 0;
+  for (int q = 0; q < d.ndim; q++)
+    for (int dd = 0; dd < d.ndim; dd++)
+      fd[(long)(slot++) * d.nc + c] = d_deriv7(prim + q * d.nc, c, strd[dd], dinv[dd]);
+  for (int dd = 0; dd < d.ndim; dd++)
+    fd[(long)(slot++) * d.nc + c] = d_deriv7(T, c, strd[dd], dinv[dd]);
+  for (int l = 0; l < IZ_NRV; l++)
+    for (int dd = 0; dd < d.ndim; dd++)
+      fd[(long)(slot++) * d.nc + c] = d_deriv7(prim + (3 + l) * d.nc, c, strd[dd], dinv[dd]);
+}
+
+// mixture-averaged transport of one cell (prim_vis_vars :989-1230)
+__device__ inline void d_transport(double rho, double T, double W_i, const double *Xa, double *dm,
+                                   double &mu, double &ct, bool want_mu) {
+  const double eps = 1.0e-12;
+  double Kl = 0.5 * 1.0e-5 / c_thd.laminf;
+  double Ku = 1.0e-1 / c_thd.muinf;
+  double lt1 = log(T * c_thd.Tinf);
+  double lt2 = lt1 * lt1;
+  double lt3 = lt2 * lt1;
+  double P = rho * T * W_i;
+  double Kd = 1.0e-4 * c_thd.p0 / (c_thd.pinf * c_thd.Dinf);
+  for (int m = 0; m < IZ_NRV; m++) {
+    double wrk1 = 0.0, s = 0.0;
+    for (int l = 0; l < IZ_NRV; l++) {
+      if (m == l) continue;
+      const double *cf = c_thd.CofD[m][l];
+      double dkj = 0.0;
+      dkj = dkj + cf[0] * 1.0;
+      dkj = dkj + cf[1] * lt1;
+      dkj = dkj + cf[2] * lt2;
+      dkj = dkj + cf[3] * lt3;
+      double xe = Xa[l] + eps;
+      wrk1 = wrk1 + xe * c_thd.Wc[l] * W_i;
+      s = s + xe * (1.0 / exp(dkj));
+    }
+    dm[m] = wrk1 * Kd / (s * P);
+  }
+  double w1 = 0.0, cc = 0.0;
+  for (int l = 0; l < IZ_NRV; l++) {
+    const double *cf = c_thd.CofLam[l];
+    double s = 0.0;
+    s = s + cf[0] * 1.0;
+    s = s + cf[1] * lt1;
+    s = s + cf[2] * lt2;
+    s = s + cf[3] * lt3;
+    double c1 = exp(s), c2 = Xa[l];
+    cc = cc + c2 / c1;
+    w1 = w1 + c2 * c1;
+  }
+  ct = Kl * (1.0 / cc + w1);
+  mu = 0.0;
+  if (!want_mu) return;
+  double eta[IZ_NRV];
+  for (int l = 0; l < IZ_NRV; l++) {
+    const double *cf = c_thd.CofEta[l];
+    double s = 0.0;
+    s = s + cf[0] * 1.0;
+    s = s + cf[1] * lt1;
+    s = s + cf[2] * lt2;
+    s = s + cf[3] * lt3;
+    eta[l] = exp(s);
+  }
+  double mus = 0.0;
+  for (int m = 0; m < IZ_NRV; m++) {
+    double c4 = 0.0;
+    for (int l = 0; l < IZ_NRV; l++) {
+      double c3 = 1.0 + c_thd.wilke_c1[m][l] * sqrt(eta[m] / eta[l]);
+      double c2 = c3 * c3 * c_thd.wilke_c2[m][l];
+      c4 = c4 + c2 * Xa[l];
+    }
+    mus = mus + Ku * (eta[m] * Xa[m] / c4);
+  }
+  mu = mus;
+}
+
+// stresses, heat flux and diffusion fluxes on a domain; the species slots of
+// fd are overwritten with rho Y_k V_k as in the reference
+__global__ void k_visc_flux(IzDims d, IzBox b, double sqgmr, double rvd0, double rlam0,
+                            const double *__restrict__ v, const double *__restrict__ T,
+                            const double *__restrict__ W_i, const double *__restrict__ ha,
+                            const double *__restrict__ prim, double *__restrict__ fd,
+                            double *__restrict__ sf) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(b, tid, i, j, k)) return;
+  const long nc = d.nc;
+  long c = iz_cidx(d, i, j, k);
+  const int ndim = d.ndim, nfluid = ndim * (ndim + 1);
+  const double twothirds = 2.0 / 3.0;
+  double X[IZ_NRV], dm[IZ_NRV], mu, ct;
+  for (int l = 0; l < IZ_NRV; l++) X[l] = prim[(3 + l) * nc + c];
+  const double r0 = v[c], Wi = W_i[c];
+  d_transport(r0, T[c], Wi, X, dm, mu, ct, true);
+  double rho_i = 1.0 / r0;
+  double uxs = v[nc + c] * rho_i, vys = v[2 * nc + c] * rho_i, wzs = v[3 * nc + c] * rho_i;
+  double rmu = -sqgmr * mu;
+  double rvd = rvd0 * r0;
+  double rlam = -rlam0 * ct;
+#define FD(s) fd[(long)((s) - 1) * nc + c]
+  if (ndim == 3) {
+    double f1 = FD(1), f2 = FD(2), f3 = FD(3), f4 = FD(4), f5 = FD(5), f6 = FD(6), f7 = FD(7),
+           f8 = FD(8), f9 = FD(9);
+    double sxx = twothirds * rmu * (f1 + f1 - (f5 + f9));
+    double sxy = rmu * (f4 + f2);
+    double sxz = rmu * (f7 + f3);
+    double syy = twothirds * rmu * (f5 + f5 - (f1 + f9));
+    double syz = rmu * (f8 + f6);
+    double szz = twothirds * rmu * (f9 + f9 - (f1 + f5));
+    double vdcx = 0.0, vdcy = 0.0, vdcz = 0.0;
+    for (int l = 1; l <= IZ_NRV; l++) {
+      double wrk = dm[l - 1] * c_thd.Wc[l - 1] * Wi;
+      vdcx = vdcx + wrk * FD(nfluid + ndim * l - 2);
+      vdcy = vdcy + wrk * FD(nfluid + ndim * l - 1);
+      vdcz = vdcz + wrk * FD(nfluid + ndim * l);
+    }
+    double vdhx = 0.0, vdhy = 0.0, vdhz = 0.0;
+    for (int l = 1; l <= IZ_NRV; l++) {
+      double Ya = v[(IZ_NIV + l - 1) * nc + c] * rho_i;
+      double vdtx = vdcx * Ya, vdty = vdcy * Ya, vdtz = vdcz * Ya;
+      double wrk = -dm[l - 1] * c_thd.Wc[l - 1] * Wi;
+      vdtx = vdtx + wrk * FD(nfluid + ndim * l - 2);
+      vdty = vdty + wrk * FD(nfluid + ndim * l - 1);
+      vdtz = vdtz + wrk * FD(nfluid + ndim * l);
+      double jx = rvd * vdtx, jy = rvd * vdty, jz = rvd * vdtz;
+      FD(nfluid + ndim * l - 2) = jx;
+      FD(nfluid + ndim * l - 1) = jy;
+      FD(nfluid + ndim * l) = jz;
+      double h = ha[(long)(l - 1) * nc + c];
+      vdhx = vdhx + h * jx;
+      vdhy = vdhy + h * jy;
+      vdhz = vdhz + h * jz;
+    }
+    sf[0 * nc + c] = sxx; sf[1 * nc + c] = sxy; sf[2 * nc + c] = sxz;
+    sf[3 * nc + c] = syy; sf[4 * nc + c] = syz; sf[5 * nc + c] = szz;
+    sf[6 * nc + c] = uxs * sxx + vys * sxy + wzs * sxz + rlam * FD(10) + vdhx;
+    sf[7 * nc + c] = uxs * sxy + vys * syy + wzs * syz + rlam * FD(11) + vdhy;
+    sf[8 * nc + c] = uxs * sxz + vys * syz + wzs * szz + rlam * FD(12) + vdhz;
+  } else {
+    double f1 = FD(1), f2 = FD(2), f3 = FD(3), f4 = FD(4);
+    double sxx = twothirds * rmu * (f1 + f1 - f4);
+    double sxy = rmu * (f3 + f2);
+    double syy = twothirds * rmu * (f4 + f4 - f1);
+    double vdcx = 0.0, vdcy = 0.0;
+    for (int l = 1; l <= IZ_NRV; l++) {
+      double wrk = dm[l - 1] * c_thd.Wc[l - 1] * Wi;
+      vdcx = vdcx + wrk * FD(nfluid + ndim * l - 1);
+      vdcy = vdcy + wrk * FD(nfluid + ndim * l);
+    }
+    double vdhx = 0.0, vdhy = 0.0;
+    for (int l = 1; l <= IZ_NRV; l++) {
+      double Ya = v[(IZ_NIV + l - 1) * nc + c] * rho_i;
+      double vdtx = vdcx * Ya, vdty = vdcy * Ya;
+      double wrk = -dm[l - 1] * c_thd.Wc[l - 1] * Wi;
+      vdtx = vdtx + wrk * FD(nfluid + ndim * l - 1);
+      vdty = vdty + wrk * FD(nfluid + ndim * l);
+      double jx = rvd * vdtx, jy = rvd * vdty;
+      FD(nfluid + ndim * l - 1) = jx;
+      FD(nfluid + ndim * l) = jy;
+      double h = ha[(long)(l - 1) * nc + c];
+      vdhx = vdhx + h * jx;
+      vdhy = vdhy + h * jy;
+    }
+    sf[0 * nc + c] = sxx; sf[1 * nc + c] = sxy; sf[2 * nc + c] = syy;
+    sf[3 * nc + c] = uxs * sxx + vys * sxy + rlam * FD(5) + vdhx;
+    sf[4 * nc + c] = uxs * sxy + vys * syy + rlam * FD(6) + vdhy;
+  }
+#undef FD
+}
+
+// viscous divergence of one cell into out[0..13] (0 for the continuity and,
+// in 2-D, the z-momentum equations), :2087-2183
+__device__ inline void d_visc_div(const IzDims &d, long c, int i, int j, int k,
+                                  const double *__restrict__ sf, const double *__restrict__ fd,
+                                  const double *__restrict__ dx_i, const double *__restrict__ dy_i,
+                                  const double *__restrict__ dz_i, double *out, bool *has) {
+  const long nc = d.nc, sx = 1, sy = d.NX, sz = d.NX * d.NY;
+  const double dxi = dx_i[i + d.ng], dyi = dy_i[j + d.ng], dzi = dz_i[k + d.ng];
+  for (int m = 0; m < IZ_NV; m++) { out[m] = 0.0; has[m] = false; }
+  if (d.ndim == 3) {
+    const int nf = 12;
+    double wx, wy, wz;
+    wx = d_deriv7(sf + 0 * nc, c, sx, dxi); wy = d_deriv7(sf + 1 * nc, c, sy, dyi);
+    wz = d_deriv7(sf + 2 * nc, c, sz, dzi); out[1] = wx + wy + wz;
+    wx = d_deriv7(sf + 1 * nc, c, sx, dxi); wy = d_deriv7(sf + 3 * nc, c, sy, dyi);
+    wz = d_deriv7(sf + 4 * nc, c, sz, dzi); out[2] = wx + wy + wz;
+    wx = d_deriv7(sf + 2 * nc, c, sx, dxi); wy = d_deriv7(sf + 4 * nc, c, sy, dyi);
+    wz = d_deriv7(sf + 5 * nc, c, sz, dzi); out[3] = wx + wy + wz;
+    wx = d_deriv7(sf + 6 * nc, c, sx, dxi); wy = d_deriv7(sf + 7 * nc, c, sy, dyi);
+    wz = d_deriv7(sf + 8 * nc, c, sz, dzi); out[4] = wx + wy + wz;
+    has[1] = has[2] = has[3] = has[4] = true;
+    for (int l = 1; l <= IZ_NRV; l++) {
+      wx = d_deriv7(fd + (long)(nf + 3 * l - 3) * nc, c, sx, dxi);
+      wy = d_deriv7(fd + (long)(nf + 3 * l - 2) * nc, c, sy, dyi);
+      wz = d_deriv7(fd + (long)(nf + 3 * l - 1) * nc, c, sz, dzi);
+      out[IZ_NIV + l - 1] = wx + wy + wz;
+      has[IZ_NIV + l - 1] = true;
+    }
+  } else {
+    const int nf = 6;
+    double wx, wy;
+    wx = d_deriv7(sf + 0 * nc, c, sx, dxi); wy = d_deriv7(sf + 1 * nc, c, sy, dyi);
+    out[1] = wx + wy;
+    wx = d_deriv7(sf + 1 * nc, c, sx, dxi); wy = d_deriv7(sf + 2 * nc, c, sy, dyi);
+    out[2] = wx + wy;
+    wx = d_deriv7(sf + 3 * nc, c, sx, dxi); wy = d_deriv7(sf + 4 * nc, c, sy, dyi);
+    out[4] = wx + wy;
+    has[1] = has[2] = has[4] = true;
+    for (int l = 1; l <= IZ_NRV; l++) {
+      wx = d_deriv7(fd + (long)(nf + 2 * l - 2) * nc, c, sx, dxi);
+      wy = d_deriv7(fd + (long)(nf + 2 * l - 1) * nc, c, sy, dyi);
+      out[IZ_NIV + l - 1] = wx + wy;
+      has[IZ_NIV + l - 1] = true;
+    }
+  }
+}
+
+__global__ void k_visc_apply(IzDims d, const double *__restrict__ sf,
+                             const double *__restrict__ fd, const double *__restrict__ dx_i,
+                             const double *__restrict__ dy_i, const double *__restrict__ dz_i,
+                             double *__restrict__ fl) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ncin = (long)d.nx * d.ny * d.nz;
+  if (tid >= ncin) return;
+  int i = (int)(tid % d.nx);
+  long r = tid / d.nx;
+  int j = (int)(r % d.ny), k = (int)(r / d.ny);
+  long c = iz_cidx(d, i, j, k);
+  double out[IZ_NV];
+  bool has[IZ_NV];
+  d_visc_div(d, c, i, j, k, sf, fd, dx_i, dy_i, dz_i, out, has);
+  for (int m = 0; m < IZ_NV; m++)
+    if (has[m]) fl[m * d.nc + c] = fl[m * d.nc + c] + out[m];
+}
+
+// ===================================================================
+// Fused right-hand side + SSP-RK3 stage update + force_sumY + Newton probe
+// (rk3, module_solver.f90:769-847).  mode 0: write f only; mode 1: update.
+// ===================================================================
+__global__ void __launch_bounds__(256)
+k_rhs_update(IzDims d, int mode, const double *__restrict__ Fx, const double *__restrict__ Fy,
+             const double *__restrict__ Fz, int vis, const double *__restrict__ sf,
+             const double *__restrict__ fd, const double *__restrict__ dx_i,
+             const double *__restrict__ dy_i, const double *__restrict__ dz_i,
+             const double *__restrict__ dtp, double bk, double ck,
+             const double *__restrict__ v0, double *__restrict__ v, double *__restrict__ fout,
+             int forcesum, const double *__restrict__ T, double eps,
+             unsigned long long *__restrict__ mask) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ncin = (long)d.nx * d.ny * d.nz;
+  unsigned long long msk = 0ull;
+  if (tid < ncin) {
+    int i = (int)(tid % d.nx);
+    long r = tid / d.nx;
+    int j = (int)(r % d.ny), k = (int)(r / d.ny);
+    long c = iz_cidx(d, i, j, k);
+    const long nfx = (long)(d.nx + 1) * d.ny * d.nz, nfy = (long)d.nx * (d.ny + 1) * d.nz,
+               nfz = (long)d.nx * d.ny * (d.nz + 1);
+    const long fxl = i + (long)(d.nx + 1) * (j + (long)d.ny * k);
+    const long fyl = i + (long)d.nx * (j + (long)(d.ny + 1) * k);
+    const long fzl = i + (long)d.nx * (j + (long)d.ny * k);
+    const double dxi = dx_i[i + d.ng], dyi = dy_i[j + d.ng], dzi = dz_i[k + d.ng];
+    double f[IZ_NV];
+#pragma unroll
+    for (int m = 0; m < IZ_NV; m++) {
+      double df = (Fx[m * nfx + fxl + 1] - Fx[m * nfx + fxl]) * dxi;
+      f[m] = df * 1.0;
+      if (d.ndim >= 2) {
+        df = (Fy[m * nfy + fyl + d.nx] - Fy[m * nfy + fyl]) * dyi;
+        f[m] = f[m] + df * 1.0;
+      }
+      if (d.ndim == 3) {
+        df = (Fz[m * nfz + fzl + (long)d.nx * d.ny] - Fz[m * nfz + fzl]) * dzi;
+        f[m] = f[m] + df * 1.0;
+      }
+    }
+    if (vis) {
+      double out[IZ_NV];
+      bool has[IZ_NV];
+      d_visc_div(d, c, i, j, k, sf, fd, dx_i, dy_i, dz_i, out, has);
+#pragma unroll
+      for (int m = 0; m < IZ_NV; m++)
+        if (has[m]) f[m] = f[m] + out[m];
+    }
+    if (mode == 0) {
+#pragma unroll
+      for (int m = 0; m < IZ_NV; m++) fout[m * d.nc + c] = f[m];
+    } else {
+      const double dt = *dtp;
+      double vc[IZ_NV];
+#pragma unroll
+      for (int m = 0; m < IZ_NV; m++)
+        vc[m] = v0[m * d.nc + c] * bk + ck * (v[m * d.nc + c] - dt * f[m]);
+      if (forcesum) d_force_sumY(vc);
+#pragma unroll
+      for (int m = 0; m < IZ_NV; m++) v[m * d.nc + c] = vc[m];
+      msk = d_newton_probe(vc, T[c], eps);
+    }
+  }
+  if (mode != 0) block_or_u64(msk, mask);
+}
+
+// ===================================================================
+// Time step: timestep (module_solver.f90:202-408)
+// ===================================================================
+__device__ inline void atomic_max_pos(double *addr, double val) {
+  // non-negative doubles order like their bit patterns
+  atomicMax((unsigned long long *)addr, (unsigned long long)__double_as_longlong(val));
+}
+__device__ inline void atomic_min_pos(double *addr, double val) {
+  atomicMin((unsigned long long *)addr, (unsigned long long)__double_as_longlong(val));
+}
+
+__global__ void k_dt_conv(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
+                          const double *__restrict__ W_i, const double *__restrict__ cp,
+                          const double *__restrict__ dx_i, const double *__restrict__ dy_i,
+                          const double *__restrict__ dz_i, double *__restrict__ red) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ncin = (long)d.nx * d.ny * d.nz;
+  double dtc = 0.0;
+  if (tid < ncin) {
+    int i = (int)(tid % d.nx);
+    long r = tid / d.nx;
+    int j = (int)(r % d.ny), k = (int)(r / d.ny);
+    long c = iz_cidx(d, i, j, k);
+    double rho_i = 1.0 / v[c];
+    double ux = v[d.nc + c] * rho_i, vy = v[2 * d.nc + c] * rho_i, wz = v[3 * d.nc + c] * rho_i;
+    double gamma = c_thd.gam2 * W_i[c] / cp[c];
+    gamma = 1.0 / (1.0 - gamma);
+    double P = v[c] * T[c] * W_i[c];
+    double cs = sqrt(gamma * P * rho_i);
+    double a = (fabs(ux) + cs) * dx_i[i + d.ng];
+    double b = (fabs(vy) + cs) * dy_i[j + d.ng];
+    double e = (fabs(wz) + cs) * dz_i[k + d.ng];
+    dtc = fmax(fmax(a, b), e);
+  }
+  for (int off = 32; off > 0; off >>= 1) dtc = fmax(dtc, __shfl_xor(dtc, off, 64));
+  if ((threadIdx.x & 63) == 0) atomic_max_pos(&red[0], dtc);
+}
+
+__device__ inline double d_filter_delta2(const IzDims &d, const double *dx_i, const double *dy_i,
+                                         const double *dz_i, int i, int j, int k) {
+  double delta;
+  if (d.ndim == 3)
+    delta = 1.0 / pow(dx_i[i + d.ng] * dy_i[j + d.ng] * dz_i[k + d.ng], 1.0 / 3.0);
+  else if (d.ndim == 2)
+    delta = 1.0 / sqrt(dx_i[i + d.ng] * dy_i[j + d.ng]);
+  else
+    delta = 1.0 / dx_i[i + d.ng];
+  return delta * delta;
+}
+
+// pass 1: per-species maximum of the mixture diffusivity
+__global__ void k_dt_visc1(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
+                           const double *__restrict__ W_i, double *__restrict__ red) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ncin = (long)d.nx * d.ny * d.nz;
+  double dm[IZ_NRV];
+  for (int l = 0; l < IZ_NRV; l++) dm[l] = 0.0;
+  if (tid < ncin) {
+    int i = (int)(tid % d.nx);
+    long r = tid / d.nx;
+    int j = (int)(r % d.ny), k = (int)(r / d.ny);
+    long c = iz_cidx(d, i, j, k);
+    double X[IZ_NRV], mu, ct;
+    for (int l = 0; l < IZ_NRV; l++)
+      X[l] = v[(IZ_NIV + l) * d.nc + c] / (v[c] * W_i[c] * c_thd.Wc[l]);
+    d_transport(v[c], T[c], W_i[c], X, dm, mu, ct, false);
+  }
+  for (int l = 0; l < IZ_NRV; l++) {
+    double x = dm[l];
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+    if ((threadIdx.x & 63) == 0) atomic_max_pos(&red[1 + l], x);
+  }
+}
+
+// pass 2: minima of Delta^2/D(ldmax) and Delta^2 rho/(lambda/cp)
+__global__ void k_dt_visc2(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
+                           const double *__restrict__ W_i, const double *__restrict__ cp,
+                           const double *__restrict__ dx_i, const double *__restrict__ dy_i,
+                           const double *__restrict__ dz_i, double *__restrict__ red) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ncin = (long)d.nx * d.ny * d.nz;
+  // species of the global maximum: the first in species order
+  double gmax = red[1];
+  int ldmax = 0;
+  for (int l = 1; l < IZ_NRV; l++)
+    if (gmax < red[1 + l]) { gmax = red[1 + l]; ldmax = l; }
+  double a = __longlong_as_double(0x7ff0000000000000LL), b = a;  // +inf
+  if (tid < ncin) {
+    int i = (int)(tid % d.nx);
+    long r = tid / d.nx;
+    int j = (int)(r % d.ny), k = (int)(r / d.ny);
+    long c = iz_cidx(d, i, j, k);
+    double X[IZ_NRV], dm[IZ_NRV], mu, ct;
+    for (int l = 0; l < IZ_NRV; l++)
+      X[l] = v[(IZ_NIV + l) * d.nc + c] / (v[c] * W_i[c] * c_thd.Wc[l]);
+    d_transport(v[c], T[c], W_i[c], X, dm, mu, ct, false);
+    double d2 = d_filter_delta2(d, dx_i, dy_i, dz_i, i, j, k);
+    a = d2 / (dm[ldmax] + 0.0);
+    b = d2 * v[c] / (ct / cp[c] + 0.0);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a = fmin(a, __shfl_xor(a, off, 64));
+    b = fmin(b, __shfl_xor(b, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) { atomic_min_pos(&red[10], a); atomic_min_pos(&red[11], b); }
+}
+
+// red layout: [0] dtc, [1..9] dm max, [10] min d2/D, [11] min d2 rho cp/lambda
+// ts: [0..3] dt_4, [4] dtmin (in: previous), [5] DTmax (K)
+__global__ void k_dt_init(double *red) {
+  int t = threadIdx.x;
+  if (t == 0) red[0] = 1.0;
+  if (t >= 1 && t <= 9) red[t] = 0.0;
+  if (t == 10 || t == 11) red[t] = __longlong_as_double(0x7ff0000000000000LL);
+}
+
+__global__ void k_dt_final(const double *__restrict__ red, double CFL, double CFLmin, double Fo,
+                           double CPR, int visdt, int reaction, double rvd0_i, double rlamcp0_i,
+                           double *__restrict__ ts) {
+  double dt4[4] = {1.0, 1.0, 1.0, 1.0};
+  dt4[0] = CFL / red[0];
+  if (visdt) {
+    dt4[1] = rvd0_i * Fo * red[10];
+    dt4[2] = rlamcp0_i * Fo * red[11];
+  }
+  if (reaction) {
+    dt4[3] = ts[4] * CPR / ts[5];
+    dt4[3] = fmin(dt4[3], 1.0);
+    dt4[3] = fmax(dt4[3], dt4[0] * CFLmin / CFL);
+  }
+  double mn = dt4[0];
+  for (int q = 1; q < 4; q++) mn = fmin(mn, dt4[q]);
+  for (int q = 0; q < 4; q++) ts[q] = dt4[q];
+  ts[4] = mn;
+}
+
+// small utility kernels (no hipMemsetAsync so that callers may capture them)
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0ull;
+}
+__global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = a[t];
+}
+
+// batched wenorec for unit tests of the reconstruction itself
+__global__ void k_wenorec_batch(int n, int js, const double *__restrict__ vp,
+                                const double *__restrict__ vm, double *__restrict__ out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double a[6], b[6];
+  for (int s = 0; s < 6; s++) { a[s] = vp[6 * t + s]; b[s] = vm[6 * t + s]; }
+  double gl, gr;
+  if (js) d_wenorec(a, b, gl, gr);
+  else d_wenorec_nw(a, b, gl, gr);
+  out[2 * t] = gl;
+  out[2 * t + 1] = gr;
+}
+
+// explicit instantiations used by the API
+template __global__ void k_euler_faces<0>(IzDims, const double *, const double *, const double *,
+                                          const double *, const double *, const uint8_t *, int,
+                                          double *, long, int *);
+template __global__ void k_euler_faces<1>(IzDims, const double *, const double *, const double *,
+                                          const double *, const double *, const uint8_t *, int,
+                                          double *, long, int *);
+template __global__ void k_euler_faces<2>(IzDims, const double *, const double *, const double *,
+                                          const double *, const double *, const uint8_t *, int,
+                                          double *, long, int *);
+template __global__ void k_euler_apply<0>(IzDims, const double *, long, const double *, int, double *);
+template __global__ void k_euler_apply<1>(IzDims, const double *, long, const double *, int, double *);
+template __global__ void k_euler_apply<2>(IzDims, const double *, long, const double *, int, double *);
+
+// ===================================================================
+// C ABI (include/izem_hip.h)
+// ===================================================================
+static IzBox domain_box(const IzDims &d, int dom) {
+  IzBox b{0, d.nx - 1, 0, d.ny - 1, 0, d.nz - 1};
+  switch (dom) {
+    case -1: b.i0 = -d.ng; b.i1 = -1; break;
+    case 1: b.i0 = d.nx; b.i1 = d.nx + d.ng - 1; break;
+    case -2: b.j0 = -d.ng; b.j1 = -1; break;
+    case 2: b.j0 = d.ny; b.j1 = d.ny + d.ng - 1; break;
+    case -3: b.k0 = -d.ng; b.k1 = -1; break;
+    case 3: b.k0 = d.nz; b.k1 = d.nz + d.ng - 1; break;
+    default: break;
+  }
+  return b;
+}
+static long box_count(const IzBox &b) {
+  return (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
+}
+static long n_interior(const IzDims &d) { return (long)d.nx * d.ny * d.nz; }
+
+extern "C" int izem_create(izem_ctx **out, const izem_grid *g, const double *thd, long nthd,
+                           const izem_opts *opts) {
+  if (!out || !g || !thd || !opts) return IZEM_E_ARG;
+  if (nthd != (long)(sizeof(IzThd) / sizeof(double))) return IZEM_E_ARG;
+  if (g->ng != 3 || g->ndim < 2 || g->ndim > 3 || g->nx < 7 || g->ny < 7) return IZEM_E_ARG;
+  if (g->ndim == 3 && g->nz < 7) return IZEM_E_ARG;
+  if (g->ndim == 2 && g->nz != 1) return IZEM_E_ARG;
+  izem_ctx *c = (izem_ctx *)calloc(1, sizeof(izem_ctx));
+  if (!c) return IZEM_E_NOMEM;
+  IzDims &d = c->d;
+  d.nx = g->nx; d.ny = g->ny; d.nz = g->nz; d.ng = g->ng; d.ndim = g->ndim;
+  d.NX = d.nx + 2 * d.ng; d.NY = d.ny + 2 * d.ng; d.NZ = d.nz + 2 * d.ng;
+  d.nc = d.NX * d.NY * d.NZ;
+  c->o = *opts;
+  if (hipStreamCreate(&c->own_stream) != hipSuccess) { free(c); return IZEM_E_HIP; }
+  c->stream = c->own_stream;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_thd), thd, sizeof(IzThd)) != hipSuccess) {
+    izem_destroy(c);
+    return IZEM_E_HIP;
+  }
+  c->nfaces[0] = (long)(d.nx + 1) * d.ny * d.nz;
+  c->nfaces[1] = (long)d.nx * (d.ny + 1) * d.nz;
+  c->nfaces[2] = (long)d.nx * d.ny * (d.nz + 1);
+  c->nderiv = d.ndim * (d.ndim + 1) + d.ndim * IZ_NRV;
+  c->nsf = (d.ndim == 3) ? 9 : 5;
+  bool ok = true;
+  for (int q = 0; q < d.ndim; q++)
+    ok = ok && hipMalloc(&c->F[q], sizeof(double) * IZ_NV * c->nfaces[q]) == hipSuccess;
+  ok = ok && hipMalloc(&c->prim, sizeof(double) * (3 + IZ_NRV) * d.nc) == hipSuccess;
+  ok = ok && hipMalloc(&c->fd, sizeof(double) * c->nderiv * d.nc) == hipSuccess;
+  ok = ok && hipMalloc(&c->sf, sizeof(double) * c->nsf * d.nc) == hipSuccess;
+  ok = ok && hipMalloc(&c->v0, sizeof(double) * IZ_NV * d.nc) == hipSuccess;
+  ok = ok && hipMalloc(&c->masks, sizeof(unsigned long long) * 8) == hipSuccess;
+  ok = ok && hipMalloc(&c->status, sizeof(int) * 2) == hipSuccess;
+  ok = ok && hipMalloc(&c->iters, sizeof(int) * 8) == hipSuccess;
+  ok = ok && hipMalloc(&c->red, sizeof(double) * 12) == hipSuccess;
+  ok = ok && hipMalloc(&c->ts, sizeof(double) * 6) == hipSuccess;
+  if (!ok) { izem_destroy(c); return IZEM_E_NOMEM; }
+  // fields are read at ghost edges/corners never; keep scratch defined
+  hipMemsetAsync(c->prim, 0, sizeof(double) * (3 + IZ_NRV) * d.nc, c->stream);
+  hipMemsetAsync(c->fd, 0, sizeof(double) * c->nderiv * d.nc, c->stream);
+  hipMemsetAsync(c->sf, 0, sizeof(double) * c->nsf * d.nc, c->stream);
+  hipMemsetAsync(c->status, 0, sizeof(int) * 2, c->stream);
+  double ts0[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 0.0};
+  hipMemcpyAsync(c->ts, ts0, sizeof(ts0), hipMemcpyHostToDevice, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) { izem_destroy(c); return IZEM_E_HIP; }
+  *out = c;
+  return IZEM_OK;
+}
+
+extern "C" int izem_destroy(izem_ctx *c) {
+  if (!c) return IZEM_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  iz_chem_destroy(c);
+  for (int q = 0; q < 3; q++) if (c->F[q]) hipFree(c->F[q]);
+  if (c->prim) hipFree(c->prim);
+  if (c->fd) hipFree(c->fd);
+  if (c->sf) hipFree(c->sf);
+  if (c->v0) hipFree(c->v0);
+  if (c->masks) hipFree(c->masks);
+  if (c->status) hipFree(c->status);
+  if (c->iters) hipFree(c->iters);
+  if (c->red) hipFree(c->red);
+  if (c->ts) hipFree(c->ts);
+  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  free(c);
+  return IZEM_OK;
+}
+
+extern "C" int izem_synchronize(izem_ctx *c) {
+  if (!c) return IZEM_E_ARG;
+  IZ_CHECK(hipStreamSynchronize(c->stream));
+  return IZEM_OK;
+}
+
+extern "C" int izem_status(izem_ctx *c, int *bits) {
+  if (!c || !bits) return IZEM_E_ARG;
+  int h = 0;
+  IZ_CHECK(hipMemcpyAsync(&h, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  IZ_CHECK(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
+  IZ_CHECK(hipStreamSynchronize(c->stream));
+  *bits = h;
+  return IZEM_OK;
+}
+
+extern "C" double *izem_time_state(izem_ctx *c) { return c ? c->ts : nullptr; }
+
+extern "C" int izem_set_stream(izem_ctx *c, void *stream) {
+  if (!c) return IZEM_E_ARG;
+  IZ_CHECK(hipStreamSynchronize(c->stream));
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  return IZEM_OK;
+}
+
+extern "C" int izem_get_time_state(izem_ctx *c, double *h) {
+  if (!c || !h) return IZEM_E_ARG;
+  IZ_CHECK(hipMemcpyAsync(h, c->ts, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  IZ_CHECK(hipStreamSynchronize(c->stream));
+  return IZEM_OK;
+}
+
+extern "C" int izem_set_time_state(izem_ctx *c, const double *h) {
+  if (!c || !h) return IZEM_E_ARG;
+  IZ_CHECK(hipMemcpyAsync(c->ts, h, 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  IZ_CHECK(hipStreamSynchronize(c->stream));
+  return IZEM_OK;
+}
+
+extern "C" int izem_wenorec(izem_ctx *c, int n, int js, const double *vp, const double *vm,
+                            double *out) {
+  if (!c || n < 0 || !vp || !vm || !out) return IZEM_E_ARG;
+  if (n == 0) return IZEM_OK;
+  hipLaunchKernelGGL(k_wenorec_batch, dim3(iz_blocks(n, 256)), dim3(256), 0, c->stream, n, js,
+                     vp, vm, out);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+static int euler_dir(izem_ctx *c, int dir, const double *d_i, const double *T,
+                     const double *W_i, const double *cp, const double *ha,
+                     const uint8_t *sensor, const double *v, double *fl, bool apply) {
+  if (!c || !d_i || !T || !W_i || !cp || !ha || !sensor || !v) return IZEM_E_ARG;
+  if (dir >= c->d.ndim) return IZEM_E_ARG;
+  long nf = c->nfaces[dir];
+  unsigned nb = iz_blocks(nf, 64);
+  int wa = c->o.weno_avg;
+  if (dir == 0)
+    hipLaunchKernelGGL(k_euler_faces<0>, dim3(nb), dim3(64), 0, c->stream, c->d, v, T, W_i, cp,
+                       ha, sensor, wa, c->F[0], nf, c->status);
+  else if (dir == 1)
+    hipLaunchKernelGGL(k_euler_faces<1>, dim3(nb), dim3(64), 0, c->stream, c->d, v, T, W_i, cp,
+                       ha, sensor, wa, c->F[1], nf, c->status);
+  else
+    hipLaunchKernelGGL(k_euler_faces<2>, dim3(nb), dim3(64), 0, c->stream, c->d, v, T, W_i, cp,
+                       ha, sensor, wa, c->F[2], nf, c->status);
+  IZ_CHECK(hipGetLastError());
+  if (apply) {
+    if (!fl) return IZEM_E_ARG;
+    unsigned nbc = iz_blocks(n_interior(c->d), 256);
+    int acc = dir > 0;
+    if (dir == 0)
+      hipLaunchKernelGGL(k_euler_apply<0>, dim3(nbc), dim3(256), 0, c->stream, c->d, c->F[0], nf,
+                         d_i, acc, fl);
+    else if (dir == 1)
+      hipLaunchKernelGGL(k_euler_apply<1>, dim3(nbc), dim3(256), 0, c->stream, c->d, c->F[1], nf,
+                         d_i, acc, fl);
+    else
+      hipLaunchKernelGGL(k_euler_apply<2>, dim3(nbc), dim3(256), 0, c->stream, c->d, c->F[2], nf,
+                         d_i, acc, fl);
+    IZ_CHECK(hipGetLastError());
+  }
+  return IZEM_OK;
+}
+
+extern "C" int izem_euler_llf_x(izem_ctx *c, const double *dx_i, const double *T,
+                                const double *W_i, const double *cp, const double *ha,
+                                const uint8_t *sensor, const double *v, double *fl) {
+  return euler_dir(c, 0, dx_i, T, W_i, cp, ha, sensor, v, fl, true);
+}
+extern "C" int izem_euler_llf_y(izem_ctx *c, const double *dy_i, const double *T,
+                                const double *W_i, const double *cp, const double *ha,
+                                const uint8_t *sensor, const double *v, double *fl) {
+  return euler_dir(c, 1, dy_i, T, W_i, cp, ha, sensor, v, fl, true);
+}
+extern "C" int izem_euler_llf_z(izem_ctx *c, const double *dz_i, const double *T,
+                                const double *W_i, const double *cp, const double *ha,
+                                const uint8_t *sensor, const double *v, double *fl) {
+  return euler_dir(c, 2, dz_i, T, W_i, cp, ha, sensor, v, fl, true);
+}
+
+int iz_comm_cons(izem_ctx *c, double *a, int nf) {
+  const IzDims &d = c->d;
+  for (int dir = 0; dir < d.ndim; dir++) {
+    long n0 = (dir == 0) ? d.ny : d.nx, n1 = (dir == 2) ? d.ny : d.nz;
+    long tot = n0 * n1 * 2 * d.ng * nf;
+    hipLaunchKernelGGL(k_ghost_fill, dim3(iz_blocks(tot, 256)), dim3(256), 0, c->stream, d, a, nf,
+                       dir);
+  }
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+extern "C" int izem_comm_cons(izem_ctx *c, double *v) {
+  if (!c || !v) return IZEM_E_ARG;
+  return iz_comm_cons(c, v, IZ_NV);
+}
+
+// viscous fluxes into the context scratch (prim, fd, sf)
+static int visc_fluxes(izem_ctx *c, const double *dx_i, const double *dy_i, const double *dz_i,
+                       const double *T, const double *W_i, const double *ha, const double *v) {
+  const IzDims &d = c->d;
+  for (int dom = -d.ndim; dom <= d.ndim; dom++) {
+    IzBox b = domain_box(d, dom);
+    hipLaunchKernelGGL(k_visc_prim, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0, c->stream,
+                       d, b, v, W_i, c->prim);
+  }
+  hipLaunchKernelGGL(k_visc_grad, dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0, c->stream,
+                     d, c->prim, T, dx_i, dy_i, dz_i, c->fd);
+  IZ_CHECK(hipGetLastError());
+  int r = iz_comm_cons(c, c->fd, c->nderiv);
+  if (r) return r;
+  const double rvd0 = c->o.sqgmr / c->o.Sc, rlam0 = c->o.sqgmr * c->o.ggmopr;
+  for (int dom = -d.ndim; dom <= d.ndim; dom++) {
+    IzBox b = domain_box(d, dom);
+    hipLaunchKernelGGL(k_visc_flux, dim3(iz_blocks(box_count(b), 128)), dim3(128), 0, c->stream,
+                       d, b, c->o.sqgmr, rvd0, rlam0, v, T, W_i, ha, c->prim, c->fd, c->sf);
+  }
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+extern "C" int izem_viscflux(izem_ctx *c, const double *dx_i, const double *dy_i,
+                             const double *dz_i, const double *T, const double *W_i,
+                             const double *ha, const double *v, double *fl) {
+  if (!c || !dx_i || !dy_i || !dz_i || !T || !W_i || !ha || !v || !fl) return IZEM_E_ARG;
+  int r = visc_fluxes(c, dx_i, dy_i, dz_i, T, W_i, ha, v);
+  if (r) return r;
+  hipLaunchKernelGGL(k_visc_apply, dim3(iz_blocks(n_interior(c->d), 256)), dim3(256), 0,
+                     c->stream, c->d, c->sf, c->fd, dx_i, dy_i, dz_i, fl);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+static int dom_ok(const IzDims &d, int dom) { return dom >= -d.ndim && dom <= d.ndim; }
+
+extern "C" int izem_force_sumY(izem_ctx *c, int dom, double *v) {
+  if (!c || !v || !dom_ok(c->d, dom)) return IZEM_E_ARG;
+  IzBox b = domain_box(c->d, dom);
+  // probe output is discarded (mask slot 7)
+  hipLaunchKernelGGL(k_force_probe, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0, c->stream,
+                     c->d, b, 1, v, (const double *)nullptr, -1.0, c->masks + 7);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+// force_sumY (optional) + probe + apply on a list of domains
+static int newton_domains(izem_ctx *c, const int *doms, int nd, int forcesum, double *v,
+                          double *W_i, double *T, double *cp, double *ha) {
+  const IzDims &d = c->d;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->masks, 7);
+  for (int q = 0; q < nd; q++) {
+    IzBox b = domain_box(d, doms[q]);
+    hipLaunchKernelGGL(k_force_probe, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0,
+                       c->stream, d, b, forcesum, v, T, 1.0e-5, c->masks + (doms[q] + 3));
+  }
+  for (int q = 0; q < nd; q++) {
+    IzBox b = domain_box(d, doms[q]);
+    hipLaunchKernelGGL(k_newton_apply, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0,
+                       c->stream, d, b, v, T, W_i, cp, ha, c->masks + (doms[q] + 3), 200,
+                       c->status, c->iters + (doms[q] + 3));
+  }
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+extern "C" int izem_prim_inv_var(izem_ctx *c, int dom, double *v, double *W_i, double *T,
+                                 double *cp, double *ha, int *iters) {
+  if (!c || !v || !W_i || !T || !cp || !ha || !dom_ok(c->d, dom)) return IZEM_E_ARG;
+  int r = newton_domains(c, &dom, 1, 0, v, W_i, T, cp, ha);
+  if (r) return r;
+  if (iters) {
+    IZ_CHECK(hipMemcpyAsync(iters, c->iters + dom + 3, sizeof(int), hipMemcpyDeviceToHost,
+                            c->stream));
+    IZ_CHECK(hipStreamSynchronize(c->stream));
+  }
+  return IZEM_OK;
+}
+
+extern "C" int izem_upd_prim_var_woT(izem_ctx *c, int dom, const double *v, const double *T,
+                                     double *W_i, double *cp, double *ha) {
+  if (!c || !v || !T || !W_i || !cp || !ha || !dom_ok(c->d, dom)) return IZEM_E_ARG;
+  return iz_upd_prim_woT(c, dom, v, T, W_i, cp, ha);
+}
+
+int iz_upd_prim_woT(izem_ctx *c, int dom, const double *v, const double *T, double *W_i,
+                    double *cp, double *ha) {
+  IzBox b = domain_box(c->d, dom);
+  hipLaunchKernelGGL(k_upd_prim_woT, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0, c->stream,
+                     c->d, b, v, T, W_i, cp, ha);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+int iz_ghost_newton(izem_ctx *c, double *T, double *W_i, double *cp, double *ha, double *v) {
+  static const int faces[6] = {-1, 1, -2, 2, -3, 3};
+  return newton_domains(c, faces, 2 * c->d.ndim, c->o.forcesum, v, W_i, T, cp, ha);
+}
+
+extern "C" int izem_upd_prim_var_ghost(izem_ctx *c, double *T, double *W_i, double *cp,
+                                       double *ha, double *v) {
+  if (!c || !T || !W_i || !cp || !ha || !v) return IZEM_E_ARG;
+  return iz_ghost_newton(c, T, W_i, cp, ha, v);
+}
+
+extern "C" int izem_shock_sensor(izem_ctx *c, const double *v, const double *T,
+                                 const double *W_i, uint8_t *sensor) {
+  if (!c || !v || !T || !W_i || !sensor) return IZEM_E_ARG;
+  hipLaunchKernelGGL(k_sensor, dim3(iz_blocks(n_interior(c->d), 256)), dim3(256), 0, c->stream,
+                     c->d, c->o.max_rel_weight, v, T, W_i, sensor);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+// sensor + fluxes of one stage into the context scratch
+static int stage_fluxes(izem_ctx *c, const double *dx_i, const double *dy_i,
+                        const double *dz_i, const double *T, const double *W_i,
+                        const double *cp, const double *ha, uint8_t *sensor, const double *v) {
+  int r = izem_shock_sensor(c, v, T, W_i, sensor);
+  if (r) return r;
+  const double *di[3] = {dx_i, dy_i, dz_i};
+  for (int q = 0; q < c->d.ndim; q++) {
+    r = euler_dir(c, q, di[q], T, W_i, cp, ha, sensor, v, nullptr, false);
+    if (r) return r;
+  }
+  if (c->o.vis) return visc_fluxes(c, dx_i, dy_i, dz_i, T, W_i, ha, v);
+  return IZEM_OK;
+}
+
+extern "C" int izem_rhs(izem_ctx *c, const double *dx_i, const double *dy_i, const double *dz_i,
+                        const double *T, const double *W_i, const double *cp, const double *ha,
+                        uint8_t *sensor, const double *v, double *f) {
+  if (!c || !dx_i || !dy_i || !dz_i || !T || !W_i || !cp || !ha || !sensor || !v || !f)
+    return IZEM_E_ARG;
+  int r = stage_fluxes(c, dx_i, dy_i, dz_i, T, W_i, cp, ha, sensor, v);
+  if (r) return r;
+  const IzDims &d = c->d;
+  hipLaunchKernelGGL(k_rhs_update, dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0, c->stream,
+                     d, 0, c->F[0], c->F[1], c->F[2], c->o.vis, c->sf, c->fd, dx_i, dy_i, dz_i,
+                     (const double *)nullptr, 0.0, 0.0, (const double *)nullptr,
+                     (double *)nullptr, f, 0, T, 0.0, (unsigned long long *)nullptr);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+extern "C" int izem_rk3(izem_ctx *c, const double *dt, const double *dx_i, const double *dy_i,
+                        const double *dz_i, double *T, double *W_i, double *cp, double *ha,
+                        uint8_t *sensor, double *v) {
+  if (!c || !dt || !dx_i || !dy_i || !dz_i || !T || !W_i || !cp || !ha || !sensor || !v)
+    return IZEM_E_ARG;
+  static const double bk3[3] = {0.00, 0.75, 1.00 / 3.00};
+  static const double ck3[3] = {1.00, 0.25, 2.00 / 3.00};
+  const IzDims &d = c->d;
+  long ntot = (long)IZ_NV * d.nc;
+  hipLaunchKernelGGL(k_copy, dim3(iz_blocks(ntot, 256)), dim3(256), 0, c->stream, v, c->v0, ntot);
+  for (int ik = 0; ik < 3; ik++) {
+    int r = stage_fluxes(c, dx_i, dy_i, dz_i, T, W_i, cp, ha, sensor, v);
+    if (r) return r;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->masks, 7);
+    hipLaunchKernelGGL(k_rhs_update, dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0,
+                       c->stream, d, 1, c->F[0], c->F[1], c->F[2], c->o.vis, c->sf, c->fd, dx_i,
+                       dy_i, dz_i, dt, bk3[ik], ck3[ik], c->v0, v, (double *)nullptr,
+                       c->o.forcesum, T, 1.0e-5, c->masks + 3);
+    IzBox b0 = domain_box(d, 0);
+    hipLaunchKernelGGL(k_newton_apply, dim3(iz_blocks(box_count(b0), 256)), dim3(256), 0,
+                       c->stream, d, b0, v, T, W_i, cp, ha, c->masks + 3, 200, c->status,
+                       c->iters + 3);
+    IZ_CHECK(hipGetLastError());
+    r = iz_comm_cons(c, v, IZ_NV);
+    if (r) return r;
+    r = iz_ghost_newton(c, T, W_i, cp, ha, v);
+    if (r) return r;
+  }
+  return IZEM_OK;
+}
+
+extern "C" int izem_timestep(izem_ctx *c, const double *dx_i, const double *dy_i,
+                             const double *dz_i, const double *cp, const double *W_i,
+                             const double *T, const double *v) {
+  if (!c || !dx_i || !dy_i || !dz_i || !cp || !W_i || !T || !v) return IZEM_E_ARG;
+  const IzDims &d = c->d;
+  unsigned nb = iz_blocks(n_interior(d), 256);
+  hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(64), 0, c->stream, c->red);
+  hipLaunchKernelGGL(k_dt_conv, dim3(nb), dim3(256), 0, c->stream, d, v, T, W_i, cp, dx_i, dy_i,
+                     dz_i, c->red);
+  int visdt = c->o.vis && c->o.vis_dt;
+  if (visdt) {
+    hipLaunchKernelGGL(k_dt_visc1, dim3(iz_blocks(n_interior(d), 128)), dim3(128), 0, c->stream,
+                       d, v, T, W_i, c->red);
+    hipLaunchKernelGGL(k_dt_visc2, dim3(iz_blocks(n_interior(d), 128)), dim3(128), 0, c->stream,
+                       d, v, T, W_i, cp, dx_i, dy_i, dz_i, c->red);
+  }
+  double rvd0_i = c->o.Sc / c->o.sqgmr, rlamcp0_i = c->o.Pr / c->o.sqgmr;
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, c->stream, c->red, c->o.CFL, c->o.CFLmin,
+                     c->o.Fo, c->o.CPR, visdt, c->o.reaction, rvd0_i, rlamcp0_i, c->ts);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}

@@ -1,0 +1,211 @@
+// flow_device.h -- per-cell FP64 device functions of the flow solver.
+//
+// Every function keeps the operation order of the reference Fortran (cited
+// per function) and is compiled with -ffp-contract=off, so that the results
+// are the reference's own roundings, not a re-associated variant.
+#pragma once
+
+#include "izem_common.h"
+
+extern __constant__ IzThd c_thd;
+
+// Wmix_i_scalar: devel-izem/src/module_thermodynamics.f90:739-756
+__device__ inline double d_wmix_scalar(const double *Y) {
+  double w = 0.0;
+#pragma unroll
+  for (int l = 0; l < IZ_NRV; l++) w = w + Y[l] * c_thd.Wc_i[l];
+  return w;
+}
+
+// cp_scalar: module_thermodynamics.f90:767-862
+__device__ inline double d_cp_scalar(double temp, const double *Y) {
+  double cp = 0.0;
+  for (int is = 0; is < IZ_NRV; is++) {
+    double tk1 = temp * c_thd.Tinf, tk2 = tk1 * tk1, tk3 = tk2 * tk1, tk4 = tk3 * tk1;
+    const double *hl = c_thd.HeatCap[is][0], *hh = c_thd.HeatCap[is][1];
+    double fac = Y[is] * c_thd.gam2 * c_thd.Wc_i[is];
+    if (tk1 < c_thd.Tpolymin[is]) {
+      tk1 = c_thd.Tpolymin[is]; tk2 = tk1 * tk1; tk3 = tk2 * tk1; tk4 = tk3 * tk1;
+      double cp0 = hl[0] + hl[1] * tk1 + hl[2] * tk2 + hl[3] * tk3 + hl[4] * tk4;
+      double dcp0 = hl[1] + hl[2] * tk1 * 2.0 + hl[3] * tk2 * 3.0 + hl[4] * tk3 * 4.0;
+      cp = cp + fac * (dcp0 * (temp * c_thd.Tinf - c_thd.Tpolymin[is]) + cp0);
+    } else if (tk1 < c_thd.Tpolylim[is]) {
+      cp = cp + fac * (hl[0] + hl[1] * tk1 + hl[2] * tk2 + hl[3] * tk3 + hl[4] * tk4);
+    } else if (tk1 < c_thd.Tpolymax[is]) {
+      cp = cp + fac * (hh[0] + hh[1] * tk1 + hh[2] * tk2 + hh[3] * tk3 + hh[4] * tk4);
+    } else {
+      tk1 = c_thd.Tpolymax[is]; tk2 = tk1 * tk1; tk3 = tk2 * tk1; tk4 = tk3 * tk1;
+      double cp0 = hh[0] + hh[1] * tk1 + hh[2] * tk2 + hh[3] * tk3 + hh[4] * tk4;
+      double dcp0 = hh[1] + hh[2] * tk1 * 2.0 + hh[3] * tk2 * 3.0 + hh[4] * tk3 * 4.0;
+      cp = cp + fac * (dcp0 * (temp * c_thd.Tinf - c_thd.Tpolymax[is]) + cp0);
+    }
+  }
+  return cp;
+}
+
+__device__ inline double d_hpoly(const double *h, double t1, double t2, double t3, double t4,
+                                 double t5, double onethird) {
+  return h[0] * t1 + h[1] * t2 * 0.50 + h[2] * t3 * onethird + h[3] * t4 * 0.25 +
+         h[4] * t5 * 0.20 + h[5];
+}
+__device__ inline double d_dhpoly(const double *h, double t1, double t2, double t3, double t4) {
+  return h[0] + h[1] * t1 + h[2] * t2 + h[3] * t3 + h[4] * t4;
+}
+
+// Per-species enthalpy and cp contribution at nondimensional T, with the
+// four-branch NASA-7 evaluation of newton / upd_prim_var_woT / ha_scalar
+// (module_thermodynamics.f90:327-498, 522-689, 873-978).
+__device__ inline void d_species_h_cp(int is, double T, double Ya, double &h, double &cpc) {
+  const double Tinf_i = 1.0 / c_thd.Tinf, onethird = 1.0 / 3.0;
+  double t1 = T * c_thd.Tinf, t2 = t1 * t1, t3 = t2 * t1, t4 = t3 * t1, t5 = t4 * t1;
+  const double *hl = c_thd.HeatCap[is][0], *hh = c_thd.HeatCap[is][1];
+  double fh = c_thd.Wc_i[is] * Tinf_i;
+  double fc = Ya * c_thd.gam2 * c_thd.Wc_i[is];
+  double hv = 0.0;
+  if (t1 < c_thd.Tpolymin[is]) {
+    t1 = c_thd.Tpolymin[is]; t2 = t1 * t1; t3 = t2 * t1; t4 = t3 * t1; t5 = t4 * t1;
+    double ha0 = d_hpoly(hl, t1, t2, t3, t4, t5, onethird);
+    double dha0 = d_dhpoly(hl, t1, t2, t3, t4);
+    hv = hv + fh * (dha0 * (T * c_thd.Tinf - c_thd.Tpolymin[is]) + ha0);
+    double dcp0 = hl[1] + hl[2] * t1 * 2.0 + hl[3] * t2 * 3.0 + hl[4] * t3 * 4.0;
+    cpc = fc * (dcp0 * (T * c_thd.Tinf - c_thd.Tpolymin[is]) + dha0);
+  } else if (t1 < c_thd.Tpolylim[is]) {
+    hv = hv + fh * d_hpoly(hl, t1, t2, t3, t4, t5, onethird);
+    cpc = fc * d_dhpoly(hl, t1, t2, t3, t4);
+  } else if (t1 < c_thd.Tpolymax[is]) {
+    hv = hv + fh * d_hpoly(hh, t1, t2, t3, t4, t5, onethird);
+    cpc = fc * d_dhpoly(hh, t1, t2, t3, t4);
+  } else {
+    t1 = c_thd.Tpolymax[is]; t2 = t1 * t1; t3 = t2 * t1; t4 = t3 * t1; t5 = t4 * t1;
+    double ha0 = d_hpoly(hh, t1, t2, t3, t4, t5, onethird);
+    double dha0 = d_dhpoly(hh, t1, t2, t3, t4);
+    hv = hv + fh * (dha0 * (T * c_thd.Tinf - c_thd.Tpolymax[is]) + ha0);
+    double dcp0 = hh[1] + hh[2] * t1 * 2.0 + hh[3] * t2 * 3.0 + hh[4] * t3 * 4.0;
+    cpc = fc * (dcp0 * (T * c_thd.Tinf - c_thd.Tpolymax[is]) + dha0);
+  }
+  h = hv;
+}
+
+// ha_scalar: module_thermodynamics.f90:873-978
+__device__ inline void d_ha_scalar(double temp, double *ha) {
+  for (int is = 0; is < IZ_NRV; is++) {
+    double h, cpc;
+    d_species_h_cp(is, temp, 0.0, h, cpc);
+    ha[is] = h;
+  }
+}
+
+// newton (one cell): module_thermodynamics.f90:271-508.  Returns DF and the
+// cp / ha of this evaluation.
+__device__ inline double d_newton_cell(const double *vc, double T, double &cp_out,
+                                       double *ha_out) {
+  const double gam2_i = 1.0 / c_thd.gam2;
+  double rho_i = 1.0 / vc[0];
+  double F = -vc[4] * rho_i + 0.5 * (vc[1] * vc[1] + vc[2] * vc[2] + vc[3] * vc[3]) *
+                                  (rho_i * rho_i);
+  double Ya[IZ_NRV];
+#pragma unroll
+  for (int is = 0; is < IZ_NRV; is++) Ya[is] = vc[IZ_NIV + is] * rho_i;
+  double cp = 0.0;
+  for (int is = 0; is < IZ_NRV; is++) {
+    double wrk1 = -T;
+    double h, cpc;
+    d_species_h_cp(is, T, Ya[is], h, cpc);
+    cp = cp + cpc;
+    ha_out[is] = h;
+    F = F + Ya[is] * (h + wrk1 * c_thd.Wc_i[is]);
+  }
+  double wrk = 0.0;
+#pragma unroll
+  for (int is = 0; is < IZ_NRV; is++) wrk = wrk + Ya[is] * c_thd.Wc_i[is];
+  cp_out = cp;
+  return F / (wrk - cp * gam2_i);
+}
+
+// ------------------------------------------------------------------ WENO
+// wenopar / wenorec / wenorec_nw, ng = 3: module_weno.f90:114-190, 236-527
+#define IZ_Z1 (13.0 / 12.0)
+#define IZ_Z2 (1.0 / 4.0)
+#define IZ_EPS_WENO 1.0e-10
+
+__device__ inline void d_betas(const double *q, int i, double *beta) {
+  // q is 1-based
+  double a0 = q[i] + (-2.0) * q[i + 1] + q[i + 2], b0 = 3.0 * q[i] + (-4.0) * q[i + 1] + q[i + 2];
+  beta[0] = IZ_Z1 * a0 * a0 + IZ_Z2 * b0 * b0;
+  double a1 = q[i - 1] + (-2.0) * q[i] + q[i + 1], b1 = q[i - 1] + (-1.0) * q[i + 1];
+  beta[1] = IZ_Z1 * a1 * a1 + IZ_Z2 * b1 * b1;
+  double a2 = q[i - 2] + (-2.0) * q[i - 1] + q[i],
+         b2 = q[i - 2] + (-4.0) * q[i - 1] + 3.0 * q[i];
+  beta[2] = IZ_Z1 * a2 * a2 + IZ_Z2 * b2 * b2;
+}
+
+__device__ inline void d_wenorec(const double *vp0, const double *vm0, double &vminus,
+                                 double &vplus) {
+  const double c[4][3] = {{11.0 / 6.0, -7.0 / 6.0, 1.0 / 3.0},
+                          {1.0 / 3.0, 5.0 / 6.0, -1.0 / 6.0},
+                          {-1.0 / 6.0, 5.0 / 6.0, 1.0 / 3.0},
+                          {1.0 / 3.0, -7.0 / 6.0, 11.0 / 6.0}};
+  const double d[3] = {3.0 / 10.0, 3.0 / 5.0, 1.0 / 10.0};
+  const double *vp = vp0 - 1, *vm = vm0 - 1;
+  double vpr[3], beta[3], alfa[3], sum;
+  int i = 3;
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    vpr[r] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) vpr[r] = vpr[r] + c[r + 1][j] * vp[i - r + j];
+  }
+  d_betas(vp, i, beta);
+  sum = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    alfa[r] = d[r] / ((IZ_EPS_WENO + beta[r]) * (IZ_EPS_WENO + beta[r]));
+    sum = sum + alfa[r];
+  }
+  sum = 1.0 / sum;
+  double out = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; r++) out = out + (alfa[r] * sum) * vpr[r];
+  vminus = out;
+  i = 4;
+  double vmr[3];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    vmr[r] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) vmr[r] = vmr[r] + c[r][j] * vm[i - r + j];
+  }
+  d_betas(vm, i, beta);
+  sum = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    alfa[r] = d[2 - r] / ((IZ_EPS_WENO + beta[r]) * (IZ_EPS_WENO + beta[r]));
+    sum = sum + alfa[r];
+  }
+  sum = 1.0 / sum;
+  out = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; r++) out = out + (alfa[r] * sum) * vmr[r];
+  vplus = out;
+}
+
+__device__ inline void d_wenorec_nw(const double *vp0, const double *vm0, double &vminus,
+                                    double &vplus) {
+  const double *vp = vp0 - 1, *vm = vm0 - 1;
+  const double denom = 1.0 / 60.0;
+  vminus = (-3.0 * vp[1] + 27.0 * vp[2] + 47.0 * vp[3] - 13.0 * vp[4] + 2.0 * vp[5]) * denom;
+  vplus = (-3.0 * vm[6] + 27.0 * vm[5] + 47.0 * vm[4] - 13.0 * vm[3] + 2.0 * vm[2]) * denom;
+}
+
+// 6th-order central first derivative (alpha6), periodic branch of
+// dx/dy/dz, d?_fixed1, d?_fixed2: module_deriv.f90:37-40
+__device__ inline double d_deriv7(const double *__restrict__ a, long c, long s, double dinv) {
+  double dd = 0.0;
+  dd = dd + (-1.0 / 60.0) * a[c - 3 * s];
+  dd = dd + (9.0 / 60.0) * a[c - 2 * s];
+  dd = dd + (-45.0 / 60.0) * a[c - s];
+  dd = dd + (45.0 / 60.0) * a[c + s];
+  dd = dd + (-9.0 / 60.0) * a[c + 2 * s];
+  dd = dd + (1.0 / 60.0) * a[c + 3 * s];
+  return dd * dinv;
+}

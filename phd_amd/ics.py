@@ -1,0 +1,207 @@
+"""Synthetic initial conditions for the reacting-flow workloads.
+
+The reference ships no input for these cases (its ``ini_thi`` HIT generator
+is commented out, devel-izem/src/module_ICs.f90:2597-2705), so the
+generators here are the project's own.  All states are returned in the
+reference's nondimensional variables (constants() of
+devel-izem/src/module_tools.f90:453-562) and in its array layout
+v(sx-ng:ex+ng, sy-ng:ey+ng, sz-ng:ez+ng, 1:nv) (Fortran order, ghosts
+included, ghosts filled by the periodic wrap).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import chemdata as cd
+
+P_ATM = 101325.0
+
+
+@dataclass
+class Box:
+    nx: int
+    ny: int
+    nz: int
+    ng: int
+    ndim: int
+    L: tuple  # physical box sides [m]
+
+    @property
+    def shape(self):
+        """(NZ, NY, NX) of one field including ghosts (C view of the Fortran array)."""
+        return (self.nz + 2 * self.ng, self.ny + 2 * self.ng, self.nx + 2 * self.ng)
+
+    @property
+    def ncell(self):
+        a, b, c = self.shape
+        return a * b * c
+
+    def interior(self):
+        g = self.ng
+        return (slice(g, g + self.nz), slice(g, g + self.ny), slice(g, g + self.nx))
+
+    def inv_spacing(self, adi: cd.Adim):
+        """Nondimensional 1/dx arrays (ghosts included) of a uniform periodic
+        grid: dx_i = L_ref / dx (metrics + adimensionalize,
+        module_tools.f90:223-442, :160-212).  Directions beyond ndim get
+        L_ref * 1 as the reference's metrics leave them at 1/m."""
+        out = []
+        for d, n in enumerate((self.nx, self.ny, self.nz)):
+            a = np.empty(n + 2 * self.ng)
+            if d < self.ndim:
+                a[:] = adi.L_ref / (self.L[d] / n)
+            else:
+                a[:] = 1.0 * adi.L_ref
+            out.append(a)
+        return out
+
+
+def periodic_fill(box: Box, a: np.ndarray):
+    """Fill the face ghost slabs of a (…, NZ, NY, NX) array by periodic wrap
+    (comm_cons on one rank, faces only)."""
+    g = box.ng
+    nx, ny, nz = box.nx, box.ny, box.nz
+    I = box.interior()
+    a[..., I[0], I[1], 0:g] = a[..., I[0], I[1], nx:nx + g]
+    a[..., I[0], I[1], nx + g:nx + 2 * g] = a[..., I[0], I[1], g:2 * g]
+    if box.ndim >= 2:
+        a[..., I[0], 0:g, I[2]] = a[..., I[0], ny:ny + g, I[2]]
+        a[..., I[0], ny + g:ny + 2 * g, I[2]] = a[..., I[0], g:2 * g, I[2]]
+    if box.ndim == 3:
+        a[..., 0:g, I[1], I[2]] = a[..., nz:nz + g, I[1], I[2]]
+        a[..., nz + g:nz + 2 * g, I[1], I[2]] = a[..., g:2 * g, I[1], I[2]]
+
+
+def _spectral_velocity(n: tuple, ndim: int, k0: float, u_rms: float, seed: int) -> np.ndarray:
+    """Solenoidal random-phase field with E(k) ~ k^4 exp(-2 (k/k0)^2), k in
+    units of 2 pi / L; returns (ndim, *n) with the requested rms per component."""
+    rng = np.random.default_rng(seed)
+    shp = n[:ndim]
+    ks = [np.fft.fftfreq(m, 1.0 / m) for m in shp]
+    K = np.meshgrid(*ks, indexing="ij")
+    k2 = sum(k * k for k in K)
+    kk = np.sqrt(k2)
+    kk[(0,) * ndim] = 1.0
+    E = kk ** 4 * np.exp(-2.0 * (kk / k0) ** 2)
+    # amplitude per mode so that the shell sum reproduces E(k)
+    shell = kk ** (ndim - 1)
+    amp = np.sqrt(E / shell)
+    amp[(0,) * ndim] = 0.0
+    u = []
+    for _ in range(ndim):
+        ph = rng.uniform(0, 2 * np.pi, shp)
+        u.append(amp * np.exp(1j * ph) * rng.standard_normal(shp))
+    u = np.array(u)
+    kdotu = sum(K[d] * u[d] for d in range(ndim))
+    k2s = np.where(k2 == 0, 1.0, k2)
+    for d in range(ndim):
+        u[d] = u[d] - K[d] * kdotu / k2s
+    ur = np.array([np.real(np.fft.ifftn(u[d])) for d in range(ndim)])
+    rms = np.sqrt(np.mean(ur ** 2))
+    return ur * (u_rms / rms)
+
+
+def _gaussian_random_field(n: tuple, ndim: int, corr_len_cells: float, rms: float, seed: int):
+    rng = np.random.default_rng(seed)
+    shp = n[:ndim]
+    w = rng.standard_normal(shp)
+    ks = [np.fft.fftfreq(m) for m in shp]
+    K = np.meshgrid(*ks, indexing="ij")
+    k2 = sum(k * k for k in K)
+    filt = np.exp(-0.5 * k2 * (2 * np.pi * corr_len_cells) ** 2)
+    f = np.real(np.fft.ifftn(np.fft.fftn(w) * filt))
+    return f * (rms / np.std(f))
+
+
+def conserved_from_primitive(box: Box, adi: cd.Adim, thd: cd.Thd, T_K: np.ndarray, p_Pa,
+                             Y: np.ndarray, u_ms: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Interior arrays (physical units) -> nondimensional conserved v with
+    ghosts and T.  T_K (nz,ny,nx), Y (9,nz,ny,nx), u_ms (3,nz,ny,nx)."""
+    from .thermo_host import ha_scalar_vec
+
+    nv = cd.NV
+    v = np.zeros((nv,) + box.shape)
+    T = np.zeros(box.shape)
+    I = box.interior()
+    Tn = T_K / adi.T_ref
+    W_i = np.tensordot(thd.Wc_i, Y, axes=(0, 0))
+    pn = np.asarray(p_Pa) / adi.p_ref
+    rho = pn / (Tn * W_i)
+    un = u_ms / adi.u_ref
+    ha = ha_scalar_vec(thd, Tn)
+    e = 0.5 * (un ** 2).sum(axis=0) + (Y * ha).sum(axis=0) - Tn * W_i
+    v[(0,) + I] = rho
+    for c in range(3):
+        v[(1 + c,) + I] = rho * un[c]
+    v[(4,) + I] = rho * e
+    for l in range(cd.NRV):
+        v[(5 + l,) + I] = rho * Y[l]
+    T[I] = Tn
+    periodic_fill(box, v)
+    periodic_fill(box, T)
+    return v, T
+
+
+def config1_state(adi, thd, n: int = 128, seed: int = 1):
+    """2-D periodic 1 cm box, stoichiometric H2/air at 1 atm, T = 900 K plus a
+    Gaussian hot spot (peak 1500 K, sigma 1 mm), u_rms = 10 m/s."""
+    L = 0.01
+    box = Box(n, n, 1, 3, 2, (L, L, L))
+    x = (np.arange(n) + 0.0) * (L / n)
+    X, Yy = np.meshgrid(x, x, indexing="xy")
+    r2 = (X - L / 2) ** 2 + (Yy - L / 2) ** 2
+    T = 900.0 + 600.0 * np.exp(-r2 / (2 * 0.001 ** 2))
+    T = T[None]
+    Ymix = cd.mass_fractions_premix(thd.names, 1.0)
+    Y = np.broadcast_to(Ymix[:, None, None, None], (9, 1, n, n)).copy()
+    uv = _spectral_velocity((n, n), 2, 4.0, 10.0, seed)  # (2, ny, nx) indexed [y][x]
+    u = np.zeros((3, 1, n, n))
+    u[0, 0] = uv[0]
+    u[1, 0] = uv[1]
+    v, Tn = conserved_from_primitive(box, adi, thd, T, P_ATM, Y, u)
+    return box, v, Tn
+
+
+def config2_state(adi, thd, n: int = 256, seed: int = 2, L: float = 0.01):
+    """3-D periodic box, premixed H2/air at phi = 0.8, T0 = 1000 K with
+    Gaussian-random fluctuations (rms 50 K, correlation length L/8),
+    u_rms = 20 m/s with the k^4 exp(-2 (k/k0)^2) spectrum, k0 = 4."""
+    box = Box(n, n, n, 3, 3, (L, L, L))
+    dT = _gaussian_random_field((n, n, n), 3, n / 8.0, 50.0, seed)
+    T = 1000.0 + dT
+    Ymix = cd.mass_fractions_premix(thd.names, 0.8)
+    Y = np.empty((9, n, n, n))
+    Y[:] = Ymix[:, None, None, None]
+    u = _spectral_velocity((n, n, n), 3, 4.0, 20.0, seed + 100)
+    v, Tn = conserved_from_primitive(box, adi, thd, T, P_ATM, Y, u)
+    return box, v, Tn
+
+
+def config3_cells(thd, ncell: int, seed: int = 3):
+    """Independent reactor cells: T ~ U[900, 2000] K, p log-uniform in
+    [1, 10] atm, phi log-normal (median 1, sigma 0.4) clipped to [0.3, 3],
+    0-20 % products blended in.  Returns (T_K, p_Pa, Y[ncell, 9])."""
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(900.0, 2000.0, ncell)
+    p = P_ATM * np.exp(rng.uniform(0.0, math.log(10.0), ncell))
+    phi = np.clip(np.exp(rng.normal(0.0, 0.4, ncell)), 0.3, 3.0)
+    prod = rng.uniform(0.0, 0.2, ncell)
+    Y = np.empty((ncell, 9))
+    # vectorised premix blend (same recipe as chemdata.mass_fractions_premix)
+    tr = cd.read_tranfit()
+    idx = {nm.upper(): i for i, nm in enumerate(thd.names)}
+    X = np.zeros((ncell, 9))
+    X[:, idx["H2"]] = 2.0 * phi
+    X[:, idx["O2"]] = 1.0
+    X[:, idx["N2"]] = 0.79 / 0.21
+    Ym = X * tr.W
+    Ym /= Ym.sum(axis=1, keepdims=True)
+    Xp = np.zeros(9)
+    Xp[idx["H2O"]], Xp[idx["OH"]], Xp[idx["N2"]] = 0.9, 0.1, 0.79 / 0.21 * 0.5
+    Yp = Xp * tr.W
+    Yp /= Yp.sum()
+    Y[:] = (1.0 - prod)[:, None] * Ym + prod[:, None] * Yp[None, :]
+    return T, p, Y

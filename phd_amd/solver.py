@@ -1,0 +1,206 @@
+"""Host-side mirror of the reference solver interface for the reacting-flow
+path, running on one MI355X through the HIP library.
+
+State lives on the device as torch tensors in the reference's array layout
+(Fortran order, ghosts included): v(14, NZ, NY, NX) is the C view of
+v(sx-ng:ex+ng, sy-ng:ey+ng, sz-ng:ez+ng, 1:nv).  The methods mirror the
+reference subroutines (devel-izem/src):
+
+  euler_llf_x/y/z  module_weno.f90:537/1055/1595
+  viscflux         module_viscflux.f90:721 (2-D) / 1660 (3-D)
+  prim_inv_var     module_thermodynamics.f90:79
+  rk3              module_solver.f90:694
+  timestep         module_solver.f90:202
+  reaction_selec   module_reaction.f90:47 (DVODE branch)
+  step             one iteration of the main loop, izem.f90:242-389
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import chemdata as cd
+from .ics import Box
+
+
+@dataclass
+class Options:
+    """Hot-path input keys (module_input.f90:266-399) with the parity
+    settings of the path: JS weights on every face (WEIGHTS = T,
+    ADAMSSHARIFF sensor with PERCENT_WEIGHT = -1)."""
+    weno_avg: int = 1
+    max_rel_weight: float = -1.0
+    vis: int = 1
+    vis_dt: int = 1
+    reaction: int = 1
+    forcesum: int = 1
+    CFL: float = 0.5
+    CFLmin: float = 0.1
+    Fo: float = 0.1
+    CPR: float = 20.0
+    rtol: float = 1.0e-6
+    atol: float = 1.0e-12
+    trim_box: int = 1   # reaction_dvode skips ng+1 planes at global faces
+
+
+def _ptr(t: torch.Tensor):
+    return C.c_void_p(t.data_ptr())
+
+
+class Solver:
+    def __init__(self, box: Box, opts: Options | None = None, device: str = "cuda:0",
+                 adi: cd.Adim | None = None, thd: cd.Thd | None = None,
+                 mech: cd.Mechanism | None = None):
+        if not torch.cuda.is_available():
+            raise _lib.IzemError("the IZEM solver needs a ROCm GPU")
+        self.lib = _lib.load()
+        self.box = box
+        self.opts = opts or Options()
+        self.dev = torch.device(device)
+        self.adi = adi or cd.Adim()
+        self.thd = thd or cd.build_thd(self.adi)
+        self.mech = mech
+        o = self.opts
+        a = self.adi
+        g = _lib.Grid(box.nx, box.ny, box.nz, box.ng, box.ndim)
+        op = _lib.Opts(o.weno_avg, o.max_rel_weight, o.vis, o.vis_dt, o.reaction, o.forcesum,
+                       o.CFL, o.CFLmin, o.Fo, o.CPR, a.sqgmr, a.Sc, a.pr, a.ggmopr, a.time_ref,
+                       a.T_ref, a.rho_ref, a.L_ref)
+        flat = self.thd.flat()
+        ctx = C.c_void_p()
+        torch.cuda.set_device(self.dev)
+        _lib.check(self.lib.izem_create(C.byref(ctx), C.byref(g), flat.ctypes.data, flat.size,
+                                        C.byref(op)), "izem_create")
+        self.ctx = ctx
+        # run on torch's current stream so that library and torch work is ordered
+        _lib.check(self.lib.izem_set_stream(ctx, C.c_void_p(
+            torch.cuda.current_stream(self.dev).cuda_stream)), "izem_set_stream")
+        shp = box.shape
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        self.v = torch.zeros((cd.NV,) + shp, **f64)
+        self.T = torch.zeros(shp, **f64)
+        self.W_i = torch.zeros(shp, **f64)
+        self.cp = torch.zeros(shp, **f64)
+        self.ha = torch.zeros((cd.NRV,) + shp, **f64)
+        self.sensor = torch.zeros(shp, dtype=torch.uint8, device=self.dev)
+        dxi = box.inv_spacing(self.adi)
+        self.d_i = [torch.tensor(x, **f64) for x in dxi]
+        self.dt = torch.zeros(1, **f64)
+        self.time = 0.0
+        if self.opts.reaction and mech is not None:
+            mf = mech_flat(mech)
+            _lib.check(self.lib.izem_chem_setup(ctx, mf.ctypes.data, mf.size, o.rtol, o.atol,
+                                                o.trim_box), "izem_chem_setup")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.izem_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- state
+    def load_state(self, v: np.ndarray, T: np.ndarray):
+        """Upload conserved variables and temperature (ghosts filled) and
+        initialise W_i, cp, ha with a Newton pass (prim_inv_var)."""
+        self.v.copy_(torch.from_numpy(np.ascontiguousarray(v)))
+        self.T.copy_(torch.from_numpy(np.ascontiguousarray(T)))
+        for dom in [0] + [s * q for q in range(1, self.box.ndim + 1) for s in (-1, 1)]:
+            self.prim_inv_var(dom)
+        self.sync()
+
+    def sync(self):
+        _lib.check(self.lib.izem_synchronize(self.ctx), "izem_synchronize")
+
+    def status(self) -> int:
+        b = C.c_int()
+        _lib.check(self.lib.izem_status(self.ctx, C.byref(b)), "izem_status")
+        return b.value
+
+    def time_state(self) -> np.ndarray:
+        """dt_4(1:4), dtmin, DTmax of the last timestep / chemistry call."""
+        out = np.zeros(6)
+        _lib.check(self.lib.izem_get_time_state(self.ctx, out.ctypes.data), "time_state")
+        return out
+
+    def set_time_state(self, ts) -> None:
+        a = np.ascontiguousarray(ts, dtype=np.float64)
+        _lib.check(self.lib.izem_set_time_state(self.ctx, a.ctypes.data), "time_state")
+
+    # ------------------------------------------------- reference subroutines
+    def euler_llf(self, direction: int, fl: torch.Tensor):
+        fn = [self.lib.izem_euler_llf_x, self.lib.izem_euler_llf_y, self.lib.izem_euler_llf_z][
+            direction]
+        _lib.check(fn(self.ctx, _ptr(self.d_i[direction]), _ptr(self.T), _ptr(self.W_i),
+                      _ptr(self.cp), _ptr(self.ha), _ptr(self.sensor), _ptr(self.v), _ptr(fl)),
+                   "euler_llf")
+
+    def viscflux(self, fl: torch.Tensor):
+        _lib.check(self.lib.izem_viscflux(self.ctx, *[_ptr(x) for x in self.d_i], _ptr(self.T),
+                                          _ptr(self.W_i), _ptr(self.ha), _ptr(self.v), _ptr(fl)),
+                   "viscflux")
+
+    def prim_inv_var(self, domain: int = 0) -> None:
+        _lib.check(self.lib.izem_prim_inv_var(self.ctx, domain, _ptr(self.v), _ptr(self.W_i),
+                                              _ptr(self.T), _ptr(self.cp), _ptr(self.ha), None),
+                   "prim_inv_var")
+
+    def shock_sensor(self):
+        _lib.check(self.lib.izem_shock_sensor(self.ctx, _ptr(self.v), _ptr(self.T),
+                                              _ptr(self.W_i), _ptr(self.sensor)), "sensor")
+
+    def rhs(self, f: torch.Tensor):
+        _lib.check(self.lib.izem_rhs(self.ctx, *[_ptr(x) for x in self.d_i], _ptr(self.T),
+                                     _ptr(self.W_i), _ptr(self.cp), _ptr(self.ha),
+                                     _ptr(self.sensor), _ptr(self.v), _ptr(f)), "rhs")
+
+    def rk3(self, dt: torch.Tensor | None = None):
+        dtp = dt if dt is not None else self.dt
+        _lib.check(self.lib.izem_rk3(self.ctx, _ptr(dtp), *[_ptr(x) for x in self.d_i],
+                                     _ptr(self.T), _ptr(self.W_i), _ptr(self.cp), _ptr(self.ha),
+                                     _ptr(self.sensor), _ptr(self.v)), "rk3")
+
+    def timestep(self):
+        _lib.check(self.lib.izem_timestep(self.ctx, *[_ptr(x) for x in self.d_i], _ptr(self.cp),
+                                          _ptr(self.W_i), _ptr(self.T), _ptr(self.v)),
+                   "timestep")
+
+    def reaction_selec(self, dt_dev_ptr: int, scale: float):
+        _lib.check(self.lib.izem_reaction_dvode(self.ctx, C.c_void_p(dt_dev_ptr), scale,
+                                                _ptr(self.T), _ptr(self.W_i), _ptr(self.cp),
+                                                _ptr(self.ha), _ptr(self.v)), "reaction_dvode")
+
+    def step(self):
+        """timestep -> reaction(dt/2) -> rk3(dt) -> reaction(dt/2)
+        (izem.f90:254-277), with dt kept on the device."""
+        self.timestep()
+        dtp = self.lib.izem_time_state(self.ctx) + 4 * 8
+        if self.opts.reaction and self.mech is not None:
+            self.reaction_selec(dtp, 0.5)
+        _lib.check(self.lib.izem_rk3(self.ctx, C.c_void_p(dtp), *[_ptr(x) for x in self.d_i],
+                                     _ptr(self.T), _ptr(self.W_i), _ptr(self.cp), _ptr(self.ha),
+                                     _ptr(self.sensor), _ptr(self.v)), "rk3")
+        if self.opts.reaction and self.mech is not None:
+            self.reaction_selec(dtp, 0.5)
+
+
+def mech_flat(mech: cd.Mechanism) -> np.ndarray:
+    """Flat float64 image of the mechanism tables in the order of the device
+    struct IzMech (chem.hip)."""
+    t = mech.tables()
+    parts = [np.array([t["kk"], t["ii"]], dtype=np.float64), t["wt"], t["a"].ravel(), t["tmid"],
+             t["A"], t["b"], t["EoR"], t["nu"].ravel().astype(np.float64),
+             t["nunk"].ravel().astype(np.float64), t["reversible"].astype(np.float64),
+             t["ntb"].astype(np.float64), t["tb_k"].ravel().astype(np.float64),
+             t["tb_eff"].ravel(), t["fal"].astype(np.float64), t["fpar"].ravel()]
+    return np.ascontiguousarray(np.concatenate(parts), dtype=np.float64)
+
+

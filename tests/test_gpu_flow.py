@@ -1,0 +1,224 @@
+"""GPU parity of the flow path against the oracle, through the C ABI.
+
+Bar: the flux / RHS kernels keep the reference's operation order, so the
+comparison is max|d f_m| / max|f_m| <= 1e-12 per equation (north-star
+tolerance); in practice the results are bit-identical apart from libm
+exp/log differences in the transport properties.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests.flowcase import OracleRun, P, small_state
+
+pytestmark = pytest.mark.gpu
+TOL = 1.0e-12
+
+
+def _gpu_solver(box, opts=None):
+    import torch
+    from phd_amd.solver import Options, Solver
+    return Solver(box, opts or Options(reaction=0), device="cuda:0")
+
+
+def _upload(s, run):
+    import torch
+    s.v.copy_(torch.from_numpy(run.v))
+    s.T.copy_(torch.from_numpy(run.T))
+    s.W_i.copy_(torch.from_numpy(run.W_i))
+    s.cp.copy_(torch.from_numpy(run.cp))
+    s.ha.copy_(torch.from_numpy(run.ha))
+    s.sensor.copy_(torch.from_numpy(run.sensor))
+    torch.cuda.synchronize()
+
+
+def _relerr(a, b, box):
+    I = (slice(None),) + box.interior()
+    a, b = a[I], b[I]
+    out = []
+    for m in range(a.shape[0]):
+        s = np.max(np.abs(b[m]))
+        d = np.max(np.abs(a[m] - b[m]))
+        out.append(0.0 if s == 0 and d == 0 else d / (s if s > 0 else 1.0))
+    return np.array(out)
+
+
+CASES = [("2d", 24), ("3d", 12)]
+
+
+@pytest.mark.parametrize("kind,n", CASES)
+def test_wenorec_bitwise(oracle, adi, thd, kind, n):
+    import torch
+    from phd_amd.ics import Box
+    rng = np.random.default_rng(3)
+    N = 4096
+    vp = rng.standard_normal((N, 6))
+    vm = rng.standard_normal((N, 6))
+    vp[::7, 3:] += 10.0  # discontinuities
+    vm[::5, :2] *= 1e-9   # near-flat stencils (eps-dominated weights)
+    s = _gpu_solver(Box(8, 8, 1, 3, 2, (1, 1, 1)))
+    for js in (1, 0):
+        dvp = torch.tensor(vp, device="cuda:0")
+        dvm = torch.tensor(vm, device="cuda:0")
+        out = torch.zeros((N, 2), dtype=torch.float64, device="cuda:0")
+        assert s.lib.izem_wenorec(s.ctx, N, js, C.c_void_p(dvp.data_ptr()),
+                                  C.c_void_p(dvm.data_ptr()), C.c_void_p(out.data_ptr())) == 0
+        s.sync()
+        g = out.cpu().numpy()
+        ref = np.zeros((N, 2))
+        fn = oracle.lib.ora_wenorec if js else oracle.lib.ora_wenorec_nw
+        for t in range(N):
+            a, b = C.c_double(), C.c_double()
+            fn(P(np.ascontiguousarray(vp[t])), P(np.ascontiguousarray(vm[t])), C.byref(a),
+               C.byref(b))
+            ref[t] = (a.value, b.value)
+        assert np.array_equal(g, ref)
+
+
+@pytest.mark.parametrize("kind,n", CASES)
+def test_euler_flux_parity(oracle, adi, thd, kind, n):
+    import torch
+    box, v, T = small_state(adi, thd, kind, n)
+    run = OracleRun(oracle, adi, thd, box, v, T)
+    run.prim_all()
+    run.sensor[box.interior()] = 1
+    s = _gpu_solver(box)
+    _upload(s, run)
+    f_ref = np.zeros((14,) + box.shape)
+    f_gpu = torch.zeros((14,) + box.shape, dtype=torch.float64, device="cuda:0")
+    for d in range(box.ndim):
+        st = oracle.lib.ora_euler_llf(d, C.byref(run.g), C.byref(run.t), P(run.d_i[d]),
+                                      P(run.T), P(run.W_i), P(run.cp), P(run.ha),
+                                      run.sensor.ctypes.data_as(C.POINTER(C.c_uint8)), 1,
+                                      P(run.v), P(f_ref))
+        assert st == 0
+        s.euler_llf(d, f_gpu)
+        s.sync()
+        err = _relerr(f_gpu.cpu().numpy(), f_ref, box)
+        print(kind, "dir", d, "max rel err", err.max())
+        assert err.max() <= TOL, err
+    assert s.status() == 0
+
+
+@pytest.mark.parametrize("kind,n", CASES)
+def test_euler_flux_linear_path(oracle, adi, thd, kind, n):
+    """Faces whose 6-cell window has no flagged cell take wenorec_nw."""
+    import torch
+    box, v, T = small_state(adi, thd, kind, n)
+    run = OracleRun(oracle, adi, thd, box, v, T)
+    run.prim_all()
+    rng = np.random.default_rng(1)
+    I = box.interior()
+    run.sensor[I] = (rng.random(run.sensor[I].shape) < 0.05).astype(np.uint8)
+    s = _gpu_solver(box)
+    _upload(s, run)
+    f_ref = np.zeros((14,) + box.shape)
+    f_gpu = torch.zeros((14,) + box.shape, dtype=torch.float64, device="cuda:0")
+    for d in range(box.ndim):
+        oracle.lib.ora_euler_llf(d, C.byref(run.g), C.byref(run.t), P(run.d_i[d]), P(run.T),
+                                 P(run.W_i), P(run.cp), P(run.ha),
+                                 run.sensor.ctypes.data_as(C.POINTER(C.c_uint8)), 1, P(run.v),
+                                 P(f_ref))
+        s.euler_llf(d, f_gpu)
+    s.sync()
+    assert _relerr(f_gpu.cpu().numpy(), f_ref, box).max() <= TOL
+
+
+@pytest.mark.parametrize("kind,n", CASES)
+def test_viscflux_parity(oracle, adi, thd, kind, n):
+    import torch
+    box, v, T = small_state(adi, thd, kind, n)
+    run = OracleRun(oracle, adi, thd, box, v, T)
+    run.prim_all()
+    s = _gpu_solver(box)
+    _upload(s, run)
+    f_ref = np.zeros((14,) + box.shape)
+    f_gpu = torch.zeros((14,) + box.shape, dtype=torch.float64, device="cuda:0")
+    oracle.lib.ora_viscflux(C.byref(run.g), C.byref(run.t), C.byref(run.a), P(run.d_i[0]),
+                            P(run.d_i[1]), P(run.d_i[2]), P(run.T), P(run.W_i), P(run.ha),
+                            P(run.v), P(f_ref))
+    s.viscflux(f_gpu)
+    s.sync()
+    err = _relerr(f_gpu.cpu().numpy(), f_ref, box)
+    print(kind, "visc max rel err", err)
+    assert err.max() <= TOL
+
+
+@pytest.mark.parametrize("kind,n", CASES)
+def test_newton_parity(oracle, adi, thd, kind, n):
+    """prim_inv_var from a perturbed warm start: same iteration count, same
+    T / cp / ha (cp, ha at the pre-update temperature)."""
+    import torch
+    box, v, T = small_state(adi, thd, kind, n)
+    rng = np.random.default_rng(5)
+    T0 = T * (1.0 + 0.01 * rng.standard_normal(T.shape))
+    run = OracleRun(oracle, adi, thd, box, v, T0)
+    s = _gpu_solver(box)
+    _upload(s, run)
+    it = run.prim_all()
+    it_gpu = C.c_int()
+    for d in run.domains():
+        assert s.lib.izem_prim_inv_var(s.ctx, d, C.c_void_p(s.v.data_ptr()),
+                                       C.c_void_p(s.W_i.data_ptr()), C.c_void_p(s.T.data_ptr()),
+                                       C.c_void_p(s.cp.data_ptr()), C.c_void_p(s.ha.data_ptr()),
+                                       C.byref(it_gpu)) == 0
+    s.sync()
+    assert it_gpu.value == it
+    mask = np.zeros(box.shape, bool)
+    mask[box.interior()] = True
+    for name, ref in (("T", run.T), ("cp", run.cp), ("W_i", run.W_i)):
+        g = getattr(s, name).cpu().numpy()
+        assert np.max(np.abs(g[mask] - ref[mask]) / np.abs(ref[mask])) <= 1e-14, name
+    h = s.ha.cpu().numpy()
+    assert np.max(np.abs(h[:, mask] - run.ha[:, mask])) <= 1e-14 * np.max(np.abs(run.ha))
+
+
+@pytest.mark.parametrize("kind,n", CASES)
+def test_rhs_and_rk3_parity(oracle, adi, thd, kind, n):
+    """Fused right-hand side (sensor, Euler, viscous) and one SSP-RK3 step
+    against the oracle's rk3 (module_solver.f90:694-856)."""
+    import torch
+    box, v, T = small_state(adi, thd, kind, n)
+    run = OracleRun(oracle, adi, thd, box, v, T)
+    run.prim_all()
+    s = _gpu_solver(box)
+    _upload(s, run)
+    # fused RHS
+    f_gpu = torch.zeros((14,) + box.shape, dtype=torch.float64, device="cuda:0")
+    s.rhs(f_gpu)
+    s.sync()
+    # oracle rk3 gives the first-stage RHS and the stepped state
+    opts = run.opts(vis=1)
+    ts = np.zeros(4)
+    dtmin = C.c_double(1.0)
+    oracle.lib.ora_timestep(C.byref(run.g), C.byref(run.t), C.byref(run.a), C.byref(opts),
+                            P(run.d_i[0]), P(run.d_i[1]), P(run.d_i[2]), P(run.cp), P(run.W_i),
+                            P(run.T), P(run.v), C.c_double(0.0), P(ts), C.byref(dtmin))
+    s.timestep()
+    tsg = s.time_state()
+    assert abs(tsg[4] - dtmin.value) <= 1e-14 * dtmin.value, (tsg, ts, dtmin.value)
+    f1 = np.zeros((14,) + box.shape)
+    sens = np.zeros(box.shape, np.uint8)
+    st = oracle.lib.ora_rk3(C.byref(run.g), C.byref(run.t), C.byref(run.a), C.byref(opts),
+                            C.c_double(dtmin.value), P(run.d_i[0]), P(run.d_i[1]),
+                            P(run.d_i[2]), P(run.T), P(run.W_i), P(run.cp), P(run.ha),
+                            sens.ctypes.data_as(C.POINTER(C.c_uint8)), P(run.v), P(f1))
+    assert st == 0
+    err = _relerr(f_gpu.cpu().numpy(), f1, box)
+    print(kind, "rhs max rel err", err.max())
+    assert err.max() <= TOL
+    dt = torch.tensor([dtmin.value], dtype=torch.float64, device="cuda:0")
+    s.sensor.zero_()
+    s.rk3(dt)
+    s.sync()
+    assert s.status() == 0
+    vg = s.v.cpu().numpy()
+    err = _relerr(vg, run.v, box)
+    print(kind, "rk3 state max rel err", err.max())
+    assert err.max() <= TOL
+    Tg = s.T.cpu().numpy()
+    I = box.interior()
+    assert np.max(np.abs(Tg[I] - run.T[I]) / run.T[I]) <= TOL
