@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Benchmark of the IZEM reacting-flow step on one MI355X.
+
+Workload (the headline configuration): 3-D periodic 256^3 decaying reacting
+HIT, premixed H2/air at phi = 0.8, T0 = 1000 K with 50 K Gaussian-random
+fluctuations, u_rms = 20 m/s, FP64, 9 species / 21-reaction mechanism.  A
+step is one iteration of the reference main loop (devel-izem/src/izem.f90:
+254-277): timestep -> chemistry(dt/2) -> SSP-RK3(dt) -> chemistry(dt/2),
+WENO5-JS characteristic LLF fluxes, viscous + mixture-averaged diffusion,
+Newton T inversion, batched BDF chemistry (rtol 1e-6, atol 1e-12).
+
+Prints one JSON line (metric, value in cell-steps/s, roofline of the
+dominant kernel measured with HIP events, CPU baseline timed on the host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-steps/s 256^3 reacting H2 (FP64); chem cells/s surrogate vs VODE @|dT|<=2K"
+# algorithmic FP64 operations of one Euler face with the JS reconstruction
+# (count per section in DESIGN.md)
+EULER_FLOP_PER_FACE = 8461
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def read_traffic(n: int):
+    """HBM bytes per Euler-face launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", f"euler_faces_{n}cube_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n: int = 48):
+    """The oracle (C restatement of the flow path, one core) plus the
+    reference's own compiled DVODE chemistry, one full reacting step on an
+    n^3 sample of the same workload recipe."""
+    import ctypes as C
+
+    import numpy as np
+
+    from oracle.oracle import Oracle, RefChem
+    from phd_amd import chemdata as cd
+    from phd_amd import ics
+    from tests.flowcase import OracleRun, P
+
+    adi = cd.Adim()
+    thd = cd.build_thd(adi)
+    box, v, T = ics.config2_state(adi, thd, n=n, seed=2, L=0.01 * n / 256)
+    run = OracleRun(Oracle(), adi, thd, box, v, T)
+    run.prim_all()
+    rc = RefChem()
+    opts = run.opts(vis=1, reaction=1)
+    t0 = time.perf_counter()
+    ts = np.zeros(4)
+    dtmin = C.c_double(1.0)
+    run.o.lib.ora_timestep(C.byref(run.g), C.byref(run.t), C.byref(run.a), C.byref(opts),
+                           P(run.d_i[0]), P(run.d_i[1]), P(run.d_i[2]), P(run.cp), P(run.W_i),
+                           P(run.T), P(run.v), C.c_double(0.0), P(ts), C.byref(dtmin))
+    dt = dtmin.value
+    rho_cgs = adi.rho_ref * 1e-3
+
+    def chem():
+        g = box.ng
+        for k in range(g, box.nz - g - 1):
+            for j in range(g, box.ny - g - 1):
+                for i in range(g, box.nx - g - 1):
+                    c = (k + g, j + g, i + g)
+                    rho0 = run.v[(0,) + c]
+                    Ya = run.v[(slice(5, 14),) + c] / rho0
+                    Tn, Yn, ist, *_ = rc.psr(run.T[c] * adi.T_ref, rho0 * rho_cgs, Ya,
+                                             0.5 * dt * adi.time_ref, 1e-6, 1e-12)
+                    run.T[c] = Tn / adi.T_ref
+                    Yc = np.clip(Yn[:8], 0, 1)
+                    run.v[(slice(5, 13),) + c] = rho0 * Yc
+                    run.v[(13,) + c] = rho0 * (1 - Yc.sum())
+        run.o.lib.ora_upd_prim_var_woT(0, C.byref(run.g), C.byref(run.t), P(run.v), P(run.T),
+                                       P(run.W_i), P(run.cp), P(run.ha))
+        run.o.lib.ora_comm_cons(C.byref(run.g), P(run.v))
+        run.o.lib.ora_upd_prim_var_ghost(C.byref(run.g), C.byref(run.t), 1, P(run.v),
+                                         P(run.W_i), P(run.T), P(run.cp), P(run.ha))
+
+    chem()
+    sens = np.zeros(box.shape, np.uint8)
+    run.o.lib.ora_rk3(C.byref(run.g), C.byref(run.t), C.byref(run.a), C.byref(opts),
+                      C.c_double(dt), P(run.d_i[0]), P(run.d_i[1]), P(run.d_i[2]), P(run.T),
+                      P(run.W_i), P(run.cp), P(run.ha), sens.ctypes.data_as(C.POINTER(C.c_uint8)),
+                      P(run.v), None)
+    chem()
+    el = time.perf_counter() - t0
+    return {"value": n ** 3 / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"one full reacting step (timestep, 2 x chemistry, SSP-RK3) on a {n}^3 "
+                      f"box of the same recipe: oracle C restatement of the flow path "
+                      f"(1 core) + the reference's compiled PSR_CHEMKIN/DVODE per cell; "
+                      f"{el:.1f} s"}
+
+
+def chem_vode_rate(s, thd, ncell: int):
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from phd_amd import ics
+
+    T, p, Y = ics.config3_cells(thd, ncell, seed=3)
+    Wmix = 1.0 / (Y / (np.array(thd.W) * 1000.0)).sum(axis=1)
+    rho = (p * 10.0) * Wmix / (8.314510e7 * T)
+    d = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
+    dT, drho, dY = d(T), d(rho), d(Y)
+    nst = torch.zeros(ncell, dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rc = s.lib.izem_psr_batch(s.ctx, ncell, C.c_void_p(dT.data_ptr()), C.c_void_p(drho.data_ptr()),
+                              C.c_void_p(dY.data_ptr()), 1.0e-6, C.c_void_p(nst.data_ptr()))
+    s.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    st = nst.cpu().numpy()
+    return {"cells": ncell, "dt_s": 1e-6, "vode_cells_per_s": ncell / el, "seconds": el,
+            "bdf_steps_mean": float(st.mean()), "bdf_steps_max": int(st.max()), "rc": rc}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--chem-cells", type=int, default=1 << 22)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported "
+              f"(the multi-GPU decomposition is described in DESIGN.md, not run)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from phd_amd import chemdata as cd
+    from phd_amd import ics
+    from phd_amd.solver import Options, Solver, profile_read, profile_start
+
+    n = args.n
+    adi = cd.Adim()
+    thd = cd.build_thd(adi)
+    mech = cd.read_mechanism()
+    log(f"building the {n}^3 reacting HIT state")
+    box, v, T = ics.config2_state(adi, thd, n=n, seed=2)
+    s = Solver(box, Options(), adi=adi, thd=thd, mech=mech)
+    s.load_state(v, T)
+    del v, T
+    log("state resident on the GPU; warm-up")
+    for _ in range(args.warmup):
+        s.step()
+    s.sync()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
+    profile_start(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for q in range(args.steps):
+        s.step()
+    s.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    prof = profile_read(s)
+    status = s.status()
+    ts = s.time_state()
+    ncell = n ** 3
+    value = ncell * args.steps / el
+    ms_step = 1e3 * el / args.steps
+    e_ms, e_cnt = prof["euler_faces"]
+    nfaces = (n + 1) * n * n
+    e_avg_s = e_ms / e_cnt / 1e3
+    achieved = EULER_FLOP_PER_FACE * nfaces / e_avg_s / 1e12
+    traffic = read_traffic(n)
+    log(f"{ms_step:.1f} ms/step; kernel time per step: " +
+        ", ".join(f"{k} {v[0] / args.steps:.1f} ms" for k, v in prof.items()))
+    chem = None
+    if args.chem_cells > 0:
+        log(f"batched VODE-path chemistry on {args.chem_cells} independent cells")
+        chem = chem_vode_rate(s, thd, args.chem_cells)
+    s.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle flow + reference DVODE, 1 core)")
+        cpu = cpu_baseline()
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3-D periodic {n}^3 decaying reacting HIT, H2/air phi=0.8, "
+                               f"T0=1000 K +- 50 K, u_rms=20 m/s, 14 conserved variables, "
+                               f"WENO5-JS LLF + viscous + mixture-averaged diffusion, SSP-RK3 "
+                               f"with Strang BDF chemistry (rtol 1e-6, atol 1e-12)",
+                   "cells": ncell, "dt_s": float(ts[4] * adi.time_ref),
+                   "status_bits": status},
+        "roofline": {"kernel": "k_euler_faces (x, y, z)", "bound": "mfma",
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "avg_launch_ms": e_avg_s * 1e3, "launches": e_cnt,
+                     "flop_per_face": EULER_FLOP_PER_FACE, "faces_per_launch": nfaces},
+        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
+        "chemistry": chem,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -79,6 +79,14 @@ int izem_get_time_state(izem_ctx *ctx, double *host6);
  * limit of the next izem_timestep) */
 int izem_set_time_state(izem_ctx *ctx, const double *host6);
 
+/* event profiler: capacity > 0 starts recording up to capacity kernel
+ * launches of the classes 0 Euler faces, 1 viscous, 2 RHS + RK update,
+ * 3 Newton, 4 chemistry, 5 other; 0 stops.  izem_profile_read sums the
+ * recorded milliseconds and launch counts per class (6 entries each) and
+ * clears the record. */
+int izem_profile(izem_ctx *ctx, int capacity);
+int izem_profile_read(izem_ctx *ctx, double *ms, int *count);
+
 /* ---- module_weno ------------------------------------------------------ */
 /* wenorec (js = 1, module_weno.f90:286) or wenorec_nw (js = 0, :236) on n
  * stencils of 6 values; out = (vminus, vplus) pairs */
@@ -145,6 +153,10 @@ int izem_timestep(izem_ctx *ctx, const double *dx_i, const double *dy_i, const d
  * chemkin/vode.f:391), flat image of phd_amd.chemdata.Mechanism */
 int izem_chem_setup(izem_ctx *ctx, const double *mech, long nmech, double rtol, double atol,
                     int trim_box);
+/* launch parameters of the persistent chemistry kernels, as chosen by the
+ * autotuner (phd_amd/autotune.py): threads per block (multiple of 64),
+ * resident blocks per CU, Jacobian reuse age in steps */
+int izem_chem_tune(izem_ctx *ctx, int block, int blocks_per_cu, int jac_age);
 /* ckwyr (chemkin/vode.f:1797) on n cells: rho [g/cm3], T [K], Y[n][kk] ->
  * wdot[n][kk] [mol/cm3/s] */
 int izem_ckwyr(izem_ctx *ctx, int n, const double *rho, const double *T, const double *Y,

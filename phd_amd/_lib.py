@@ -13,11 +13,12 @@ LIB_PATH = os.path.join(HERE, "libizem_hip.so")
 
 EXPORTS = [
     "izem_create", "izem_destroy", "izem_synchronize", "izem_set_stream", "izem_status",
-    "izem_time_state", "izem_get_time_state", "izem_set_time_state",
+    "izem_time_state", "izem_get_time_state", "izem_set_time_state", "izem_profile",
+    "izem_profile_read",
     "izem_wenorec", "izem_euler_llf_x", "izem_euler_llf_y", "izem_euler_llf_z",
     "izem_viscflux", "izem_force_sumY", "izem_prim_inv_var", "izem_upd_prim_var_woT",
     "izem_comm_cons", "izem_upd_prim_var_ghost", "izem_shock_sensor", "izem_rk3", "izem_rhs",
-    "izem_timestep", "izem_chem_setup", "izem_ckwyr", "izem_psr_batch", "izem_reaction_dvode",
+    "izem_timestep", "izem_chem_setup", "izem_chem_tune", "izem_ckwyr", "izem_psr_batch", "izem_reaction_dvode",
 ]
 
 
@@ -63,6 +64,8 @@ def load(path: str | None = None):
     lib.izem_set_stream.argtypes = [vp, vp]
     lib.izem_get_time_state.argtypes = [vp, vp]
     lib.izem_set_time_state.argtypes = [vp, vp]
+    lib.izem_profile.argtypes = [vp, C.c_int]
+    lib.izem_profile_read.argtypes = [vp, vp, vp]
     lib.izem_wenorec.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
     for n in ("x", "y", "z"):
         getattr(lib, f"izem_euler_llf_{n}").argtypes = [vp] * 9
@@ -77,6 +80,7 @@ def load(path: str | None = None):
     lib.izem_rhs.argtypes = [vp] * 11
     lib.izem_timestep.argtypes = [vp] * 8
     lib.izem_chem_setup.argtypes = [vp, vp, C.c_long, C.c_double, C.c_double, C.c_int]
+    lib.izem_chem_tune.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     lib.izem_ckwyr.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     lib.izem_psr_batch.argtypes = [vp, C.c_int, vp, vp, vp, C.c_double, vp]
     lib.izem_reaction_dvode.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp]

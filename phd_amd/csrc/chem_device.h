@@ -1,0 +1,404 @@
+// chem_device.h -- CHEMKIN-form kinetics and the BDF integrator of one
+// reactor cell.  Device code; the functions are also host-compilable (IZ_HD)
+// so that the integrator can be exercised on a CPU build.
+#pragma once
+
+#include <math.h>
+
+#ifndef IZ_HD
+#define IZ_HD __device__
+#endif
+#ifndef IZ_CMECH
+#define IZ_CMECH c_mech
+#define IZ_CBDF c_bdf
+#endif
+
+#include "chem_tables.h"
+
+// --------------------------------------------------------------- kinetics
+// CKSMH (vode.f:1474-1535): S/R - H/RT per species
+IZ_HD inline void d_smh(double T, double *smh) {
+  double lt = log(T);
+  double t2 = T * T, t3 = t2 * T, t4 = t3 * T;
+  double tn0 = lt - 1.0, tn1 = T / 2.0, tn2 = t2 / 6.0, tn3 = t3 / 12.0, tn4 = t4 / 20.0;
+  double Ti = 1.0 / T;
+  for (int k = 0; k < IZ_CMECH.kk; k++) {
+    const double *a = IZ_CMECH.a[k][(T > IZ_CMECH.tmid[k]) ? 1 : 0];
+    smh[k] = tn0 * a[0] + tn1 * a[1] + tn2 * a[2] + tn3 * a[3] + tn4 * a[4] + a[6] - a[5] * Ti;
+  }
+}
+
+IZ_HD inline double d_ipow(double x, int n) {
+  double r = 1.0;
+  for (int q = 0; q < n; q++) r *= x;
+  return r;
+}
+
+// CKWYR = CKRATT + CKYTCR + CKRATX + stoichiometric sum (vode.f:935-1314,
+// 1797-1904, 3171-3221); mol/cm^3/s
+IZ_HD inline void d_ckwyr(double rho, double T, const double *Y, double *wdot) {
+  const int ii = IZ_CMECH.ii, kk = IZ_CMECH.kk;
+  double smh[CK_KMAX], c[CK_KMAX];
+  d_smh(T, smh);
+  const double alogt = log(T), Ti = 1.0 / T;
+  const double pfac = 1.01325e6 / (8.314510e7 * T);
+  double ctot = 0.0;
+  for (int k = 0; k < kk; k++) {
+    c[k] = rho * Y[k] / IZ_CMECH.wt[k];
+    ctot += c[k];
+  }
+  for (int k = 0; k < kk; k++) wdot[k] = 0.0;
+  for (int i = 0; i < ii; i++) {
+    double kf = IZ_CMECH.A[i] * exp(IZ_CMECH.b[i] * alogt - IZ_CMECH.EoR[i] * Ti);
+    double kr = 0.0;
+    if (IZ_CMECH.reversible[i]) {
+      double s = 0.0;
+      int nus = 0;
+      for (int n = 0; n < 6; n++) {
+        int k = IZ_CMECH.nunk[i][n];
+        if (k) { s += (double)IZ_CMECH.nu[i][n] * smh[k - 1]; nus += IZ_CMECH.nu[i][n]; }
+      }
+      double eqk = exp(fmin(s, 690.77552789821368));  // EXPARG = ln(1e300)
+      eqk *= (nus >= 0) ? d_ipow(pfac, nus) : 1.0 / d_ipow(pfac, -nus);
+      kr = kf / fmax(eqk, 1e-300);
+    }
+    double ctb = 1.0;
+    if (IZ_CMECH.ntb[i] >= 0) {
+      ctb = ctot;
+      for (int l = 0; l < IZ_CMECH.ntb[i]; l++)
+        ctb += (IZ_CMECH.tb_eff[i][l] - 1.0) * c[IZ_CMECH.tb_k[i][l] - 1];
+    }
+    if (IZ_CMECH.fal[i]) {
+      const double *fp = IZ_CMECH.fpar[i];
+      double rklow = fp[0] * exp(fp[1] * alogt - fp[2] * Ti);
+      double pr = rklow * ctb / kf;
+      ctb = 1.0;
+      double pcor = pr / (1.0 + pr);
+      if (IZ_CMECH.fal[i] > 1) {
+        double prlog = log10(fmax(pr, 1e-300));
+        double fcent = (1.0 - fp[3]) * exp(-T / fp[4]) + fp[3] * exp(-T / fp[5]);
+        if (IZ_CMECH.fal[i] == 4) fcent += exp(-fp[6] * Ti);
+        double fclog = log10(fmax(fcent, 1e-300));
+        double xn = 0.75 - 1.27 * fclog;
+        double cprlog = prlog - (0.4 + 0.67 * fclog);
+        double q = cprlog / (xn - 0.14 * cprlog);
+        double flog = fclog / (1.0 + q * q);
+        pcor *= exp(flog * 2.302585092994046);  // 10**flog
+      }
+      kf *= pcor;
+      kr *= pcor;
+    }
+    kf *= ctb;
+    kr *= ctb;
+    double rf = kf, rr = kr;
+    for (int n = 0; n < 3; n++) {
+      int k = IZ_CMECH.nunk[i][n];
+      if (k) rf *= d_ipow(c[k - 1], -IZ_CMECH.nu[i][n]);
+    }
+    for (int n = 3; n < 6; n++) {
+      int k = IZ_CMECH.nunk[i][n];
+      if (k) rr *= d_ipow(c[k - 1], IZ_CMECH.nu[i][n]);
+    }
+    double rop = rf - rr;
+    for (int n = 0; n < 6; n++) {
+      int k = IZ_CMECH.nunk[i][n];
+      if (k) wdot[k - 1] += rop * (double)IZ_CMECH.nu[i][n];
+    }
+  }
+}
+
+// FUN (reaction_v.f:69-114) with CKCVBS (vode.f:64) and CKHMS (vode.f:228):
+// constant-volume adiabatic reactor, z = (T, Y_1..Y_kk)
+IZ_HD inline void d_reactor_rhs(double rho, const double *z, double *zp) {
+  const int kk = IZ_CMECH.kk;
+  const double T = z[0], ru = 8.314510e7;
+  double wdot[CK_KMAX];
+  d_ckwyr(rho, T, z + 1, wdot);
+  double t2 = T * T, t3 = t2 * T, t4 = t3 * T;
+  double cvb = 0.0, sum = 0.0;
+  const double rho_i = 1.0 / rho;
+  for (int k = 0; k < kk; k++) {
+    const double *a = IZ_CMECH.a[k][(T > IZ_CMECH.tmid[k]) ? 1 : 0];
+    const double w = IZ_CMECH.wt[k];
+    double cp = ru * (a[0] + a[1] * T + a[2] * t2 + a[3] * t3 + a[4] * t4) / w;
+    cvb += z[1 + k] * (cp - ru / w);
+    double h = ru * T * (a[0] + a[1] * T / 2.0 + a[2] * t2 / 3.0 + a[3] * t3 / 4.0 +
+                         a[4] * t4 / 5.0 + a[5] / T) / w;
+    zp[1 + k] = wdot[k] * w * rho_i;
+    sum += h * wdot[k] * w - ru * T * wdot[k];
+  }
+  zp[0] = -sum / (rho * cvb);
+}
+
+// ------------------------------------------------------------ BDF solver
+#define N_ CK_NEQ
+
+IZ_HD inline double d_wrms(const double *v, const double *ewt) {
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < N_; i++) { double x = v[i] / ewt[i]; s += x * x; }
+  return sqrt(s / N_);
+}
+
+// LU with partial pivoting (the dgefa / dgesl pair, vode.f:7033, 7630)
+IZ_HD inline bool d_lu(double (*A)[N_], int *piv) {
+  bool ok = true;
+  for (int k = 0; k < N_; k++) {
+    int p = k;
+    double amax = fabs(A[k][k]);
+    for (int i = k + 1; i < N_; i++) if (fabs(A[i][k]) > amax) { amax = fabs(A[i][k]); p = i; }
+    piv[k] = p;
+    if (amax == 0.0) { ok = false; continue; }
+    if (p != k)
+      for (int j = 0; j < N_; j++) { double t = A[k][j]; A[k][j] = A[p][j]; A[p][j] = t; }
+    double inv = 1.0 / A[k][k];
+    for (int i = k + 1; i < N_; i++) {
+      double m = A[i][k] * inv;
+      A[i][k] = m;
+      for (int j = k + 1; j < N_; j++) A[i][j] -= m * A[k][j];
+    }
+  }
+  return ok;
+}
+IZ_HD inline void d_lusolve(const double (*A)[N_], const int *piv, double *x) {
+  for (int k = 0; k < N_; k++) {
+    int p = piv[k];
+    if (p != k) { double t = x[k]; x[k] = x[p]; x[p] = t; }
+    for (int i = k + 1; i < N_; i++) x[i] -= A[i][k] * x[k];
+  }
+  for (int k = N_ - 1; k >= 0; k--) {
+    double s = x[k];
+    for (int j = k + 1; j < N_; j++) s -= A[k][j] * x[j];
+    x[k] = s / A[k][k];
+  }
+}
+
+// Pascal-triangle update of the Nordsieck history (sign +1: predict to t+h,
+// sign -1: retract); sweeps jb = 1..q over columns q-jb..q-1 in ascending
+// order, which makes the -1 sweep the exact inverse of the +1 sweep.
+IZ_HD inline void d_nordsieck_shift(double (*z)[N_], int q, double sign) {
+  for (int jb = 1; jb <= q; jb++)
+    for (int i = q - jb; i <= q - 1; i++)
+#pragma unroll
+      for (int n = 0; n < N_; n++) z[i][n] += sign * z[i + 1][n];
+}
+
+// rescale the Nordsieck history for a step change h -> rh*h
+IZ_HD inline void d_nordsieck_rescale(double (*z)[N_], int q, double rh) {
+  double f = rh;
+  for (int j = 1; j <= q; j++) {
+#pragma unroll
+    for (int n = 0; n < N_; n++) z[j][n] *= f;
+    f *= rh;
+  }
+}
+
+struct BdfResult {
+  int nst, nfe, nje, ok;
+};
+
+// Integrate the reactor of one cell from 0 to tout (ITASK = 1: step past tout
+// and interpolate the Nordsieck history back to it).
+IZ_HD inline BdfResult d_bdf_integrate(double rho, double *y, double tout, double rtol,
+                                     double atol, int jac_age) {
+  const double uround = 2.220446049250313e-16, srur = 1.4901161193847656e-08;
+  const int maxstep = 10000, maxcor = 3;
+  BdfResult res{0, 0, 0, 1};
+  double z[6][N_], acor[N_], ewt[N_], savf[N_], yy[N_];
+  double P[N_][N_];
+  int piv[N_];
+#pragma unroll
+  for (int i = 0; i < N_; i++) { z[0][i] = y[i]; ewt[i] = rtol * fabs(y[i]) + atol; }
+  d_reactor_rhs(rho, y, savf);
+  res.nfe++;
+  // initial step (the LSODE-style estimate also used by DVHIN's first guess)
+  double tol = fmax(rtol, 100.0 * uround);
+  double fn = d_wrms(savf, ewt);
+  double sum = 1.0 / (tol * tout * tout) + tol * fn * fn;
+  double h = 1.0 / sqrt(sum);
+  h = fmin(h, tout);
+#pragma unroll
+  for (int i = 0; i < N_; i++) z[1][i] = h * savf[i];
+  int q = 1, ialth = 2, ncf = 0, kflag = 0;
+  double t = 0.0, rc = 0.0, crate = 0.7, hl0 = 0.0, rmax = 1.0e4;
+  bool need_jac = true;
+  int nslj = 0;
+  double acorp_norm = 0.0;  // unused placeholder for clarity
+  (void)acorp_norm;
+  while (t < tout) {
+    if (res.nst >= maxstep) { res.ok = 0; break; }
+    // error weights at the start of the step
+#pragma unroll
+    for (int i = 0; i < N_; i++) ewt[i] = rtol * fabs(z[0][i]) + atol;
+    // predictor: multiply the Nordsieck array by the Pascal triangle
+    d_nordsieck_shift(z, q, 1.0);
+    const double l0 = IZ_CBDF.el[q][0];
+    hl0 = h * l0;
+    if (rc != 0.0 && fabs(hl0 / rc - 1.0) > 0.3) need_jac = true;
+    if (res.nst >= nslj + jac_age) need_jac = true;
+    // corrector (chord Newton on G(e) = h f(z0 + l0 e) - z1 - e)
+#pragma unroll
+    for (int n = 0; n < N_; n++) { yy[n] = z[0][n]; acor[n] = 0.0; }
+    d_reactor_rhs(rho, yy, savf);
+    res.nfe++;
+    bool jcur = false;
+    if (need_jac) {
+      double fac = d_wrms(savf, ewt);
+      double r0 = 1000.0 * fabs(h) * uround * N_ * fac;
+      if (r0 == 0.0) r0 = 1.0;
+      double ftmp[N_];
+      for (int j = 0; j < N_; j++) {
+        double yj = yy[j];
+        double r = fmax(srur * fabs(yj), r0 * ewt[j]);
+        yy[j] = yj + r;
+        d_reactor_rhs(rho, yy, ftmp);
+        yy[j] = yj;
+        double ri = 1.0 / r;
+#pragma unroll
+        for (int i = 0; i < N_; i++) P[i][j] = -hl0 * (ftmp[i] - savf[i]) * ri;
+      }
+      res.nfe += N_;
+      res.nje++;
+#pragma unroll
+      for (int i = 0; i < N_; i++) P[i][i] += 1.0;
+      d_lu(P, piv);
+      rc = hl0;
+      nslj = res.nst;
+      need_jac = false;
+      jcur = true;
+      crate = 0.7;
+    }
+    bool conv = false;
+    double del = 0.0, delp = 0.0;
+    for (int m = 0; m < maxcor; m++) {
+      double d[N_];
+#pragma unroll
+      for (int n = 0; n < N_; n++) d[n] = h * savf[n] - z[1][n] - acor[n];
+      d_lusolve(P, piv, d);
+      if (rc != hl0) {  // chord correction for a stale h*l0
+        double cs = 2.0 / (1.0 + hl0 / rc);
+#pragma unroll
+        for (int n = 0; n < N_; n++) d[n] *= cs;
+      }
+      del = d_wrms(d, ewt);
+#pragma unroll
+      for (int n = 0; n < N_; n++) { acor[n] += d[n]; yy[n] = z[0][n] + l0 * acor[n]; }
+      if (m > 0) crate = fmax(0.2 * crate, del / delp);
+      double dcon = del * fmin(1.0, 1.5 * crate) / (IZ_CBDF.tesco[q][1] * (0.5 / (q + 2)));
+      if (dcon <= 1.0) { conv = true; break; }
+      if (m >= 1 && del > 2.0 * delp) break;
+      delp = del;
+      d_reactor_rhs(rho, yy, savf);
+      res.nfe++;
+    }
+    if (!conv) {
+      // undo the prediction
+      d_nordsieck_shift(z, q, -1.0);
+      ncf++;
+      need_jac = true;
+      if (jcur || ncf > 10) {
+        if (ncf > 10) { res.ok = 0; break; }
+        d_nordsieck_rescale(z, q, 0.25);
+        h *= 0.25;
+        rmax = 2.0;
+      }
+      continue;
+    }
+    // local error test
+    double dsm = d_wrms(acor, ewt) / IZ_CBDF.tesco[q][1];
+#ifdef IZ_BDF_TRACE
+    printf("t %.4e h %.3e q %d dsm %.3e conv %d jcur %d T %.6f\n", t, h, q, dsm, (int)conv,
+           (int)jcur, z[0][0]);
+#endif
+    if (dsm > 1.0) {
+      d_nordsieck_shift(z, q, -1.0);
+      kflag--;
+      if (kflag <= -3) {
+        // repeated failures: restart at order 1 from a fresh derivative
+        h *= 0.1;
+#pragma unroll
+        for (int n = 0; n < N_; n++) yy[n] = z[0][n];
+        d_reactor_rhs(rho, yy, savf);
+        res.nfe++;
+#pragma unroll
+        for (int n = 0; n < N_; n++) z[1][n] = h * savf[n];
+        q = 1;
+        ialth = 5;
+        need_jac = true;
+        if (kflag <= -10) { res.ok = 0; break; }
+        continue;
+      }
+      double rh = 1.0 / (1.2 * pow(dsm, 1.0 / (q + 1)) + 1.2e-6);
+      int newq = q;
+      if (q > 1) {
+        double ddn = d_wrms(z[q], ewt) / IZ_CBDF.tesco[q][0];
+        double rhdn = 1.0 / (1.3 * pow(ddn, 1.0 / q) + 1.3e-6);
+        if (rhdn > rh) { rh = rhdn; newq = q - 1; }
+      }
+      if (kflag <= -2) rh = fmin(rh, 0.2);
+      q = newq;
+      d_nordsieck_rescale(z, q, rh);
+      h *= rh;
+      ialth = q + 1;
+      rmax = 2.0;
+      continue;
+    }
+    // accept
+    kflag = 0;
+    ncf = 0;
+    t += h;
+    res.nst++;
+    for (int j = 0; j <= q; j++) {
+      double e = IZ_CBDF.el[q][j];
+#pragma unroll
+      for (int n = 0; n < N_; n++) z[j][n] += e * acor[n];
+    }
+    ialth--;
+    if (ialth == 1 && q < 5) {
+#pragma unroll
+      for (int n = 0; n < N_; n++) z[5][n] = acor[n];  // kept for the order-up estimate
+    }
+    if (ialth > 0) continue;
+    // order and step selection (every q+1 steps)
+    double rhsm = 1.0 / (1.2 * pow(dsm, 1.0 / (q + 1)) + 1.2e-6);
+    double rhdn = 0.0, rhup = 0.0;
+    if (q > 1) {
+      double ddn = d_wrms(z[q], ewt) / IZ_CBDF.tesco[q][0];
+      rhdn = 1.0 / (1.3 * pow(ddn, 1.0 / q) + 1.3e-6);
+    }
+    if (q < 5) {
+      double dd[N_];
+#pragma unroll
+      for (int n = 0; n < N_; n++) dd[n] = acor[n] - z[5][n];
+      double dup = d_wrms(dd, ewt) / IZ_CBDF.tesco[q][2];
+      rhup = 1.0 / (1.4 * pow(dup, 1.0 / (q + 2)) + 1.4e-6);
+    }
+    int newq = q;
+    double rh = rhsm;
+    if (rhup > rh && rhup >= rhdn) { newq = q + 1; rh = rhup; }
+    else if (rhdn > rh) { newq = q - 1; rh = rhdn; }
+    if (rh < 1.1) { ialth = 3; continue; }
+    rh = fmin(rh, rmax);
+    rmax = 10.0;
+    if (newq > q) {
+      // new highest-order column from the last correction
+      double e = IZ_CBDF.el[q][q] / (q + 1);
+#pragma unroll
+      for (int n = 0; n < N_; n++) z[newq][n] = acor[n] * e;
+    }
+    q = newq;
+    d_nordsieck_rescale(z, q, rh);
+    h *= rh;
+    ialth = q + 1;
+  }
+  // interpolate back to tout: y = sum_j z[j] s^j
+  double s = (tout - t) / h;
+#pragma unroll
+  for (int n = 0; n < N_; n++) {
+    double acc = z[q][n];
+    for (int j = q - 1; j >= 0; j--) acc = acc * s + z[j][n];
+    y[n] = acc;
+  }
+  return res;
+}
+

@@ -1,0 +1,32 @@
+// chem_tables.h -- mechanism and BDF coefficient tables of the chemistry
+// kernels (the content of the CHEMKIN linking file, chemkin/vode.f:391).
+#pragma once
+
+#define CK_KMAX 16
+#define CK_IMAX 48
+#define CK_TBMAX 12
+#define CK_NEQ 10  // T + 9 species (the H2/air mechanism); checked at setup
+
+struct IzMech {
+  int kk, ii;
+  double wt[CK_KMAX];
+  double a[CK_KMAX][2][7];
+  double tmid[CK_KMAX];
+  double A[CK_IMAX], b[CK_IMAX], EoR[CK_IMAX];
+  int nu[CK_IMAX][6], nunk[CK_IMAX][6];
+  int reversible[CK_IMAX];
+  int ntb[CK_IMAX];
+  int tb_k[CK_IMAX][CK_TBMAX];
+  double tb_eff[CK_IMAX][CK_TBMAX];
+  int fal[CK_IMAX];
+  double fpar[CK_IMAX][8];
+};
+
+
+// BDF coefficients (Nordsieck l-vectors and error constants) for orders 1..5
+struct IzBdf {
+  double el[6][6];     // el[q][j], j = 0..q, el[q][1] = 1
+  double tesco[6][3];  // [q][0] order-down, [1] same order, [2] order-up constants
+};
+
+

@@ -1066,9 +1066,47 @@ extern "C" int izem_create(izem_ctx **out, const izem_grid *g, const double *thd
   return IZEM_OK;
 }
 
+extern "C" int izem_profile(izem_ctx *c, int capacity) {
+  if (!c || capacity < 0) return IZEM_E_ARG;
+  IzProf &p = c->prof;
+  IZ_CHECK(hipStreamSynchronize(c->stream));
+  if (p.ev) {
+    for (int k = 0; k < 2 * p.cap; k++) hipEventDestroy(p.ev[k]);
+    free(p.ev);
+    free(p.cls);
+    p.ev = nullptr;
+    p.cls = nullptr;
+  }
+  p.on = 0; p.n = 0; p.cap = 0;
+  if (capacity == 0) return IZEM_OK;
+  p.ev = (hipEvent_t *)calloc(2 * capacity, sizeof(hipEvent_t));
+  p.cls = (int *)calloc(capacity, sizeof(int));
+  if (!p.ev || !p.cls) return IZEM_E_NOMEM;
+  for (int k = 0; k < 2 * capacity; k++) IZ_CHECK(hipEventCreate(&p.ev[k]));
+  p.cap = capacity;
+  p.on = 1;
+  return IZEM_OK;
+}
+
+extern "C" int izem_profile_read(izem_ctx *c, double *ms, int *count) {
+  if (!c || !ms || !count) return IZEM_E_ARG;
+  IzProf &p = c->prof;
+  IZ_CHECK(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < IZP_NCLASS; q++) { ms[q] = 0.0; count[q] = 0; }
+  for (int k = 0; k < p.n; k++) {
+    float t = 0.f;
+    IZ_CHECK(hipEventElapsedTime(&t, p.ev[2 * k], p.ev[2 * k + 1]));
+    ms[p.cls[k]] += t;
+    count[p.cls[k]] += 1;
+  }
+  p.n = 0;
+  return IZEM_OK;
+}
+
 extern "C" int izem_destroy(izem_ctx *c) {
   if (!c) return IZEM_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
+  izem_profile(c, 0);
   iz_chem_destroy(c);
   for (int q = 0; q < 3; q++) if (c->F[q]) hipFree(c->F[q]);
   if (c->prim) hipFree(c->prim);
@@ -1142,6 +1180,7 @@ static int euler_dir(izem_ctx *c, int dir, const double *d_i, const double *T,
   long nf = c->nfaces[dir];
   unsigned nb = iz_blocks(nf, 64);
   int wa = c->o.weno_avg;
+  int pk = iz_prof_begin(c, IZP_EULER);
   if (dir == 0)
     hipLaunchKernelGGL(k_euler_faces<0>, dim3(nb), dim3(64), 0, c->stream, c->d, v, T, W_i, cp,
                        ha, sensor, wa, c->F[0], nf, c->status);
@@ -1151,6 +1190,7 @@ static int euler_dir(izem_ctx *c, int dir, const double *d_i, const double *T,
   else
     hipLaunchKernelGGL(k_euler_faces<2>, dim3(nb), dim3(64), 0, c->stream, c->d, v, T, W_i, cp,
                        ha, sensor, wa, c->F[2], nf, c->status);
+  iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
   if (apply) {
     if (!fl) return IZEM_E_ARG;
@@ -1207,6 +1247,7 @@ extern "C" int izem_comm_cons(izem_ctx *c, double *v) {
 static int visc_fluxes(izem_ctx *c, const double *dx_i, const double *dy_i, const double *dz_i,
                        const double *T, const double *W_i, const double *ha, const double *v) {
   const IzDims &d = c->d;
+  int pk = iz_prof_begin(c, IZP_VISC);
   for (int dom = -d.ndim; dom <= d.ndim; dom++) {
     IzBox b = domain_box(d, dom);
     hipLaunchKernelGGL(k_visc_prim, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0, c->stream,
@@ -1223,6 +1264,7 @@ static int visc_fluxes(izem_ctx *c, const double *dx_i, const double *dy_i, cons
     hipLaunchKernelGGL(k_visc_flux, dim3(iz_blocks(box_count(b), 128)), dim3(128), 0, c->stream,
                        d, b, c->o.sqgmr, rvd0, rlam0, v, T, W_i, ha, c->prim, c->fd, c->sf);
   }
+  iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
 }
@@ -1364,14 +1406,18 @@ extern "C" int izem_rk3(izem_ctx *c, const double *dt, const double *dx_i, const
     int r = stage_fluxes(c, dx_i, dy_i, dz_i, T, W_i, cp, ha, sensor, v);
     if (r) return r;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->masks, 7);
+    int pk = iz_prof_begin(c, IZP_RHS);
     hipLaunchKernelGGL(k_rhs_update, dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0,
                        c->stream, d, 1, c->F[0], c->F[1], c->F[2], c->o.vis, c->sf, c->fd, dx_i,
                        dy_i, dz_i, dt, bk3[ik], ck3[ik], c->v0, v, (double *)nullptr,
                        c->o.forcesum, T, 1.0e-5, c->masks + 3);
+    iz_prof_end(c, pk);
     IzBox b0 = domain_box(d, 0);
+    pk = iz_prof_begin(c, IZP_NEWTON);
     hipLaunchKernelGGL(k_newton_apply, dim3(iz_blocks(box_count(b0), 256)), dim3(256), 0,
                        c->stream, d, b0, v, T, W_i, cp, ha, c->masks + 3, 200, c->status,
                        c->iters + 3);
+    iz_prof_end(c, pk);
     IZ_CHECK(hipGetLastError());
     r = iz_comm_cons(c, v, IZ_NV);
     if (r) return r;

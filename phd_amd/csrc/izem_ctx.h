@@ -15,6 +15,23 @@
 
 struct IzChem;  // chem.hip
 
+// kernel classes timed by the event profiler (izem_profile_*)
+enum IzProfClass {
+  IZP_EULER = 0,   // Euler face kernels (x, y, z)
+  IZP_VISC = 1,    // viscous prim / gradient / flux kernels
+  IZP_RHS = 2,     // fused flux difference + RK stage update + Newton probe
+  IZP_NEWTON = 3,  // Newton apply / ghost Newton
+  IZP_CHEM = 4,    // chemistry queue kernels
+  IZP_OTHER = 5,   // ghost fills, sensor, time step
+  IZP_NCLASS = 6
+};
+
+struct IzProf {
+  int on, n, cap;
+  hipEvent_t *ev;  // 2 per record
+  int *cls;
+};
+
 struct izem_ctx {
   IzDims d;
   izem_opts o;
@@ -35,7 +52,21 @@ struct izem_ctx {
   double *red;  // timestep reductions [12]
   double *ts;   // dt_4[4], dtmin, DTmax
   IzChem *chem;
+  IzProf prof;
 };
+
+// bracket a kernel launch with events on the context stream
+static inline int iz_prof_begin(izem_ctx *c, int cls) {
+  IzProf &p = c->prof;
+  if (!p.on || p.n >= p.cap) return -1;
+  int k = p.n++;
+  p.cls[k] = cls;
+  hipEventRecord(p.ev[2 * k], c->stream);
+  return k;
+}
+static inline void iz_prof_end(izem_ctx *c, int k) {
+  if (k >= 0) hipEventRecord(c->prof.ev[2 * k + 1], c->stream);
+}
 
 static inline unsigned iz_blocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 

@@ -204,3 +204,17 @@ def mech_flat(mech: cd.Mechanism) -> np.ndarray:
     return np.ascontiguousarray(np.concatenate(parts), dtype=np.float64)
 
 
+
+
+PROFILE_CLASSES = ("euler_faces", "viscous", "rhs_update", "newton", "chemistry", "other")
+
+
+def profile_start(s: Solver, capacity: int = 8192):
+    _lib.check(s.lib.izem_profile(s.ctx, capacity), "izem_profile")
+
+
+def profile_read(s: Solver) -> dict:
+    ms = np.zeros(6)
+    cnt = np.zeros(6, dtype=np.int32)
+    _lib.check(s.lib.izem_profile_read(s.ctx, ms.ctypes.data, cnt.ctypes.data), "profile_read")
+    return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PROFILE_CLASSES)}
