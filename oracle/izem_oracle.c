@@ -195,6 +195,91 @@ void ora_wenorec_nw(const double *vp0, const double *vm0, double *vminus, double
 
 /* --------------------------------------------------------- Euler fluxes */
 
+/* Left and right eigenvector matrices at an averaged state
+ * (module_weno.f90:763-951 and the y/z rotations).  Shared by the flux
+ * routine and the eigenvector tests. */
+void ora_eigvecs(int dir, const ora_thd *thd, const double *uc, const double *Yc,
+                 const double *psi_c, double ht_c, double cs_c, double gam_c,
+                 double el[ORA_NV][ORA_NV], double er[ORA_NV][ORA_NV]) {
+  const int cn = 1 + dir;
+  const int t1 = (dir == 0) ? 2 : 1;
+  const int t2 = (dir == 2) ? 2 : 3;
+  double wrk;
+  (void)thd;
+  double cs_c_i = 1.0 / cs_c, cs_c_2 = cs_c * cs_c;
+  double ux_c = uc[0], vy_c = uc[1], wz_c = uc[2];
+  double q = 0.5 * (ux_c * ux_c + vy_c * vy_c + wz_c * wz_c);
+  double b2 = (gam_c - 1.0) / cs_c_2;
+  double b1 = b2 * q;
+  double xi = b1 - b2 * ht_c;
+  double un_c = uc[cn - 1];
+  /* left eigenvectors: src/module_weno.f90:775-874 (and y/z rotations) */
+  memset(el, 0, sizeof(double) * NV * NV);
+  memset(er, 0, sizeof(double) * NV * NV);
+  el[0][0] = 0.5 * (b1 + un_c * cs_c_i);
+  el[2][0] = 0.5 * (b1 - un_c * cs_c_i);
+  for (int c = 1; c <= 3; c++) {
+    if (c == cn) {
+      el[0][c] = -0.5 * (b2 * uc[c - 1] + cs_c_i);
+      el[2][c] = -0.5 * (b2 * uc[c - 1] - cs_c_i);
+    } else {
+      el[0][c] = -0.5 * (b2 * uc[c - 1]);
+      el[2][c] = -0.5 * (b2 * uc[c - 1]);
+    }
+  }
+  el[0][4] = 0.5 * b2;
+  el[2][4] = 0.5 * b2;
+  el[1][0] = 1.0 - b1;
+  el[1][1] = b2 * ux_c; el[1][2] = b2 * vy_c; el[1][3] = b2 * wz_c;
+  el[1][4] = -b2;
+  el[3][0] = -uc[t1 - 1]; el[3][t1] = 1.0;
+  el[4][0] = -uc[t2 - 1]; el[4][t2] = 1.0;
+  wrk = 0.5 * b2;
+  for (int l = NIV; l < NV; l++) el[0][l] = wrk * psi_c[l - NIV];
+  for (int l = NIV; l < NV; l++) el[1][l] = -el[0][l] - el[0][l];
+  for (int l = NIV; l < NV; l++) el[2][l] = el[0][l];
+  el[5][0] = (1 + xi) * q;
+  el[5][1] = -(1 + xi) * ux_c;
+  el[5][2] = -(1 + xi) * vy_c;
+  el[5][3] = -(1 + xi) * wz_c;
+  el[5][4] = (1 + xi);
+  for (int l = NIV; l < NV; l++) el[5][l] = xi * psi_c[l - NIV];
+  for (int l = 6; l < NV; l++) {
+    double y = Yc[l - 6];
+    el[l][0] = -b1 * y;
+    el[l][1] = b2 * ux_c * y;
+    el[l][2] = b2 * vy_c * y;
+    el[l][3] = b2 * wz_c * y;
+    el[l][4] = -b2 * y;
+  }
+  for (int s1 = 0; s1 < NRV - 1; s1++)
+    for (int s2 = 0; s2 < NRV; s2++) el[s1 + 6][s2 + 5] = -b2 * Yc[s1] * psi_c[s2];
+  for (int l = 0; l < NRV - 1; l++) el[l + 6][l + 5] = el[l + 6][l + 5] + 1.0;
+  /* right eigenvectors: src/module_weno.f90:881-951 */
+  er[0][0] = 1.0; er[0][1] = 1.0; er[0][2] = 1.0;
+  for (int c = 1; c <= 3; c++) {
+    if (c == cn) {
+      er[c][0] = uc[c - 1] - cs_c;
+      er[c][1] = uc[c - 1];
+      er[c][2] = uc[c - 1] + cs_c;
+    } else {
+      er[c][0] = uc[c - 1]; er[c][1] = uc[c - 1]; er[c][2] = uc[c - 1];
+    }
+  }
+  er[t1][3] = 1.0;
+  er[t2][4] = 1.0;
+  er[4][0] = ht_c - un_c * cs_c;
+  er[4][1] = q;
+  er[4][2] = ht_c + un_c * cs_c;
+  er[4][3] = uc[t1 - 1];
+  er[4][4] = uc[t2 - 1];
+  for (int l = NIV; l < NV; l++) { er[l][0] = Yc[l - NIV]; er[l][2] = Yc[l - NIV]; }
+  for (int l = NIV; l < NV; l++) er[l - 1][l] = 1.0;
+  er[NV - 1][5] = -1.0 / psi_c[NRV - 1];
+  wrk = -1.0 / psi_c[NRV - 1];
+  for (int l = 0; l < NRV - 1; l++) er[NV - 1][l + NIV + 1] = wrk * psi_c[l];
+}
+
 /* euler_LLF_x / _y / _z: src/module_weno.f90:537-1044, 1055-1584, 1595-2118.
  * dir 0/1/2.  x assigns fl, y and z accumulate (src/module_weno.f90:1002-1018
  * vs the += form of the y/z routines). */
@@ -289,79 +374,8 @@ int ora_euler_llf(int dir, const ora_grid *g, const ora_thd *thd, const double *
           psi_c[l] = -(r * ha_c[l] + ha_c[l]) * r1;
           psi_c[l] = psi_c[l] + wrk * thd->Wc_i[l];
         }
-        double cs_c_i = 1.0 / cs_c, cs_c_2 = cs_c * cs_c;
-        double ux_c = uc[0], vy_c = uc[1], wz_c = uc[2];
-        double q = 0.5 * (ux_c * ux_c + vy_c * vy_c + wz_c * wz_c);
-        double b2 = (gam_c - 1.0) / cs_c_2;
-        double b1 = b2 * q;
-        double xi = b1 - b2 * ht_c;
-        double un_c = uc[cn - 1];
-        /* left eigenvectors: src/module_weno.f90:775-874 (and y/z rotations) */
         double el[NV][NV], er[NV][NV];
-        memset(el, 0, sizeof el);
-        memset(er, 0, sizeof er);
-        el[0][0] = 0.5 * (b1 + un_c * cs_c_i);
-        el[2][0] = 0.5 * (b1 - un_c * cs_c_i);
-        for (int c = 1; c <= 3; c++) {
-          if (c == cn) {
-            el[0][c] = -0.5 * (b2 * uc[c - 1] + cs_c_i);
-            el[2][c] = -0.5 * (b2 * uc[c - 1] - cs_c_i);
-          } else {
-            el[0][c] = -0.5 * (b2 * uc[c - 1]);
-            el[2][c] = -0.5 * (b2 * uc[c - 1]);
-          }
-        }
-        el[0][4] = 0.5 * b2;
-        el[2][4] = 0.5 * b2;
-        el[1][0] = 1.0 - b1;
-        el[1][1] = b2 * ux_c; el[1][2] = b2 * vy_c; el[1][3] = b2 * wz_c;
-        el[1][4] = -b2;
-        el[3][0] = -uc[t1 - 1]; el[3][t1] = 1.0;
-        el[4][0] = -uc[t2 - 1]; el[4][t2] = 1.0;
-        wrk = 0.5 * b2;
-        for (int l = NIV; l < NV; l++) el[0][l] = wrk * psi_c[l - NIV];
-        for (int l = NIV; l < NV; l++) el[1][l] = -el[0][l] - el[0][l];
-        for (int l = NIV; l < NV; l++) el[2][l] = el[0][l];
-        el[5][0] = (1 + xi) * q;
-        el[5][1] = -(1 + xi) * ux_c;
-        el[5][2] = -(1 + xi) * vy_c;
-        el[5][3] = -(1 + xi) * wz_c;
-        el[5][4] = (1 + xi);
-        for (int l = NIV; l < NV; l++) el[5][l] = xi * psi_c[l - NIV];
-        for (int l = 6; l < NV; l++) {
-          double y = Yc[l - 6];
-          el[l][0] = -b1 * y;
-          el[l][1] = b2 * ux_c * y;
-          el[l][2] = b2 * vy_c * y;
-          el[l][3] = b2 * wz_c * y;
-          el[l][4] = -b2 * y;
-        }
-        for (int s1 = 0; s1 < NRV - 1; s1++)
-          for (int s2 = 0; s2 < NRV; s2++) el[s1 + 6][s2 + 5] = -b2 * Yc[s1] * psi_c[s2];
-        for (int l = 0; l < NRV - 1; l++) el[l + 6][l + 5] = el[l + 6][l + 5] + 1.0;
-        /* right eigenvectors: src/module_weno.f90:881-951 */
-        er[0][0] = 1.0; er[0][1] = 1.0; er[0][2] = 1.0;
-        for (int c = 1; c <= 3; c++) {
-          if (c == cn) {
-            er[c][0] = uc[c - 1] - cs_c;
-            er[c][1] = uc[c - 1];
-            er[c][2] = uc[c - 1] + cs_c;
-          } else {
-            er[c][0] = uc[c - 1]; er[c][1] = uc[c - 1]; er[c][2] = uc[c - 1];
-          }
-        }
-        er[t1][3] = 1.0;
-        er[t2][4] = 1.0;
-        er[4][0] = ht_c - un_c * cs_c;
-        er[4][1] = q;
-        er[4][2] = ht_c + un_c * cs_c;
-        er[4][3] = uc[t1 - 1];
-        er[4][4] = uc[t2 - 1];
-        for (int l = NIV; l < NV; l++) { er[l][0] = Yc[l - NIV]; er[l][2] = Yc[l - NIV]; }
-        for (int l = NIV; l < NV; l++) er[l - 1][l] = 1.0;
-        er[NV - 1][5] = -1.0 / psi_c[NRV - 1];
-        wrk = -1.0 / psi_c[NRV - 1];
-        for (int l = 0; l < NRV - 1; l++) er[NV - 1][l + NIV + 1] = wrk * psi_c[l];
+        ora_eigvecs(dir, thd, uc, Yc, psi_c, ht_c, cs_c, gam_c, el, er);
         /* characteristic LLF + WENO per field: src/module_weno.f90:955-989 */
         int shk = 0;
         for (int p = il - ng + 1; p <= il + ng; p++) {
@@ -1171,4 +1185,10 @@ void ora_reactor_rhs(const ora_mech *m, double rho, const double *z, double *zp)
     sum = sum + h[k] * wdot[k] * m->wt[k] + mru * z[0] * wdot[k];
   }
   zp[0] = -sum / (rho * cvb) * 1.0;
+}
+
+/* one line of the 6th-order central derivative (alpha6), for tests:
+ * out[i] = d/dx a at a[i], i = 3..n-4 */
+void ora_deriv7_line(int n, const double *a, double dinv, double *out) {
+  for (int i = 3; i < n - 3; i++) out[i] = deriv7(a, i, 1, dinv);
 }

@@ -95,6 +95,11 @@ void ora_ha_scalar(const ora_thd *thd, double temp, double *ha);
 void ora_wenorec(const double *vp, const double *vm, double *vminus, double *vplus);
 void ora_wenorec_nw(const double *vp, const double *vm, double *vminus, double *vplus);
 
+void ora_eigvecs(int dir, const ora_thd *thd, const double *uc, const double *Yc,
+                 const double *psi_c, double ht_c, double cs_c, double gam_c,
+                 double el[ORA_NV][ORA_NV], double er[ORA_NV][ORA_NV]);
+void ora_deriv7_line(int n, const double *a, double dinv, double *out);
+
 /* flow-solver kernels; all return 0 or a negative status */
 int ora_euler_llf(int dir, const ora_grid *g, const ora_thd *thd, const double *d_i,
                   const double *T, const double *W_i, const double *cp, const double *ha,

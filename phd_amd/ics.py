@@ -101,7 +101,9 @@ def _spectral_velocity(n: tuple, ndim: int, k0: float, u_rms: float, seed: int) 
         u[d] = u[d] - K[d] * kdotu / k2s
     ur = np.array([np.real(np.fft.ifftn(u[d])) for d in range(ndim)])
     rms = np.sqrt(np.mean(ur ** 2))
-    return ur * (u_rms / rms)
+    # component d above is along array axis d; the state arrays are indexed
+    # [z][y][x], so return the components in (x, y[, z]) order
+    return ur[::-1] * (u_rms / rms)
 
 
 def _gaussian_random_field(n: tuple, ndim: int, corr_len_cells: float, rms: float, seed: int):
@@ -157,7 +159,7 @@ def config1_state(adi, thd, n: int = 128, seed: int = 1):
     T = T[None]
     Ymix = cd.mass_fractions_premix(thd.names, 1.0)
     Y = np.broadcast_to(Ymix[:, None, None, None], (9, 1, n, n)).copy()
-    uv = _spectral_velocity((n, n), 2, 4.0, 10.0, seed)  # (2, ny, nx) indexed [y][x]
+    uv = _spectral_velocity((n, n), 2, 4.0, 10.0, seed)  # (u, v) on the [y][x] grid
     u = np.zeros((3, 1, n, n))
     u[0, 0] = uv[0]
     u[1, 0] = uv[1]

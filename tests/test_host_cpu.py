@@ -1,0 +1,102 @@
+"""CPU tests of the host side: data readers, initial conditions, the C ABI
+library (loads and exports every declared symbol; no compute without a
+GPU), the bench command line and the Fortran ISO_C_BINDING shim."""
+from __future__ import annotations
+
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_tranfit_reader(thd):
+    from phd_amd import chemdata as cd
+    tr = cd.read_tranfit()
+    assert tr.names == ["H", "H2", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2"]
+    assert np.allclose(tr.cofd, np.transpose(tr.cofd, (1, 0, 2)))
+    # molecular weights of the fixture (g/mol) against the CHEMKIN output
+    assert tr.W * 1000.0 == pytest.approx([1.00797, 2.01594, 15.9994, 31.9988, 17.00737,
+                                           18.01534, 33.00677, 34.01474, 28.0134], rel=1e-6)
+    assert tr.kncf.shape == (3, 9)
+
+
+def test_therm_reader_columns(thd):
+    from phd_amd import chemdata as cd
+    nasa, defaults = cd.read_therm()
+    assert defaults == (300.0, 1000.0, 5000.0)
+    assert set(nasa) == {"H", "H2", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2"}
+    n2 = nasa["N2"]
+    assert (n2.tlo, n2.thi, n2.tmid) == (300.0, 5000.0, 1000.0)
+    assert n2.high[0] == pytest.approx(2.92664)
+    assert thd.heatcap.shape == (9, 2, 7)
+
+
+def test_thd_flat_layout(thd):
+    flat = thd.flat()
+    n = 9
+    expected = 3 * n + n * 14 + 3 * n + n * n * 4 + 2 * n * 4 + 2 * n * n + 7
+    assert flat.size == expected
+    assert flat[0] == thd.W[0] and flat[-1] == thd.muinf
+
+
+def test_initial_conditions(adi, thd):
+    from phd_amd import ics
+    box, v, T = ics.config1_state(adi, thd, n=64)
+    I = (slice(None),) + box.interior()
+    rho = v[0][box.interior()]
+    assert np.all(rho > 0)
+    Ysum = v[5:14][I[1:] if False else (slice(None),) + box.interior()].sum(axis=0) / rho
+    assert np.max(np.abs(Ysum - 1.0)) < 1e-13
+    TK = T[box.interior()] * adi.T_ref
+    assert TK.min() == pytest.approx(900.0, abs=1.0) and TK.max() == pytest.approx(1500.0, abs=5)
+    u = v[1][box.interior()] / rho * adi.u_ref
+    w = v[2][box.interior()] / rho * adi.u_ref
+    urms = np.sqrt(np.mean(u ** 2 + w ** 2) / 2)
+    assert urms == pytest.approx(10.0, rel=1e-6)
+    # solenoidal: spectral divergence vanishes
+    uh, wh = np.fft.fft2(u[0]), np.fft.fft2(w[0])
+    k = np.fft.fftfreq(64, 1 / 64)
+    KY, KX = np.meshgrid(k, k, indexing="ij")
+    div = KX * uh + KY * wh
+    assert np.max(np.abs(div)) <= 1e-9 * np.max(np.abs(KX * uh))
+
+
+def test_config3_cells(thd):
+    from phd_amd import ics
+    T, p, Y = ics.config3_cells(thd, 4096)
+    assert T.min() >= 900 and T.max() <= 2000
+    assert p.min() >= 101325 * 0.999 and p.max() <= 1013250 * 1.001
+    assert np.allclose(Y.sum(axis=1), 1.0)
+
+
+def _declared_symbols():
+    hdr = open(os.path.join(ROOT, "include", "izem_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|double\s*\*)\s*(izem_\w+)\s*\(", hdr, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    so = os.path.join(ROOT, "phd_amd", "libizem_hip.so")
+    if not os.path.exists(so):
+        pytest.skip("libizem_hip.so not built (run __graft_entry__.build())")
+    out = subprocess.run(["nm", "-D", "--defined-only", so], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (izem_\w+)", out))
+    missing = [s for s in _declared_symbols() if s not in exported]
+    assert not missing, missing
+    from phd_amd import _lib
+    lib = _lib.load()
+    for s in _declared_symbols():
+        assert hasattr(lib, s)
+    assert set(_lib.EXPORTS) == set(_declared_symbols())
+
+
+def test_bench_refuses_several_gpus():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one MI355X" in r.stderr
