@@ -153,10 +153,14 @@ int izem_timestep(izem_ctx *ctx, const double *dx_i, const double *dy_i, const d
  * chemkin/vode.f:391), flat image of phd_amd.chemdata.Mechanism */
 int izem_chem_setup(izem_ctx *ctx, const double *mech, long nmech, double rtol, double atol,
                     int trim_box);
-/* launch parameters of the persistent chemistry kernels, as chosen by the
- * autotuner (phd_amd/autotune.py): threads per block (multiple of 64),
- * resident blocks per CU, Jacobian reuse age in steps */
-int izem_chem_tune(izem_ctx *ctx, int block, int blocks_per_cu, int jac_age);
+/* parameters of the persistent chemistry kernels, as chosen by the
+ * autotuner (phd_amd/autotune.py): register budget as waves per SIMD the
+ * kernel variant is compiled for (1, 2 or 4), resident 64-lane blocks per
+ * CU, Jacobian reuse age in BDF steps, kinetics form (0 table-driven,
+ * 1 straight-line code generated for the bundled mechanism -- IZEM_E_ARG if
+ * the loaded mechanism is another one --, -1 straight-line when it
+ * matches).  No counterpart in the reference (DVODE runs on the CPU). */
+int izem_chem_tune(izem_ctx *ctx, int waves, int blocks_per_cu, int jac_age, int kinetics);
 /* ckwyr (chemkin/vode.f:1797) on n cells: rho [g/cm3], T [K], Y[n][kk] ->
  * wdot[n][kk] [mol/cm3/s] */
 int izem_ckwyr(izem_ctx *ctx, int n, const double *rho, const double *T, const double *Y,

@@ -80,7 +80,7 @@ def load(path: str | None = None):
     lib.izem_rhs.argtypes = [vp] * 11
     lib.izem_timestep.argtypes = [vp] * 8
     lib.izem_chem_setup.argtypes = [vp, vp, C.c_long, C.c_double, C.c_double, C.c_int]
-    lib.izem_chem_tune.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    lib.izem_chem_tune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     lib.izem_ckwyr.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     lib.izem_psr_batch.argtypes = [vp, C.c_int, vp, vp, vp, C.c_double, vp]
     lib.izem_reaction_dvode.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp]

@@ -197,10 +197,51 @@ struct BdfResult {
   int nst, nfe, nje, ok;
 };
 
+// finite-difference increment of Jacobian column j: DVJAC's
+// max(srur |y_j|, r0 w_j) (vode.f:3744-3750, w_j = rtol |y_j| + atol),
+// with a floor of srur * 1e-6 on the mass-fraction columns.  For a species
+// that is absent (y_j = 0) DVJAC's increment is r0 * atol, of order
+// 1e-20 here, and the temperature row of that column, a difference of two
+// evaluations of dT/dt ~ 1e8 K/s, is then a few ulps over 1e-20: noise of
+// 1e8..1e10 in an entry whose true value is of that size.  The Newton
+// iteration converges or fails with that noise from one build to the next;
+// with the floor the entries are accurate to about 1e-3.  The species
+// rates are polynomial in the concentrations, so the larger increment
+// costs no truncation accuracy that matters to a chord Newton iteration.
+IZ_HD inline double d_fd_increment(int j, double yj, double ewtj, double r0) {
+  const double srur = 1.4901161193847656e-08;
+  double r = fmax(srur * fabs(yj), r0 * ewtj);
+  return j == 0 ? r : fmax(r, srur * 1.0e-6);
+}
+
 // Integrate the reactor of one cell from 0 to tout (ITASK = 1: step past tout
 // and interpolate the Nordsieck history back to it).
-IZ_HD inline BdfResult d_bdf_integrate(double rho, double *y, double tout, double rtol,
-                                     double atol, int jac_age) {
+// Kinetics policies: rhs(rho, y, f) evaluates the reactor right-hand side;
+// jac(rho, y, f0, hl0, ewt, r0, P) writes P = I - hl0 * J with a
+// finite-difference Jacobian (DVJAC with MF = 22, vode.f:3728-3770).
+struct KinTables {
+  IZ_MHD inline void rhs(double rho, const double *y, double *f) const {
+    d_reactor_rhs(rho, y, f);
+  }
+  IZ_MHD inline void jac(double rho, double *y, const double *f0, double hl0, const double *ewt,
+                        double r0, double (*P)[N_]) const {
+    double ftmp[N_];
+    for (int j = 0; j < N_; j++) {
+      double yj = y[j];
+      double r = d_fd_increment(j, yj, ewt[j], r0);
+      y[j] = yj + r;
+      d_reactor_rhs(rho, y, ftmp);
+      y[j] = yj;
+      double ri = 1.0 / r;
+#pragma unroll
+      for (int i = 0; i < N_; i++) P[i][j] = -hl0 * (ftmp[i] - f0[i]) * ri;
+    }
+  }
+};
+
+template <class Kin>
+IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, double tout,
+                                       double rtol, double atol, int jac_age) {
   const double uround = 2.220446049250313e-16, srur = 1.4901161193847656e-08;
   const int maxstep = 10000, maxcor = 3;
   BdfResult res{0, 0, 0, 1};
@@ -209,7 +250,7 @@ IZ_HD inline BdfResult d_bdf_integrate(double rho, double *y, double tout, doubl
   int piv[N_];
 #pragma unroll
   for (int i = 0; i < N_; i++) { z[0][i] = y[i]; ewt[i] = rtol * fabs(y[i]) + atol; }
-  d_reactor_rhs(rho, y, savf);
+  kin.rhs(rho, y, savf);
   res.nfe++;
   // initial step (the LSODE-style estimate also used by DVHIN's first guess)
   double tol = fmax(rtol, 100.0 * uround);
@@ -239,24 +280,14 @@ IZ_HD inline BdfResult d_bdf_integrate(double rho, double *y, double tout, doubl
     // corrector (chord Newton on G(e) = h f(z0 + l0 e) - z1 - e)
 #pragma unroll
     for (int n = 0; n < N_; n++) { yy[n] = z[0][n]; acor[n] = 0.0; }
-    d_reactor_rhs(rho, yy, savf);
+    kin.rhs(rho, yy, savf);
     res.nfe++;
     bool jcur = false;
     if (need_jac) {
       double fac = d_wrms(savf, ewt);
       double r0 = 1000.0 * fabs(h) * uround * N_ * fac;
       if (r0 == 0.0) r0 = 1.0;
-      double ftmp[N_];
-      for (int j = 0; j < N_; j++) {
-        double yj = yy[j];
-        double r = fmax(srur * fabs(yj), r0 * ewt[j]);
-        yy[j] = yj + r;
-        d_reactor_rhs(rho, yy, ftmp);
-        yy[j] = yj;
-        double ri = 1.0 / r;
-#pragma unroll
-        for (int i = 0; i < N_; i++) P[i][j] = -hl0 * (ftmp[i] - savf[i]) * ri;
-      }
+      kin.jac(rho, yy, savf, hl0, ewt, r0, P);
       res.nfe += N_;
       res.nje++;
 #pragma unroll
@@ -288,7 +319,7 @@ IZ_HD inline BdfResult d_bdf_integrate(double rho, double *y, double tout, doubl
       if (dcon <= 1.0) { conv = true; break; }
       if (m >= 1 && del > 2.0 * delp) break;
       delp = del;
-      d_reactor_rhs(rho, yy, savf);
+      kin.rhs(rho, yy, savf);
       res.nfe++;
     }
     if (!conv) {
@@ -318,7 +349,7 @@ IZ_HD inline BdfResult d_bdf_integrate(double rho, double *y, double tout, doubl
         h *= 0.1;
 #pragma unroll
         for (int n = 0; n < N_; n++) yy[n] = z[0][n];
-        d_reactor_rhs(rho, yy, savf);
+        kin.rhs(rho, yy, savf);
         res.nfe++;
 #pragma unroll
         for (int n = 0; n < N_; n++) z[1][n] = h * savf[n];
@@ -402,3 +433,42 @@ IZ_HD inline BdfResult d_bdf_integrate(double rho, double *y, double tout, doubl
   return res;
 }
 
+
+#ifdef GEN_KK
+// straight-line kinetics of the compiled-in mechanism; the T-only part kt is
+// shared by the Jacobian columns that perturb mass fractions
+struct KinGen {
+  IZ_MHD inline void rhs(double rho, const double *y, double *f) const {
+    double kt[GEN_NKT];
+    gen_kT(y[0], kt);
+    gen_rhs(rho, y, kt, f);
+  }
+  IZ_MHD inline void jac(double rho, double *y, const double *f0, double hl0, const double *ewt,
+                        double r0, double (*P)[N_]) const {
+    double kt[GEN_NKT], ftmp[N_];
+    gen_kT(y[0], kt);
+#pragma unroll
+    for (int j = 1; j < N_; j++) {
+      double yj = y[j];
+      double r = d_fd_increment(j, yj, ewt[j], r0);
+      y[j] = yj + r;
+      gen_rhs(rho, y, kt, ftmp);
+      y[j] = yj;
+      double ri = 1.0 / r;
+#pragma unroll
+      for (int i = 0; i < N_; i++) P[i][j] = -hl0 * (ftmp[i] - f0[i]) * ri;
+    }
+    {
+      double yj = y[0];
+      double r = d_fd_increment(0, yj, ewt[0], r0);
+      y[0] = yj + r;
+      gen_kT(y[0], kt);
+      gen_rhs(rho, y, kt, ftmp);
+      y[0] = yj;
+      double ri = 1.0 / r;
+#pragma unroll
+      for (int i = 0; i < N_; i++) P[i][0] = -hl0 * (ftmp[i] - f0[i]) * ri;
+    }
+  }
+};
+#endif

@@ -2,6 +2,15 @@
 // kernels (the content of the CHEMKIN linking file, chemkin/vode.f:391).
 #pragma once
 
+// qualifier of device free functions and of policy member functions; a CPU
+// build of these headers (for debugging the integrator) defines them away
+#ifndef IZ_HD
+#define IZ_HD __device__
+#endif
+#ifndef IZ_MHD
+#define IZ_MHD __device__
+#endif
+
 #define CK_KMAX 16
 #define CK_IMAX 48
 #define CK_TBMAX 12

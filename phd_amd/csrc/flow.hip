@@ -15,100 +15,106 @@
 __constant__ IzThd c_thd;
 
 // ===================================================================
-// Euler fluxes: one thread per cell face.  The face value F^ of all 14
-// equations is written to a face array; the flux difference is applied by
-// a separate (or fused) pass.  euler_LLF_x/_y/_z: module_weno.f90:537-1044,
-// 1055-1584, 1595-2118.
+// Euler fluxes (euler_LLF_x/_y/_z: module_weno.f90:537-1044, 1055-1584,
+// 1595-2118): one thread per cell face in 256-thread blocks.  The cells of
+// a block's stencils are staged once in LDS, with the per-cell quantities
+// every stencil point needs (u_n, P, c), so a cell is read from HBM once per
+// block instead of once per face and stencil point:
+//   x:    each wave takes 64 consecutive faces of the (face, j, k) order
+//         (they may wrap into the next row); its tile is the run of cells
+//         their stencils cover;
+//   y, z: a block takes IZ_EU_TX cells along x by IZ_EU_TF faces along the
+//         direction and stages IZ_EU_TF + 5 cell rows; logical tiles are
+//         numbered so that tiles adjacent along the direction run on the
+//         same XCD, whose L2 then serves the shared halo rows.
+// The face value F^ of the 14 equations goes to a face array; the flux
+// difference is a separate pass (k_euler_apply).
 // ===================================================================
-template <int DIR>
-__global__ void __launch_bounds__(64)
-k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
-              const double *__restrict__ W_i, const double *__restrict__ cp,
-              const double *__restrict__ ha, const uint8_t *__restrict__ sensor, int weno_avg,
-              double *__restrict__ F, long nfaces, int *__restrict__ status) {
-  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= nfaces) return;
-  const long nc = d.nc;
-  int a0, a1, a2;  // (i, j, k) of the left cell of the face; the face index runs along DIR
-  long stride;
-  if (DIR == 0) {
-    int nf = d.nx + 1;
-    a0 = (int)(tid % nf) - 1; long r = tid / nf; a1 = (int)(r % d.ny); a2 = (int)(r / d.ny);
-    stride = 1;
-  } else if (DIR == 1) {
-    a0 = (int)(tid % d.nx); long r = tid / d.nx; int nf = d.ny + 1;
-    a1 = (int)(r % nf) - 1; a2 = (int)(r / nf);
-    stride = d.NX;
-  } else {
-    a0 = (int)(tid % d.nx); long r = tid / d.nx; a1 = (int)(r % d.ny); a2 = (int)(r / d.ny) - 1;
-    stride = d.NX * d.NY;
-  }
-  constexpr int cn = 1 + DIR;               // normal momentum column
-  constexpr int t1 = (DIR == 0) ? 2 : 1;    // tangential columns (el rows 4, 5)
-  constexpr int t2 = (DIR == 2) ? 2 : 3;
-  const long cl = iz_cidx(d, a0, a1, a2);   // left cell of the face (i_l)
-  const long c0 = cl - 2 * stride;          // first stencil cell (i_l - ng + 1)
+#define IZ_EU_BLOCK 256
+#define IZ_EU_TX 32
+#define IZ_EU_TF (IZ_EU_BLOCK / IZ_EU_TX)
+#define IZ_EU_ROWS (IZ_EU_TF + 5)
+#define IZ_EU_NQ 17  // staged per cell: 14 conserved variables, u_n, P, c
+#ifndef IZ_EULER_WAVES
+#define IZ_EULER_WAVES 2
+#endif
 
-  // stencil presolve (module_weno.f90:634-650)
-  double rho_i[6], un[6], P[6];
+// per-cell part of the stencil presolve (module_weno.f90:634-650), with the
+// expressions of the per-stencil loop evaluated once per cell
+__device__ inline void eu_stage_cell(const double *__restrict__ v, const double *__restrict__ T,
+                                     const double *__restrict__ W_i,
+                                     const double *__restrict__ cp, long nc, long c, int cn,
+                                     double *dst, int fs, int &neg) {
+  const double r0 = v[c];
+  const double rho_i = 1.0 / r0;
+  dst[0] = r0;
+#pragma unroll
+  for (int l = 1; l < IZ_NV; l++) dst[l * fs] = v[l * nc + c];
+  const double un = v[cn * nc + c] * rho_i;
+  const double P = r0 * T[c] * W_i[c];
+  if (P <= 0.0) neg = 1;
+  double gam = c_thd.gam2 * W_i[c] / cp[c];
+  gam = 1.0 / (1.0 - gam);
+  const double cs = sqrt(gam * P * rho_i);
+  dst[14 * fs] = un;
+  dst[15 * fs] = P;
+  dst[16 * fs] = cs;
+}
+
+// F^ of one face from its six staged stencil cells: Q(q, st) is staged
+// quantity q of stencil point st (st = 2 is i_l, st = 3 is i_r); cL, cR are
+// the HBM indices of i_l, i_r (for h_k); F^ goes to F[m * nfaces + fidx].
+template <int DIR, class Acc>
+__device__ inline void eu_face(const Acc &Q, bool js, const double *__restrict__ ha, long nc,
+                               long cL, long cR, double *__restrict__ F, long nfaces, long fidx) {
+  constexpr int cn = 1 + DIR;             // normal momentum column
+  constexpr int t1 = (DIR == 0) ? 2 : 1;  // tangential columns (el rows 4, 5)
+  constexpr int t2 = (DIR == 2) ? 2 : 3;
   double lmax0 = -1.0, lmax1 = -1.0, lmax2 = -1.0;
-  int neg = 0;
 #pragma unroll
   for (int st = 0; st < 6; st++) {
-    long c = c0 + st * stride;
-    double r0 = v[c];
-    rho_i[st] = 1.0 / r0;
-    un[st] = v[cn * nc + c] * rho_i[st];
-    P[st] = r0 * T[c] * W_i[c];
-    if (P[st] <= 0.0) neg = 1;
-    double gam = c_thd.gam2 * W_i[c] / cp[c];
-    gam = 1.0 / (1.0 - gam);
-    double cs = sqrt(gam * P[st] * rho_i[st]);
-    lmax0 = fmax(fabs(un[st] - cs), lmax0);
-    lmax1 = fmax(fabs(un[st]), lmax1);
-    lmax2 = fmax(fabs(un[st] + cs), lmax2);
+    const double u = Q(14, st), cs = Q(16, st);
+    lmax0 = fmax(fabs(u - cs), lmax0);
+    lmax1 = fmax(fabs(u), lmax1);
+    lmax2 = fmax(fabs(u + cs), lmax2);
   }
-  if (neg) atomicOr(status, IZ_ST_NEG_PRESSURE);
-
-  // cell quantities of i_l (st = 2) and i_r (st = 3) for the Roe average
-  // (module_weno.f90:654-667, 726-735)
+  // Roe average of i_l and i_r (module_weno.f90:654-667, 726-735)
   double uc[3], Yc[IZ_NRV], ht_c, et_c, r, r1;
   {
-    const long cL = cl, cR = cl + stride;
-    const double riL = rho_i[2], riR = rho_i[3];
+    const double riL = 1.0 / Q(0, 2), riR = 1.0 / Q(0, 3);
     double uL[3], uR[3];
 #pragma unroll
-    for (int q = 0; q < 3; q++) { uL[q] = v[(1 + q) * nc + cL] * riL; uR[q] = v[(1 + q) * nc + cR] * riR; }
+    for (int q = 0; q < 3; q++) { uL[q] = Q(1 + q, 2) * riL; uR[q] = Q(1 + q, 3) * riR; }
     double htL = 0.0, htR = 0.0;
-    r = sqrt(v[cR] * riL);
+    r = sqrt(Q(0, 3) * riL);
     r1 = 1.0 / (r + 1.0);
 #pragma unroll
     for (int l = 0; l < IZ_NRV; l++) {
-      double yL = v[(IZ_NIV + l) * nc + cL] * riL, yR = v[(IZ_NIV + l) * nc + cR] * riR;
+      const double yL = Q(IZ_NIV + l, 2) * riL, yR = Q(IZ_NIV + l, 3) * riR;
       htL = htL + ha[l * nc + cL] * yL;
       htR = htR + ha[l * nc + cR] * yR;
       Yc[l] = (r * (yR) + yL) * r1;
     }
     htL = htL + 0.5 * (uL[0] * uL[0] + uL[1] * uL[1] + uL[2] * uL[2]);
     htR = htR + 0.5 * (uR[0] * uR[0] + uR[1] * uR[1] + uR[2] * uR[2]);
-    double etL = v[4 * nc + cL] * riL, etR = v[4 * nc + cR] * riR;
+    const double etL = Q(4, 2) * riL, etR = Q(4, 3) * riR;
 #pragma unroll
     for (int q = 0; q < 3; q++) uc[q] = (r * uR[q] + uL[q]) * r1;
     ht_c = (r * htR + htL) * r1;
     et_c = (r * etR + etL) * r1;
   }
   // EOS-compatible average state (module_weno.f90:750-761)
-  double W_i_c = d_wmix_scalar(Yc);
-  double T_c = (ht_c - et_c) / W_i_c;
-  double cp_c = d_cp_scalar(T_c, Yc);
+  const double W_i_c = d_wmix_scalar(Yc);
+  const double T_c = (ht_c - et_c) / W_i_c;
+  const double cp_c = d_cp_scalar(T_c, Yc);
   double gam_c = c_thd.gam2 * W_i_c / cp_c;
   gam_c = 1.0 / (1.0 - gam_c);
-  double cs_c = sqrt(gam_c * T_c * W_i_c);
+  const double cs_c = sqrt(gam_c * T_c * W_i_c);
   double psi[IZ_NRV];
   {
     double ha_c[IZ_NRV];
     d_ha_scalar(T_c, ha_c);
-    double wrk = (gam_c / (gam_c - 1.0)) * T_c;
+    const double wrk = (gam_c / (gam_c - 1.0)) * T_c;
 #pragma unroll
     for (int l = 0; l < IZ_NRV; l++) {
       psi[l] = -(r * ha_c[l] + ha_c[l]) * r1;
@@ -123,12 +129,6 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
   const double xi = b1 - b2 * ht_c;
   const double un_c = uc[cn - 1];
   const double hb2 = 0.5 * b2;
-
-  // shock-sensor switch for this face (module_weno.f90:973)
-  int shk = 0;
-#pragma unroll
-  for (int st = 0; st < 6; st++) shk |= sensor[c0 + st * stride];
-  const bool js = shk && weno_avg;
 
   double ghat[IZ_NV];
 #pragma unroll 1
@@ -150,7 +150,7 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
       e[1] = b2 * ux_c; e[2] = b2 * vy_c; e[3] = b2 * wz_c;
       e[4] = -b2;
 #pragma unroll
-      for (int l = 0; l < IZ_NRV; l++) { double a = hb2 * psi[l]; e[5 + l] = -a - a; }
+      for (int l = 0; l < IZ_NRV; l++) { const double a = hb2 * psi[l]; e[5 + l] = -a - a; }
       lam = lmax1;
     } else if (m == 2) {
       e[0] = 0.5 * (b1 - un_c * cs_c_i);
@@ -162,7 +162,7 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
       for (int l = 0; l < IZ_NRV; l++) e[5 + l] = hb2 * psi[l];
       lam = lmax2;
     } else if (m == 5) {
-      double opx = (1 + xi);
+      const double opx = (1 + xi);
       e[0] = opx * q;
       e[1] = -opx * ux_c; e[2] = -opx * vy_c; e[3] = -opx * wz_c;
       e[4] = opx;
@@ -178,7 +178,7 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
       const double by = b2 * y;
 #pragma unroll
       for (int l = 0; l < IZ_NRV; l++) {
-        double x = -(by * psi[l]);
+        const double x = -(by * psi[l]);
         e[5 + l] = (l == s1) ? x + 1.0 : x;
       }
       lam = lmax1;
@@ -188,33 +188,32 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
     double gp[6], gm[6];
 #pragma unroll
     for (int st = 0; st < 6; st++) {
-      const long c = c0 + st * stride;
-      const double u = un[st], p = P[st];
+      const double u = Q(14, st), p = Q(15, st);
       double wc, gc;
       if (m == 3 || m == 4) {
         const int tc = (m == 3) ? t1 : t2;
         const double et0 = -uc[tc - 1];
-        const double v0 = v[c], vt = v[tc * nc + c];
-        const double f0 = v[cn * nc + c];
+        const double v0 = Q(0, st), vt = Q(tc, st);
+        const double f0 = Q(cn, st);
         const double ft = (tc == cn) ? vt * u + p : vt * u;
         wc = et0 * v0 + vt;
         gc = et0 * f0 + ft;
       } else {
-        const double v0 = v[c];
+        const double v0 = Q(0, st);
         wc = e[0] * v0;
-        gc = e[0] * v[cn * nc + c];
+        gc = e[0] * Q(cn, st);
 #pragma unroll
         for (int cc = 1; cc <= 3; cc++) {
-          const double vc = v[cc * nc + c];
+          const double vc = Q(cc, st);
           wc = wc + e[cc] * vc;
           gc = gc + e[cc] * ((cc == cn) ? vc * u + p : vc * u);
         }
-        const double v4 = v[4 * nc + c];
+        const double v4 = Q(4, st);
         wc = wc + e[4] * v4;
         gc = gc + e[4] * (u * (v4 + p));
 #pragma unroll
         for (int l = IZ_NIV; l < IZ_NV; l++) {
-          const double vl = v[l * nc + c];
+          const double vl = Q(l, st);
           wc = wc + e[l] * vl;
           gc = gc + e[l] * (vl * u);
         }
@@ -229,33 +228,32 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
   }
 
   // F^ = er . ghat with the nonzero pattern of er (module_weno.f90:881-989)
-  const long nfc = nfaces;
   double fh;
   fh = ghat[0] + ghat[1];
   fh = fh + ghat[2];
-  F[0 * nfc + tid] = fh;
+  F[fidx] = fh;
 #pragma unroll
   for (int c = 1; c <= 3; c++) {
     const double u = uc[c - 1];
-    double e0 = (c == cn) ? u - cs_c : u;
-    double e2 = (c == cn) ? u + cs_c : u;
+    const double e0 = (c == cn) ? u - cs_c : u;
+    const double e2 = (c == cn) ? u + cs_c : u;
     fh = e0 * ghat[0] + u * ghat[1];
     fh = fh + e2 * ghat[2];
     if (c == t1) fh = fh + ghat[3];
     if (c == t2) fh = fh + ghat[4];
-    F[(long)c * nfc + tid] = fh;
+    F[(long)c * nfaces + fidx] = fh;
   }
   fh = (ht_c - un_c * cs_c) * ghat[0] + q * ghat[1];
   fh = fh + (ht_c + un_c * cs_c) * ghat[2];
   fh = fh + uc[t1 - 1] * ghat[3];
   fh = fh + uc[t2 - 1] * ghat[4];
   fh = fh + ghat[5];
-  F[4 * nfc + tid] = fh;
+  F[4 * nfaces + fidx] = fh;
 #pragma unroll
   for (int l = 0; l < IZ_NRV - 1; l++) {
     fh = Yc[l] * ghat[0] + Yc[l] * ghat[2];
     fh = fh + ghat[6 + l];
-    F[(long)(IZ_NIV + l) * nfc + tid] = fh;
+    F[(long)(IZ_NIV + l) * nfaces + fidx] = fh;
   }
   {
     const double wrk = -1.0 / psi[IZ_NRV - 1];
@@ -263,8 +261,117 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
     fh = fh + wrk * ghat[5];
 #pragma unroll
     for (int l = 0; l < IZ_NRV - 1; l++) fh = fh + (wrk * psi[l]) * ghat[6 + l];
-    F[(long)(IZ_NV - 1) * nfc + tid] = fh;
+    F[(long)(IZ_NV - 1) * nfaces + fidx] = fh;
   }
+}
+
+// LDS bytes of one block of k_euler_faces<DIR>; tile0 = cells staged per
+// wave for x (69 + 5 per row boundary that 64 consecutive faces can cross)
+static inline int eu_tile0(const IzDims &d) { return 69 + 5 * (63 / (d.nx + 1) + 1); }
+static inline size_t eu_lds_bytes(int dir, int tile0) {
+  if (dir == 0) return (size_t)4 * tile0 * (IZ_EU_NQ * sizeof(double) + 1);
+  return (size_t)IZ_EU_ROWS * IZ_EU_TX * (IZ_EU_NQ * sizeof(double) + 1);
+}
+// logical tiles of k_euler_faces<1/2> (x tiles * face tiles * other index)
+static inline long eu_ntiles(const IzDims &d, int dir) {
+  const int nd = dir == 1 ? d.ny : d.nz;
+  const long ntf = (nd + 1 + IZ_EU_TF - 1) / IZ_EU_TF;
+  const long ntx = (d.nx + IZ_EU_TX - 1) / IZ_EU_TX;
+  return ntf * ntx * (dir == 1 ? d.nz : d.ny);
+}
+
+template <int DIR>
+__global__ void __attribute__((amdgpu_flat_work_group_size(IZ_EU_BLOCK, IZ_EU_BLOCK),
+                               amdgpu_waves_per_eu(IZ_EULER_WAVES, IZ_EULER_WAVES)))
+k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
+              const double *__restrict__ W_i, const double *__restrict__ cp,
+              const double *__restrict__ ha, const uint8_t *__restrict__ sensor, int weno_avg,
+              double *__restrict__ F, long nfaces, int *__restrict__ status, int tile0,
+              long ntiles) {
+  extern __shared__ double lds[];
+  int neg = 0;
+  if constexpr (DIR == 0) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long f0 = ((long)blockIdx.x * 4 + wave) * 64;  // first face of this wave
+    const int nf = d.nx + 1, NXc = d.nx + 6;             // faces, cells per row
+    const long nrows = (long)d.ny * d.nz;
+    double *tl = lds + (long)wave * tile0 * IZ_EU_NQ;
+    uint8_t *tsn = (uint8_t *)(lds + 4L * tile0 * IZ_EU_NQ) + wave * tile0;
+    // cells in row-flattened order r * NXc + (x + 3); face f's stencil
+    // starts at S(f) = r * NXc + x + 1 for its left cell x
+    long S0 = 0;
+    if (f0 < nfaces) S0 = (f0 / nf) * NXc + (f0 % nf);
+    for (int t = lane; t < tile0; t += 64) {
+      const long cf = S0 + t;
+      const long r = cf / NXc;
+      if (f0 < nfaces && r < nrows) {
+        const int xc = (int)(cf - r * NXc) - 3;
+        const long c = iz_cidx(d, xc, (int)(r % d.ny), (int)(r / d.ny));
+        eu_stage_cell(v, T, W_i, cp, d.nc, c, 1, tl + t, tile0, neg);
+        tsn[t] = sensor[c];
+      }
+    }
+    __syncthreads();
+    const long f = f0 + lane;
+    if (f < nfaces) {
+      const long r = f / nf;
+      const int x = (int)(f - r * nf) - 1;
+      const int o = (int)(r * NXc + x + 1 - S0);
+      const long cL = iz_cidx(d, x, (int)(r % d.ny), (int)(r / d.ny));
+      auto Q = [&](int q, int st) { return tl[q * tile0 + o + st]; };
+      int shk = 0;
+#pragma unroll
+      for (int st = 0; st < 6; st++) shk |= tsn[o + st];
+      eu_face<0>(Q, shk && weno_avg, ha, d.nc, cL, cL + 1, F, nfaces, f);
+    }
+  } else {
+    // hardware blocks b, b + 8, ... run on one XCD: give them consecutive
+    // logical tiles, face tiles fastest
+    const long per = (ntiles + 7) / 8;
+    const long L = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    const bool blk_ok = L < ntiles;
+    const int nd = DIR == 1 ? d.ny : d.nz;
+    const int ntf = (nd + 1 + IZ_EU_TF - 1) / IZ_EU_TF;
+    const int ntx = (d.nx + IZ_EU_TX - 1) / IZ_EU_TX;
+    const int tf = (int)(L % ntf);
+    const int tx = (int)((L / ntf) % ntx);
+    const int oth = (int)(L / ((long)ntf * ntx));  // k for y faces, j for z faces
+    const int row0 = tf * IZ_EU_TF - 3;              // first staged cell along DIR
+    constexpr int FS = IZ_EU_ROWS * IZ_EU_TX;        // field stride in LDS
+    uint8_t *tsn = (uint8_t *)(lds + (long)IZ_EU_NQ * FS);
+    for (int t = threadIdx.x; t < FS; t += IZ_EU_BLOCK) {
+      const int rr = t / IZ_EU_TX, cx = tx * IZ_EU_TX + t % IZ_EU_TX;
+      const int cdir = row0 + rr;
+      if (blk_ok && cx < d.nx && cdir <= nd + 2) {
+        const long c = DIR == 1 ? iz_cidx(d, cx, cdir, oth) : iz_cidx(d, cx, oth, cdir);
+        eu_stage_cell(v, T, W_i, cp, d.nc, c, 1 + DIR, lds + t, FS, neg);
+        tsn[t] = sensor[c];
+      }
+    }
+    __syncthreads();
+    const int xl = threadIdx.x % IZ_EU_TX, fl = threadIdx.x / IZ_EU_TX;
+    const int x = tx * IZ_EU_TX + xl;
+    const int f = tf * IZ_EU_TF + fl - 1;  // left cell of the face along DIR
+    if (blk_ok && x < d.nx && f <= nd - 1) {
+      // stencil point st of this face is staged row fl + st
+      auto Q = [&](int q, int st) { return lds[q * FS + (fl + st) * IZ_EU_TX + xl]; };
+      int shk = 0;
+#pragma unroll
+      for (int st = 0; st < 6; st++) shk |= tsn[(fl + st) * IZ_EU_TX + xl];
+      long cL, cR, fidx;
+      if (DIR == 1) {
+        cL = iz_cidx(d, x, f, oth);
+        cR = cL + d.NX;
+        fidx = x + (long)d.nx * ((f + 1) + (long)(d.ny + 1) * oth);
+      } else {
+        cL = iz_cidx(d, x, oth, f);
+        cR = cL + d.NX * d.NY;
+        fidx = x + (long)d.nx * (oth + (long)d.ny * (f + 1));
+      }
+      eu_face<DIR>(Q, shk && weno_avg, ha, d.nc, cL, cR, F, nfaces, fidx);
+    }
+  }
+  if (neg) atomicOr(status, IZ_ST_NEG_PRESSURE);
 }
 
 // fl = (F^_r - F^_l) * d_i  (x assigns, y/z accumulate): module_weno.f90:998-1020
@@ -984,13 +1091,13 @@ __global__ void k_wenorec_batch(int n, int js, const double *__restrict__ vp,
 // explicit instantiations used by the API
 template __global__ void k_euler_faces<0>(IzDims, const double *, const double *, const double *,
                                           const double *, const double *, const uint8_t *, int,
-                                          double *, long, int *);
+                                          double *, long, int *, int, long);
 template __global__ void k_euler_faces<1>(IzDims, const double *, const double *, const double *,
                                           const double *, const double *, const uint8_t *, int,
-                                          double *, long, int *);
+                                          double *, long, int *, int, long);
 template __global__ void k_euler_faces<2>(IzDims, const double *, const double *, const double *,
                                           const double *, const double *, const uint8_t *, int,
-                                          double *, long, int *);
+                                          double *, long, int *, int, long);
 template __global__ void k_euler_apply<0>(IzDims, const double *, long, const double *, int, double *);
 template __global__ void k_euler_apply<1>(IzDims, const double *, long, const double *, int, double *);
 template __global__ void k_euler_apply<2>(IzDims, const double *, long, const double *, int, double *);
@@ -1178,18 +1285,23 @@ static int euler_dir(izem_ctx *c, int dir, const double *d_i, const double *T,
   if (!c || !d_i || !T || !W_i || !cp || !ha || !sensor || !v) return IZEM_E_ARG;
   if (dir >= c->d.ndim) return IZEM_E_ARG;
   long nf = c->nfaces[dir];
-  unsigned nb = iz_blocks(nf, 64);
   int wa = c->o.weno_avg;
+  const int tile0 = eu_tile0(c->d);
+  const size_t lds = eu_lds_bytes(dir, tile0);
+  const long ntiles = dir == 0 ? 0 : eu_ntiles(c->d, dir);
+  // x: four 64-face waves per block; y, z: logical tiles rounded up to a
+  // multiple of the 8 XCDs (blocks past the last tile only pass the barrier)
+  const unsigned nb = dir == 0 ? iz_blocks(nf, IZ_EU_BLOCK) : (unsigned)(((ntiles + 7) / 8) * 8);
   int pk = iz_prof_begin(c, IZP_EULER);
   if (dir == 0)
-    hipLaunchKernelGGL(k_euler_faces<0>, dim3(nb), dim3(64), 0, c->stream, c->d, v, T, W_i, cp,
-                       ha, sensor, wa, c->F[0], nf, c->status);
+    hipLaunchKernelGGL(k_euler_faces<0>, dim3(nb), dim3(IZ_EU_BLOCK), lds, c->stream, c->d, v,
+                       T, W_i, cp, ha, sensor, wa, c->F[0], nf, c->status, tile0, ntiles);
   else if (dir == 1)
-    hipLaunchKernelGGL(k_euler_faces<1>, dim3(nb), dim3(64), 0, c->stream, c->d, v, T, W_i, cp,
-                       ha, sensor, wa, c->F[1], nf, c->status);
+    hipLaunchKernelGGL(k_euler_faces<1>, dim3(nb), dim3(IZ_EU_BLOCK), lds, c->stream, c->d, v,
+                       T, W_i, cp, ha, sensor, wa, c->F[1], nf, c->status, tile0, ntiles);
   else
-    hipLaunchKernelGGL(k_euler_faces<2>, dim3(nb), dim3(64), 0, c->stream, c->d, v, T, W_i, cp,
-                       ha, sensor, wa, c->F[2], nf, c->status);
+    hipLaunchKernelGGL(k_euler_faces<2>, dim3(nb), dim3(IZ_EU_BLOCK), lds, c->stream, c->d, v,
+                       T, W_i, cp, ha, sensor, wa, c->F[2], nf, c->status, tile0, ntiles);
   iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
   if (apply) {

@@ -61,11 +61,11 @@ static inline int iz_prof_begin(izem_ctx *c, int cls) {
   if (!p.on || p.n >= p.cap) return -1;
   int k = p.n++;
   p.cls[k] = cls;
-  hipEventRecord(p.ev[2 * k], c->stream);
+  (void)hipEventRecord(p.ev[2 * k], c->stream);
   return k;
 }
 static inline void iz_prof_end(izem_ctx *c, int k) {
-  if (k >= 0) hipEventRecord(c->prof.ev[2 * k + 1], c->stream);
+  if (k >= 0) (void)hipEventRecord(c->prof.ev[2 * k + 1], c->stream);
 }
 
 static inline unsigned iz_blocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }

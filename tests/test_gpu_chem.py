@@ -92,11 +92,15 @@ def test_element_conservation_of_rates(mech):
         assert np.max(np.abs(tot) / scale) <= 1e-12
 
 
+@pytest.mark.parametrize("kinetics,waves", [(0, 2), (1, 2), (1, 4)])
 @pytest.mark.parametrize("rtol,atol", [(1e-6, 1e-12)])
-def test_psr_batch_vs_reference_dvode(mech, thd, rtol, atol):
+def test_psr_batch_vs_reference_dvode(mech, thd, rtol, atol, kinetics, waves):
+    """Both kinetics forms (table-driven, straight-line generated) and two
+    register budgets of the persistent kernel against DVODE."""
     import torch
     rc = _refchem()
     s = _solver(mech, rtol=rtol, atol=atol)
+    assert s.lib.izem_chem_tune(s.ctx, waves, 4 * waves, 20, kinetics) == 0
     n = 96
     T, rho, Y = _cells(thd, n)
     dt = 1.0e-6
