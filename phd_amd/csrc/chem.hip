@@ -209,7 +209,7 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
     if (!ctx->chem) return IZEM_E_NOMEM;
     if (hipMalloc(&ctx->chem->counter, sizeof(int)) != hipSuccess) return IZEM_E_NOMEM;
     // defaults measured on MI355X for 2^18 independent cells at dt = 1e-6 s:
-    // four resident waves per SIMD, table-driven kinetics
+    // four resident waves per SIMD
     ctx->chem->waves = 4;
     ctx->chem->blocks_per_cu = 16;
     ctx->chem->jac_age = 20;
@@ -222,7 +222,7 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
     IZ_CHECK(hipStreamSynchronize(ctx->stream));
   }
   ctx->chem->generated = same;
-  ctx->chem->use_gen = 0;
+  ctx->chem->use_gen = same;
   ctx->chem->rtol = rtol * IZ_BDF_TOL_SCALE;
   ctx->chem->atol = atol * IZ_BDF_TOL_SCALE;
   ctx->chem->trim = trim;
