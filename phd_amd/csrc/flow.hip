@@ -64,9 +64,12 @@ __device__ inline void eu_stage_cell(const double *__restrict__ v, const double 
 // F^ of one face from its six staged stencil cells: Q(q, st) is staged
 // quantity q of stencil point st (st = 2 is i_l, st = 3 is i_r); cL, cR are
 // the HBM indices of i_l, i_r (for h_k); F^ goes to F[m * nfaces + fidx].
-template <int DIR, class Acc>
-__device__ inline void eu_face(const Acc &Q, bool js, const double *__restrict__ ha, long nc,
-                               long cL, long cR, double *__restrict__ F, long nfaces, long fidx) {
+template <int DIR, int QS, int SS>
+__device__ inline void eu_face(const double *tile, int off, bool js,
+                               const double *__restrict__ ha, long nc, long cL, long cR,
+                               double *__restrict__ F, long nfaces, long fidx) {
+  // staged quantity q of stencil point st
+  auto Q = [&](int q, int st) { return tile[off + q * QS + st * SS]; };
   constexpr int cn = 1 + DIR;             // normal momentum column
   constexpr int t1 = (DIR == 0) ? 2 : 1;  // tangential columns (el rows 4, 5)
   constexpr int t2 = (DIR == 2) ? 2 : 3;
@@ -133,6 +136,12 @@ __device__ inline void eu_face(const Acc &Q, bool js, const double *__restrict__
   double ghat[IZ_NV];
 #pragma unroll 1
   for (int m = 0; m < IZ_NV; m++) {
+    // the stencil is re-read from LDS for every characteristic field: an
+    // opaque copy of the offset keeps the compiler from hoisting the 96
+    // loop-invariant reads into registers (which spills at two waves/SIMD)
+    int offm = off;
+    asm volatile("" : "+v"(offm));
+    auto Qm = [&](int q, int st) { return tile[offm + q * QS + st * SS]; };
     // row m of the left eigenvector matrix (module_weno.f90:775-874)
     double e[IZ_NV];
     double lam;
@@ -188,32 +197,32 @@ __device__ inline void eu_face(const Acc &Q, bool js, const double *__restrict__
     double gp[6], gm[6];
 #pragma unroll
     for (int st = 0; st < 6; st++) {
-      const double u = Q(14, st), p = Q(15, st);
+      const double u = Qm(14, st), p = Qm(15, st);
       double wc, gc;
       if (m == 3 || m == 4) {
         const int tc = (m == 3) ? t1 : t2;
         const double et0 = -uc[tc - 1];
-        const double v0 = Q(0, st), vt = Q(tc, st);
-        const double f0 = Q(cn, st);
+        const double v0 = Qm(0, st), vt = Qm(tc, st);
+        const double f0 = Qm(cn, st);
         const double ft = (tc == cn) ? vt * u + p : vt * u;
         wc = et0 * v0 + vt;
         gc = et0 * f0 + ft;
       } else {
-        const double v0 = Q(0, st);
+        const double v0 = Qm(0, st);
         wc = e[0] * v0;
-        gc = e[0] * Q(cn, st);
+        gc = e[0] * Qm(cn, st);
 #pragma unroll
         for (int cc = 1; cc <= 3; cc++) {
-          const double vc = Q(cc, st);
+          const double vc = Qm(cc, st);
           wc = wc + e[cc] * vc;
           gc = gc + e[cc] * ((cc == cn) ? vc * u + p : vc * u);
         }
-        const double v4 = Q(4, st);
+        const double v4 = Qm(4, st);
         wc = wc + e[4] * v4;
         gc = gc + e[4] * (u * (v4 + p));
 #pragma unroll
         for (int l = IZ_NIV; l < IZ_NV; l++) {
-          const double vl = Q(l, st);
+          const double vl = Qm(l, st);
           wc = wc + e[l] * vl;
           gc = gc + e[l] * (vl * u);
         }
@@ -291,38 +300,42 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
   extern __shared__ double lds[];
   int neg = 0;
   if constexpr (DIR == 0) {
+    // face, row and slot numbers fit 32 bits (faces < 2^31 is checked at
+    // context creation); only HBM cell indices are 64-bit
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long f0 = ((long)blockIdx.x * 4 + wave) * 64;  // first face of this wave
-    const int nf = d.nx + 1, NXc = d.nx + 6;             // faces, cells per row
-    const long nrows = (long)d.ny * d.nz;
-    double *tl = lds + (long)wave * tile0 * IZ_EU_NQ;
-    uint8_t *tsn = (uint8_t *)(lds + 4L * tile0 * IZ_EU_NQ) + wave * tile0;
+    const unsigned f0 = (blockIdx.x * 4u + wave) * 64u;  // first face of this wave
+    const unsigned nf = d.nx + 1, NXc = d.nx + 6;          // faces, cells per row
+    const unsigned nrows = (unsigned)d.ny * d.nz, nfc = (unsigned)nfaces;
+    // cell-major tile (slot * IZ_EU_NQ + q): every stencil access is a
+    // compile-time offset from the lane's first stencil cell
+    double *tl = lds + wave * tile0 * IZ_EU_NQ;
+    uint8_t *tsn = (uint8_t *)(lds + 4 * tile0 * IZ_EU_NQ) + wave * tile0;
     // cells in row-flattened order r * NXc + (x + 3); face f's stencil
     // starts at S(f) = r * NXc + x + 1 for its left cell x
-    long S0 = 0;
-    if (f0 < nfaces) S0 = (f0 / nf) * NXc + (f0 % nf);
+    unsigned S0 = 0;
+    if (f0 < nfc) S0 = (f0 / nf) * NXc + (f0 % nf);
     for (int t = lane; t < tile0; t += 64) {
-      const long cf = S0 + t;
-      const long r = cf / NXc;
-      if (f0 < nfaces && r < nrows) {
+      const unsigned cf = S0 + t;
+      const unsigned r = cf / NXc;
+      if (f0 < nfc && r < nrows) {
         const int xc = (int)(cf - r * NXc) - 3;
         const long c = iz_cidx(d, xc, (int)(r % d.ny), (int)(r / d.ny));
-        eu_stage_cell(v, T, W_i, cp, d.nc, c, 1, tl + t, tile0, neg);
+        eu_stage_cell(v, T, W_i, cp, d.nc, c, 1, tl + t * IZ_EU_NQ, 1, neg);
         tsn[t] = sensor[c];
       }
     }
     __syncthreads();
-    const long f = f0 + lane;
-    if (f < nfaces) {
-      const long r = f / nf;
+    const unsigned f = f0 + lane;
+    if (f < nfc) {
+      const unsigned r = f / nf;
       const int x = (int)(f - r * nf) - 1;
       const int o = (int)(r * NXc + x + 1 - S0);
       const long cL = iz_cidx(d, x, (int)(r % d.ny), (int)(r / d.ny));
-      auto Q = [&](int q, int st) { return tl[q * tile0 + o + st]; };
       int shk = 0;
 #pragma unroll
       for (int st = 0; st < 6; st++) shk |= tsn[o + st];
-      eu_face<0>(Q, shk && weno_avg, ha, d.nc, cL, cL + 1, F, nfaces, f);
+      eu_face<0, 1, IZ_EU_NQ>(lds, (wave * tile0 + o) * IZ_EU_NQ, shk && weno_avg, ha, d.nc,
+                              cL, cL + 1, F, nfaces, (long)f);
     }
   } else {
     // hardware blocks b, b + 8, ... run on one XCD: give them consecutive
@@ -354,7 +367,6 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
     const int f = tf * IZ_EU_TF + fl - 1;  // left cell of the face along DIR
     if (blk_ok && x < d.nx && f <= nd - 1) {
       // stencil point st of this face is staged row fl + st
-      auto Q = [&](int q, int st) { return lds[q * FS + (fl + st) * IZ_EU_TX + xl]; };
       int shk = 0;
 #pragma unroll
       for (int st = 0; st < 6; st++) shk |= tsn[(fl + st) * IZ_EU_TX + xl];
@@ -368,7 +380,8 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
         cR = cL + d.NX * d.NY;
         fidx = x + (long)d.nx * (oth + (long)d.ny * (f + 1));
       }
-      eu_face<DIR>(Q, shk && weno_avg, ha, d.nc, cL, cR, F, nfaces, fidx);
+      eu_face<DIR, FS, IZ_EU_TX>(lds, fl * IZ_EU_TX + xl, shk && weno_avg, ha, d.nc, cL, cR, F,
+                                 nfaces, fidx);
     }
   }
   if (neg) atomicOr(status, IZ_ST_NEG_PRESSURE);
@@ -447,19 +460,63 @@ __device__ inline unsigned long long d_newton_probe(const double *vc, double T, 
   return mask;
 }
 
+// Block-wide reductions that end in one atomic per block.  Kernels that
+// reduce walk the cells with a grid-stride loop over a grid capped at a
+// few blocks per CU, so a launch issues a few thousand atomics on the
+// target words instead of one per wave (same-address atomics serialise in
+// L2).  blockDim.x is a multiple of 64 and at most 1024; every thread of
+// the block calls these.
 __device__ inline void block_or_u64(unsigned long long m, unsigned long long *dst) {
-  // wave-level OR then one atomic per wave
+  __shared__ unsigned long long s_or[16];
   for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off, 64);
-  if ((threadIdx.x & 63) == 0 && m) atomicOr(dst, m);
+  if ((threadIdx.x & 63) == 0) s_or[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0ull;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) a |= s_or[w];
+    if (a) atomicOr(dst, a);
+  }
+}
+
+__device__ inline void atomic_max_pos(double *addr, double val) {
+  // non-negative doubles order like their bit patterns
+  atomicMax((unsigned long long *)addr, (unsigned long long)__double_as_longlong(val));
+}
+__device__ inline void atomic_min_pos(double *addr, double val) {
+  atomicMin((unsigned long long *)addr, (unsigned long long)__double_as_longlong(val));
+}
+
+// x[0..N) reduced over the block by max (MAXR) or min, then one atomic per
+// value into dst[0..N); values are non-negative (or +inf)
+template <int N, bool MAXR>
+__device__ inline void block_extreme_pos(double *x, double *dst) {
+  __shared__ double s_ex[N][16];
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    double a = x[q];
+    for (int off = 32; off > 0; off >>= 1) {
+      const double b = __shfl_xor(a, off, 64);
+      a = MAXR ? fmax(a, b) : fmin(a, b);
+    }
+    if ((threadIdx.x & 63) == 0) s_ex[q][threadIdx.x >> 6] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    const int q = threadIdx.x;
+    double a = s_ex[q][0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) a = MAXR ? fmax(a, s_ex[q][w]) : fmin(a, s_ex[q][w]);
+    if (MAXR) atomic_max_pos(&dst[q], a);
+    else atomic_min_pos(&dst[q], a);
+  }
 }
 
 __global__ void k_force_probe(IzDims d, IzBox b, int forcesum, double *__restrict__ v,
                               const double *__restrict__ T, double eps,
                               unsigned long long *__restrict__ mask) {
-  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   unsigned long long m = 0ull;
-  if (box_cell(b, tid, i, j, k)) {
+  for (long tid = (long)blockIdx.x * blockDim.x + threadIdx.x; box_cell(b, tid, i, j, k);
+       tid += (long)gridDim.x * blockDim.x) {
     long c = iz_cidx(d, i, j, k);
     double vc[IZ_NV];
 #pragma unroll
@@ -469,7 +526,7 @@ __global__ void k_force_probe(IzDims d, IzBox b, int forcesum, double *__restric
 #pragma unroll
       for (int l = IZ_NIV; l < IZ_NV; l++) v[l * d.nc + c] = vc[l];
     }
-    if (eps > 0.0) m = d_newton_probe(vc, T[c], eps);
+    if (eps > 0.0) m |= d_newton_probe(vc, T[c], eps);
   }
   block_or_u64(m, mask);
 }
@@ -865,7 +922,11 @@ __global__ void k_visc_apply(IzDims d, const double *__restrict__ sf,
 // Fused right-hand side + SSP-RK3 stage update + force_sumY + Newton probe
 // (rk3, module_solver.f90:769-847).  mode 0: write f only; mode 1: update.
 // ===================================================================
-__global__ void __launch_bounds__(256)
+#ifndef IZ_RHS_WAVES
+#define IZ_RHS_WAVES 2
+#endif
+__global__ void __attribute__((amdgpu_flat_work_group_size(256, 256),
+                               amdgpu_waves_per_eu(IZ_RHS_WAVES, IZ_RHS_WAVES)))
 k_rhs_update(IzDims d, int mode, const double *__restrict__ Fx, const double *__restrict__ Fy,
              const double *__restrict__ Fz, int vis, const double *__restrict__ sf,
              const double *__restrict__ fd, const double *__restrict__ dx_i,
@@ -874,9 +935,11 @@ k_rhs_update(IzDims d, int mode, const double *__restrict__ Fx, const double *__
              const double *__restrict__ v0, double *__restrict__ v, double *__restrict__ fout,
              int forcesum, const double *__restrict__ T, double eps,
              unsigned long long *__restrict__ mask) {
-  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long ncin = (long)d.nx * d.ny * d.nz;
+  // one cell per thread (a grid-stride loop here was 2.7x slower: the
+  // y/z face and viscous stencil neighbours lose their L2 reuse)
+  const long ncin = (long)d.nx * d.ny * d.nz;
   unsigned long long msk = 0ull;
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid < ncin) {
     int i = (int)(tid % d.nx);
     long r = tid / d.nx;
@@ -931,22 +994,14 @@ k_rhs_update(IzDims d, int mode, const double *__restrict__ Fx, const double *__
 // ===================================================================
 // Time step: timestep (module_solver.f90:202-408)
 // ===================================================================
-__device__ inline void atomic_max_pos(double *addr, double val) {
-  // non-negative doubles order like their bit patterns
-  atomicMax((unsigned long long *)addr, (unsigned long long)__double_as_longlong(val));
-}
-__device__ inline void atomic_min_pos(double *addr, double val) {
-  atomicMin((unsigned long long *)addr, (unsigned long long)__double_as_longlong(val));
-}
-
 __global__ void k_dt_conv(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
                           const double *__restrict__ W_i, const double *__restrict__ cp,
                           const double *__restrict__ dx_i, const double *__restrict__ dy_i,
                           const double *__restrict__ dz_i, double *__restrict__ red) {
-  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long ncin = (long)d.nx * d.ny * d.nz;
+  const long ncin = (long)d.nx * d.ny * d.nz;
   double dtc = 0.0;
-  if (tid < ncin) {
+  for (long tid = (long)blockIdx.x * blockDim.x + threadIdx.x; tid < ncin;
+       tid += (long)gridDim.x * blockDim.x) {
     int i = (int)(tid % d.nx);
     long r = tid / d.nx;
     int j = (int)(r % d.ny), k = (int)(r / d.ny);
@@ -960,10 +1015,9 @@ __global__ void k_dt_conv(IzDims d, const double *__restrict__ v, const double *
     double a = (fabs(ux) + cs) * dx_i[i + d.ng];
     double b = (fabs(vy) + cs) * dy_i[j + d.ng];
     double e = (fabs(wz) + cs) * dz_i[k + d.ng];
-    dtc = fmax(fmax(a, b), e);
+    dtc = fmax(dtc, fmax(fmax(a, b), e));
   }
-  for (int off = 32; off > 0; off >>= 1) dtc = fmax(dtc, __shfl_xor(dtc, off, 64));
-  if ((threadIdx.x & 63) == 0) atomic_max_pos(&red[0], dtc);
+  block_extreme_pos<1, true>(&dtc, red);
 }
 
 __device__ inline double d_filter_delta2(const IzDims &d, const double *dx_i, const double *dy_i,
@@ -981,25 +1035,22 @@ __device__ inline double d_filter_delta2(const IzDims &d, const double *dx_i, co
 // pass 1: per-species maximum of the mixture diffusivity
 __global__ void k_dt_visc1(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
                            const double *__restrict__ W_i, double *__restrict__ red) {
-  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long ncin = (long)d.nx * d.ny * d.nz;
-  double dm[IZ_NRV];
-  for (int l = 0; l < IZ_NRV; l++) dm[l] = 0.0;
-  if (tid < ncin) {
+  const long ncin = (long)d.nx * d.ny * d.nz;
+  double dmax[IZ_NRV];
+  for (int l = 0; l < IZ_NRV; l++) dmax[l] = 0.0;
+  for (long tid = (long)blockIdx.x * blockDim.x + threadIdx.x; tid < ncin;
+       tid += (long)gridDim.x * blockDim.x) {
     int i = (int)(tid % d.nx);
     long r = tid / d.nx;
     int j = (int)(r % d.ny), k = (int)(r / d.ny);
     long c = iz_cidx(d, i, j, k);
-    double X[IZ_NRV], mu, ct;
+    double X[IZ_NRV], dm[IZ_NRV], mu, ct;
     for (int l = 0; l < IZ_NRV; l++)
       X[l] = v[(IZ_NIV + l) * d.nc + c] / (v[c] * W_i[c] * c_thd.Wc[l]);
     d_transport(v[c], T[c], W_i[c], X, dm, mu, ct, false);
+    for (int l = 0; l < IZ_NRV; l++) dmax[l] = fmax(dmax[l], dm[l]);
   }
-  for (int l = 0; l < IZ_NRV; l++) {
-    double x = dm[l];
-    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
-    if ((threadIdx.x & 63) == 0) atomic_max_pos(&red[1 + l], x);
-  }
+  block_extreme_pos<IZ_NRV, true>(dmax, red + 1);
 }
 
 // pass 2: minima of Delta^2/D(ldmax) and Delta^2 rho/(lambda/cp)
@@ -1007,15 +1058,16 @@ __global__ void k_dt_visc2(IzDims d, const double *__restrict__ v, const double 
                            const double *__restrict__ W_i, const double *__restrict__ cp,
                            const double *__restrict__ dx_i, const double *__restrict__ dy_i,
                            const double *__restrict__ dz_i, double *__restrict__ red) {
-  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long ncin = (long)d.nx * d.ny * d.nz;
+  const long ncin = (long)d.nx * d.ny * d.nz;
   // species of the global maximum: the first in species order
   double gmax = red[1];
   int ldmax = 0;
   for (int l = 1; l < IZ_NRV; l++)
     if (gmax < red[1 + l]) { gmax = red[1 + l]; ldmax = l; }
-  double a = __longlong_as_double(0x7ff0000000000000LL), b = a;  // +inf
-  if (tid < ncin) {
+  double ab[2];
+  ab[0] = ab[1] = __longlong_as_double(0x7ff0000000000000LL);  // +inf
+  for (long tid = (long)blockIdx.x * blockDim.x + threadIdx.x; tid < ncin;
+       tid += (long)gridDim.x * blockDim.x) {
     int i = (int)(tid % d.nx);
     long r = tid / d.nx;
     int j = (int)(r % d.ny), k = (int)(r / d.ny);
@@ -1025,14 +1077,10 @@ __global__ void k_dt_visc2(IzDims d, const double *__restrict__ v, const double 
       X[l] = v[(IZ_NIV + l) * d.nc + c] / (v[c] * W_i[c] * c_thd.Wc[l]);
     d_transport(v[c], T[c], W_i[c], X, dm, mu, ct, false);
     double d2 = d_filter_delta2(d, dx_i, dy_i, dz_i, i, j, k);
-    a = d2 / (dm[ldmax] + 0.0);
-    b = d2 * v[c] / (ct / cp[c] + 0.0);
+    ab[0] = fmin(ab[0], d2 / (dm[ldmax] + 0.0));
+    ab[1] = fmin(ab[1], d2 * v[c] / (ct / cp[c] + 0.0));
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    a = fmin(a, __shfl_xor(a, off, 64));
-    b = fmin(b, __shfl_xor(b, off, 64));
-  }
-  if ((threadIdx.x & 63) == 0) { atomic_min_pos(&red[10], a); atomic_min_pos(&red[11], b); }
+  block_extreme_pos<2, false>(ab, red + 10);
 }
 
 // red layout: [0] dtc, [1..9] dm max, [10] min d2/D, [11] min d2 rho cp/lambda
@@ -1130,8 +1178,16 @@ extern "C" int izem_create(izem_ctx **out, const izem_grid *g, const double *thd
   if (g->ng != 3 || g->ndim < 2 || g->ndim > 3 || g->nx < 7 || g->ny < 7) return IZEM_E_ARG;
   if (g->ndim == 3 && g->nz < 7) return IZEM_E_ARG;
   if (g->ndim == 2 && g->nz != 1) return IZEM_E_ARG;
+  // face indices of a direction are 32-bit in the Euler kernels
+  if ((long)(g->nx + 1) * (g->ny + 1) * (g->nz + 1) >= (1L << 31)) return IZEM_E_ARG;
   izem_ctx *c = (izem_ctx *)calloc(1, sizeof(izem_ctx));
   if (!c) return IZEM_E_NOMEM;
+  {
+    int dev = 0;
+    c->ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  }
   IzDims &d = c->d;
   d.nx = g->nx; d.ny = g->ny; d.nz = g->nz; d.ng = g->ng; d.ndim = g->ndim;
   d.NX = d.nx + 2 * d.ng; d.NY = d.ny + 2 * d.ng; d.NZ = d.nz + 2 * d.ng;
@@ -1399,7 +1455,7 @@ extern "C" int izem_force_sumY(izem_ctx *c, int dom, double *v) {
   if (!c || !v || !dom_ok(c->d, dom)) return IZEM_E_ARG;
   IzBox b = domain_box(c->d, dom);
   // probe output is discarded (mask slot 7)
-  hipLaunchKernelGGL(k_force_probe, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0, c->stream,
+  hipLaunchKernelGGL(k_force_probe, dim3(iz_blocks_capped(c, box_count(b), 256, 16)), dim3(256), 0, c->stream,
                      c->d, b, 1, v, (const double *)nullptr, -1.0, c->masks + 7);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
@@ -1412,7 +1468,7 @@ static int newton_domains(izem_ctx *c, const int *doms, int nd, int forcesum, do
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->masks, 7);
   for (int q = 0; q < nd; q++) {
     IzBox b = domain_box(d, doms[q]);
-    hipLaunchKernelGGL(k_force_probe, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_force_probe, dim3(iz_blocks_capped(c, box_count(b), 256, 16)), dim3(256), 0,
                        c->stream, d, b, forcesum, v, T, 1.0e-5, c->masks + (doms[q] + 3));
   }
   for (int q = 0; q < nd; q++) {
@@ -1544,15 +1600,15 @@ extern "C" int izem_timestep(izem_ctx *c, const double *dx_i, const double *dy_i
                              const double *T, const double *v) {
   if (!c || !dx_i || !dy_i || !dz_i || !cp || !W_i || !T || !v) return IZEM_E_ARG;
   const IzDims &d = c->d;
-  unsigned nb = iz_blocks(n_interior(d), 256);
+  unsigned nb = iz_blocks_capped(c, n_interior(d), 256, 16);
   hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(64), 0, c->stream, c->red);
   hipLaunchKernelGGL(k_dt_conv, dim3(nb), dim3(256), 0, c->stream, d, v, T, W_i, cp, dx_i, dy_i,
                      dz_i, c->red);
   int visdt = c->o.vis && c->o.vis_dt;
   if (visdt) {
-    hipLaunchKernelGGL(k_dt_visc1, dim3(iz_blocks(n_interior(d), 128)), dim3(128), 0, c->stream,
+    hipLaunchKernelGGL(k_dt_visc1, dim3(iz_blocks_capped(c, n_interior(d), 128, 32)), dim3(128), 0, c->stream,
                        d, v, T, W_i, c->red);
-    hipLaunchKernelGGL(k_dt_visc2, dim3(iz_blocks(n_interior(d), 128)), dim3(128), 0, c->stream,
+    hipLaunchKernelGGL(k_dt_visc2, dim3(iz_blocks_capped(c, n_interior(d), 128, 32)), dim3(128), 0, c->stream,
                        d, v, T, W_i, cp, dx_i, dy_i, dz_i, c->red);
   }
   double rvd0_i = c->o.Sc / c->o.sqgmr, rlamcp0_i = c->o.Pr / c->o.sqgmr;

@@ -53,6 +53,7 @@ struct izem_ctx {
   double *ts;   // dt_4[4], dtmin, DTmax
   IzChem *chem;
   IzProf prof;
+  int ncu;  // compute units of the device (grid caps of reducing kernels)
 };
 
 // bracket a kernel launch with events on the context stream
@@ -69,6 +70,13 @@ static inline void iz_prof_end(izem_ctx *c, int k) {
 }
 
 static inline unsigned iz_blocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+// grid of a grid-stride kernel: enough blocks for n items, at most per_cu
+// blocks per compute unit
+static inline unsigned iz_blocks_capped(const izem_ctx *c, long n, int bs, int per_cu) {
+  long nb = (n + bs - 1) / bs, cap = (long)c->ncu * per_cu;
+  if (nb > cap) nb = cap;
+  return (unsigned)(nb < 1 ? 1 : nb);
+}
 
 // internal entry points across translation units
 int iz_ghost_newton(izem_ctx *ctx, double *T, double *W_i, double *cp, double *ha, double *v);
