@@ -144,6 +144,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--chem-cells", type=int, default=1 << 22)
+    ap.add_argument("--chem-tune", type=str, default=None,
+                    help="waves,blocks_per_cu,jac_age,kinetics for the chemistry kernels "
+                         "(default: autotuned on a sample of the reacting cells)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported "
@@ -163,9 +166,22 @@ def main():
     mech = cd.read_mechanism()
     log(f"building the {n}^3 reacting HIT state")
     box, v, T = ics.config2_state(adi, thd, n=n, seed=2)
-    s = Solver(box, Options(), adi=adi, thd=thd, mech=mech)
+    tune = tuple(int(x) for x in args.chem_tune.split(",")) if args.chem_tune else None
+    s = Solver(box, Options(chem_tune=tune), adi=adi, thd=thd, mech=mech)
     s.load_state(v, T)
     del v, T
+    tuned = None
+    if tune is None:
+        from phd_amd import autotune
+        ts0 = s.time_state()
+        s.timestep()  # the step's dt, to sample the chemistry at its half step
+        dt_half = 0.5 * float(s.time_state()[4]) * adi.time_ref
+        s.set_time_state(ts0)
+        Ts, rs, Ys = autotune.sample_reacting_cells(s, 1 << 20)
+        tuned = autotune.tune(s, Ts, rs, Ys, dt_half, key=f"reacting-hit-{n}", use_cache=False)
+        log(f"chemistry kernels tuned: waves {tuned[0]}, blocks/CU {tuned[1]}, "
+            f"kinetics {'straight-line' if tuned[3] else 'tables'} ({tuned[4]:.3e} cells/s)")
+        del Ts, rs, Ys
     log("state resident on the GPU; warm-up")
     for _ in range(args.warmup):
         s.step()
@@ -219,6 +235,10 @@ def main():
                      "avg_launch_ms": e_avg_s * 1e3, "launches": e_cnt,
                      "flop_per_face": EULER_FLOP_PER_FACE, "faces_per_launch": nfaces},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
+        "chem_tuning": (None if tuned is None else
+                        {"waves_per_simd": tuned[0], "blocks_per_cu": tuned[1],
+                         "jac_age": tuned[2], "kinetics": "straight-line" if tuned[3] else
+                         "tables", "sample_cells_per_s": tuned[4]}),
         "chemistry": chem,
         "cpu_baseline": cpu,
     }

@@ -45,6 +45,9 @@ class Options:
     rtol: float = 1.0e-6
     atol: float = 1.0e-12
     trim_box: int = 1   # reaction_dvode skips ng+1 planes at global faces
+    # (waves, blocks_per_cu, jac_age, kinetics) for izem_chem_tune; None keeps
+    # the library defaults
+    chem_tune: tuple | None = None
 
 
 def _ptr(t: torch.Tensor):
@@ -95,6 +98,13 @@ class Solver:
             mf = mech_flat(mech)
             _lib.check(self.lib.izem_chem_setup(ctx, mf.ctypes.data, mf.size, o.rtol, o.atol,
                                                 o.trim_box), "izem_chem_setup")
+            if o.chem_tune is not None:
+                self.chem_tune(*o.chem_tune)
+
+    def chem_tune(self, waves: int, blocks_per_cu: int, jac_age: int, kinetics: int):
+        """Launch parameters of the chemistry kernels (izem_chem_tune)."""
+        _lib.check(self.lib.izem_chem_tune(self.ctx, waves, blocks_per_cu, jac_age, kinetics),
+                   "izem_chem_tune")
 
     def close(self):
         if getattr(self, "ctx", None):
