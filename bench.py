@@ -36,10 +36,10 @@ def log(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def read_traffic(n: int):
-    """HBM bytes per Euler-face launch from the committed rocprofv3 --pmc
+def read_traffic(kernel: str, n: int):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
-    path = os.path.join(ROOT, "profiles", f"euler_faces_{n}cube_pmc.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{n}cube_pmc.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
@@ -188,6 +188,9 @@ def main():
     s.sync()
     torch.cuda.synchronize()
     log(f"timing {args.steps} steps")
+    chem_stats = np.zeros(4, dtype=np.uint64)
+    if s.opts.reaction:
+        s.lib.izem_chem_stats(s.ctx, chem_stats.ctypes.data)  # clear the counters
     profile_start(s)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -197,6 +200,8 @@ def main():
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     prof = profile_read(s)
+    if s.opts.reaction:
+        s.lib.izem_chem_stats(s.ctx, chem_stats.ctypes.data)
     status = s.status()
     ts = s.time_state()
     ncell = n ** 3
@@ -206,7 +211,32 @@ def main():
     nfaces = (n + 1) * n * n
     e_avg_s = e_ms / e_cnt / 1e3
     achieved = EULER_FLOP_PER_FACE * nfaces / e_avg_s / 1e12
-    traffic = read_traffic(n)
+    euler_roof = {"kernel": "k_euler_faces (x, y, z)", "bound": "mfma",
+                  "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                  "frac": achieved / PEAK_FP64_TFLOPS,
+                  "traffic": read_traffic("euler_faces", n),
+                  "avg_launch_ms": e_avg_s * 1e3, "launches": e_cnt,
+                  "flop_per_face": EULER_FLOP_PER_FACE, "faces_per_launch": nfaces}
+    c_ms, c_cnt = prof["chemistry"]
+    chem_roof = None
+    if c_cnt:
+        from phd_amd.flops import chem_flops
+        c_avg_s = c_ms / c_cnt / 1e3
+        c_flop = chem_flops(chem_stats) / c_cnt
+        c_ach = c_flop / c_avg_s / 1e12
+        chem_roof = {"kernel": "k_reaction_queue", "bound": "mfma", "achieved": c_ach,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": c_ach / PEAK_FP64_TFLOPS,
+                     "traffic": read_traffic("reaction_queue", n),
+                     "avg_launch_ms": c_avg_s * 1e3, "launches": c_cnt,
+                     "flop_per_launch": c_flop,
+                     "work_per_launch": {"bdf_steps": float(chem_stats[0]) / c_cnt,
+                                         "rhs_evals": float(chem_stats[1]) / c_cnt,
+                                         "jacobians": float(chem_stats[2]) / c_cnt,
+                                         "lu_solves": float(chem_stats[3]) / c_cnt}}
+    # the roofline object describes the kernel with the largest share of
+    # the step; the other one is reported beside it
+    dominant = chem_roof if (chem_roof and c_ms > e_ms) else euler_roof
     log(f"{ms_step:.1f} ms/step; kernel time per step: " +
         ", ".join(f"{k} {v[0] / args.steps:.1f} ms" for k, v in prof.items()))
     chem = None
@@ -229,11 +259,8 @@ def main():
                                f"with Strang BDF chemistry (rtol 1e-6, atol 1e-12)",
                    "cells": ncell, "dt_s": float(ts[4] * adi.time_ref),
                    "status_bits": status},
-        "roofline": {"kernel": "k_euler_faces (x, y, z)", "bound": "mfma",
-                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                     "avg_launch_ms": e_avg_s * 1e3, "launches": e_cnt,
-                     "flop_per_face": EULER_FLOP_PER_FACE, "faces_per_launch": nfaces},
+        "roofline": dominant,
+        "roofline_kernels": {"k_reaction_queue": chem_roof, "k_euler_faces": euler_roof},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
         "chem_tuning": (None if tuned is None else
                         {"waves_per_simd": tuned[0], "blocks_per_cu": tuned[1],

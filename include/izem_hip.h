@@ -161,6 +161,12 @@ int izem_chem_setup(izem_ctx *ctx, const double *mech, long nmech, double rtol, 
  * the loaded mechanism is another one --, -1 straight-line when it
  * matches).  No counterpart in the reference (DVODE runs on the CPU). */
 int izem_chem_tune(izem_ctx *ctx, int waves, int blocks_per_cu, int jac_age, int kinetics);
+/* work counters of the chemistry integrations since the previous call:
+ * out[0] BDF steps, out[1] RHS evaluations, out[2] Jacobians (LU
+ * factorisations), out[3] LU solves; synchronises and clears them.
+ * Instrumentation without a reference counterpart (DVODE keeps the same
+ * counts per call in IWORK(11..13), vode.f:4286). */
+int izem_chem_stats(izem_ctx *ctx, unsigned long long out[4]);
 /* ckwyr (chemkin/vode.f:1797) on n cells: rho [g/cm3], T [K], Y[n][kk] ->
  * wdot[n][kk] [mol/cm3/s] */
 int izem_ckwyr(izem_ctx *ctx, int n, const double *rho, const double *T, const double *Y,

@@ -18,7 +18,7 @@ EXPORTS = [
     "izem_wenorec", "izem_euler_llf_x", "izem_euler_llf_y", "izem_euler_llf_z",
     "izem_viscflux", "izem_force_sumY", "izem_prim_inv_var", "izem_upd_prim_var_woT",
     "izem_comm_cons", "izem_upd_prim_var_ghost", "izem_shock_sensor", "izem_rk3", "izem_rhs",
-    "izem_timestep", "izem_chem_setup", "izem_chem_tune", "izem_ckwyr", "izem_psr_batch", "izem_reaction_dvode",
+    "izem_timestep", "izem_chem_setup", "izem_chem_tune", "izem_chem_stats", "izem_ckwyr", "izem_psr_batch", "izem_reaction_dvode",
 ]
 
 
@@ -81,6 +81,7 @@ def load(path: str | None = None):
     lib.izem_timestep.argtypes = [vp] * 8
     lib.izem_chem_setup.argtypes = [vp, vp, C.c_long, C.c_double, C.c_double, C.c_int]
     lib.izem_chem_tune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.izem_chem_stats.argtypes = [vp, vp]
     lib.izem_ckwyr.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     lib.izem_psr_batch.argtypes = [vp, C.c_int, vp, vp, vp, C.c_double, vp]
     lib.izem_reaction_dvode.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp]

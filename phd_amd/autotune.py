@@ -8,7 +8,7 @@ the cells' stiffness (steps per cell), the device and the mechanism.  The
 tuner times each candidate with izem_psr_batch on a sample of the cells the
 caller will integrate, keeps the fastest and applies it with
 izem_chem_tune.  Every candidate integrates the same BDF method with the
-same tolerances and Jacobian reuse age (DVODE's MSBP = 20), so the choice
+same tolerances and Jacobian reuse ages (DVODE's MSBP = 20, MSBJ = 50), so the choice
 changes speed only; the two kinetics forms differ at round-off level.
 
 Results are cached per (device, mechanism hash, workload key) in a JSON
@@ -28,7 +28,7 @@ from . import _lib
 from .codegen_mech import mechanism_hash
 
 CANDIDATES = [(w, b, k) for k in (1, 0) for w in (4, 2, 1) for b in (4 * w, 2 * w)]
-JAC_AGE = 20
+JAC_AGE = 50  # DVODE MSBJ
 
 
 def _cache_path() -> str:

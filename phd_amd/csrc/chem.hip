@@ -40,6 +40,9 @@ struct IzChem {
   // for: 1, 2 or 4), resident 64-lane blocks per CU, Jacobian reuse age
   int waves, blocks_per_cu, jac_age;
   int generated;  // the loaded mechanism is the compiled-in one (mech_gen.h)
+  // work counters of the integrations since the last izem_chem_stats:
+  // BDF steps, RHS evaluations, Jacobians, LU solves
+  unsigned long long *stats;
   int use_gen;    // run the straight-line kinetics (generated && not disabled)
 };
 
@@ -55,6 +58,24 @@ __global__ void k_ckwyr(int n, const double *__restrict__ rho, const double *__r
   for (int k = 0; k < kk; k++) wdot[(long)t * kk + k] = w[k];
 }
 
+// add per-lane work counters to stats[0..3], one atomic per wave and counter
+__device__ inline void chem_stats_add(unsigned long long *stats, unsigned long long a,
+                                      unsigned long long b, unsigned long long c,
+                                      unsigned long long d) {
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+    c += __shfl_xor(c, off, 64);
+    d += __shfl_xor(d, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&stats[0], a);
+    atomicAdd(&stats[1], b);
+    atomicAdd(&stats[2], c);
+    atomicAdd(&stats[3], d);
+  }
+}
+
 // persistent work-queue kernel over independent cells: each lane takes the
 // next cell from the queue as soon as its previous one is finished, so
 // cells with very different step counts keep every lane busy
@@ -62,7 +83,8 @@ template <class Kin, int WAVES>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(WAVES, WAVES))) k_psr_queue(int n, double *__restrict__ T, const double *__restrict__ rho,
                             double *__restrict__ Y, double dt, double rtol, double atol,
                             int jac_age, int *__restrict__ counter, int *__restrict__ nsteps,
-                            int *__restrict__ status) {
+                            int *__restrict__ status, unsigned long long *__restrict__ stats) {
+  unsigned long long w_nst = 0, w_nfe = 0, w_nje = 0, w_nsol = 0;
   for (;;) {
     int cell = atomicAdd(counter, 1);
     if (cell >= n) break;
@@ -74,7 +96,9 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     for (int k = 0; k < N_ - 1; k++) Y[(long)cell * (N_ - 1) + k] = y[1 + k];
     if (nsteps) nsteps[cell] = r.nst;
     if (!r.ok) atomicOr(status, IZ_ST_VODE_FAIL);
+    w_nst += r.nst; w_nfe += r.nfe; w_nje += r.nje; w_nsol += r.nsol;
   }
+  chem_stats_add(stats, w_nst, w_nfe, w_nje, w_nsol);
 }
 
 __device__ inline void atomic_max_posd(double *addr, double val) {
@@ -89,12 +113,14 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                                  double T_ref, double rho_cgs, double rtol, double atol,
                                  int jac_age, double *__restrict__ T, double *__restrict__ v,
                                  int *__restrict__ counter, double *__restrict__ ts,
-                                 int *__restrict__ status) {
+                                 int *__restrict__ status,
+                                 unsigned long long *__restrict__ stats) {
   const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
   const long n = (long)ni * nj * nk;
   const double dt_phys = (scale * (*dtp)) * time_ref;
   const double Tref_i = 1.0 / T_ref;
   double dtmax = 0.0;
+  unsigned long long w_nst = 0, w_nfe = 0, w_nje = 0, w_nsol = 0;
   for (;;) {
     long cell = atomicAdd(counter, 1);
     if (cell >= n) break;
@@ -110,6 +136,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     for (int l = 0; l < N_ - 1; l++) y[1 + l] = v[(IZ_NIV + l) * d.nc + c] * rho_i;
     BdfResult res = d_bdf_integrate(Kin(), rho0 * rho_cgs, y, dt_phys, rtol, atol, jac_age);
     if (!res.ok) atomicOr(status, IZ_ST_VODE_FAIL);
+    w_nst += res.nst; w_nfe += res.nfe; w_nje += res.nje; w_nsol += res.nsol;
     T[c] = y[0] * Tref_i;
     dtmax = fmax(dtmax, fabs(y[0] - T0_phys));
     double sumY = 0.0;
@@ -123,6 +150,11 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
   }
   for (int off = 32; off > 0; off >>= 1) dtmax = fmax(dtmax, __shfl_xor(dtmax, off, 64));
   if ((threadIdx.x & 63) == 0) atomic_max_posd(&ts[5], dtmax);
+  chem_stats_add(stats, w_nst, w_nfe, w_nje, w_nsol);
+}
+
+__global__ void k_stats_zero(unsigned long long *stats) {
+  if (threadIdx.x < 4) stats[threadIdx.x] = 0ull;
 }
 
 __global__ void k_chem_prep(int *counter, double *ts, int zero_dtmax) {
@@ -167,6 +199,7 @@ static void bdf_tables(IzBdf &t) {
 void iz_chem_destroy(izem_ctx *ctx) {
   if (!ctx || !ctx->chem) return;
   if (ctx->chem->counter) (void)hipFree(ctx->chem->counter);
+  if (ctx->chem->stats) (void)hipFree(ctx->chem->stats);
   free(ctx->chem);
   ctx->chem = nullptr;
 }
@@ -208,11 +241,14 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
     ctx->chem = (IzChem *)calloc(1, sizeof(IzChem));
     if (!ctx->chem) return IZEM_E_NOMEM;
     if (hipMalloc(&ctx->chem->counter, sizeof(int)) != hipSuccess) return IZEM_E_NOMEM;
+    if (hipMalloc(&ctx->chem->stats, 4 * sizeof(unsigned long long)) != hipSuccess)
+      return IZEM_E_NOMEM;
+    hipLaunchKernelGGL(k_stats_zero, dim3(1), dim3(64), 0, ctx->stream, ctx->chem->stats);
     // defaults measured on MI355X for 2^18 independent cells at dt = 1e-6 s:
     // four resident waves per SIMD
     ctx->chem->waves = 4;
     ctx->chem->blocks_per_cu = 16;
-    ctx->chem->jac_age = 20;
+    ctx->chem->jac_age = 50;  // DVODE MSBJ
   }
   int same = 0;
   if (m.kk == GEN_KK && m.ii == GEN_II) {
@@ -275,7 +311,7 @@ extern "C" int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho
   dim3 grid(persistent_grid(ctx, n)), blk(64);
 #define IZ_PSR(K, W)                                                                        \
   hipLaunchKernelGGL((k_psr_queue<K, W>), grid, blk, 0, ctx->stream, n, T, rho, Y, dt, ch->rtol, \
-                     ch->atol, ch->jac_age, ch->counter, nsteps, ctx->status)
+                     ch->atol, ch->jac_age, ch->counter, nsteps, ctx->status, ch->stats)
   if (ch->use_gen) {
     if (ch->waves == 1) IZ_PSR(KinGen, 1);
     else if (ch->waves == 2) IZ_PSR(KinGen, 2);
@@ -318,7 +354,7 @@ extern "C" int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale
 #define IZ_RQ(K, W)                                                                            \
   hipLaunchKernelGGL((k_reaction_queue<K, W>), grid, blk, 0, ctx->stream, d, i0, i1, j0, j1, k0, \
                      k1, dt, scale, ctx->o.time_ref, ctx->o.T_ref, rho_cgs, ch->rtol, ch->atol,   \
-                     ch->jac_age, T, v, ch->counter, ctx->ts, ctx->status)
+                     ch->jac_age, T, v, ch->counter, ctx->ts, ctx->status, ch->stats)
     if (ch->use_gen) {
       if (ch->waves == 1) IZ_RQ(KinGen, 1);
       else if (ch->waves == 2) IZ_RQ(KinGen, 2);
@@ -337,4 +373,16 @@ extern "C" int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale
   r = iz_comm_cons(ctx, v, IZ_NV);
   if (r) return r;
   return iz_ghost_newton(ctx, T, W_i, cp, ha, v);
+}
+
+// work counters of the chemistry integrations since the previous call
+// (BDF steps, RHS evaluations, Jacobians, LU solves); synchronises the
+// context stream and clears the counters
+extern "C" int izem_chem_stats(izem_ctx *ctx, unsigned long long *out) {
+  if (!ctx || !ctx->chem || !out) return IZEM_E_ARG;
+  IZ_CHECK(hipMemcpyAsync(out, ctx->chem->stats, 4 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, ctx->stream));
+  hipLaunchKernelGGL(k_stats_zero, dim3(1), dim3(64), 0, ctx->stream, ctx->chem->stats);
+  IZ_CHECK(hipStreamSynchronize(ctx->stream));
+  return IZEM_OK;
 }

@@ -195,6 +195,7 @@ IZ_HD inline void d_nordsieck_rescale(double (*z)[N_], int q, double rh) {
 
 struct BdfResult {
   int nst, nfe, nje, ok;
+  int nsol;  // LU back-substitutions (one per corrector iteration)
 };
 
 // finite-difference increment of Jacobian column j: DVJAC's
@@ -217,37 +218,53 @@ IZ_HD inline double d_fd_increment(int j, double yj, double ewtj, double r0) {
 // Integrate the reactor of one cell from 0 to tout (ITASK = 1: step past tout
 // and interpolate the Nordsieck history back to it).
 // Kinetics policies: rhs(rho, y, f) evaluates the reactor right-hand side;
-// jac(rho, y, f0, hl0, ewt, r0, P) writes P = I - hl0 * J with a
-// finite-difference Jacobian (DVJAC with MF = 22, vode.f:3728-3770).
+// jac(rho, y, f0, hl0, ewt, r0, P) writes P = -hl0 * J with a
+// finite-difference Jacobian (DVJAC with MF = 22, vode.f:3728-3770); both
+// leave y unchanged.
 struct KinTables {
   IZ_MHD inline void rhs(double rho, const double *y, double *f) const {
     d_reactor_rhs(rho, y, f);
   }
-  IZ_MHD inline void jac(double rho, double *y, const double *f0, double hl0, const double *ewt,
-                        double r0, double (*P)[N_]) const {
-    double ftmp[N_];
+  IZ_MHD inline void jac(double rho, const double *y, const double *f0, double hl0,
+                         const double *ewt, double r0, double (*P)[N_]) const {
     for (int j = 0; j < N_; j++) {
-      double yj = y[j];
-      double r = d_fd_increment(j, yj, ewt[j], r0);
-      y[j] = yj + r;
-      d_reactor_rhs(rho, y, ftmp);
-      y[j] = yj;
-      double ri = 1.0 / r;
+      double yl[N_], ftmp[N_];
+#pragma unroll
+      for (int n = 0; n < N_; n++) yl[n] = y[n];
+      const double yj = yl[j];
+      const double r = d_fd_increment(j, yj, ewt[j], r0);
+      yl[j] = yj + r;
+      d_reactor_rhs(rho, yl, ftmp);
+      const double ri = 1.0 / r;
 #pragma unroll
       for (int i = 0; i < N_; i++) P[i][j] = -hl0 * (ftmp[i] - f0[i]) * ri;
     }
   }
 };
 
+// Jacobian and iteration-matrix reuse follow DVODE (DVNLSD / DVJAC,
+// vode.f:4032-4259, 3596-3780): the finite-difference Jacobian J is kept,
+// and the iteration matrix P = I - h l0 J is refactored from it whenever
+// h l0 has changed by more than CCMAX = 0.3 or MSBP = 20 steps have
+// passed; J itself is re-evaluated only at the first step, after jac_age
+// (DVODE's MSBJ, default 50) steps, or after a corrector failure that a
+// new h l0 cannot explain (DRC < CCMXJ = 0.2) or that a current J did not
+// prevent.
 template <class Kin>
 IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, double tout,
                                        double rtol, double atol, int jac_age) {
-  const double uround = 2.220446049250313e-16, srur = 1.4901161193847656e-08;
-  const int maxstep = 10000, maxcor = 3;
-  BdfResult res{0, 0, 0, 1};
+  const double uround = 2.220446049250313e-16;
+  const int maxstep = 10000, maxcor = 3, msbp = 20;
+  const double ccmax = 0.3, ccmxj = 0.2;
+  BdfResult res{0, 0, 0, 1, 0};
   double z[6][N_], acor[N_], ewt[N_], savf[N_], yy[N_];
-  double P[N_][N_];
+  double J[N_][N_], P[N_][N_];
   int piv[N_];
+  // J is written by the first step's Jacobian before any use; cleared so
+  // that no path of the loop reads memory the function has not written
+  for (int i = 0; i < N_; i++)
+#pragma unroll
+    for (int j = 0; j < N_; j++) J[i][j] = 0.0;
 #pragma unroll
   for (int i = 0; i < N_; i++) { z[0][i] = y[i]; ewt[i] = rtol * fabs(y[i]) + atol; }
   kin.rhs(rho, y, savf);
@@ -262,10 +279,9 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
   for (int i = 0; i < N_; i++) z[1][i] = h * savf[i];
   int q = 1, ialth = 2, ncf = 0, kflag = 0;
   double t = 0.0, rc = 0.0, crate = 0.7, hl0 = 0.0, rmax = 1.0e4;
-  bool need_jac = true;
-  int nslj = 0;
-  double acorp_norm = 0.0;  // unused placeholder for clarity
-  (void)acorp_norm;
+  bool need_p = true;   // DVODE's IPUP
+  int nslp = 0, nslj = 0;
+  int icf = 0;          // 1: corrector failed with a saved J, 2: with a current J
   while (t < tout) {
     if (res.nst >= maxstep) { res.ok = 0; break; }
     // error weights at the start of the step
@@ -275,28 +291,38 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
     d_nordsieck_shift(z, q, 1.0);
     const double l0 = IZ_CBDF.el[q][0];
     hl0 = h * l0;
-    if (rc != 0.0 && fabs(hl0 / rc - 1.0) > 0.3) need_jac = true;
-    if (res.nst >= nslj + jac_age) need_jac = true;
+    const double drc = (rc != 0.0) ? fabs(hl0 / rc - 1.0) : 0.0;
+    if (drc > ccmax || res.nst >= nslp + msbp) need_p = true;
     // corrector (chord Newton on G(e) = h f(z0 + l0 e) - z1 - e)
 #pragma unroll
     for (int n = 0; n < N_; n++) { yy[n] = z[0][n]; acor[n] = 0.0; }
     kin.rhs(rho, yy, savf);
     res.nfe++;
     bool jcur = false;
-    if (need_jac) {
-      double fac = d_wrms(savf, ewt);
-      double r0 = 1000.0 * fabs(h) * uround * N_ * fac;
-      if (r0 == 0.0) r0 = 1.0;
-      kin.jac(rho, yy, savf, hl0, ewt, r0, P);
-      res.nfe += N_;
-      res.nje++;
+    if (need_p) {
+      const bool new_j = res.nst == 0 || res.nst > nslj + jac_age || (icf == 1 && drc < ccmxj) ||
+                         icf == 2;
+      if (new_j) {
+        double fac = d_wrms(savf, ewt);
+        double r0 = 1000.0 * fabs(h) * uround * N_ * fac;
+        if (r0 == 0.0) r0 = 1.0;
+        kin.jac(rho, yy, savf, -1.0, ewt, r0, J);  // J = (f(y + r e_j) - f(y)) / r
+        res.nfe += N_;
+        res.nje++;
+        nslj = res.nst;
+        jcur = true;
+      }
+      const double con = -hl0;
+#pragma unroll
+      for (int i = 0; i < N_; i++)
+#pragma unroll
+        for (int j = 0; j < N_; j++) P[i][j] = con * J[i][j];
 #pragma unroll
       for (int i = 0; i < N_; i++) P[i][i] += 1.0;
       d_lu(P, piv);
       rc = hl0;
-      nslj = res.nst;
-      need_jac = false;
-      jcur = true;
+      nslp = res.nst;
+      need_p = false;
       crate = 0.7;
     }
     bool conv = false;
@@ -306,6 +332,7 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
 #pragma unroll
       for (int n = 0; n < N_; n++) d[n] = h * savf[n] - z[1][n] - acor[n];
       d_lusolve(P, piv, d);
+      res.nsol++;
       if (rc != hl0) {  // chord correction for a stale h*l0
         double cs = 2.0 / (1.0 + hl0 / rc);
 #pragma unroll
@@ -323,10 +350,12 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
       res.nfe++;
     }
     if (!conv) {
-      // undo the prediction
+      // undo the prediction; retry with a new iteration matrix, and with a
+      // smaller step when the Jacobian was already current
       d_nordsieck_shift(z, q, -1.0);
       ncf++;
-      need_jac = true;
+      need_p = true;
+      icf = jcur ? 2 : 1;
       if (jcur || ncf > 10) {
         if (ncf > 10) { res.ok = 0; break; }
         d_nordsieck_rescale(z, q, 0.25);
@@ -335,6 +364,7 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
       }
       continue;
     }
+    icf = 0;
     // local error test
     double dsm = d_wrms(acor, ewt) / IZ_CBDF.tesco[q][1];
 #ifdef IZ_BDF_TRACE
@@ -355,7 +385,7 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
         for (int n = 0; n < N_; n++) z[1][n] = h * savf[n];
         q = 1;
         ialth = 5;
-        need_jac = true;
+        need_p = true;
         if (kflag <= -10) { res.ok = 0; break; }
         continue;
       }
@@ -443,31 +473,31 @@ struct KinGen {
     gen_kT(y[0], kt);
     gen_rhs(rho, y, kt, f);
   }
-  IZ_MHD inline void jac(double rho, double *y, const double *f0, double hl0, const double *ewt,
-                        double r0, double (*P)[N_]) const {
-    double kt[GEN_NKT], ftmp[N_];
+  // the species columns share kt (T is not perturbed).  Each column
+  // perturbs a local copy of the state: perturbing the caller's vector in
+  // place and restoring it gave wrong Jacobians in a GPU build of the
+  // unrolled loop (correct on the CPU), so the state is never written here
+  IZ_MHD inline void jac(double rho, const double *y, const double *f0, double hl0,
+                         const double *ewt, double r0, double (*P)[N_]) const {
+    double kt[GEN_NKT];
     gen_kT(y[0], kt);
+    for (int j = 0; j < N_; j++) {
+      double yl[N_], ftmp[N_];
 #pragma unroll
-    for (int j = 1; j < N_; j++) {
-      double yj = y[j];
-      double r = d_fd_increment(j, yj, ewt[j], r0);
-      y[j] = yj + r;
-      gen_rhs(rho, y, kt, ftmp);
-      y[j] = yj;
-      double ri = 1.0 / r;
+      for (int n = 0; n < N_; n++) yl[n] = y[n];
+      const double yj = yl[j];
+      const double r = d_fd_increment(j, yj, ewt[j], r0);
+      yl[j] = yj + r;
+      if (j == 0) {
+        double kt0[GEN_NKT];
+        gen_kT(yl[0], kt0);
+        gen_rhs(rho, yl, kt0, ftmp);
+      } else {
+        gen_rhs(rho, yl, kt, ftmp);
+      }
+      const double ri = 1.0 / r;
 #pragma unroll
       for (int i = 0; i < N_; i++) P[i][j] = -hl0 * (ftmp[i] - f0[i]) * ri;
-    }
-    {
-      double yj = y[0];
-      double r = d_fd_increment(0, yj, ewt[0], r0);
-      y[0] = yj + r;
-      gen_kT(y[0], kt);
-      gen_rhs(rho, y, kt, ftmp);
-      y[0] = yj;
-      double ri = 1.0 / r;
-#pragma unroll
-      for (int i = 0; i < N_; i++) P[i][0] = -hl0 * (ftmp[i] - f0[i]) * ri;
     }
   }
 };

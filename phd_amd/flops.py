@@ -1,0 +1,99 @@
+"""Algorithmic operation counts of the hot kernels (for the roofline).
+
+Counts are of the arithmetic the algorithm performs, not of the machine
+instructions: +, -, *, / are one FP64 operation each; a transcendental
+(exp, log, log10, pow, sqrt) counts as one operation too (its polynomial
+evaluation inside the math library is implementation).  Comparisons,
+fmin/fmax/fabs and selects are not counted.
+
+Chemistry: the work of a batch follows from the integrator's counters
+(izem_chem_stats: BDF steps, RHS evaluations, Jacobians, LU solves) and the
+per-call counts of the straight-line kinetics read from the generated
+source (phd_amd/csrc/mech_gen.h); both kinetics forms do the same
+arithmetic in a different order.
+"""
+from __future__ import annotations
+
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NEQ = 10  # T + 9 species
+
+_TOKEN = re.compile(r"\d+\.\d*(?:[eE][+-]?\d+)?|\d+(?:[eE][+-]?\d+)|\d+|[A-Za-z_]\w*|"
+                    r"\+=|-=|\*=|/=|[-+*/()\[\],;=?:<>!&|{}]")
+_TRANS = {"exp", "log", "log10", "pow", "sqrt"}
+
+
+def count_ops(src: str) -> tuple[int, int]:
+    """(arithmetic operations, transcendental calls) in a C function body."""
+    arith = trans = 0
+    prev = None
+    for tok in _TOKEN.findall(src):
+        if tok in ("+=", "-=", "*=", "/="):
+            arith += 1
+        elif tok in ("*", "/"):
+            arith += 1
+        elif tok in ("+", "-"):
+            # binary when it follows an operand
+            if prev is not None and (prev[0].isalnum() or prev[0] == "_" or prev in (")", "]")):
+                arith += 1
+        elif tok in _TRANS:
+            trans += 1
+        prev = tok
+    return arith, trans
+
+
+def _function_body(src: str, name: str) -> str:
+    i = src.index(name + "(")
+    i = src.index("{", i)
+    depth, j = 0, i
+    while True:
+        if src[j] == "{":
+            depth += 1
+        elif src[j] == "}":
+            depth -= 1
+            if depth == 0:
+                return src[i:j + 1]
+        j += 1
+
+
+def kinetics_counts(path: str | None = None) -> dict:
+    """Per-call operation counts of the temperature part (gen_kT) and the
+    concentration part plus reactor RHS (gen_rhs) of the kinetics."""
+    path = path or os.path.join(HERE, "csrc", "mech_gen.h")
+    src = open(path).read()
+    kt = count_ops(_function_body(src, "gen_kT"))
+    rhs = count_ops(_function_body(src, "gen_rhs"))
+    return {"kT": kt[0] + kt[1], "conc": rhs[0] + rhs[1], "kT_trans": kt[1],
+            "conc_trans": rhs[1]}
+
+
+def lu_ops(n: int = NEQ) -> int:
+    """LU with partial pivoting of an n x n matrix: per column one reciprocal,
+    per row below one multiplier and 2 operations per updated entry."""
+    return sum(1 + m * (1 + 2 * m) for m in range(n))
+
+
+def solve_ops(n: int = NEQ) -> int:
+    """Forward and back substitution."""
+    return n * (n - 1) + n * (n - 1) + n
+
+
+def chem_flops(stats, counts: dict | None = None) -> float:
+    """Operations of a batch of integrations from its counters
+    (steps, RHS evaluations, Jacobians, LU solves).  A Jacobian is one
+    temperature part and one full RHS per perturbed column, the species
+    columns sharing the temperature part, plus the difference quotients
+    (3 per entry, 6 per column for the increment) and the diagonal."""
+    nst, nfe, nje, nsol = [float(x) for x in stats]
+    c = counts or kinetics_counts()
+    rhs = c["kT"] + c["conc"]
+    jac = 2 * c["kT"] + NEQ * c["conc"] + NEQ * (3 * NEQ + 6) + NEQ
+    plain_rhs = nfe - NEQ * nje  # evaluations outside the Jacobians
+    return plain_rhs * rhs + nje * (jac + lu_ops()) + nsol * solve_ops()
+
+
+if __name__ == "__main__":
+    c = kinetics_counts()
+    print(c, "LU", lu_ops(), "solve", solve_ops())
