@@ -25,9 +25,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "cell-steps/s 256^3 reacting H2 (FP64); chem cells/s surrogate vs VODE @|dT|<=2K"
-# algorithmic FP64 operations of one Euler face with the JS reconstruction
-# (count per section in DESIGN.md)
-EULER_FLOP_PER_FACE = 8461
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -150,7 +147,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported "
-              f"(the multi-GPU decomposition is described in DESIGN.md, not run)",
+              f"(one process drives one GPU; a multi-GPU decomposition is not built)",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
@@ -158,6 +155,7 @@ def main():
 
     from phd_amd import chemdata as cd
     from phd_amd import ics
+    from phd_amd.flops import EULER_FLOP_PER_FACE
     from phd_amd.solver import Options, Solver, profile_read, profile_start
 
     n = args.n

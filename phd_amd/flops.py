@@ -69,6 +69,33 @@ def kinetics_counts(path: str | None = None) -> dict:
             "conc_trans": rhs[1]}
 
 
+# Operations of one Euler face flux (euler_LLF_x/_y/_z, module_weno.f90:
+# 537-1044) with the JS reconstruction, counted from the reference's
+# expressions per section (the per-stencil presolve is counted per face,
+# as the reference evaluates it):
+EULER_FACE_SECTIONS = {
+    # rho^-1, u_n, P, gamma, c and the |u_n -+ c| of 6 stencil points
+    "stencil presolve (6 x 13)": 78,
+    # Roe weights, u, Y_k, h_t, e_t averages of i_l and i_r
+    "Roe average": 124,
+    # W^-1, T, c_p (NASA-7, 9 species), gamma, c, h_k(T_c), psi_k
+    "EOS-consistent average state": 408,
+    # 1/c, c^2, |u|^2/2, b1, b2, xi
+    "eigenvector scalars": 14,
+    # rows of el: 20 + 22 + 20 + 14 + 9 x 19
+    "left eigenvector rows": 247,
+    # w = el.U and g = el.F with the flux at 6 points for 12 dense rows
+    # (73 per point) and 2 two-entry rows (9.5 per point)
+    "characteristic projections": 12 * 438 + 2 * 57,
+    # two JS reconstructions (betas 34, weights 14, candidates 15,
+    # combination 5) and g^ = g- + g+ for 14 fields
+    "WENO5-JS reconstructions": 14 * 137,
+    # F^ = er . g^ with er's nonzero pattern
+    "back projection": 97,
+}
+EULER_FLOP_PER_FACE = sum(EULER_FACE_SECTIONS.values())
+
+
 def lu_ops(n: int = NEQ) -> int:
     """LU with partial pivoting of an n x n matrix: per column one reciprocal,
     per row below one multiplier and 2 operations per updated entry."""
@@ -96,4 +123,4 @@ def chem_flops(stats, counts: dict | None = None) -> float:
 
 if __name__ == "__main__":
     c = kinetics_counts()
-    print(c, "LU", lu_ops(), "solve", solve_ops())
+    print(c, "LU", lu_ops(), "solve", solve_ops(), "Euler face", EULER_FLOP_PER_FACE)
