@@ -133,6 +133,37 @@ def chem_vode_rate(s, thd, ncell: int):
             "bdf_steps_mean": float(st.mean()), "bdf_steps_max": int(st.max()), "rc": rc}
 
 
+def surrogate_rate(s, adi, n: int, dt_half: float):
+    """Train the chemistry surrogate on chemistry trajectories of cells of
+    the benchmark state (labelled by the BDF path), calibrate its indicator
+    on other cells, and time surrogate + fallback against the BDF path on a
+    third sample at the step's half dt."""
+    from phd_amd import autotune
+    from phd_amd import surrogate as su
+
+    t0 = time.perf_counter()
+    dts = [0.5 * dt_half, dt_half, 2.0 * dt_half] * 2
+    train = []
+    for seed in (11, 12):
+        Ts, rs, Ys = autotune.sample_reacting_cells(s, n, seed=seed)
+        train += su.trajectory(s, Ts.cpu().numpy(), rs.cpu().numpy(), Ys.cpu().numpy(), dts)
+    sur = su.Surrogate(s, seed=0)
+    sur.set_normalisation(train)
+    hist = sur.fit(train, epochs=8, batch=65536, lr=1e-3)
+    Tc, rc, Yc = autotune.sample_reacting_cells(s, n // 2, seed=13)
+    cal = sur.calibrate(su.concat(su.trajectory(s, Tc.cpu().numpy(), rc.cpu().numpy(),
+                                                Yc.cpu().numpy(), [dt_half] * 3)))
+    t_train = time.perf_counter() - t0
+    Tt, rt, Yt = autotune.sample_reacting_cells(s, 2 * n, seed=14)
+    L = su.label_cells(s, Tt.cpu().numpy(), rt.cpu().numpy(), Yt.cpu().numpy(), dt_half)
+    res = su.evaluate(sur, L)
+    res.update({"train_samples": sum(p.T.numel() for p in train), "train_epochs": 8,
+                "final_loss": hist[-1], "train_and_calibrate_s": t_train,
+                "tol_T_K": cal["tol_T"], "dT_max_K": cal["dT_max"],
+                "cells": "cells of the 256^3 reacting state, Strang half step"})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +172,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--chem-cells", type=int, default=1 << 22)
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--chem-tune", type=str, default=None,
                     help="waves,blocks_per_cu,jac_age,kinetics for the chemistry kernels "
                          "(default: autotuned on a sample of the reacting cells)")
@@ -237,6 +269,13 @@ def main():
     dominant = chem_roof if (chem_roof and c_ms > e_ms) else euler_roof
     log(f"{ms_step:.1f} ms/step; kernel time per step: " +
         ", ".join(f"{k} {v[0] / args.steps:.1f} ms" for k, v in prof.items()))
+    surr = None
+    if s.opts.reaction and not args.no_surrogate:
+        log("chemistry surrogate: train, calibrate, time against the BDF path")
+        surr = surrogate_rate(s, adi, 1 << 19, 0.5 * float(ts[4]) * adi.time_ref)
+        log(f"surrogate {surr['surrogate_cells_per_s']:.3e} cells/s vs BDF "
+            f"{surr['vode_cells_per_s']:.3e} (x{surr['speedup']:.2f}), fallback "
+            f"{100 * surr['fallback_rate']:.2f} %, max |dT| {surr['max_abs_dT_K']:.2e} K")
     chem = None
     if args.chem_cells > 0:
         log(f"batched VODE-path chemistry on {args.chem_cells} independent cells")
@@ -265,6 +304,7 @@ def main():
                          "jac_age": tuned[2], "kinetics": "straight-line" if tuned[3] else
                          "tables", "sample_cells_per_s": tuned[4]}),
         "chemistry": chem,
+        "chem_surrogate": surr,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

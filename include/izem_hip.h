@@ -182,6 +182,59 @@ int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho, double *Y
 int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale, double *T, double *W_i,
                         double *cp, double *ha, double *v);
 
+/* ---- chemistry surrogate (no reference counterpart) -------------------
+ * A BF16 MLP (12 -> 256 -> 256 -> 256 -> 10, GELU) stands in for the
+ * constant-volume reactor substep of izem_psr_batch on the cells its error
+ * indicator accepts; the others fall back to izem_psr_batch.  BF16 operands
+ * are raw 16-bit arrays.  Matrices are row-major with leading dimensions in
+ * elements. */
+/* C[M][N] = A[M][K] . B[K][N] + epilogue on the matrix cores:
+ * ta = 0: A row-major, 1: A stored [K][M]; tb = 0: B stored [N][K],
+ * 1: B stored [K][N]; epi 0: C = AB (fp32 Cf), 1: Z = AB + bias (fp32 Cf)
+ * and gelu(Z) (bf16 Cb), 2: Cf = AB + bias, 3: Cb = AB * gelu'(aux),
+ * 5: Cb = gelu(AB + bias) only (inference) */
+int izem_mlp_gemm(izem_ctx *ctx, int ta, int tb, int epi, int M, int N, int K, const void *A,
+                  long lda, const void *B, long ldb, const float *bias, const float *aux,
+                  long ldaux, float *Cf, long ldcf, void *Cb, long ldcb);
+/* C[M][N] = A . B with the K range split over nslices blocks (weight
+ * gradients, K = batch); work holds nslices * M * N floats */
+int izem_mlp_gemm_splitk(izem_ctx *ctx, int ta, int tb, int M, int N, int K, const void *A,
+                         long lda, const void *B, long ldb, int nslices, float *work, float *C);
+/* out[n] = sum over rows of the bf16 matrix G[M][N] (bias gradients);
+ * work holds nslices * N floats */
+int izem_mlp_colsum(izem_ctx *ctx, int M, int N, const void *G, long ld, int nslices,
+                    float *work, float *out);
+/* Adam step `step` (>= 1) on the fp32 master matrix P[rows][cols]; writes
+ * bf16 copies P^T (ldt) and P (ldw) where given (NULL skips) */
+int izem_mlp_adam(izem_ctx *ctx, int rows, int cols, float *P, const float *G, float *m,
+                  float *v, float lr, float b1, float b2, float eps, int step, void *Pt,
+                  long ldt, void *Pw, long ldw);
+int izem_mlp_to_bf16(izem_ctx *ctx, int rows, int cols, const float *P, void *Pt, long ldt,
+                     void *Pw, long ldw);
+/* mean-square loss gradient dY = 2 (pred - tgt) / (Bn nout), bf16, padded
+ * columns zero; *loss (device) accumulates the sum of squares */
+int izem_mlp_mse(izem_ctx *ctx, int Bn, int nout, int npad, const float *pred, long ldp,
+                 const float *tgt, long ldt, void *dY, long ldd, float *loss);
+/* normalised bf16 features (ln p, T, Y_k, ln dt) of n cells, rows of 32 */
+int izem_surr_features(izem_ctx *ctx, int n, const double *T, const double *rho, const double *Y,
+                       double dt, const float *mean, const float *istd, void *X);
+/* apply the network output pred[n][ldp] (dT, dY_k as standardised
+ * asinh(change / oc): change = oc sinh(pred ostd + omean)): project dY onto
+ * element conservation (proj: 9 x 9), T += dT; flag[c] = 1 (cell
+ * untouched) when the indicator rejects it:
+ * a feature beyond feat_lim, a mass fraction below -1e-7, |T(energy of the
+ * projected Y) - (T0 + dT)| > tol_T or |dT| > dT_max; T_energy (may be
+ * NULL) receives that energy temperature */
+int izem_surr_apply(izem_ctx *ctx, int n, double *T, double *Y, const float *pred, long ldp,
+                    const float *omean, const float *ostd, const float *oc, const double *proj,
+                    const void *X,
+                    float feat_lim, double tol_T, double dT_max, int *flag, double *T_energy);
+/* the flagged cells through izem_psr_batch (device work arrays: counter[1],
+ * idx[n], Tw[n], rw[n], Yw[9 n]); *nfall (host) receives their number */
+int izem_surr_fallback(izem_ctx *ctx, int n, double *T, const double *rho, double *Y, double dt,
+                       const int *flag, int *counter, int *idx, double *Tw, double *rw,
+                       double *Yw, int *nfall);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,7 +6,8 @@ kernels are tolerance-checked, but their BDF step control was seen to
 depend on which multiply-adds the compiler fused (with fusion, one build of
 the straight-line kinetics took twice DVODE's steps on the same cells);
 without fusion a GPU build follows the same trajectories as a CPU build of
-the same source.
+the same source.  The surrogate's BF16 matrix-core kernels (mlp.hip) have
+no parity constraint and keep contraction.
 """
 from __future__ import annotations
 
@@ -26,6 +27,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall",
 SOURCES = {
     "flow.hip": ["-ffp-contract=off"],
     "chem.hip": ["-ffp-contract=off"],
+    "mlp.hip": ["-ffp-contract=fast"],
 }
 
 

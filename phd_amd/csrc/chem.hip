@@ -386,3 +386,204 @@ extern "C" int izem_chem_stats(izem_ctx *ctx, unsigned long long *out) {
   IZ_CHECK(hipStreamSynchronize(ctx->stream));
   return IZEM_OK;
 }
+
+// ===================================================================
+// Chemistry surrogate: features, conservation projection, energy-
+// consistent temperature, error indicator and the VODE fallback.  No
+// counterpart in the reference; the fallback is izem_psr_batch.
+// ===================================================================
+#define IZ_SURR_NF 12   // features: ln p, T, Y_k (9), ln dt
+#define IZ_SURR_XLD 32  // padded feature row (BF16)
+#define IZ_SURR_NO 10   // outputs: dT, dY_k (9)
+
+__device__ inline unsigned short surr_f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// mixture internal energy [erg/g] and c_v [erg/(g K)] from the mechanism's
+// NASA-7 tables (CKUMS/CKCVBS forms, chemkin/vode.f:64, 228)
+__device__ inline double d_u_mix(double T, const double *Y, double &cv) {
+  const double ru = 8.314510e7;  // erg/(mol K), as CKINIT
+  double u = 0.0;
+  cv = 0.0;
+  for (int k = 0; k < CK_NEQ - 1; k++) {
+    const double *a = IZ_CMECH.a[k][(T > IZ_CMECH.tmid[k]) ? 1 : 0];
+    const double w = IZ_CMECH.wt[k];
+    const double h = ru * T * (a[0] + a[1] * T / 2.0 + a[2] * T * T / 3.0 +
+                               a[3] * T * T * T / 4.0 + a[4] * T * T * T * T / 5.0 + a[5] / T) / w;
+    const double cp = ru * (a[0] + a[1] * T + a[2] * T * T + a[3] * T * T * T +
+                            a[4] * T * T * T * T) / w;
+    u += Y[k] * (h - ru * T / w);
+    cv += Y[k] * (cp - ru / w);
+  }
+  return u;
+}
+
+__global__ void k_surr_features(int n, const double *__restrict__ T, const double *__restrict__ rho,
+                                const double *__restrict__ Y, double lndt,
+                                const float *__restrict__ mean, const float *__restrict__ istd,
+                                unsigned short *__restrict__ X) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double wi = 0.0;
+  float f[IZ_SURR_NF];
+  for (int k = 0; k < CK_NEQ - 1; k++) {
+    const double y = Y[(long)c * (CK_NEQ - 1) + k];
+    wi += y / IZ_CMECH.wt[k];
+    f[2 + k] = (float)y;
+  }
+  const double p = rho[c] * 8.314510e7 * T[c] * wi;  // dyn/cm^2
+  f[0] = (float)log(p);
+  f[1] = (float)T[c];
+  f[11] = (float)lndt;
+  unsigned short *row = X + (long)c * IZ_SURR_XLD;
+  for (int j = 0; j < IZ_SURR_NF; j++) row[j] = surr_f2bf((f[j] - mean[j]) * istd[j]);
+  for (int j = IZ_SURR_NF; j < IZ_SURR_XLD; j++) row[j] = 0;
+}
+
+// accept or reject the network's substep for each cell.  pred holds the
+// scaled outputs (dT, dY_k); proj is the 9 x 9 projector onto element
+// (hence mass) conservation; on acceptance Y += proj.dY and T += dT.  The
+// temperature that keeps the mixture internal energy of the constant-
+// volume reactor with the projected Y is the indicator's check: the
+// network's T and the one its species imply must agree within tol_T.
+// flag: 0 accepted, 1 fallback (T, Y untouched).
+__global__ void k_surr_apply(int n, double *__restrict__ T, double *__restrict__ Y,
+                             const float *__restrict__ pred, long ldp,
+                             const float *__restrict__ omean, const float *__restrict__ ostd,
+                             const float *__restrict__ oc,
+                             const double *__restrict__ proj, const unsigned short *__restrict__ X,
+                             float feat_lim, double tol_T, double dT_max,
+                             int *__restrict__ flag, double *__restrict__ T_energy) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int K = CK_NEQ - 1;
+  double y0[CK_NEQ - 1], dy[CK_NEQ - 1], y1[CK_NEQ - 1];
+  for (int k = 0; k < K; k++) y0[k] = Y[(long)c * K + k];
+  const float *pr = pred + (long)c * ldp;
+  // outputs are standardised asinh(change / oc): the transform resolves
+  // small and large changes with the same relative accuracy
+  const double dT = oc[0] * sinh((double)(pr[0] * ostd[0] + omean[0]));
+  for (int k = 0; k < K; k++)
+    dy[k] = oc[1 + k] * sinh((double)(pr[1 + k] * ostd[1 + k] + omean[1 + k]));
+  int bad = 0;
+  // inputs outside the range the network was trained on
+  for (int j = 0; j < IZ_SURR_NF; j++) {
+    const float x = __uint_as_float(((unsigned)X[(long)c * IZ_SURR_XLD + j]) << 16);
+    if (!(fabsf(x) <= feat_lim)) bad = 1;
+  }
+  double ymin = 1.0;
+  for (int i = 0; i < K; i++) {
+    double s = 0.0;
+    for (int k = 0; k < K; k++) s += proj[i * K + k] * dy[k];
+    y1[i] = y0[i] + s;
+    ymin = fmin(ymin, y1[i]);
+  }
+  // a projected mass fraction below -1e-7 rejects the cell; smaller
+  // undershoots (absent radicals) are kept, which keeps the element
+  // masses exact
+  if (!(ymin >= -1.0e-7)) bad = 1;
+  // energy-consistent temperature (Newton on u(T, Y1) = u(T0, Y0))
+  const double T0 = T[c];
+  double cv;
+  const double u0 = d_u_mix(T0, y0, cv);
+  double Te = T0 + dT;
+  for (int it = 0; it < 8; it++) {
+    const double u = d_u_mix(Te, y1, cv);
+    const double d = (u - u0) / cv;
+    Te -= d;
+    if (fabs(d) < 1.0e-10 * Te) break;
+  }
+  if (!(fabs(Te - (T0 + dT)) <= tol_T) || !(fabs(dT) <= dT_max) || !(Te > 0.0)) bad = 1;
+  if (T_energy) T_energy[c] = Te;
+  flag[c] = bad;
+  if (!bad) {
+    T[c] = T0 + dT;
+    for (int k = 0; k < K; k++) Y[(long)c * K + k] = y1[k];
+  }
+}
+
+// pack the flagged cells (order not kept) and return their indices
+__global__ void k_surr_gather(int n, const int *__restrict__ flag, const double *__restrict__ T,
+                              const double *__restrict__ rho, const double *__restrict__ Y,
+                              int *__restrict__ count, int *__restrict__ idx,
+                              double *__restrict__ Tw, double *__restrict__ rw,
+                              double *__restrict__ Yw) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n || !flag[c]) return;
+  const int j = atomicAdd(count, 1);
+  idx[j] = c;
+  Tw[j] = T[c];
+  rw[j] = rho[c];
+  for (int k = 0; k < CK_NEQ - 1; k++) Yw[(long)j * (CK_NEQ - 1) + k] = Y[(long)c * (CK_NEQ - 1) + k];
+}
+
+__global__ void k_surr_scatter(int m, const int *__restrict__ idx, const double *__restrict__ Tw,
+                               const double *__restrict__ Yw, double *__restrict__ T,
+                               double *__restrict__ Y) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const int c = idx[j];
+  T[c] = Tw[j];
+  for (int k = 0; k < CK_NEQ - 1; k++) Y[(long)c * (CK_NEQ - 1) + k] = Yw[(long)j * (CK_NEQ - 1) + k];
+}
+
+__global__ void k_zero_int(int *p) { *p = 0; }
+
+extern "C" int izem_surr_features(izem_ctx *ctx, int n, const double *T, const double *rho,
+                                  const double *Y, double dt, const float *mean,
+                                  const float *istd, void *X) {
+  if (!ctx || !ctx->chem || n < 0 || !T || !rho || !Y || !(dt > 0) || !mean || !istd || !X)
+    return IZEM_E_ARG;
+  if (n == 0) return IZEM_OK;
+  hipLaunchKernelGGL(k_surr_features, dim3(iz_blocks(n, 256)), dim3(256), 0, ctx->stream, n, T,
+                     rho, Y, log(dt), mean, istd, (unsigned short *)X);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+extern "C" int izem_surr_apply(izem_ctx *ctx, int n, double *T, double *Y, const float *pred,
+                               long ldp, const float *omean, const float *ostd, const float *oc,
+                               const double *proj, const void *X, float feat_lim, double tol_T,
+                               double dT_max, int *flag, double *T_energy) {
+  if (!ctx || !ctx->chem || n < 0 || !T || !Y || !pred || !omean || !ostd || !oc || !proj ||
+      !X || !flag)
+    return IZEM_E_ARG;
+  if (n == 0) return IZEM_OK;
+  int pk = iz_prof_begin(ctx, IZP_CHEM);
+  hipLaunchKernelGGL(k_surr_apply, dim3(iz_blocks(n, 128)), dim3(128), 0, ctx->stream, n, T, Y,
+                     pred, ldp, omean, ostd, oc, proj, (const unsigned short *)X, feat_lim, tol_T,
+                     dT_max, flag, T_energy);
+  iz_prof_end(ctx, pk);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+// integrate the flagged cells with the BDF path: gather into the caller's
+// work arrays (idx[n], Tw[n], rw[n], Yw[9 n], counter: 1 int), run
+// izem_psr_batch on them, scatter back; *nfall receives their number
+extern "C" int izem_surr_fallback(izem_ctx *ctx, int n, double *T, const double *rho,
+                                  double *Y, double dt, const int *flag, int *counter, int *idx,
+                                  double *Tw, double *rw, double *Yw, int *nfall) {
+  if (!ctx || !ctx->chem || n < 0 || !T || !rho || !Y || !flag || !counter || !idx || !Tw ||
+      !rw || !Yw || !nfall || !(dt > 0))
+    return IZEM_E_ARG;
+  *nfall = 0;
+  if (n == 0) return IZEM_OK;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, ctx->stream, counter);
+  hipLaunchKernelGGL(k_surr_gather, dim3(iz_blocks(n, 256)), dim3(256), 0, ctx->stream, n, flag,
+                     T, rho, Y, counter, idx, Tw, rw, Yw);
+  int m = 0;
+  IZ_CHECK(hipMemcpyAsync(&m, counter, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  IZ_CHECK(hipStreamSynchronize(ctx->stream));
+  *nfall = m;
+  if (m == 0) return IZEM_OK;
+  int r = izem_psr_batch(ctx, m, Tw, rw, Yw, dt, nullptr);
+  if (r != IZEM_OK) return r;
+  hipLaunchKernelGGL(k_surr_scatter, dim3(iz_blocks(m, 256)), dim3(256), 0, ctx->stream, m, idx,
+                     Tw, Yw, T, Y);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}

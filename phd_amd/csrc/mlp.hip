@@ -1,0 +1,349 @@
+// mlp.hip -- BF16 matrix-core kernels of the chemistry surrogate MLP
+// (12 -> 256 -> 256 -> 256 -> 10, GELU) for CDNA4 (gfx950): forward with
+// fused bias + GELU, backward (data and weight gradients, bias sums),
+// loss gradient and Adam.  Operands are BF16 (raw 16-bit storage),
+// accumulation FP32 on v_mfma_f32_16x16x32_bf16.  The surrogate has no
+// counterpart in the reference; it replaces PSR_CHEMKIN/DVODE substeps
+// (chemkin/psr_chemkin.f:1) for the cells its error indicator accepts.
+#include "izem_ctx.h"
+
+#include <stdint.h>
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16;
+
+#define MT 64   // tile rows (M)
+#define NT 64   // tile columns (N)
+#define KT 32   // tile depth (K), one MFMA 16x16x32 per wave sub-tile
+#define LDT 40  // LDS row stride in bf16 (80 B: 16-byte aligned rows)
+
+enum {
+  EPI_F32 = 0,
+  EPI_BIAS_GELU = 1,
+  EPI_BIAS_F32 = 2,
+  EPI_GELU_BWD = 3,
+  EPI_GELU_ONLY = 5,  // inference: gelu(AB + bias) as bf16, no pre-activation kept
+  EPI_PARTIAL = 4
+};
+
+__device__ inline float bf2f(u16 h) { return __uint_as_float(((unsigned)h) << 16); }
+__device__ inline u16 f2bf(float f) {
+  // round to nearest even (finite inputs)
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (u16)(u >> 16);
+}
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ inline float gelu_d(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
+         z * 0.39894228040143268f * __expf(-0.5f * z * z);
+}
+
+// Stage a 64 x 32 tile of the logical matrix X[R][Kd] into lds[r][k]:
+// TR = 0: X stored row-major X[r*ld + k] (k contiguous);
+// TR = 1: X stored transposed X[k*ld + r] (r contiguous).
+// Out-of-range elements are zero.
+template <int TR>
+__device__ inline void stage_tile(u16 (*lds)[LDT], const u16 *__restrict__ X, long ld, int R,
+                                  int Kd, int r0, int k0) {
+  const int t = threadIdx.x;
+  if (TR == 0) {
+    const int r = t >> 2, kc = (t & 3) * 8;
+    const int gr = r0 + r, gk = k0 + kc;
+    u16 v[8];
+    if (gr < R && gk + 7 < Kd && (ld & 7) == 0 && (((uintptr_t)X) & 15) == 0) {
+      const uint4 q = *(const uint4 *)(X + (long)gr * ld + gk);
+      const u16 *p = (const u16 *)&q;
+#pragma unroll
+      for (int i = 0; i < 8; i++) v[i] = p[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; i++) v[i] = (gr < R && gk + i < Kd) ? X[(long)gr * ld + gk + i] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) lds[r][kc + i] = v[i];
+  } else {
+    const int k = t >> 3, rc = (t & 7) * 8;
+    const int gk = k0 + k, gr = r0 + rc;
+    u16 v[8];
+    if (gk < Kd && gr + 7 < R && (ld & 7) == 0 && (((uintptr_t)X) & 15) == 0) {
+      const uint4 q = *(const uint4 *)(X + (long)gk * ld + gr);
+      const u16 *p = (const u16 *)&q;
+#pragma unroll
+      for (int i = 0; i < 8; i++) v[i] = p[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; i++) v[i] = (gk < Kd && gr + i < R) ? X[(long)gk * ld + gr + i] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) lds[rc + i][k] = v[i];
+  }
+}
+
+// C[M][N] = A[M][K] . B[K][N] (+ epilogue).  A: TA = 0 row-major, 1 stored
+// as [K][M]; B: TB = 0 stored as [N][K] (weights transposed), 1 stored as
+// [K][N].  Block: 256 threads = 4 waves on a 64 x 64 tile, each wave a
+// 32 x 32 quarter as 2 x 2 MFMA 16x16x32 accumulators.  Split-K: blockIdx.z
+// takes K range [z * kslice, (z + 1) * kslice) and (EPI_PARTIAL) writes its
+// partial product to Cf + z * M * ldcf.
+template <int TA, int TB, int EPI>
+__global__ void __launch_bounds__(256)
+k_gemm(int M, int N, int K, const u16 *__restrict__ A, long lda, const u16 *__restrict__ B,
+       long ldb, const float *__restrict__ bias, const float *__restrict__ aux, long ldaux,
+       float *__restrict__ Cf, long ldcf, u16 *__restrict__ Cb, long ldcb, int kslice) {
+  __shared__ __attribute__((aligned(16))) u16 As[MT][LDT];
+  __shared__ __attribute__((aligned(16))) u16 Bs[NT][LDT];
+  const int m0 = blockIdx.y * MT, n0 = blockIdx.x * NT;
+  const int kb = blockIdx.z * kslice, ke = min(K, kb + kslice);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kb; k0 < ke; k0 += KT) {
+    stage_tile<TA>(As, A, lda, M, ke, m0, k0);
+    stage_tile<TB == 0 ? 0 : 1>(Bs, B, ldb, N, ke, n0, k0);
+    __syncthreads();
+    bf16x8 af[2], bfr[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const uint4 q = *(const uint4 *)&As[wm * 32 + i * 16 + (lane & 15)][(lane >> 4) * 8];
+      af[i] = __builtin_bit_cast(bf16x8, q);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const uint4 q = *(const uint4 *)&Bs[wn * 32 + j * 16 + (lane & 15)][(lane >> 4) * 8];
+      bfr[j] = __builtin_bit_cast(bf16x8, q);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  float *Cz = Cf;
+  if (EPI == EPI_PARTIAL) Cz = Cf + (long)blockIdx.z * M * ldcf;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + r;
+        const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (m >= M || n >= N) continue;
+        const float a = acc[i][j][r];
+        if (EPI == EPI_F32 || EPI == EPI_PARTIAL) {
+          Cz[(long)m * ldcf + n] = a;
+        } else if (EPI == EPI_BIAS_GELU) {
+          const float z = a + bias[n];
+          Cf[(long)m * ldcf + n] = z;
+          Cb[(long)m * ldcb + n] = f2bf(gelu(z));
+        } else if (EPI == EPI_BIAS_F32) {
+          Cf[(long)m * ldcf + n] = a + bias[n];
+        } else if (EPI == EPI_GELU_ONLY) {
+          Cb[(long)m * ldcb + n] = f2bf(gelu(a + bias[n]));
+        } else {  // EPI_GELU_BWD
+          Cb[(long)m * ldcb + n] = f2bf(a * gelu_d(aux[(long)m * ldaux + n]));
+        }
+      }
+}
+
+// out[i] = sum over s of part[s * n + i] (fixed order: deterministic)
+__global__ void k_sum_slices(long n, int ns, const float *__restrict__ part,
+                             float *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int z = 0; z < ns; z++) s += part[(long)z * n + i];
+  out[i] = s;
+}
+
+// column sums of a BF16 matrix G[M][N] over row slices: part[s][n]
+__global__ void k_colsum(int M, int N, const u16 *__restrict__ G, long ld, int rows,
+                         float *__restrict__ part) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  if (n >= N) return;
+  const int r0 = s * rows, r1 = min(M, r0 + rows);
+  float a = 0.f;
+  for (int r = r0; r < r1; r++) a += bf2f(G[(long)r * ld + n]);
+  part[(long)s * N + n] = a;
+}
+
+// Adam (Kingma & Ba 2015) on an FP32 master matrix P[rows][cols] (row-major)
+// with gradient G, moments m, v; writes BF16 copies P^T (ldt) and P (ldw)
+// when given.  Step t >= 1.
+__global__ void k_adam(int rows, int cols, float *__restrict__ P, const float *__restrict__ G,
+                       float *__restrict__ m, float *__restrict__ v, float lr, float b1, float b2,
+                       float eps, float c1, float c2, u16 *__restrict__ Pt, long ldt,
+                       u16 *__restrict__ Pw, long ldw) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)rows * cols) return;
+  const float g = G[i];
+  const float mi = b1 * m[i] + (1.f - b1) * g;
+  const float vi = b2 * v[i] + (1.f - b2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  const float p = P[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+  P[i] = p;
+  const int r = (int)(i / cols), c = (int)(i % cols);
+  if (Pt) Pt[(long)c * ldt + r] = f2bf(p);
+  if (Pw) Pw[(long)r * ldw + c] = f2bf(p);
+}
+
+// write BF16 copies of an FP32 matrix (initialisation)
+__global__ void k_to_bf16(int rows, int cols, const float *__restrict__ P, u16 *__restrict__ Pt,
+                          long ldt, u16 *__restrict__ Pw, long ldw) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)rows * cols) return;
+  const int r = (int)(i / cols), c = (int)(i % cols);
+  if (Pt) Pt[(long)c * ldt + r] = f2bf(P[i]);
+  if (Pw) Pw[(long)r * ldw + c] = f2bf(P[i]);
+}
+
+// mean-square loss over nout of the ldp output columns: dY = 2 (pred -
+// target) / (B nout) as BF16 (padding columns zero); loss sum of squares
+// accumulated into *loss (one atomic per block)
+__global__ void k_mse(int Bn, int nout, int npad, const float *__restrict__ pred, long ldp,
+                      const float *__restrict__ tgt, long ldt, float scale,
+                      u16 *__restrict__ dY, long ldd, float *__restrict__ loss) {
+  __shared__ float red[4];
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  float sq = 0.f;
+  if (i < (long)Bn * npad) {
+    const long b = i / npad;
+    const int j = (int)(i % npad);
+    float g = 0.f;
+    if (j < nout) {
+      const float d = pred[b * ldp + j] - tgt[b * ldt + j];
+      sq = d * d;
+      g = 2.f * d * scale;
+    }
+    dY[b * ldd + j] = f2bf(g);
+  }
+  for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float s = 0.f;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) s += red[q];
+    atomicAdd(loss, s);
+  }
+}
+
+// ------------------------------------------------------------------ C ABI
+#define LAUNCH_GEMM(TA_, TB_, EPI_)                                                           \
+  hipLaunchKernelGGL((k_gemm<TA_, TB_, EPI_>), grid, dim3(256), 0, ctx->stream, M, N, K,      \
+                     (const u16 *)A, lda, (const u16 *)B, ldb, bias, aux, ldaux, Cf, ldcf,     \
+                     (u16 *)Cb, ldcb, kslice)
+
+template <int EPI>
+static void gemm_dispatch(izem_ctx *ctx, dim3 grid, int ta, int tb, int M, int N, int K,
+                          const void *A, long lda, const void *B, long ldb, const float *bias,
+                          const float *aux, long ldaux, float *Cf, long ldcf, void *Cb,
+                          long ldcb, int kslice) {
+  if (!ta && !tb) LAUNCH_GEMM(0, 0, EPI);
+  else if (!ta && tb) LAUNCH_GEMM(0, 1, EPI);
+  else if (ta && !tb) LAUNCH_GEMM(1, 0, EPI);
+  else LAUNCH_GEMM(1, 1, EPI);
+}
+
+extern "C" int izem_mlp_gemm(izem_ctx *ctx, int ta, int tb, int epi, int M, int N, int K,
+                             const void *A, long lda, const void *B, long ldb,
+                             const float *bias, const float *aux, long ldaux, float *Cf,
+                             long ldcf, void *Cb, long ldcb) {
+  if (!ctx || !A || !B || M <= 0 || N <= 0 || K <= 0 || epi < 0 || epi > 5 || epi == 4)
+    return IZEM_E_ARG;
+  if ((epi == EPI_BIAS_GELU || epi == EPI_BIAS_F32 || epi == EPI_GELU_ONLY) && !bias)
+    return IZEM_E_ARG;
+  if ((epi == EPI_BIAS_GELU || epi == EPI_GELU_BWD || epi == EPI_GELU_ONLY) && !Cb)
+    return IZEM_E_ARG;
+  if (epi == EPI_GELU_BWD && !aux) return IZEM_E_ARG;
+  if ((epi == EPI_F32 || epi == EPI_BIAS_GELU || epi == EPI_BIAS_F32) && !Cf) return IZEM_E_ARG;
+  dim3 grid((N + NT - 1) / NT, (M + MT - 1) / MT, 1);
+  const int ks = ((K + KT - 1) / KT) * KT;
+  int pk = iz_prof_begin(ctx, IZP_OTHER);
+  switch (epi) {
+    case EPI_F32: gemm_dispatch<EPI_F32>(ctx, grid, ta, tb, M, N, K, A, lda, B, ldb, bias, aux, ldaux, Cf, ldcf, Cb, ldcb, ks); break;
+    case EPI_BIAS_GELU: gemm_dispatch<EPI_BIAS_GELU>(ctx, grid, ta, tb, M, N, K, A, lda, B, ldb, bias, aux, ldaux, Cf, ldcf, Cb, ldcb, ks); break;
+    case EPI_BIAS_F32: gemm_dispatch<EPI_BIAS_F32>(ctx, grid, ta, tb, M, N, K, A, lda, B, ldb, bias, aux, ldaux, Cf, ldcf, Cb, ldcb, ks); break;
+    case EPI_GELU_ONLY: gemm_dispatch<EPI_GELU_ONLY>(ctx, grid, ta, tb, M, N, K, A, lda, B, ldb, bias, aux, ldaux, Cf, ldcf, Cb, ldcb, ks); break;
+    default: gemm_dispatch<EPI_GELU_BWD>(ctx, grid, ta, tb, M, N, K, A, lda, B, ldb, bias, aux, ldaux, Cf, ldcf, Cb, ldcb, ks); break;
+  }
+  iz_prof_end(ctx, pk);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+extern "C" int izem_mlp_gemm_splitk(izem_ctx *ctx, int ta, int tb, int M, int N, int K,
+                                    const void *A, long lda, const void *B, long ldb,
+                                    int nslices, float *work, float *C) {
+  if (!ctx || !A || !B || !work || !C || M <= 0 || N <= 0 || K <= 0 || nslices < 1)
+    return IZEM_E_ARG;
+  int ks = (K + nslices - 1) / nslices;
+  ks = ((ks + KT - 1) / KT) * KT;
+  const int nz = (K + ks - 1) / ks;
+  dim3 grid((N + NT - 1) / NT, (M + MT - 1) / MT, nz);
+  const float *nul = nullptr;
+  gemm_dispatch<EPI_PARTIAL>(ctx, grid, ta, tb, M, N, K, A, lda, B, ldb, nul, nul, 0, work,
+                             N, nullptr, 0, ks);
+  const long n = (long)M * N;
+  hipLaunchKernelGGL(k_sum_slices, dim3(iz_blocks(n, 256)), dim3(256), 0, ctx->stream, n, nz,
+                     work, C);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+extern "C" int izem_mlp_colsum(izem_ctx *ctx, int M, int N, const void *G, long ld,
+                               int nslices, float *work, float *out) {
+  if (!ctx || !G || !work || !out || M <= 0 || N <= 0 || nslices < 1) return IZEM_E_ARG;
+  const int rows = (M + nslices - 1) / nslices;
+  const int ns = (M + rows - 1) / rows;
+  hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256, ns), dim3(256), 0, ctx->stream, M, N,
+                     (const u16 *)G, ld, rows, work);
+  hipLaunchKernelGGL(k_sum_slices, dim3(iz_blocks(N, 256)), dim3(256), 0, ctx->stream, (long)N,
+                     ns, work, out);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+extern "C" int izem_mlp_adam(izem_ctx *ctx, int rows, int cols, float *P, const float *G,
+                             float *m, float *v, float lr, float b1, float b2, float eps,
+                             int step, void *Pt, long ldt, void *Pw, long ldw) {
+  if (!ctx || !P || !G || !m || !v || rows <= 0 || cols <= 0 || step < 1) return IZEM_E_ARG;
+  const float c1 = 1.f / (1.f - powf(b1, (float)step));
+  const float c2 = 1.f / (1.f - powf(b2, (float)step));
+  const long n = (long)rows * cols;
+  hipLaunchKernelGGL(k_adam, dim3(iz_blocks(n, 256)), dim3(256), 0, ctx->stream, rows, cols, P,
+                     G, m, v, lr, b1, b2, eps, c1, c2, (u16 *)Pt, ldt, (u16 *)Pw, ldw);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+extern "C" int izem_mlp_to_bf16(izem_ctx *ctx, int rows, int cols, const float *P, void *Pt,
+                                long ldt, void *Pw, long ldw) {
+  if (!ctx || !P || rows <= 0 || cols <= 0) return IZEM_E_ARG;
+  const long n = (long)rows * cols;
+  hipLaunchKernelGGL(k_to_bf16, dim3(iz_blocks(n, 256)), dim3(256), 0, ctx->stream, rows, cols,
+                     P, (u16 *)Pt, ldt, (u16 *)Pw, ldw);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+extern "C" int izem_mlp_mse(izem_ctx *ctx, int Bn, int nout, int npad, const float *pred,
+                            long ldp, const float *tgt, long ldt, void *dY, long ldd,
+                            float *loss) {
+  if (!ctx || !pred || !tgt || !dY || !loss || Bn <= 0 || nout <= 0 || npad < nout)
+    return IZEM_E_ARG;
+  const long n = (long)Bn * npad;
+  const float scale = 1.f / ((float)Bn * (float)nout);
+  hipLaunchKernelGGL(k_mse, dim3(iz_blocks(n, 256)), dim3(256), 0, ctx->stream, Bn, nout, npad,
+                     pred, ldp, tgt, ldt, scale, (u16 *)dY, ldd, loss);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
