@@ -215,6 +215,12 @@ int izem_mlp_to_bf16(izem_ctx *ctx, int rows, int cols, const float *P, void *Pt
  * columns zero; *loss (device) accumulates the sum of squares */
 int izem_mlp_mse(izem_ctx *ctx, int Bn, int nout, int npad, const float *pred, long ldp,
                  const float *tgt, long ldt, void *dY, long ldd, float *loss);
+/* the whole network on M cells in one kernel (inference): X[M][32] bf16
+ * features, Wt[l] the bf16 weights of layer l stored [N][K] (K = 32, 256,
+ * 256, 256; N = 256, 256, 256, 64), b[l] fp32 biases, pred[M][64] fp32
+ * outputs; activations stay on chip */
+int izem_mlp_infer(izem_ctx *ctx, int M, const void *X, const void *const *Wt,
+                   const float *const *b, float *pred);
 /* normalised bf16 features (ln p, T, Y_k, ln dt) of n cells, rows of 32 */
 int izem_surr_features(izem_ctx *ctx, int n, const double *T, const double *rho, const double *Y,
                        double dt, const float *mean, const float *istd, void *X);

@@ -20,7 +20,7 @@ EXPORTS = [
     "izem_comm_cons", "izem_upd_prim_var_ghost", "izem_shock_sensor", "izem_rk3", "izem_rhs",
     "izem_timestep", "izem_chem_setup", "izem_chem_tune", "izem_chem_stats", "izem_ckwyr", "izem_psr_batch", "izem_reaction_dvode",
     "izem_mlp_gemm", "izem_mlp_gemm_splitk", "izem_mlp_colsum", "izem_mlp_adam", "izem_mlp_to_bf16",
-    "izem_mlp_mse", "izem_surr_features", "izem_surr_apply", "izem_surr_fallback",
+    "izem_mlp_mse", "izem_mlp_infer", "izem_surr_features", "izem_surr_apply", "izem_surr_fallback",
 ]
 
 
@@ -91,6 +91,7 @@ def load(path: str | None = None):
     lib.izem_mlp_adam.argtypes = [vp, I, I, vp, vp, vp, vp, F, F, F, F, I, vp, L, vp, L]
     lib.izem_mlp_to_bf16.argtypes = [vp, I, I, vp, vp, L, vp, L]
     lib.izem_mlp_mse.argtypes = [vp, I, I, I, vp, L, vp, L, vp, L, vp]
+    lib.izem_mlp_infer.argtypes = [vp, I, vp, vp, vp, vp]
     lib.izem_surr_features.argtypes = [vp, I, vp, vp, vp, D, vp, vp, vp]
     lib.izem_surr_apply.argtypes = [vp, I, vp, vp, vp, L, vp, vp, vp, vp, vp, F, D, D, vp, vp]
     lib.izem_surr_fallback.argtypes = [vp, I, vp, vp, vp, D, vp, vp, vp, vp, vp, vp, vp]

@@ -8,7 +8,8 @@ inference the predicted dY is projected onto element (hence mass)
 conservation and T advances by the predicted dT; a per-cell error
 indicator sends cells back to the BDF path:
 
-  * a feature outside the training range (|normalised feature| > feat_lim),
+  * a feature outside the training range (|normalised feature| > feat_lim,
+    6 standard deviations of the training data),
   * a mass fraction below -1e-7 after the projection,
   * |T(energy, projected Y) - (T0 + dT_net)| > tol_T: the network's
     temperature disagrees with the one its species imply through the
@@ -146,7 +147,7 @@ class Surrogate:
         self.proj = torch.tensor(element_projector(solver.thd).ravel(), dtype=torch.float64,
                                  device=self.dev)
         self.in_mean = self.in_istd = self.out_mean = self.out_std = None
-        self.feat_lim, self.tol_T, self.dT_max = 4.0, 1.0, 50.0
+        self.feat_lim, self.tol_T, self.dT_max = 6.0, 1.0, 50.0
         self._ws = {}
 
     # ------------------------------------------------------------ buffers
@@ -305,12 +306,21 @@ class Surrogate:
         return hist
 
     # ------------------------------------------------------------ inference
+    def infer(self, X: torch.Tensor) -> torch.Tensor:
+        """Scaled outputs [n][64] from the fused single-kernel network."""
+        n = X.shape[0]
+        out = self._buf("pred", (n, 64), torch.float32)
+        wp = (C.c_void_p * 4)(*[t.data_ptr() for t in self.Wt])
+        bp = (C.c_void_p * 4)(*[t.data_ptr() for t in self.b])
+        _lib.check(self.lib.izem_mlp_infer(self.ctx, n, _p(X), wp, bp, _p(out)), "mlp_infer")
+        return out
+
     def apply(self, T, rho, Y, dt, tol_T=None, dT_max=None, feat_lim=None, T_energy=None):
         """Network substep on every cell in place for the accepted ones;
         returns the fallback flags (int32 [n])."""
         n = T.numel()
         X = self.features(T, rho, Y, dt)
-        pred = self.forward(X)
+        pred = self.infer(X)
         flag = self._buf("flag", (n,), torch.int32)
         _lib.check(self.lib.izem_surr_apply(
             self.ctx, n, _p(T), _p(Y), _p(pred), 64, _p(self.out_mean), _p(self.out_std),
@@ -374,7 +384,15 @@ class Surrogate:
                     continue
                 if err[acc].max() <= margin * target_K and acc.mean() > best[0]:
                     best = (float(acc.mean()), float(tt), float(dm))
-        self.tol_T, self.dT_max = best[1], best[2]
+        tt, dm = best[1], best[2]
+        if best[0] == 1.0 and ok.all():
+            # every held-out cell passed with room to spare: the data bound
+            # the thresholds only from below, so widen them by the ratio of
+            # the error budget to the largest error seen (at most 10x)
+            room = margin * target_K / max(float(err.max()), 1e-300)
+            f = min(10.0, room)
+            tt, dm = tt * f, dm * f
+        self.tol_T, self.dT_max = tt, dm
         return {"accept_fraction": best[0], "tol_T": self.tol_T, "dT_max": self.dT_max,
                 **self.last_calibration}
 

@@ -144,6 +144,26 @@ def test_forward_and_gradients_vs_fp64_oracle(mech):
             grads[li][0].cpu().numpy()[:, nn:] == 0)
 
 
+def test_fused_inference_equals_layered_forward(mech):
+    """The single-kernel network (activations in LDS) and the per-layer
+    kernels accumulate in the same order: identical outputs."""
+    import torch
+    from phd_amd import surrogate as su
+    s = _solver(mech)
+    sur = su.Surrogate(s, seed=5)
+    for li in range(4):  # nonzero biases
+        sur.b[li].normal_(0.0, 0.1)
+    n = 1000  # not a multiple of the 64-row tile
+    rng = np.random.default_rng(8)
+    X = np.zeros((n, su.XLD)); X[:, :su.NF] = rng.normal(size=(n, su.NF))
+    Xb, _ = _bf16_bits(X)
+    dX = torch.tensor(Xb, device="cuda")
+    a = sur.forward(dX).clone()
+    b = sur.infer(dX).clone()
+    s.sync()
+    assert torch.equal(a[:, :su.NO], b[:, :su.NO])
+
+
 def test_adam_step_vs_oracle(mech):
     import torch
     from oracle import mlp_oracle as mo
