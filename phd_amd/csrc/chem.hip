@@ -18,18 +18,13 @@
 #include <string.h>
 
 #include "chem_tables.h"
+#include "chem_setup.h"
 
 __constant__ IzMech c_mech;
 __constant__ IzBdf c_bdf;
 
 #include "mech_gen.h"
 #include "chem_device.h"
-
-// The Nordsieck BDF with fixed-coefficient rescaling has a larger global
-// error per unit of local tolerance than DVODE's variable-coefficient form;
-// controlling the local error at this fraction of the requested rtol/atol
-// brings its end states to the accuracy DVODE reaches at rtol/atol.
-#define IZ_BDF_TOL_SCALE 0.1
 
 struct IzChem {
   double rtol, atol;
@@ -76,6 +71,29 @@ __device__ inline void chem_stats_add(unsigned long long *stats, unsigned long l
   }
 }
 
+// Where a kernel variant keeps the Nordsieck history.  At one wave per SIMD
+// the lane has 512 registers for the kinetics temporaries and the LU
+// factors, and the history (6 x N_ doubles, indexed by the current order)
+// goes to LDS: 30 KB per 64-lane block, four blocks per CU.  Variants with
+// more waves per SIMD keep it in private memory.
+#ifndef IZ_ZLDS_MAX_WAVES
+#define IZ_ZLDS_MAX_WAVES 1
+#endif
+
+// integrate one cell with the history storage of the variant
+template <class Kin, int WAVES>
+__device__ IZ_FORCEINLINE BdfResult chem_integrate(double *zs, double rho, double *y, double dt,
+                                                   double rtol, double atol, int jac_age) {
+  if constexpr (WAVES <= IZ_ZLDS_MAX_WAVES) {
+    ZLds z{zs + threadIdx.x};
+    return d_bdf_integrate(Kin(), z, rho, y, dt, rtol, atol, jac_age);
+  } else {
+    ZArr z;
+    return d_bdf_integrate(Kin(), z, rho, y, dt, rtol, atol, jac_age);
+  }
+}
+#define IZ_ZLDS_DOUBLES(WAVES) ((WAVES) <= IZ_ZLDS_MAX_WAVES ? 6 * N_ * 64 : 1)
+
 // persistent work-queue kernel over independent cells: each lane takes the
 // next cell from the queue as soon as its previous one is finished, so
 // cells with very different step counts keep every lane busy
@@ -84,6 +102,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                             double *__restrict__ Y, double dt, double rtol, double atol,
                             int jac_age, int *__restrict__ counter, int *__restrict__ nsteps,
                             int *__restrict__ status, unsigned long long *__restrict__ stats) {
+  __shared__ double zs[IZ_ZLDS_DOUBLES(WAVES)];
   unsigned long long w_nst = 0, w_nfe = 0, w_nje = 0, w_nsol = 0;
   for (;;) {
     int cell = atomicAdd(counter, 1);
@@ -91,7 +110,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     double y[N_];
     y[0] = T[cell];
     for (int k = 0; k < N_ - 1; k++) y[1 + k] = Y[(long)cell * (N_ - 1) + k];
-    BdfResult r = d_bdf_integrate(Kin(), rho[cell], y, dt, rtol, atol, jac_age);
+    BdfResult r = chem_integrate<Kin, WAVES>(zs, rho[cell], y, dt, rtol, atol, jac_age);
     T[cell] = y[0];
     for (int k = 0; k < N_ - 1; k++) Y[(long)cell * (N_ - 1) + k] = y[1 + k];
     if (nsteps) nsteps[cell] = r.nst;
@@ -119,6 +138,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
   const long n = (long)ni * nj * nk;
   const double dt_phys = (scale * (*dtp)) * time_ref;
   const double Tref_i = 1.0 / T_ref;
+  __shared__ double zs[IZ_ZLDS_DOUBLES(WAVES)];
   double dtmax = 0.0;
   unsigned long long w_nst = 0, w_nfe = 0, w_nje = 0, w_nsol = 0;
   for (;;) {
@@ -134,7 +154,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     double T0_phys = T[c] * T_ref;
     y[0] = T0_phys;
     for (int l = 0; l < N_ - 1; l++) y[1 + l] = v[(IZ_NIV + l) * d.nc + c] * rho_i;
-    BdfResult res = d_bdf_integrate(Kin(), rho0 * rho_cgs, y, dt_phys, rtol, atol, jac_age);
+    BdfResult res = chem_integrate<Kin, WAVES>(zs, rho0 * rho_cgs, y, dt_phys, rtol, atol,
+                                               jac_age);
     if (!res.ok) atomicOr(status, IZ_ST_VODE_FAIL);
     w_nst += res.nst; w_nfe += res.nfe; w_nje += res.nje; w_nsol += res.nsol;
     T[c] = y[0] * Tref_i;
@@ -176,24 +197,15 @@ __global__ void k_gen_check(int *ok) {
     KinTables().rhs(3e-4, y, f2);
     for (int i = 0; i < CK_NEQ; i++)
       if (!(fabs(f1[i] - f2[i]) <= 1e-9 * fabs(f2[i]) + 1e-20)) good = 0;
+    // the fused evaluation and the split one that the Jacobian columns use
+    // must round identically, or a column difference picks up their gap
+    double kt[GEN_NKT], f3[CK_NEQ];
+    gen_kT(y[0], kt);
+    gen_rhs(3e-4, y, kt, f3);
+    for (int i = 0; i < CK_NEQ; i++)
+      if (!(f1[i] == f3[i])) good = 0;
   }
   *ok = good;
-}
-
-static void bdf_tables(IzBdf &t) {
-  memset(&t, 0, sizeof t);
-  for (int q = 1; q <= 5; q++) {
-    // coefficients of prod_{i=1..q} (1 + x/i), normalised so that l1 = 1
-    double c[7] = {1, 0, 0, 0, 0, 0, 0};
-    for (int i = 1; i <= q; i++)
-      for (int j = i; j >= 1; j--) c[j] += c[j - 1] / i;
-    for (int j = 0; j <= q; j++) t.el[q][j] = c[j] / c[1];
-    double fact = 1.0;
-    for (int i = 2; i <= q; i++) fact *= i;  // q!
-    t.tesco[q][0] = 1.0 / fact;
-    t.tesco[q][1] = (q + 1) / t.el[q][0];
-    t.tesco[q][2] = (q + 2) / t.el[q][0];
-  }
 }
 
 void iz_chem_destroy(izem_ctx *ctx) {
@@ -208,33 +220,9 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
                                double atol, int trim) {
   if (!ctx || !mf || rtol <= 0 || atol <= 0) return IZEM_E_ARG;
   IzMech m;
-  memset(&m, 0, sizeof m);
-  long p = 0;
-  auto take = [&](long n) { long s = p; p += n; return s; };
-  const long need = 2 + CK_KMAX + CK_KMAX * 14 + CK_KMAX + 3 * CK_IMAX + 12 * CK_IMAX +
-                    2 * CK_IMAX + 2 * CK_IMAX * CK_TBMAX + CK_IMAX + 8 * CK_IMAX;
-  if (nmech != need) return IZEM_E_ARG;
-  long o;
-  o = take(2); m.kk = (int)mf[o]; m.ii = (int)mf[o + 1];
-  if (m.kk != CK_NEQ - 1 || m.ii < 1 || m.ii > CK_IMAX) return IZEM_E_ARG;
-  o = take(CK_KMAX); for (int k = 0; k < CK_KMAX; k++) m.wt[k] = mf[o + k];
-  o = take(CK_KMAX * 14); memcpy(m.a, mf + o, sizeof(double) * CK_KMAX * 14);
-  o = take(CK_KMAX); for (int k = 0; k < CK_KMAX; k++) m.tmid[k] = mf[o + k];
-  o = take(CK_IMAX); for (int i = 0; i < CK_IMAX; i++) m.A[i] = mf[o + i];
-  o = take(CK_IMAX); for (int i = 0; i < CK_IMAX; i++) m.b[i] = mf[o + i];
-  o = take(CK_IMAX); for (int i = 0; i < CK_IMAX; i++) m.EoR[i] = mf[o + i];
-  o = take(6 * CK_IMAX); for (int i = 0; i < 6 * CK_IMAX; i++) (&m.nu[0][0])[i] = (int)mf[o + i];
-  o = take(6 * CK_IMAX); for (int i = 0; i < 6 * CK_IMAX; i++) (&m.nunk[0][0])[i] = (int)mf[o + i];
-  o = take(CK_IMAX); for (int i = 0; i < CK_IMAX; i++) m.reversible[i] = (int)mf[o + i];
-  o = take(CK_IMAX); for (int i = 0; i < CK_IMAX; i++) m.ntb[i] = (int)mf[o + i];
-  o = take(CK_IMAX * CK_TBMAX);
-  for (int i = 0; i < CK_IMAX * CK_TBMAX; i++) (&m.tb_k[0][0])[i] = (int)mf[o + i];
-  o = take(CK_IMAX * CK_TBMAX);
-  for (int i = 0; i < CK_IMAX * CK_TBMAX; i++) (&m.tb_eff[0][0])[i] = mf[o + i];
-  o = take(CK_IMAX); for (int i = 0; i < CK_IMAX; i++) m.fal[i] = (int)mf[o + i];
-  o = take(8 * CK_IMAX); memcpy(m.fpar, mf + o, sizeof(double) * 8 * CK_IMAX);
+  if (!iz_mech_from_flat(mf, nmech, CK_NEQ - 1, m)) return IZEM_E_ARG;
   IzBdf bt;
-  bdf_tables(bt);
+  iz_bdf_tables(bt);
   IZ_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_mech), &m, sizeof m));
   IZ_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_bdf), &bt, sizeof bt));
   if (!ctx->chem) {
@@ -259,8 +247,8 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
   }
   ctx->chem->generated = same;
   ctx->chem->use_gen = same;
-  ctx->chem->rtol = rtol * IZ_BDF_TOL_SCALE;
-  ctx->chem->atol = atol * IZ_BDF_TOL_SCALE;
+  ctx->chem->rtol = rtol;
+  ctx->chem->atol = atol;
   ctx->chem->trim = trim;
   return IZEM_OK;
 }

@@ -140,55 +140,117 @@ IZ_HD inline double d_wrms(const double *v, const double *ewt) {
   return sqrt(s / N_);
 }
 
-// LU with partial pivoting (the dgefa / dgesl pair, vode.f:7033, 7630)
-IZ_HD inline bool d_lu(double (*A)[N_], int *piv) {
+// LU with partial pivoting in LINPACK's form (dgefa / dgesl, vode.f:7033,
+// 7630): step k swaps rows k and p over columns k..N-1 only, so the
+// multipliers of the earlier steps stay in the rows where the solve
+// applies them, interleaved with the same swaps.  Every index is a
+// compile-time constant, so that the factors can stay in registers: the
+// row swap is a run of selects, skipped when no lane of the wave pivots.
+IZ_HD inline bool d_lu(double (&A)[N_][N_], int (&piv)[N_]) {
   bool ok = true;
+#pragma unroll
   for (int k = 0; k < N_; k++) {
     int p = k;
     double amax = fabs(A[k][k]);
-    for (int i = k + 1; i < N_; i++) if (fabs(A[i][k]) > amax) { amax = fabs(A[i][k]); p = i; }
-    piv[k] = p;
-    if (amax == 0.0) { ok = false; continue; }
-    if (p != k)
-      for (int j = 0; j < N_; j++) { double t = A[k][j]; A[k][j] = A[p][j]; A[p][j] = t; }
-    double inv = 1.0 / A[k][k];
+#pragma unroll
     for (int i = k + 1; i < N_; i++) {
-      double m = A[i][k] * inv;
+      const double a = fabs(A[i][k]);
+      p = (a > amax) ? i : p;
+      amax = (a > amax) ? a : amax;
+    }
+    piv[k] = p;
+    if (IZ_ANY(p != k)) {
+#pragma unroll
+      for (int j = k; j < N_; j++) {
+        const double akj = A[k][j];
+        double apj = akj;
+#pragma unroll
+        for (int i = k + 1; i < N_; i++) {
+          apj = (p == i) ? A[i][j] : apj;
+          A[i][j] = (p == i) ? akj : A[i][j];
+        }
+        A[k][j] = apj;
+      }
+    }
+    if (amax == 0.0) ok = false;  // dgefa's INFO = k: column k has no pivot
+    const double inv = (amax == 0.0) ? 0.0 : 1.0 / A[k][k];
+#pragma unroll
+    for (int i = k + 1; i < N_; i++) {
+      const double m = A[i][k] * inv;
       A[i][k] = m;
+#pragma unroll
       for (int j = k + 1; j < N_; j++) A[i][j] -= m * A[k][j];
     }
   }
   return ok;
 }
-IZ_HD inline void d_lusolve(const double (*A)[N_], const int *piv, double *x) {
+IZ_HD inline void d_lusolve(const double (&A)[N_][N_], const int (&piv)[N_], double (&x)[N_]) {
+#pragma unroll
   for (int k = 0; k < N_; k++) {
-    int p = piv[k];
-    if (p != k) { double t = x[k]; x[k] = x[p]; x[p] = t; }
-    for (int i = k + 1; i < N_; i++) x[i] -= A[i][k] * x[k];
+    const int p = piv[k];
+    if (IZ_ANY(p != k)) {
+      const double xk = x[k];
+      double xp = xk;
+#pragma unroll
+      for (int i = k + 1; i < N_; i++) {
+        xp = (p == i) ? x[i] : xp;
+        x[i] = (p == i) ? xk : x[i];
+      }
+      x[k] = xp;
+    }
+    const double t = x[k];
+#pragma unroll
+    for (int i = k + 1; i < N_; i++) x[i] -= A[i][k] * t;
   }
+#pragma unroll
   for (int k = N_ - 1; k >= 0; k--) {
     double s = x[k];
+#pragma unroll
     for (int j = k + 1; j < N_; j++) s -= A[k][j] * x[j];
     x[k] = s / A[k][k];
   }
 }
 
+// Nordsieck history of one cell, z(j, n) for column j = 0..5 and component
+// n.  ZArr holds it in the lane's own (private) memory; ZLds holds it in
+// LDS with one 64-lane wave's worth of stride between elements, so that
+// element (j, n) of lane l sits at base[(j * N_ + n) * 64 + l] and a wave
+// reads a column component as one contiguous 512-byte row.
+struct ZArr {
+  double a[6][N_];
+  IZ_MHD inline double &operator()(int j, int n) { return a[j][n]; }
+};
+struct ZLds {
+  double *p;  // base + lane
+  IZ_MHD inline double &operator()(int j, int n) const { return p[(j * N_ + n) * 64]; }
+};
+
+template <class Z>
+IZ_HD inline double d_wrms_col(Z &z, int j, const double *ewt) {
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < N_; i++) { double x = z(j, i) / ewt[i]; s += x * x; }
+  return sqrt(s / N_);
+}
+
 // Pascal-triangle update of the Nordsieck history (sign +1: predict to t+h,
 // sign -1: retract); sweeps jb = 1..q over columns q-jb..q-1 in ascending
 // order, which makes the -1 sweep the exact inverse of the +1 sweep.
-IZ_HD inline void d_nordsieck_shift(double (*z)[N_], int q, double sign) {
+template <class Z>
+IZ_HD inline void d_nordsieck_shift(Z &z, int q, double sign) {
   for (int jb = 1; jb <= q; jb++)
     for (int i = q - jb; i <= q - 1; i++)
 #pragma unroll
-      for (int n = 0; n < N_; n++) z[i][n] += sign * z[i + 1][n];
+      for (int n = 0; n < N_; n++) z(i, n) += sign * z(i + 1, n);
 }
 
 // rescale the Nordsieck history for a step change h -> rh*h
-IZ_HD inline void d_nordsieck_rescale(double (*z)[N_], int q, double rh) {
+template <class Z>
+IZ_HD inline void d_nordsieck_rescale(Z &z, int q, double rh) {
   double f = rh;
   for (int j = 1; j <= q; j++) {
 #pragma unroll
-    for (int n = 0; n < N_; n++) z[j][n] *= f;
+    for (int n = 0; n < N_; n++) z(j, n) *= f;
     f *= rh;
   }
 }
@@ -250,14 +312,15 @@ struct KinTables {
 // (DVODE's MSBJ, default 50) steps, or after a corrector failure that a
 // new h l0 cannot explain (DRC < CCMXJ = 0.2) or that a current J did not
 // prevent.
-template <class Kin>
-IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, double tout,
-                                       double rtol, double atol, int jac_age) {
+template <class Kin, class Z>
+IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double rho, double *y,
+                                               double tout, double rtol, double atol,
+                                               int jac_age) {
   const double uround = 2.220446049250313e-16;
   const int maxstep = 10000, maxcor = 3, msbp = 20;
   const double ccmax = 0.3, ccmxj = 0.2;
   BdfResult res{0, 0, 0, 1, 0};
-  double z[6][N_], acor[N_], ewt[N_], savf[N_], yy[N_];
+  double acor[N_], ewt[N_], savf[N_], yy[N_];
   double J[N_][N_], P[N_][N_];
   int piv[N_];
   // J is written by the first step's Jacobian before any use; cleared so
@@ -266,7 +329,7 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
 #pragma unroll
     for (int j = 0; j < N_; j++) J[i][j] = 0.0;
 #pragma unroll
-  for (int i = 0; i < N_; i++) { z[0][i] = y[i]; ewt[i] = rtol * fabs(y[i]) + atol; }
+  for (int i = 0; i < N_; i++) { z(0, i) = y[i]; ewt[i] = rtol * fabs(y[i]) + atol; }
   kin.rhs(rho, y, savf);
   res.nfe++;
   // initial step (the LSODE-style estimate also used by DVHIN's first guess)
@@ -276,17 +339,18 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
   double h = 1.0 / sqrt(sum);
   h = fmin(h, tout);
 #pragma unroll
-  for (int i = 0; i < N_; i++) z[1][i] = h * savf[i];
+  for (int i = 0; i < N_; i++) z(1, i) = h * savf[i];
   int q = 1, ialth = 2, ncf = 0, kflag = 0;
   double t = 0.0, rc = 0.0, crate = 0.7, hl0 = 0.0, rmax = 1.0e4;
   bool need_p = true;   // DVODE's IPUP
+  bool p_ok = true;     // the last factorisation found a pivot in every column
   int nslp = 0, nslj = 0;
   int icf = 0;          // 1: corrector failed with a saved J, 2: with a current J
   while (t < tout) {
     if (res.nst >= maxstep) { res.ok = 0; break; }
     // error weights at the start of the step
 #pragma unroll
-    for (int i = 0; i < N_; i++) ewt[i] = rtol * fabs(z[0][i]) + atol;
+    for (int i = 0; i < N_; i++) ewt[i] = rtol * fabs(z(0, i)) + atol;
     // predictor: multiply the Nordsieck array by the Pascal triangle
     d_nordsieck_shift(z, q, 1.0);
     const double l0 = IZ_CBDF.el[q][0];
@@ -295,13 +359,20 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
     if (drc > ccmax || res.nst >= nslp + msbp) need_p = true;
     // corrector (chord Newton on G(e) = h f(z0 + l0 e) - z1 - e)
 #pragma unroll
-    for (int n = 0; n < N_; n++) { yy[n] = z[0][n]; acor[n] = 0.0; }
+    for (int n = 0; n < N_; n++) { yy[n] = z(0, n); acor[n] = 0.0; }
     kin.rhs(rho, yy, savf);
     res.nfe++;
     bool jcur = false;
-    if (need_p) {
-      const bool new_j = res.nst == 0 || res.nst > nslj + jac_age || (icf == 1 && drc < ccmxj) ||
-                         icf == 2;
+    // The lanes of a wave factor together: a lane that asked for a new
+    // iteration matrix builds it from h l0 (after a new J where one is
+    // due), every other lane rebuilds its current one from the same J and
+    // the h l0 it was built with (rc), which reproduces it bit for bit.
+    // The wave executes the factorisation for any lane that needs it, so
+    // the other lanes' copy costs no time, and the old factors are not
+    // live across the Jacobian evaluation of the lanes that make one.
+    if (IZ_ANY(need_p)) {
+      const bool new_j = need_p && (res.nst == 0 || res.nst > nslj + jac_age ||
+                                    (icf == 1 && drc < ccmxj) || icf == 2);
       if (new_j) {
         double fac = d_wrms(savf, ewt);
         double r0 = 1000.0 * fabs(h) * uround * N_ * fac;
@@ -312,25 +383,29 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
         nslj = res.nst;
         jcur = true;
       }
-      const double con = -hl0;
+      const double con = need_p ? -hl0 : -rc;
 #pragma unroll
       for (int i = 0; i < N_; i++)
 #pragma unroll
         for (int j = 0; j < N_; j++) P[i][j] = con * J[i][j];
 #pragma unroll
       for (int i = 0; i < N_; i++) P[i][i] += 1.0;
-      d_lu(P, piv);
-      rc = hl0;
-      nslp = res.nst;
-      need_p = false;
-      crate = 0.7;
+      p_ok = d_lu(P, piv);
+      if (need_p) {
+        rc = hl0;
+        nslp = res.nst;
+        need_p = false;
+        crate = 0.7;
+      }
     }
     bool conv = false;
     double del = 0.0, delp = 0.0;
-    for (int m = 0; m < maxcor; m++) {
+    // a singular iteration matrix counts as a corrector failure (DVNLSD,
+    // IERPJ != 0)
+    for (int m = 0; m < maxcor && p_ok; m++) {
       double d[N_];
 #pragma unroll
-      for (int n = 0; n < N_; n++) d[n] = h * savf[n] - z[1][n] - acor[n];
+      for (int n = 0; n < N_; n++) d[n] = h * savf[n] - z(1, n) - acor[n];
       d_lusolve(P, piv, d);
       res.nsol++;
       if (rc != hl0) {  // chord correction for a stale h*l0
@@ -340,7 +415,7 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
       }
       del = d_wrms(d, ewt);
 #pragma unroll
-      for (int n = 0; n < N_; n++) { acor[n] += d[n]; yy[n] = z[0][n] + l0 * acor[n]; }
+      for (int n = 0; n < N_; n++) { acor[n] += d[n]; yy[n] = z(0, n) + l0 * acor[n]; }
       if (m > 0) crate = fmax(0.2 * crate, del / delp);
       double dcon = del * fmin(1.0, 1.5 * crate) / (IZ_CBDF.tesco[q][1] * (0.5 / (q + 2)));
       if (dcon <= 1.0) { conv = true; break; }
@@ -355,8 +430,9 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
       d_nordsieck_shift(z, q, -1.0);
       ncf++;
       need_p = true;
-      icf = jcur ? 2 : 1;
-      if (jcur || ncf > 10) {
+      const bool hard = jcur || !p_ok;
+      icf = hard ? 2 : 1;
+      if (hard || ncf > 10) {
         if (ncf > 10) { res.ok = 0; break; }
         d_nordsieck_rescale(z, q, 0.25);
         h *= 0.25;
@@ -369,7 +445,7 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
     double dsm = d_wrms(acor, ewt) / IZ_CBDF.tesco[q][1];
 #ifdef IZ_BDF_TRACE
     printf("t %.4e h %.3e q %d dsm %.3e conv %d jcur %d T %.6f\n", t, h, q, dsm, (int)conv,
-           (int)jcur, z[0][0]);
+           (int)jcur, z(0, 0));
 #endif
     if (dsm > 1.0) {
       d_nordsieck_shift(z, q, -1.0);
@@ -378,11 +454,11 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
         // repeated failures: restart at order 1 from a fresh derivative
         h *= 0.1;
 #pragma unroll
-        for (int n = 0; n < N_; n++) yy[n] = z[0][n];
+        for (int n = 0; n < N_; n++) yy[n] = z(0, n);
         kin.rhs(rho, yy, savf);
         res.nfe++;
 #pragma unroll
-        for (int n = 0; n < N_; n++) z[1][n] = h * savf[n];
+        for (int n = 0; n < N_; n++) z(1, n) = h * savf[n];
         q = 1;
         ialth = 5;
         need_p = true;
@@ -392,7 +468,7 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
       double rh = 1.0 / (1.2 * pow(dsm, 1.0 / (q + 1)) + 1.2e-6);
       int newq = q;
       if (q > 1) {
-        double ddn = d_wrms(z[q], ewt) / IZ_CBDF.tesco[q][0];
+        double ddn = d_wrms_col(z, q, ewt) / IZ_CBDF.tesco[q][0];
         double rhdn = 1.0 / (1.3 * pow(ddn, 1.0 / q) + 1.3e-6);
         if (rhdn > rh) { rh = rhdn; newq = q - 1; }
       }
@@ -412,25 +488,25 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
     for (int j = 0; j <= q; j++) {
       double e = IZ_CBDF.el[q][j];
 #pragma unroll
-      for (int n = 0; n < N_; n++) z[j][n] += e * acor[n];
+      for (int n = 0; n < N_; n++) z(j, n) += e * acor[n];
     }
     ialth--;
     if (ialth == 1 && q < 5) {
 #pragma unroll
-      for (int n = 0; n < N_; n++) z[5][n] = acor[n];  // kept for the order-up estimate
+      for (int n = 0; n < N_; n++) z(5, n) = acor[n];  // kept for the order-up estimate
     }
     if (ialth > 0) continue;
     // order and step selection (every q+1 steps)
     double rhsm = 1.0 / (1.2 * pow(dsm, 1.0 / (q + 1)) + 1.2e-6);
     double rhdn = 0.0, rhup = 0.0;
     if (q > 1) {
-      double ddn = d_wrms(z[q], ewt) / IZ_CBDF.tesco[q][0];
+      double ddn = d_wrms_col(z, q, ewt) / IZ_CBDF.tesco[q][0];
       rhdn = 1.0 / (1.3 * pow(ddn, 1.0 / q) + 1.3e-6);
     }
     if (q < 5) {
       double dd[N_];
 #pragma unroll
-      for (int n = 0; n < N_; n++) dd[n] = acor[n] - z[5][n];
+      for (int n = 0; n < N_; n++) dd[n] = acor[n] - z(5, n);
       double dup = d_wrms(dd, ewt) / IZ_CBDF.tesco[q][2];
       rhup = 1.0 / (1.4 * pow(dup, 1.0 / (q + 2)) + 1.4e-6);
     }
@@ -445,19 +521,19 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
       // new highest-order column from the last correction
       double e = IZ_CBDF.el[q][q] / (q + 1);
 #pragma unroll
-      for (int n = 0; n < N_; n++) z[newq][n] = acor[n] * e;
+      for (int n = 0; n < N_; n++) z(newq, n) = acor[n] * e;
     }
     q = newq;
     d_nordsieck_rescale(z, q, rh);
     h *= rh;
     ialth = q + 1;
   }
-  // interpolate back to tout: y = sum_j z[j] s^j
+  // interpolate back to tout: y = sum_j z(j) s^j
   double s = (tout - t) / h;
 #pragma unroll
   for (int n = 0; n < N_; n++) {
-    double acc = z[q][n];
-    for (int j = q - 1; j >= 0; j--) acc = acc * s + z[j][n];
+    double acc = z(q, n);
+    for (int j = q - 1; j >= 0; j--) acc = acc * s + z(j, n);
     y[n] = acc;
   }
   return res;
@@ -469,9 +545,7 @@ IZ_HD inline BdfResult d_bdf_integrate(const Kin &kin, double rho, double *y, do
 // shared by the Jacobian columns that perturb mass fractions
 struct KinGen {
   IZ_MHD inline void rhs(double rho, const double *y, double *f) const {
-    double kt[GEN_NKT];
-    gen_kT(y[0], kt);
-    gen_rhs(rho, y, kt, f);
+    gen_rhs_full(rho, y, f);
   }
   // the species columns share kt (T is not perturbed).  Each column
   // perturbs a local copy of the state: perturbing the caller's vector in

@@ -10,6 +10,16 @@
 #ifndef IZ_MHD
 #define IZ_MHD __device__
 #endif
+// the integrator is inlined into each kernel variant, so that every
+// register budget gets its own allocation of the per-cell state
+#ifndef IZ_FORCEINLINE
+#define IZ_FORCEINLINE __attribute__((always_inline)) inline
+#endif
+// true when the predicate holds on any active lane of the wave (a CPU
+// build, one cell at a time, defines it as the predicate itself)
+#ifndef IZ_ANY
+#define IZ_ANY(pred) (__any((int)(pred)) != 0)
+#endif
 
 #define CK_KMAX 16
 #define CK_IMAX 48
