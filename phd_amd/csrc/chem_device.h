@@ -545,7 +545,9 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double rho,
 // shared by the Jacobian columns that perturb mass fractions
 struct KinGen {
   IZ_MHD inline void rhs(double rho, const double *y, double *f) const {
-    gen_rhs_full(rho, y, f);
+    double kt[GEN_NKT];
+    gen_kT(y[0], kt);
+    gen_rhs(rho, y, kt, f);
   }
   // the species columns share kt (T is not perturbed).  Each column
   // perturbs a local copy of the state: perturbing the caller's vector in

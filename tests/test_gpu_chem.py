@@ -257,3 +257,53 @@ def test_variants_follow_host_build(mech, thd, host_bdf, kinetics, waves):
     assert np.all(rel <= 1e-6)
     assert same >= 0.8 * n
     assert abs(ng.mean() - nst_h.mean()) <= 0.03 * nst_h.mean()
+
+
+@pytest.mark.parametrize("kinetics,waves", [(0, 1), (0, 2), (0, 4), (1, 1), (1, 2), (1, 4)])
+def test_reaction_queue_variants_follow_host_build(oracle, adi, thd, mech, host_bdf, kinetics,
+                                                   waves):
+    """The reacting-step kernel (k_reaction_queue, behind reaction_selec) in
+    every variant against the g++ build of the integrator on the same cells
+    of a 2-D hot-spot state: end temperatures within 1e-6 relative, and the
+    total BDF step count (the library's work counters) within 3 % of the
+    host build's."""
+    import torch
+    from phd_amd.solver import Options, Solver
+    from tests.flowcase import OracleRun, small_state
+    box, v, T = small_state(adi, thd, "2d", 32)
+    run = OracleRun(oracle, adi, thd, box, v, T)
+    run.prim_all()
+    s = Solver(box, Options(), adi=adi, thd=thd, mech=mech)
+    s.v.copy_(torch.from_numpy(run.v)); s.T.copy_(torch.from_numpy(run.T))
+    s.W_i.copy_(torch.from_numpy(run.W_i)); s.cp.copy_(torch.from_numpy(run.cp))
+    s.ha.copy_(torch.from_numpy(run.ha))
+    assert s.lib.izem_chem_tune(s.ctx, waves, 4 * waves, 50, kinetics) == 0
+    stats = np.zeros(4, dtype=np.uint64)
+    s.lib.izem_chem_stats(s.ctx, stats.ctypes.data)  # clears the counters
+    dtmin = 2.0e-6 / adi.time_ref
+    dt = torch.tensor([dtmin], dtype=torch.float64, device="cuda:0")
+    s.reaction_selec(dt.data_ptr(), 0.5)
+    s.sync()
+    assert s.status() == 0
+    s.lib.izem_chem_stats(s.ctx, stats.ctypes.data)
+    # the same cells (the trimmed reacting box) through the host build
+    ng, nx, ny = box.ng, box.nx, box.ny
+    cells = [(ng, j + ng, i + ng) for j in range(ng, ny - ng - 1) for i in range(ng, nx - ng - 1)]
+    n = len(cells)
+    rho_cgs = adi.rho_ref * 1.0e-3
+    T0 = np.array([run.T[c] * adi.T_ref for c in cells])
+    rho = np.array([run.v[(0,) + c] * rho_cgs for c in cells])
+    Y = np.ascontiguousarray([run.v[(slice(5, 14),) + c] / run.v[(0,) + c] for c in cells])
+    Th = T0.copy()
+    nst, ok = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    host_bdf.host_psr(kinetics, n, Th.ctypes.data, rho.ctypes.data, Y.ctypes.data,
+                      0.5 * dtmin * adi.time_ref, 1e-6, 1e-12, 50, nst.ctypes.data,
+                      ok.ctypes.data)
+    assert ok.all()
+    Tg = s.T.cpu().numpy()
+    Tgc = np.array([Tg[c] * adi.T_ref for c in cells])
+    rel = np.abs(Tgc - Th) / Th
+    print(f"kinetics {kinetics} waves {waves}: {n} cells, max |dT|/T {rel.max():.2e}, "
+          f"steps {int(stats[0])} (host {int(nst.sum())}), max |T - T0| {np.max(np.abs(Th - T0)):.1f} K")
+    assert rel.max() <= 1e-6
+    assert abs(float(stats[0]) - float(nst.sum())) <= 0.03 * float(nst.sum())
