@@ -1032,12 +1032,20 @@ __device__ inline double d_filter_delta2(const IzDims &d, const double *dx_i, co
   return delta * delta;
 }
 
-// pass 1: per-species maximum of the mixture diffusivity
-__global__ void k_dt_visc1(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
-                           const double *__restrict__ W_i, double *__restrict__ red) {
+// one pass over the transport properties: per species the maximum of the
+// mixture diffusivity and the minimum of Delta^2 / D_l, and the minimum of
+// Delta^2 rho / (lambda / cp).  The species whose diffusivity maximum is
+// the largest is only known after the whole grid is reduced; keeping the
+// minimum for every species lets k_dt_final pick that species' value, which
+// is the minimum the reference takes over that one species.
+__global__ void k_dt_visc(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
+                          const double *__restrict__ W_i, const double *__restrict__ cp,
+                          const double *__restrict__ dx_i, const double *__restrict__ dy_i,
+                          const double *__restrict__ dz_i, double *__restrict__ red) {
   const long ncin = (long)d.nx * d.ny * d.nz;
-  double dmax[IZ_NRV];
+  double dmax[IZ_NRV], mins[IZ_NRV + 1];
   for (int l = 0; l < IZ_NRV; l++) dmax[l] = 0.0;
+  for (int l = 0; l <= IZ_NRV; l++) mins[l] = __longlong_as_double(0x7ff0000000000000LL);
   for (long tid = (long)blockIdx.x * blockDim.x + threadIdx.x; tid < ncin;
        tid += (long)gridDim.x * blockDim.x) {
     int i = (int)(tid % d.nx);
@@ -1048,48 +1056,26 @@ __global__ void k_dt_visc1(IzDims d, const double *__restrict__ v, const double 
     for (int l = 0; l < IZ_NRV; l++)
       X[l] = v[(IZ_NIV + l) * d.nc + c] / (v[c] * W_i[c] * c_thd.Wc[l]);
     d_transport(v[c], T[c], W_i[c], X, dm, mu, ct, false);
-    for (int l = 0; l < IZ_NRV; l++) dmax[l] = fmax(dmax[l], dm[l]);
+    const double d2 = d_filter_delta2(d, dx_i, dy_i, dz_i, i, j, k);
+#pragma unroll
+    for (int l = 0; l < IZ_NRV; l++) {
+      dmax[l] = fmax(dmax[l], dm[l]);
+      mins[l] = fmin(mins[l], d2 / (dm[l] + 0.0));
+    }
+    mins[IZ_NRV] = fmin(mins[IZ_NRV], d2 * v[c] / (ct / cp[c] + 0.0));
   }
   block_extreme_pos<IZ_NRV, true>(dmax, red + 1);
+  block_extreme_pos<IZ_NRV + 1, false>(mins, red + 12);
 }
 
-// pass 2: minima of Delta^2/D(ldmax) and Delta^2 rho/(lambda/cp)
-__global__ void k_dt_visc2(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
-                           const double *__restrict__ W_i, const double *__restrict__ cp,
-                           const double *__restrict__ dx_i, const double *__restrict__ dy_i,
-                           const double *__restrict__ dz_i, double *__restrict__ red) {
-  const long ncin = (long)d.nx * d.ny * d.nz;
-  // species of the global maximum: the first in species order
-  double gmax = red[1];
-  int ldmax = 0;
-  for (int l = 1; l < IZ_NRV; l++)
-    if (gmax < red[1 + l]) { gmax = red[1 + l]; ldmax = l; }
-  double ab[2];
-  ab[0] = ab[1] = __longlong_as_double(0x7ff0000000000000LL);  // +inf
-  for (long tid = (long)blockIdx.x * blockDim.x + threadIdx.x; tid < ncin;
-       tid += (long)gridDim.x * blockDim.x) {
-    int i = (int)(tid % d.nx);
-    long r = tid / d.nx;
-    int j = (int)(r % d.ny), k = (int)(r / d.ny);
-    long c = iz_cidx(d, i, j, k);
-    double X[IZ_NRV], dm[IZ_NRV], mu, ct;
-    for (int l = 0; l < IZ_NRV; l++)
-      X[l] = v[(IZ_NIV + l) * d.nc + c] / (v[c] * W_i[c] * c_thd.Wc[l]);
-    d_transport(v[c], T[c], W_i[c], X, dm, mu, ct, false);
-    double d2 = d_filter_delta2(d, dx_i, dy_i, dz_i, i, j, k);
-    ab[0] = fmin(ab[0], d2 / (dm[ldmax] + 0.0));
-    ab[1] = fmin(ab[1], d2 * v[c] / (ct / cp[c] + 0.0));
-  }
-  block_extreme_pos<2, false>(ab, red + 10);
-}
-
-// red layout: [0] dtc, [1..9] dm max, [10] min d2/D, [11] min d2 rho cp/lambda
+// red layout: [0] dtc, [1..9] dm max, [12..20] min d2/D_l per species,
+// [21] min d2 rho cp/lambda
 // ts: [0..3] dt_4, [4] dtmin (in: previous), [5] DTmax (K)
 __global__ void k_dt_init(double *red) {
   int t = threadIdx.x;
   if (t == 0) red[0] = 1.0;
   if (t >= 1 && t <= 9) red[t] = 0.0;
-  if (t == 10 || t == 11) red[t] = __longlong_as_double(0x7ff0000000000000LL);
+  if (t >= 12 && t <= 21) red[t] = __longlong_as_double(0x7ff0000000000000LL);
 }
 
 __global__ void k_dt_final(const double *__restrict__ red, double CFL, double CFLmin, double Fo,
@@ -1098,8 +1084,13 @@ __global__ void k_dt_final(const double *__restrict__ red, double CFL, double CF
   double dt4[4] = {1.0, 1.0, 1.0, 1.0};
   dt4[0] = CFL / red[0];
   if (visdt) {
-    dt4[1] = rvd0_i * Fo * red[10];
-    dt4[2] = rlamcp0_i * Fo * red[11];
+    // species of the global diffusivity maximum: the first in species order
+    double gmax = red[1];
+    int ldmax = 0;
+    for (int l = 1; l < IZ_NRV; l++)
+      if (gmax < red[1 + l]) { gmax = red[1 + l]; ldmax = l; }
+    dt4[1] = rvd0_i * Fo * red[12 + ldmax];
+    dt4[2] = rlamcp0_i * Fo * red[12 + IZ_NRV];
   }
   if (reaction) {
     dt4[3] = ts[4] * CPR / ts[5];
@@ -1214,7 +1205,7 @@ extern "C" int izem_create(izem_ctx **out, const izem_grid *g, const double *thd
   ok = ok && hipMalloc(&c->masks, sizeof(unsigned long long) * 8) == hipSuccess;
   ok = ok && hipMalloc(&c->status, sizeof(int) * 2) == hipSuccess;
   ok = ok && hipMalloc(&c->iters, sizeof(int) * 8) == hipSuccess;
-  ok = ok && hipMalloc(&c->red, sizeof(double) * 12) == hipSuccess;
+  ok = ok && hipMalloc(&c->red, sizeof(double) * 24) == hipSuccess;
   ok = ok && hipMalloc(&c->ts, sizeof(double) * 6) == hipSuccess;
   if (!ok) { izem_destroy(c); return IZEM_E_NOMEM; }
   // fields are read at ghost edges/corners never; keep scratch defined
@@ -1606,10 +1597,8 @@ extern "C" int izem_timestep(izem_ctx *c, const double *dx_i, const double *dy_i
                      dz_i, c->red);
   int visdt = c->o.vis && c->o.vis_dt;
   if (visdt) {
-    hipLaunchKernelGGL(k_dt_visc1, dim3(iz_blocks_capped(c, n_interior(d), 128, 32)), dim3(128), 0, c->stream,
-                       d, v, T, W_i, c->red);
-    hipLaunchKernelGGL(k_dt_visc2, dim3(iz_blocks_capped(c, n_interior(d), 128, 32)), dim3(128), 0, c->stream,
-                       d, v, T, W_i, cp, dx_i, dy_i, dz_i, c->red);
+    hipLaunchKernelGGL(k_dt_visc, dim3(iz_blocks_capped(c, n_interior(d), 128, 32)), dim3(128),
+                       0, c->stream, d, v, T, W_i, cp, dx_i, dy_i, dz_i, c->red);
   }
   double rvd0_i = c->o.Sc / c->o.sqgmr, rlamcp0_i = c->o.Pr / c->o.sqgmr;
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, c->stream, c->red, c->o.CFL, c->o.CFLmin,

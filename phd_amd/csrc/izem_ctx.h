@@ -49,7 +49,7 @@ struct izem_ctx {
   unsigned long long *masks;
   int *status;
   int *iters;
-  double *red;  // timestep reductions [12]
+  double *red;  // timestep reductions [24]
   double *ts;   // dt_4[4], dtmin, DTmax
   IzChem *chem;
   IzProf prof;

@@ -175,6 +175,43 @@ __global__ void k_colsum(int M, int N, const u16 *__restrict__ G, long ld, int r
   part[(long)s * N + n] = a;
 }
 
+// the same for N and ld multiples of 8 (16-byte rows of 8 columns): each
+// thread sums 8 columns of every (256 / (N / 8))-th row of its slice, and
+// the row groups of a block are added in a fixed order, so the sums do
+// not depend on timing
+__global__ void __launch_bounds__(256) k_colsum8(int M, int N, const u16 *__restrict__ G, long ld,
+                                                 int rows, float *__restrict__ part) {
+  __shared__ float red[256 * 8];
+  const int cpr = N / 8;               // 16-byte chunks per row
+  const int rg = 256 / cpr;            // rows per block iteration
+  const int t = threadIdx.x;
+  const int ch = t % cpr, g = t / cpr;
+  const int s = blockIdx.x;
+  const int r0 = s * rows, r1 = min(M, r0 + rows);
+  float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (g < rg) {
+    for (int r = r0 + g; r < r1; r += rg) {
+      const uint4 w = *reinterpret_cast<const uint4 *>(G + (long)r * ld + ch * 8);
+      const unsigned u[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        a[2 * q] += __uint_as_float(u[q] << 16);
+        a[2 * q + 1] += __uint_as_float(u[q] & 0xffff0000u);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) red[t * 8 + q] = a[q];
+  __syncthreads();
+  // thread (chunk, column) adds the row groups in order
+  if (t < N) {
+    const int c = t / 8, q = t % 8;
+    float sum = 0.f;
+    for (int gg = 0; gg < rg; gg++) sum += red[(gg * cpr + c) * 8 + q];
+    part[(long)s * N + t] = sum;
+  }
+}
+
 // Adam (Kingma & Ba 2015) on an FP32 master matrix P[rows][cols] (row-major)
 // with gradient G, moments m, v; writes BF16 copies P^T (ldt) and P (ldw)
 // when given.  Step t >= 1.
@@ -213,15 +250,16 @@ __global__ void k_mse(int Bn, int nout, int npad, const float *__restrict__ pred
                       const float *__restrict__ tgt, long ldt, float scale,
                       u16 *__restrict__ dY, long ldd, float *__restrict__ loss) {
   __shared__ float red[4];
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)Bn * npad;
   float sq = 0.f;
-  if (i < (long)Bn * npad) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
     const long b = i / npad;
     const int j = (int)(i % npad);
     float g = 0.f;
     if (j < nout) {
       const float d = pred[b * ldp + j] - tgt[b * ldt + j];
-      sq = d * d;
+      sq += d * d;
       g = 2.f * d * scale;
     }
     dY[b * ldd + j] = f2bf(g);
@@ -304,8 +342,12 @@ extern "C" int izem_mlp_colsum(izem_ctx *ctx, int M, int N, const void *G, long 
   if (!ctx || !G || !work || !out || M <= 0 || N <= 0 || nslices < 1) return IZEM_E_ARG;
   const int rows = (M + nslices - 1) / nslices;
   const int ns = (M + rows - 1) / rows;
-  hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256, ns), dim3(256), 0, ctx->stream, M, N,
-                     (const u16 *)G, ld, rows, work);
+  if (N % 8 == 0 && ld % 8 == 0 && N <= 256 && ((uintptr_t)G & 15) == 0)
+    hipLaunchKernelGGL(k_colsum8, dim3(ns), dim3(256), 0, ctx->stream, M, N, (const u16 *)G, ld,
+                       rows, work);
+  else
+    hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256, ns), dim3(256), 0, ctx->stream, M, N,
+                       (const u16 *)G, ld, rows, work);
   hipLaunchKernelGGL(k_sum_slices, dim3(iz_blocks(N, 256)), dim3(256), 0, ctx->stream, (long)N,
                      ns, work, out);
   IZ_CHECK(hipGetLastError());
@@ -342,7 +384,9 @@ extern "C" int izem_mlp_mse(izem_ctx *ctx, int Bn, int nout, int npad, const flo
     return IZEM_E_ARG;
   const long n = (long)Bn * npad;
   const float scale = 1.f / ((float)Bn * (float)nout);
-  hipLaunchKernelGGL(k_mse, dim3(iz_blocks(n, 256)), dim3(256), 0, ctx->stream, Bn, nout, npad,
+  // a few blocks per CU: one same-address atomic per block
+  hipLaunchKernelGGL(k_mse, dim3(iz_blocks_capped(ctx, n, 256, 2)), dim3(256), 0, ctx->stream,
+                     Bn, nout, npad,
                      pred, ldp, tgt, ldt, scale, (u16 *)dY, ldd, loss);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
