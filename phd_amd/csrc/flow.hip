@@ -722,6 +722,7 @@ __device__ inline void d_transport(double rho, double T, double W_i, const doubl
     dm[m] = wrk1 * Kd / (s * P);
   }
   double w1 = 0.0, cc = 0.0;
+#pragma unroll
   for (int l = 0; l < IZ_NRV; l++) {
     const double *cf = c_thd.CofLam[l];
     double s = 0.0;
@@ -737,6 +738,7 @@ __device__ inline void d_transport(double rho, double T, double W_i, const doubl
   mu = 0.0;
   if (!want_mu) return;
   double eta[IZ_NRV];
+#pragma unroll
   for (int l = 0; l < IZ_NRV; l++) {
     const double *cf = c_thd.CofEta[l];
     double s = 0.0;
@@ -747,8 +749,10 @@ __device__ inline void d_transport(double rho, double T, double W_i, const doubl
     eta[l] = exp(s);
   }
   double mus = 0.0;
+#pragma unroll
   for (int m = 0; m < IZ_NRV; m++) {
     double c4 = 0.0;
+#pragma unroll
     for (int l = 0; l < IZ_NRV; l++) {
       double c3 = 1.0 + c_thd.wilke_c1[m][l] * sqrt(eta[m] / eta[l]);
       double c2 = c3 * c3 * c_thd.wilke_c2[m][l];
@@ -1038,13 +1042,16 @@ __device__ inline double d_filter_delta2(const IzDims &d, const double *dx_i, co
 // the largest is only known after the whole grid is reduced; keeping the
 // minimum for every species lets k_dt_final pick that species' value, which
 // is the minimum the reference takes over that one species.
-__global__ void k_dt_visc(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
+__global__ void __attribute__((amdgpu_flat_work_group_size(128, 128), amdgpu_waves_per_eu(2, 2)))
+k_dt_visc(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
                           const double *__restrict__ W_i, const double *__restrict__ cp,
                           const double *__restrict__ dx_i, const double *__restrict__ dy_i,
                           const double *__restrict__ dz_i, double *__restrict__ red) {
   const long ncin = (long)d.nx * d.ny * d.nz;
   double dmax[IZ_NRV], mins[IZ_NRV + 1];
+#pragma unroll
   for (int l = 0; l < IZ_NRV; l++) dmax[l] = 0.0;
+#pragma unroll
   for (int l = 0; l <= IZ_NRV; l++) mins[l] = __longlong_as_double(0x7ff0000000000000LL);
   for (long tid = (long)blockIdx.x * blockDim.x + threadIdx.x; tid < ncin;
        tid += (long)gridDim.x * blockDim.x) {
@@ -1053,6 +1060,7 @@ __global__ void k_dt_visc(IzDims d, const double *__restrict__ v, const double *
     int j = (int)(r % d.ny), k = (int)(r / d.ny);
     long c = iz_cidx(d, i, j, k);
     double X[IZ_NRV], dm[IZ_NRV], mu, ct;
+#pragma unroll
     for (int l = 0; l < IZ_NRV; l++)
       X[l] = v[(IZ_NIV + l) * d.nc + c] / (v[c] * W_i[c] * c_thd.Wc[l]);
     d_transport(v[c], T[c], W_i[c], X, dm, mu, ct, false);

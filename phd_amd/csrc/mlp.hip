@@ -29,14 +29,26 @@ enum {
 
 __device__ inline float bf2f(u16 h) { return __uint_as_float(((unsigned)h) << 16); }
 __device__ inline u16 f2bf(float f) {
-  // round to nearest even (finite inputs)
-  unsigned u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (u16)(u >> 16);
+  // round to nearest even: one v_cvt_pk_bf16_f32 on gfx950
+  return __builtin_bit_cast(u16, (__bf16)f);
 }
-__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, 7.1.26:
+// erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2) with
+// t = 1 / (1 + p x), absolute error below 1.5e-7 for x >= 0, odd extension.
+// Branch-free: the library erff takes a different polynomial on either side
+// of |x| = 1 and a wave whose lanes straddle it runs both; its extra accuracy
+// is lost in the BF16 activations anyway.
+__device__ inline float erf_as(float x) {
+  const float a = fabsf(x);
+  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * a);
+  const float poly =
+      t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f +
+                                                                         t * 1.061405429f))));
+  return copysignf(1.0f - poly * __expf(-a * a), x);
+}
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
 __device__ inline float gelu_d(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
+  return 0.5f * (1.0f + erf_as(z * 0.70710678118654752f)) +
          z * 0.39894228040143268f * __expf(-0.5f * z * z);
 }
 
