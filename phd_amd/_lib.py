@@ -46,11 +46,12 @@ _lib = None
 
 
 def load(path: str | None = None):
-    """Load the HIP library (raises when it has not been built)."""
+    """Load the HIP library (raises when it has not been built).  The
+    environment variable PHD_AMD_LIB names another build of it."""
     global _lib
     if _lib is not None:
         return _lib
-    path = path or LIB_PATH
+    path = path or os.environ.get("PHD_AMD_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise IzemError(f"{path} not built: run `python -m phd_amd.build` (hipcc, gfx950)")
     lib = C.CDLL(path)

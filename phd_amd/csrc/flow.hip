@@ -510,7 +510,8 @@ __device__ inline void block_extreme_pos(double *x, double *dst) {
   }
 }
 
-__global__ void k_force_probe(IzDims d, IzBox b, int forcesum, double *__restrict__ v,
+__global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(2, 2)))
+k_force_probe(IzDims d, IzBox b, int forcesum, double *__restrict__ v,
                               const double *__restrict__ T, double eps,
                               unsigned long long *__restrict__ mask) {
   int i, j, k;
@@ -540,7 +541,8 @@ __device__ inline int d_newton_count(unsigned long long mask, int itmax) {
 
 // Apply exactly N iterations, write T, W_i, cp (last evaluation, i.e. at the
 // pre-update T), ha (module_thermodynamics.f90:79-186, Wmix_i :700-728).
-__global__ void k_newton_apply(IzDims d, IzBox b, const double *__restrict__ v,
+__global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(2, 2)))
+k_newton_apply(IzDims d, IzBox b, const double *__restrict__ v,
                                double *__restrict__ T, double *__restrict__ W_i,
                                double *__restrict__ cp, double *__restrict__ ha,
                                const unsigned long long *__restrict__ mask, int itmax,
@@ -765,7 +767,8 @@ __device__ inline void d_transport(double rho, double T, double W_i, const doubl
 
 // stresses, heat flux and diffusion fluxes on a domain; the species slots of
 // fd are overwritten with rho Y_k V_k as in the reference
-__global__ void k_visc_flux(IzDims d, IzBox b, double sqgmr, double rvd0, double rlam0,
+__global__ void __attribute__((amdgpu_flat_work_group_size(128, 128), amdgpu_waves_per_eu(2)))
+k_visc_flux(IzDims d, IzBox b, double sqgmr, double rvd0, double rlam0,
                             const double *__restrict__ v, const double *__restrict__ T,
                             const double *__restrict__ W_i, const double *__restrict__ ha,
                             const double *__restrict__ prim, double *__restrict__ fd,
@@ -1217,13 +1220,14 @@ extern "C" int izem_create(izem_ctx **out, const izem_grid *g, const double *thd
   ok = ok && hipMalloc(&c->ts, sizeof(double) * 6) == hipSuccess;
   if (!ok) { izem_destroy(c); return IZEM_E_NOMEM; }
   // fields are read at ghost edges/corners never; keep scratch defined
-  hipMemsetAsync(c->prim, 0, sizeof(double) * (3 + IZ_NRV) * d.nc, c->stream);
-  hipMemsetAsync(c->fd, 0, sizeof(double) * c->nderiv * d.nc, c->stream);
-  hipMemsetAsync(c->sf, 0, sizeof(double) * c->nsf * d.nc, c->stream);
-  hipMemsetAsync(c->status, 0, sizeof(int) * 2, c->stream);
+  ok = hipMemsetAsync(c->prim, 0, sizeof(double) * (3 + IZ_NRV) * d.nc, c->stream) == hipSuccess;
+  ok = ok && hipMemsetAsync(c->fd, 0, sizeof(double) * c->nderiv * d.nc, c->stream) == hipSuccess;
+  ok = ok && hipMemsetAsync(c->sf, 0, sizeof(double) * c->nsf * d.nc, c->stream) == hipSuccess;
+  ok = ok && hipMemsetAsync(c->status, 0, sizeof(int) * 2, c->stream) == hipSuccess;
   double ts0[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 0.0};
-  hipMemcpyAsync(c->ts, ts0, sizeof(ts0), hipMemcpyHostToDevice, c->stream);
-  if (hipStreamSynchronize(c->stream) != hipSuccess) { izem_destroy(c); return IZEM_E_HIP; }
+  ok = ok && hipMemcpyAsync(c->ts, ts0, sizeof(ts0), hipMemcpyHostToDevice, c->stream) ==
+                 hipSuccess;
+  if (!ok || hipStreamSynchronize(c->stream) != hipSuccess) { izem_destroy(c); return IZEM_E_HIP; }
   *out = c;
   return IZEM_OK;
 }
@@ -1233,7 +1237,7 @@ extern "C" int izem_profile(izem_ctx *c, int capacity) {
   IzProf &p = c->prof;
   IZ_CHECK(hipStreamSynchronize(c->stream));
   if (p.ev) {
-    for (int k = 0; k < 2 * p.cap; k++) hipEventDestroy(p.ev[k]);
+    for (int k = 0; k < 2 * p.cap; k++) (void)hipEventDestroy(p.ev[k]);
     free(p.ev);
     free(p.cls);
     p.ev = nullptr;
@@ -1267,20 +1271,20 @@ extern "C" int izem_profile_read(izem_ctx *c, double *ms, int *count) {
 
 extern "C" int izem_destroy(izem_ctx *c) {
   if (!c) return IZEM_OK;
-  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   izem_profile(c, 0);
   iz_chem_destroy(c);
-  for (int q = 0; q < 3; q++) if (c->F[q]) hipFree(c->F[q]);
-  if (c->prim) hipFree(c->prim);
-  if (c->fd) hipFree(c->fd);
-  if (c->sf) hipFree(c->sf);
-  if (c->v0) hipFree(c->v0);
-  if (c->masks) hipFree(c->masks);
-  if (c->status) hipFree(c->status);
-  if (c->iters) hipFree(c->iters);
-  if (c->red) hipFree(c->red);
-  if (c->ts) hipFree(c->ts);
-  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  for (int q = 0; q < 3; q++) if (c->F[q]) (void)hipFree(c->F[q]);
+  if (c->prim) (void)hipFree(c->prim);
+  if (c->fd) (void)hipFree(c->fd);
+  if (c->sf) (void)hipFree(c->sf);
+  if (c->v0) (void)hipFree(c->v0);
+  if (c->masks) (void)hipFree(c->masks);
+  if (c->status) (void)hipFree(c->status);
+  if (c->iters) (void)hipFree(c->iters);
+  if (c->red) (void)hipFree(c->red);
+  if (c->ts) (void)hipFree(c->ts);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   free(c);
   return IZEM_OK;
 }

@@ -88,6 +88,7 @@ __device__ inline void d_species_h_cp(int is, double T, double Ya, double &h, do
 
 // ha_scalar: module_thermodynamics.f90:873-978
 __device__ inline void d_ha_scalar(double temp, double *ha) {
+#pragma unroll
   for (int is = 0; is < IZ_NRV; is++) {
     double h, cpc;
     d_species_h_cp(is, temp, 0.0, h, cpc);
@@ -107,6 +108,7 @@ __device__ inline double d_newton_cell(const double *vc, double T, double &cp_ou
 #pragma unroll
   for (int is = 0; is < IZ_NRV; is++) Ya[is] = vc[IZ_NIV + is] * rho_i;
   double cp = 0.0;
+#pragma unroll
   for (int is = 0; is < IZ_NRV; is++) {
     double wrk1 = -T;
     double h, cpc;
