@@ -671,31 +671,6 @@ __global__ void k_visc_prim(IzDims d, IzBox b, const double *__restrict__ v,
   for (int q = 0; q < d.ndim; q++) prim[q * d.nc + c] = v[(1 + q) * d.nc + c] * rho_i;
 }
 
-// first derivatives on the interior (d?_fixed1): fd slots in reference order
-__global__ void k_visc_grad(IzDims d, const double *__restrict__ prim,
-                            const double *__restrict__ T, const double *__restrict__ dx_i,
-                            const double *__restrict__ dy_i, const double *__restrict__ dz_i,
-                            double *__restrict__ fd) {
-  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long ncin = (long)d.nx * d.ny * d.nz;
-  if (tid >= ncin) return;
-  int i = (int)(tid % d.nx);
-  long r = tid / d.nx;
-  int j = (int)(r % d.ny), k = (int)(r / d.ny);
-  long c = iz_cidx(d, i, j, k);
-  const long strd[3] = {1, d.NX, d.NX * d.NY};
-  const double dinv[3] = {dx_i[i + d.ng], dy_i[j + d.ng], dz_i[k + d.ng]};
-  int slot = 0;
-  for (int q = 0; q < d.ndim; q++)
-    for (int dd = 0; dd < d.ndim; dd++)
-      fd[(long)(slot++) * d.nc + c] = d_deriv7(prim + q * d.nc, c, strd[dd], dinv[dd]);
-  for (int dd = 0; dd < d.ndim; dd++)
-    fd[(long)(slot++) * d.nc + c] = d_deriv7(T, c, strd[dd], dinv[dd]);
-  for (int l = 0; l < IZ_NRV; l++)
-    for (int dd = 0; dd < d.ndim; dd++)
-      fd[(long)(slot++) * d.nc + c] = d_deriv7(prim + (3 + l) * d.nc, c, strd[dd], dinv[dd]);
-}
-
 // mixture-averaged transport of one cell (prim_vis_vars :989-1230)
 __device__ inline void d_transport(double rho, double T, double W_i, const double *Xa, double *dm,
                                    double &mu, double &ct, bool want_mu) {
@@ -765,22 +740,40 @@ __device__ inline void d_transport(double rho, double T, double W_i, const doubl
   mu = mus;
 }
 
-// stresses, heat flux and diffusion fluxes on a domain; the species slots of
-// fd are overwritten with rho Y_k V_k as in the reference
+// Gradients and viscous fluxes in one pass (d?_fixed1 + the flux part of
+// viscflux_xy/_xyz, module_viscflux.f90:721-1135, 1660-2189).  The
+// reference stores the first derivatives of u, v, w, T and X_k for the
+// interior, copies them to the ghost slabs (comm) and evaluates the fluxes on
+// the interior and on each ghost slab.  Here each cell takes its derivatives
+// straight from the primitives: an interior cell at itself, a ghost-slab cell
+// at its periodic image in the interior, which is the value the copy would
+// have brought.  Everything else (transport, v, T, W_i, h_k) is read at the
+// cell itself, as in the reference.  Only the diffusion fluxes rho Y_k V_k
+// (into the species slots of fd) and the stresses and heat flux (sf) are
+// stored; the derivative slots of fd are not written.
 __global__ void __attribute__((amdgpu_flat_work_group_size(128, 128), amdgpu_waves_per_eu(2)))
-k_visc_flux(IzDims d, IzBox b, double sqgmr, double rvd0, double rlam0,
-                            const double *__restrict__ v, const double *__restrict__ T,
-                            const double *__restrict__ W_i, const double *__restrict__ ha,
-                            const double *__restrict__ prim, double *__restrict__ fd,
-                            double *__restrict__ sf) {
+k_visc_fused(IzDims d, IzBox b, int dom, double sqgmr, double rvd0, double rlam0,
+             const double *__restrict__ v, const double *__restrict__ T,
+             const double *__restrict__ W_i, const double *__restrict__ ha,
+             const double *__restrict__ prim, const double *__restrict__ dx_i,
+             const double *__restrict__ dy_i, const double *__restrict__ dz_i,
+             double *__restrict__ fd, double *__restrict__ sf) {
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (!box_cell(b, tid, i, j, k)) return;
   const long nc = d.nc;
-  long c = iz_cidx(d, i, j, k);
-  const int ndim = d.ndim, nfluid = ndim * (ndim + 1);
+  const long c = iz_cidx(d, i, j, k);
+  // derivative location: the periodic image of a ghost-slab cell
+  int ii = i, jj = j, kk = k;
+  if (dom == -1) ii += d.nx; else if (dom == 1) ii -= d.nx;
+  else if (dom == -2) jj += d.ny; else if (dom == 2) jj -= d.ny;
+  else if (dom == -3) kk += d.nz; else if (dom == 3) kk -= d.nz;
+  const long cd = iz_cidx(d, ii, jj, kk);
+  const long sx = 1, sy = d.NX, sz = d.NX * d.NY;
+  const double dxi = dx_i[ii + d.ng], dyi = dy_i[jj + d.ng], dzi = dz_i[kk + d.ng];
   const double twothirds = 2.0 / 3.0;
   double X[IZ_NRV], dm[IZ_NRV], mu, ct;
+#pragma unroll
   for (int l = 0; l < IZ_NRV; l++) X[l] = prim[(3 + l) * nc + c];
   const double r0 = v[c], Wi = W_i[c];
   d_transport(r0, T[c], Wi, X, dm, mu, ct, true);
@@ -789,75 +782,103 @@ k_visc_flux(IzDims d, IzBox b, double sqgmr, double rvd0, double rlam0,
   double rmu = -sqgmr * mu;
   double rvd = rvd0 * r0;
   double rlam = -rlam0 * ct;
-#define FD(s) fd[(long)((s) - 1) * nc + c]
-  if (ndim == 3) {
-    double f1 = FD(1), f2 = FD(2), f3 = FD(3), f4 = FD(4), f5 = FD(5), f6 = FD(6), f7 = FD(7),
-           f8 = FD(8), f9 = FD(9);
+  if (d.ndim == 3) {
+    const int nfluid = 12;
+    const double f1 = d_deriv7(prim, cd, sx, dxi), f2 = d_deriv7(prim, cd, sy, dyi),
+                 f3 = d_deriv7(prim, cd, sz, dzi);
+    const double f4 = d_deriv7(prim + nc, cd, sx, dxi), f5 = d_deriv7(prim + nc, cd, sy, dyi),
+                 f6 = d_deriv7(prim + nc, cd, sz, dzi);
+    const double f7 = d_deriv7(prim + 2 * nc, cd, sx, dxi),
+                 f8 = d_deriv7(prim + 2 * nc, cd, sy, dyi),
+                 f9 = d_deriv7(prim + 2 * nc, cd, sz, dzi);
     double sxx = twothirds * rmu * (f1 + f1 - (f5 + f9));
     double sxy = rmu * (f4 + f2);
     double sxz = rmu * (f7 + f3);
     double syy = twothirds * rmu * (f5 + f5 - (f1 + f9));
     double syz = rmu * (f8 + f6);
     double szz = twothirds * rmu * (f9 + f9 - (f1 + f5));
+    double gx[IZ_NRV], gy[IZ_NRV], gz[IZ_NRV];
+#pragma unroll
+    for (int l = 0; l < IZ_NRV; l++) {
+      const double *a = prim + (3 + l) * nc;
+      gx[l] = d_deriv7(a, cd, sx, dxi);
+      gy[l] = d_deriv7(a, cd, sy, dyi);
+      gz[l] = d_deriv7(a, cd, sz, dzi);
+    }
     double vdcx = 0.0, vdcy = 0.0, vdcz = 0.0;
+#pragma unroll
     for (int l = 1; l <= IZ_NRV; l++) {
       double wrk = dm[l - 1] * c_thd.Wc[l - 1] * Wi;
-      vdcx = vdcx + wrk * FD(nfluid + ndim * l - 2);
-      vdcy = vdcy + wrk * FD(nfluid + ndim * l - 1);
-      vdcz = vdcz + wrk * FD(nfluid + ndim * l);
+      vdcx = vdcx + wrk * gx[l - 1];
+      vdcy = vdcy + wrk * gy[l - 1];
+      vdcz = vdcz + wrk * gz[l - 1];
     }
     double vdhx = 0.0, vdhy = 0.0, vdhz = 0.0;
+#pragma unroll
     for (int l = 1; l <= IZ_NRV; l++) {
       double Ya = v[(IZ_NIV + l - 1) * nc + c] * rho_i;
       double vdtx = vdcx * Ya, vdty = vdcy * Ya, vdtz = vdcz * Ya;
       double wrk = -dm[l - 1] * c_thd.Wc[l - 1] * Wi;
-      vdtx = vdtx + wrk * FD(nfluid + ndim * l - 2);
-      vdty = vdty + wrk * FD(nfluid + ndim * l - 1);
-      vdtz = vdtz + wrk * FD(nfluid + ndim * l);
+      vdtx = vdtx + wrk * gx[l - 1];
+      vdty = vdty + wrk * gy[l - 1];
+      vdtz = vdtz + wrk * gz[l - 1];
       double jx = rvd * vdtx, jy = rvd * vdty, jz = rvd * vdtz;
-      FD(nfluid + ndim * l - 2) = jx;
-      FD(nfluid + ndim * l - 1) = jy;
-      FD(nfluid + ndim * l) = jz;
+      fd[(long)(nfluid + 3 * l - 3) * nc + c] = jx;
+      fd[(long)(nfluid + 3 * l - 2) * nc + c] = jy;
+      fd[(long)(nfluid + 3 * l - 1) * nc + c] = jz;
       double h = ha[(long)(l - 1) * nc + c];
       vdhx = vdhx + h * jx;
       vdhy = vdhy + h * jy;
       vdhz = vdhz + h * jz;
     }
+    const double tx = d_deriv7(T, cd, sx, dxi), ty = d_deriv7(T, cd, sy, dyi),
+                 tz = d_deriv7(T, cd, sz, dzi);
     sf[0 * nc + c] = sxx; sf[1 * nc + c] = sxy; sf[2 * nc + c] = sxz;
     sf[3 * nc + c] = syy; sf[4 * nc + c] = syz; sf[5 * nc + c] = szz;
-    sf[6 * nc + c] = uxs * sxx + vys * sxy + wzs * sxz + rlam * FD(10) + vdhx;
-    sf[7 * nc + c] = uxs * sxy + vys * syy + wzs * syz + rlam * FD(11) + vdhy;
-    sf[8 * nc + c] = uxs * sxz + vys * syz + wzs * szz + rlam * FD(12) + vdhz;
+    sf[6 * nc + c] = uxs * sxx + vys * sxy + wzs * sxz + rlam * tx + vdhx;
+    sf[7 * nc + c] = uxs * sxy + vys * syy + wzs * syz + rlam * ty + vdhy;
+    sf[8 * nc + c] = uxs * sxz + vys * syz + wzs * szz + rlam * tz + vdhz;
   } else {
-    double f1 = FD(1), f2 = FD(2), f3 = FD(3), f4 = FD(4);
+    const int nfluid = 6;
+    const double f1 = d_deriv7(prim, cd, sx, dxi), f2 = d_deriv7(prim, cd, sy, dyi);
+    const double f3 = d_deriv7(prim + nc, cd, sx, dxi), f4 = d_deriv7(prim + nc, cd, sy, dyi);
     double sxx = twothirds * rmu * (f1 + f1 - f4);
     double sxy = rmu * (f3 + f2);
     double syy = twothirds * rmu * (f4 + f4 - f1);
+    double gx[IZ_NRV], gy[IZ_NRV];
+#pragma unroll
+    for (int l = 0; l < IZ_NRV; l++) {
+      const double *a = prim + (3 + l) * nc;
+      gx[l] = d_deriv7(a, cd, sx, dxi);
+      gy[l] = d_deriv7(a, cd, sy, dyi);
+    }
     double vdcx = 0.0, vdcy = 0.0;
+#pragma unroll
     for (int l = 1; l <= IZ_NRV; l++) {
       double wrk = dm[l - 1] * c_thd.Wc[l - 1] * Wi;
-      vdcx = vdcx + wrk * FD(nfluid + ndim * l - 1);
-      vdcy = vdcy + wrk * FD(nfluid + ndim * l);
+      vdcx = vdcx + wrk * gx[l - 1];
+      vdcy = vdcy + wrk * gy[l - 1];
     }
     double vdhx = 0.0, vdhy = 0.0;
+#pragma unroll
     for (int l = 1; l <= IZ_NRV; l++) {
       double Ya = v[(IZ_NIV + l - 1) * nc + c] * rho_i;
       double vdtx = vdcx * Ya, vdty = vdcy * Ya;
       double wrk = -dm[l - 1] * c_thd.Wc[l - 1] * Wi;
-      vdtx = vdtx + wrk * FD(nfluid + ndim * l - 1);
-      vdty = vdty + wrk * FD(nfluid + ndim * l);
+      vdtx = vdtx + wrk * gx[l - 1];
+      vdty = vdty + wrk * gy[l - 1];
       double jx = rvd * vdtx, jy = rvd * vdty;
-      FD(nfluid + ndim * l - 1) = jx;
-      FD(nfluid + ndim * l) = jy;
+      fd[(long)(nfluid + 2 * l - 2) * nc + c] = jx;
+      fd[(long)(nfluid + 2 * l - 1) * nc + c] = jy;
       double h = ha[(long)(l - 1) * nc + c];
       vdhx = vdhx + h * jx;
       vdhy = vdhy + h * jy;
     }
+    const double tx = d_deriv7(T, cd, sx, dxi), ty = d_deriv7(T, cd, sy, dyi);
     sf[0 * nc + c] = sxx; sf[1 * nc + c] = sxy; sf[2 * nc + c] = syy;
-    sf[3 * nc + c] = uxs * sxx + vys * sxy + rlam * FD(5) + vdhx;
-    sf[4 * nc + c] = uxs * sxy + vys * syy + rlam * FD(6) + vdhy;
+    sf[3 * nc + c] = uxs * sxx + vys * sxy + rlam * tx + vdhx;
+    sf[4 * nc + c] = uxs * sxy + vys * syy + rlam * ty + vdhy;
   }
-#undef FD
 }
 
 // viscous divergence of one cell into out[0..13] (0 for the continuity and,
@@ -1424,16 +1445,12 @@ static int visc_fluxes(izem_ctx *c, const double *dx_i, const double *dy_i, cons
     hipLaunchKernelGGL(k_visc_prim, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0, c->stream,
                        d, b, v, W_i, c->prim);
   }
-  hipLaunchKernelGGL(k_visc_grad, dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0, c->stream,
-                     d, c->prim, T, dx_i, dy_i, dz_i, c->fd);
-  IZ_CHECK(hipGetLastError());
-  int r = iz_comm_cons(c, c->fd, c->nderiv);
-  if (r) return r;
   const double rvd0 = c->o.sqgmr / c->o.Sc, rlam0 = c->o.sqgmr * c->o.ggmopr;
   for (int dom = -d.ndim; dom <= d.ndim; dom++) {
     IzBox b = domain_box(d, dom);
-    hipLaunchKernelGGL(k_visc_flux, dim3(iz_blocks(box_count(b), 128)), dim3(128), 0, c->stream,
-                       d, b, c->o.sqgmr, rvd0, rlam0, v, T, W_i, ha, c->prim, c->fd, c->sf);
+    hipLaunchKernelGGL(k_visc_fused, dim3(iz_blocks(box_count(b), 128)), dim3(128), 0,
+                       c->stream, d, b, dom, c->o.sqgmr, rvd0, rlam0, v, T, W_i, ha, c->prim,
+                       dx_i, dy_i, dz_i, c->fd, c->sf);
   }
   iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
