@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-steps/s 256^3 reacting H2 (FP64); chem cells/s surrogate vs VODE @|dT|<=2K"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_BF16_TFLOPS = 2500.0  # dense MFMA
 PEAK_HBM_GBS = 8000.0
 
 
@@ -157,6 +158,21 @@ def surrogate_rate(s, adi, n: int, dt_half: float):
     Tt, rt, Yt = autotune.sample_reacting_cells(s, 2 * n, seed=14)
     L = su.label_cells(s, Tt.cpu().numpy(), rt.cpu().numpy(), Yt.cpu().numpy(), dt_half)
     res = su.evaluate(sur, L)
+    # the fused inference kernel alone, event-timed on the library stream
+    from phd_amd.solver import profile_read, profile_start
+    X = sur.features(L.T, L.rho, L.Y, L.dt)
+    sur.infer(X)
+    profile_start(s)
+    for _ in range(5):
+        sur.infer(X)
+    c_ms, c_cnt = profile_read(s)["chemistry"]
+    avg_s = c_ms / c_cnt / 1e3
+    flop = 2.0 * X.shape[0] * sum(k * n for k, n in su.TRUE_DIMS)
+    res["roofline_mlp"] = {"kernel": "k_mlp_fused", "bound": "mfma",
+                           "achieved": flop / avg_s / 1e12, "peak": PEAK_BF16_TFLOPS,
+                           "unit": "TFLOP/s", "frac": flop / avg_s / 1e12 / PEAK_BF16_TFLOPS,
+                           "avg_launch_ms": avg_s * 1e3, "rows": int(X.shape[0]),
+                           "flop_per_row": flop / X.shape[0]}
     res.update({"train_samples": sum(p.T.numel() for p in train), "train_epochs": 8,
                 "final_loss": hist[-1], "train_and_calibrate_s": t_train,
                 "tol_T_K": cal["tol_T"], "dT_max_K": cal["dT_max"],
