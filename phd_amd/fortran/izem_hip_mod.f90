@@ -45,6 +45,7 @@ module izem_hip_mod
             izem_upd_prim_var_woT, izem_comm_cons, izem_upd_prim_var_ghost, &
             izem_shock_sensor, izem_rk3, izem_rhs, izem_timestep, izem_chem_setup, &
             izem_chem_tune, izem_chem_stats, izem_ckwyr, izem_psr_batch, izem_reaction_dvode
+  public :: izem_surr_features, izem_mlp_infer, izem_surr_apply, izem_surr_fallback
   public :: izem_check, hip_rk3, hip_timestep, hip_euler_llf, hip_viscflux, hip_reaction_selec
 
   ! ------------------------------------------------------------ part 1
@@ -206,6 +207,38 @@ module izem_hip_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx, dt, T, W_i, cp, ha, v
       real(c_double), value :: scale
+    end function
+    ! chemistry surrogate (no reference counterpart): apply a trained network
+    ! to n cells, with the BDF fallback for the flagged ones
+    integer(c_int) function izem_surr_features(ctx, n, T, rho, Y, dt, mean, istd, X) &
+        bind(C, name="izem_surr_features")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, T, rho, Y, mean, istd, X
+      integer(c_int), value :: n
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function izem_mlp_infer(ctx, M, X, Wt, b, pred) bind(C, name="izem_mlp_infer")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, X, pred
+      integer(c_int), value :: M
+      type(c_ptr), intent(in) :: Wt(4), b(4)
+    end function
+    integer(c_int) function izem_surr_apply(ctx, n, T, Y, pred, ldp, omean, ostd, oc, proj, X, &
+                                            feat_lim, tol_T, dT_max, flag, T_energy) &
+        bind(C, name="izem_surr_apply")
+      import :: c_int, c_ptr, c_long, c_float, c_double
+      type(c_ptr), value :: ctx, T, Y, pred, omean, ostd, oc, proj, X, flag, T_energy
+      integer(c_int), value :: n
+      integer(c_long), value :: ldp
+      real(c_float), value :: feat_lim
+      real(c_double), value :: tol_T, dT_max
+    end function
+    integer(c_int) function izem_surr_fallback(ctx, n, T, rho, Y, dt, flag, counter, idx, Tw, rw, &
+                                               Yw, nfall) bind(C, name="izem_surr_fallback")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, T, rho, Y, flag, counter, idx, Tw, rw, Yw, nfall
+      integer(c_int), value :: n
+      real(c_double), value :: dt
     end function
   end interface
 
