@@ -46,7 +46,7 @@ def _relerr(a, b, box):
     return np.array(out)
 
 
-CASES = [("2d", 24), ("3d", 12)]
+CASES = [("2d", 24), ("3d", 12), ("2d", 7), ("3d", 8)]  # the last two: minimum and small boxes
 
 
 @pytest.mark.parametrize("kind,n", CASES)
