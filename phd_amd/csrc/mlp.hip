@@ -406,44 +406,62 @@ extern "C" int izem_mlp_mse(izem_ctx *ctx, int Bn, int nout, int npad, const flo
 
 // -------------------------------------------------------------------
 // Fused inference: all four layers of the surrogate for a tile of 64
-// cells in one block, activations kept in LDS (two 64 x 256 BF16 buffers)
-// and weights read as MFMA operands straight from L2 (every block uses the
-// same 273 KB).  Persistent grid over row tiles.  Weights Wt_l are stored
-// [N][K] (the B operand of a 16x16x32 MFMA is then one 16-byte load per
-// lane); layer widths K = 32 | 256 | 256 | 256, N = 256 | 256 | 256 | 64.
+// cells in one block.  The activations stay in one LDS buffer (64 x 256
+// BF16), overwritten in place: every wave reads the whole input tile, so
+// the block synchronises once after the products and once after the
+// epilogue has written the layer's output.  The weights are read as MFMA
+// operands straight from L2 (every block uses the same 273 KB), the
+// fragments of the next 32-deep slice loaded while the current one is in
+// the matrix cores.  Persistent grid over row tiles.  Weights Wt_l are
+// stored [N][K] (the B operand of a 16x16x32 MFMA is then one 16-byte load
+// per lane); layer widths K = 32 | 256 | 256 | 256, N = 256 | 256 | 256 | 64.
 // -------------------------------------------------------------------
 #define FX 264  // LDS row stride of an activation tile (bf16): 528 B, 16-byte aligned
 
 template <int K, int N, bool LAST>
-__device__ inline void fused_layer(const u16 (*Xin)[FX], u16 (*Xout)[FX],
-                                   const u16 *__restrict__ Wt, const float *__restrict__ b,
-                                   float *__restrict__ pred, int row0, int M) {
+__device__ inline void fused_layer(u16 (*X)[FX], const u16 *__restrict__ Wt,
+                                   const float *__restrict__ b, float *__restrict__ pred,
+                                   int row0, int M) {
   constexpr int NW = N / 4;        // columns per wave
   constexpr int CB = NW / 16;      // 16-column blocks per wave
+  constexpr int KS = K / 32;       // 32-deep slices
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   f32x4 acc[4][CB];
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < CB; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    bf16x8 bfr[CB];
+  const u16 *wrow[CB];
 #pragma unroll
-    for (int j = 0; j < CB; j++) {
-      const int n = w * NW + j * 16 + (lane & 15);
-      const uint4 q = *(const uint4 *)(Wt + (long)n * K + k0 + (lane >> 4) * 8);
-      bfr[j] = __builtin_bit_cast(bf16x8, q);
+  for (int j = 0; j < CB; j++)
+    wrow[j] = Wt + (long)(w * NW + j * 16 + (lane & 15)) * K + (lane >> 4) * 8;
+  bf16x8 bcur[CB];
+#pragma unroll
+  for (int j = 0; j < CB; j++) bcur[j] = __builtin_bit_cast(bf16x8, *(const uint4 *)wrow[j]);
+#pragma unroll 1
+  for (int ks = 0; ks < KS; ks++) {
+    const int k0 = ks * 32;
+    // next slice's weight fragments, in flight during this slice's products
+    bf16x8 bnext[CB];
+    if (ks + 1 < KS) {
+#pragma unroll
+      for (int j = 0; j < CB; j++)
+        bnext[j] = __builtin_bit_cast(bf16x8, *(const uint4 *)(wrow[j] + k0 + 32));
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const uint4 q = *(const uint4 *)&Xin[i * 16 + (lane & 15)][k0 + (lane >> 4) * 8];
+      const uint4 q = *(const uint4 *)&X[i * 16 + (lane & 15)][k0 + (lane >> 4) * 8];
       const bf16x8 af = __builtin_bit_cast(bf16x8, q);
 #pragma unroll
       for (int j = 0; j < CB; j++)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bcur[j], acc[i][j], 0, 0, 0);
+    }
+    if (ks + 1 < KS) {
+#pragma unroll
+      for (int j = 0; j < CB; j++) bcur[j] = bnext[j];
     }
   }
+  if (!LAST) __syncthreads();  // every wave has read the input tile
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -456,19 +474,25 @@ __device__ inline void fused_layer(const u16 (*Xin)[FX], u16 (*Xout)[FX],
         if (LAST) {
           if (row0 + m < M) pred[(long)(row0 + m) * N + n] = z;
         } else {
-          Xout[m][n] = f2bf(gelu(z));
+          X[m][n] = f2bf(gelu(z));
         }
       }
+  if (!LAST) __syncthreads();  // the output tile is complete
 }
 
-__global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(2, 2)))
+// resident 256-thread blocks per CU (one wave per SIMD each)
+#ifndef IZ_MLP_WAVES
+#define IZ_MLP_WAVES 3
+#endif
+
+__global__ void __attribute__((amdgpu_flat_work_group_size(256, 256),
+                               amdgpu_waves_per_eu(IZ_MLP_WAVES, IZ_MLP_WAVES)))
 k_mlp_fused(int M, const u16 *__restrict__ X, const u16 *__restrict__ W0,
             const u16 *__restrict__ W1, const u16 *__restrict__ W2, const u16 *__restrict__ W3,
             const float *__restrict__ b0, const float *__restrict__ b1,
             const float *__restrict__ b2, const float *__restrict__ b3,
             float *__restrict__ pred) {
-  __shared__ __attribute__((aligned(16))) u16 A0[64][FX];
-  __shared__ __attribute__((aligned(16))) u16 A1[64][FX];
+  __shared__ __attribute__((aligned(16))) u16 A[64][FX];
   const int ntiles = (M + 63) / 64;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int row0 = tile * 64;
@@ -477,17 +501,14 @@ k_mlp_fused(int M, const u16 *__restrict__ X, const u16 *__restrict__ W0,
       const int t = threadIdx.x, r = t >> 2, c = (t & 3) * 8;
       uint4 q = make_uint4(0, 0, 0, 0);
       if (row0 + r < M) q = *(const uint4 *)(X + (long)(row0 + r) * 32 + c);
-      *(uint4 *)&A0[r][c] = q;
+      *(uint4 *)&A[r][c] = q;
     }
     __syncthreads();
-    fused_layer<32, 256, false>(A0, A1, W0, b0, nullptr, row0, M);
-    __syncthreads();
-    fused_layer<256, 256, false>(A1, A0, W1, b1, nullptr, row0, M);
-    __syncthreads();
-    fused_layer<256, 256, false>(A0, A1, W2, b2, nullptr, row0, M);
-    __syncthreads();
-    fused_layer<256, 64, true>(A1, nullptr, W3, b3, pred, row0, M);
-    __syncthreads();
+    fused_layer<32, 256, false>(A, W0, b0, nullptr, row0, M);
+    fused_layer<256, 256, false>(A, W1, b1, nullptr, row0, M);
+    fused_layer<256, 256, false>(A, W2, b2, nullptr, row0, M);
+    fused_layer<256, 64, true>(A, W3, b3, pred, row0, M);
+    __syncthreads();  // the next tile's input overwrites A
   }
 }
 
@@ -498,7 +519,7 @@ extern "C" int izem_mlp_infer(izem_ctx *ctx, int M, const void *X, const void *c
     if (!Wt[l] || !b[l] || (((uintptr_t)Wt[l]) & 15)) return IZEM_E_ARG;
   if (((uintptr_t)X) & 15) return IZEM_E_ARG;
   const long ntiles = (M + 63) / 64;
-  long g = (long)ctx->ncu * 2;
+  long g = (long)ctx->ncu * IZ_MLP_WAVES;
   if (g > ntiles) g = ntiles;
   int pk = iz_prof_begin(ctx, IZP_CHEM);
   hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)g), dim3(256), 0, ctx->stream, M,
