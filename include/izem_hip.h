@@ -158,8 +158,9 @@ int izem_chem_setup(izem_ctx *ctx, const double *mech, long nmech, double rtol, 
  * kernel variant is compiled for (1, 2 or 4), resident 64-lane blocks per
  * CU, Jacobian reuse age in BDF steps, kinetics form (0 table-driven,
  * 1 straight-line code generated for the bundled mechanism -- IZEM_E_ARG if
- * the loaded mechanism is another one --, -1 straight-line when it
- * matches).  No counterpart in the reference (DVODE runs on the CPU). */
+ * the loaded mechanism is another one or waves is not 1 --, -1 straight-line
+ * when it matches and waves is 1, table-driven otherwise).  No counterpart
+ * in the reference (DVODE runs on the CPU). */
 int izem_chem_tune(izem_ctx *ctx, int waves, int blocks_per_cu, int jac_age, int kinetics);
 /* work counters of the chemistry integrations since the previous call:
  * out[0] BDF steps, out[1] RHS evaluations, out[2] Jacobians (LU

@@ -1,9 +1,9 @@
 """Autotuner of the batched chemistry kernels.
 
 The persistent work-queue kernels (izem_psr_batch, izem_reaction_dvode) are
-compiled in several variants: register budget (1, 2 or 4 waves per SIMD),
-table-driven or straight-line kinetics, and launched with a chosen number
-of resident 64-lane blocks per CU.  Which combination is fastest depends on
+compiled in several variants: table-driven kinetics at a register budget of
+1, 2 or 4 waves per SIMD, straight-line kinetics at 1 wave per SIMD, and
+launched with a chosen number of resident 64-lane blocks per CU.  Which combination is fastest depends on
 the cells' stiffness (steps per cell), the device and the mechanism.  The
 tuner times each candidate with izem_psr_batch on a sample of the cells the
 caller will integrate, keeps the fastest and applies it with
@@ -27,7 +27,8 @@ import torch
 from . import _lib
 from .codegen_mech import mechanism_hash
 
-CANDIDATES = [(w, b, k) for k in (1, 0) for w in (4, 2, 1) for b in (4 * w, 2 * w)]
+CANDIDATES = ([(1, b, 1) for b in (4, 2)] +
+              [(w, b, 0) for w in (4, 2, 1) for b in (4 * w, 2 * w)])
 JAC_AGE = 50  # DVODE MSBJ
 
 

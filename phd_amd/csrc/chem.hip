@@ -249,14 +249,19 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
 
 // parameters chosen by phd_amd.autotune: register budget, persistent blocks
 // per CU, Jacobian reuse age, kinetics form (0 tables, 1 straight-line,
-// -1 straight-line when it matches the loaded mechanism)
+// -1 straight-line when it matches the loaded mechanism and the budget).
+// The straight-line kinetics run at one wave per SIMD only: builds of them
+// at two and four waves per SIMD (history in private memory, several KB of
+// spills per lane) were seen to take other steps than a host build of the
+// same source, with the variant affected changing from one build to the
+// next, while the one-wave build follows it.
 extern "C" int izem_chem_tune(izem_ctx *ctx, int waves, int blocks_per_cu, int jac_age,
                               int kinetics) {
   if (!ctx || !ctx->chem || !(waves == 1 || waves == 2 || waves == 4) || blocks_per_cu < 1 ||
       blocks_per_cu > 64 || jac_age < 1 || kinetics < -1 || kinetics > 1)
     return IZEM_E_ARG;
-  if (kinetics == 1 && !ctx->chem->generated) return IZEM_E_ARG;
-  ctx->chem->use_gen = kinetics == 0 ? 0 : ctx->chem->generated;
+  if (kinetics == 1 && (!ctx->chem->generated || waves != 1)) return IZEM_E_ARG;
+  ctx->chem->use_gen = (kinetics == 0 || waves != 1) ? 0 : ctx->chem->generated;
   ctx->chem->waves = waves;
   ctx->chem->blocks_per_cu = blocks_per_cu;
   ctx->chem->jac_age = jac_age;
@@ -295,9 +300,7 @@ extern "C" int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho
   hipLaunchKernelGGL((k_psr_queue<K, W>), grid, blk, 0, ctx->stream, n, T, rho, Y, dt, ch->rtol, \
                      ch->atol, ch->jac_age, ch->counter, nsteps, ctx->status, ch->stats)
   if (ch->use_gen) {
-    if (ch->waves == 1) IZ_PSR(KinGen, 1);
-    else if (ch->waves == 2) IZ_PSR(KinGen, 2);
-    else IZ_PSR(KinGen, 4);
+    IZ_PSR(KinGen, 1);
   } else {
     if (ch->waves == 1) IZ_PSR(KinTables, 1);
     else if (ch->waves == 2) IZ_PSR(KinTables, 2);
@@ -338,9 +341,7 @@ extern "C" int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale
                      k1, dt, scale, ctx->o.time_ref, ctx->o.T_ref, rho_cgs, ch->rtol, ch->atol,   \
                      ch->jac_age, T, v, ch->counter, ctx->ts, ctx->status, ch->stats)
     if (ch->use_gen) {
-      if (ch->waves == 1) IZ_RQ(KinGen, 1);
-      else if (ch->waves == 2) IZ_RQ(KinGen, 2);
-      else IZ_RQ(KinGen, 4);
+      IZ_RQ(KinGen, 1);
     } else {
       if (ch->waves == 1) IZ_RQ(KinTables, 1);
       else if (ch->waves == 2) IZ_RQ(KinTables, 2);

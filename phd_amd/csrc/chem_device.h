@@ -553,24 +553,33 @@ struct KinGen {
   // perturbs a local copy of the state: perturbing the caller's vector in
   // place and restoring it gave wrong Jacobians in a GPU build of the
   // unrolled loop (correct on the CPU), so the state is never written here
+  // The temperature column comes first, with the rate constants of the
+  // perturbed temperature; the constants of the base state are built after
+  // it, so that only one set of them is live at a time.
   IZ_MHD inline void jac(double rho, const double *y, const double *f0, double hl0,
                          const double *ewt, double r0, double (*P)[N_]) const {
+    {
+      double yl[N_], ftmp[N_], kt0[GEN_NKT];
+#pragma unroll
+      for (int n = 0; n < N_; n++) yl[n] = y[n];
+      const double r = d_fd_increment(0, yl[0], ewt[0], r0);
+      yl[0] = yl[0] + r;
+      gen_kT(yl[0], kt0);
+      gen_rhs(rho, yl, kt0, ftmp);
+      const double ri = 1.0 / r;
+#pragma unroll
+      for (int i = 0; i < N_; i++) P[i][0] = -hl0 * (ftmp[i] - f0[i]) * ri;
+    }
     double kt[GEN_NKT];
     gen_kT(y[0], kt);
-    for (int j = 0; j < N_; j++) {
+    for (int j = 1; j < N_; j++) {
       double yl[N_], ftmp[N_];
 #pragma unroll
       for (int n = 0; n < N_; n++) yl[n] = y[n];
       const double yj = yl[j];
       const double r = d_fd_increment(j, yj, ewt[j], r0);
       yl[j] = yj + r;
-      if (j == 0) {
-        double kt0[GEN_NKT];
-        gen_kT(yl[0], kt0);
-        gen_rhs(rho, yl, kt0, ftmp);
-      } else {
-        gen_rhs(rho, yl, kt, ftmp);
-      }
+      gen_rhs(rho, yl, kt, ftmp);
       const double ri = 1.0 / r;
 #pragma unroll
       for (int i = 0; i < N_; i++) P[i][j] = -hl0 * (ftmp[i] - f0[i]) * ri;
