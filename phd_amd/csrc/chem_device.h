@@ -133,12 +133,18 @@ IZ_HD inline void d_reactor_rhs(double rho, const double *z, double *zp) {
 // ------------------------------------------------------------ BDF solver
 #define N_ CK_NEQ
 
-IZ_HD inline double d_wrms(const double *v, const double *ewt) {
+// weighted RMS norm with the reciprocal error weights w = 1 / (rtol |y| + atol)
+// that DVODE keeps in EWT (vode.f:5624; DVNORM, vode.f:4260-4282)
+IZ_HD inline double d_wrms(const double *v, const double *w) {
   double s = 0.0;
 #pragma unroll
-  for (int i = 0; i < N_; i++) { double x = v[i] / ewt[i]; s += x * x; }
+  for (int i = 0; i < N_; i++) { double x = v[i] * w[i]; s += x * x; }
   return sqrt(s / N_);
 }
+
+// x^e for x >= 0: the step-ratio powers of the step and order selection
+// (DVSTEP's (BIAS*DSM)**(1/L), vode.f:6516, 6693-6708)
+IZ_HD inline double d_powpos(double x, double e) { return exp(e * log(x)); }
 
 // LU with partial pivoting in LINPACK's form (dgefa / dgesl, vode.f:7033,
 // 7630): step k swaps rows k and p over columns k..N-1 only, so the
@@ -146,6 +152,8 @@ IZ_HD inline double d_wrms(const double *v, const double *ewt) {
 // applies them, interleaved with the same swaps.  Every index is a
 // compile-time constant, so that the factors can stay in registers: the
 // row swap is a run of selects, skipped when no lane of the wave pivots.
+// The diagonal keeps the reciprocal of each pivot (0 for a missing one), so
+// the back substitution multiplies where dgesl divides.
 IZ_HD inline bool d_lu(double (&A)[N_][N_], int (&piv)[N_]) {
   bool ok = true;
 #pragma unroll
@@ -174,6 +182,7 @@ IZ_HD inline bool d_lu(double (&A)[N_][N_], int (&piv)[N_]) {
     }
     if (amax == 0.0) ok = false;  // dgefa's INFO = k: column k has no pivot
     const double inv = (amax == 0.0) ? 0.0 : 1.0 / A[k][k];
+    A[k][k] = inv;
 #pragma unroll
     for (int i = k + 1; i < N_; i++) {
       const double m = A[i][k] * inv;
@@ -207,7 +216,7 @@ IZ_HD inline void d_lusolve(const double (&A)[N_][N_], const int (&piv)[N_], dou
     double s = x[k];
 #pragma unroll
     for (int j = k + 1; j < N_; j++) s -= A[k][j] * x[j];
-    x[k] = s / A[k][k];
+    x[k] = s * A[k][k];
   }
 }
 
@@ -226,10 +235,10 @@ struct ZLds {
 };
 
 template <class Z>
-IZ_HD inline double d_wrms_col(Z &z, int j, const double *ewt) {
+IZ_HD inline double d_wrms_col(Z &z, int j, const double *w) {
   double s = 0.0;
 #pragma unroll
-  for (int i = 0; i < N_; i++) { double x = z(j, i) / ewt[i]; s += x * x; }
+  for (int i = 0; i < N_; i++) { double x = z(j, i) * w[i]; s += x * x; }
   return sqrt(s / N_);
 }
 
@@ -261,7 +270,7 @@ struct BdfResult {
 };
 
 // finite-difference increment of Jacobian column j: DVJAC's
-// max(srur |y_j|, r0 w_j) (vode.f:3744-3750, w_j = rtol |y_j| + atol),
+// max(srur |y_j|, r0 / w_j) (vode.f:3739, w_j = 1 / (rtol |y_j| + atol)),
 // with a floor of srur * 1e-6 on the mass-fraction columns.  For a species
 // that is absent (y_j = 0) DVJAC's increment is r0 * atol, of order
 // 1e-20 here, and the temperature row of that column, a difference of two
@@ -271,16 +280,16 @@ struct BdfResult {
 // with the floor the entries are accurate to about 1e-3.  The species
 // rates are polynomial in the concentrations, so the larger increment
 // costs no truncation accuracy that matters to a chord Newton iteration.
-IZ_HD inline double d_fd_increment(int j, double yj, double ewtj, double r0) {
+IZ_HD inline double d_fd_increment(int j, double yj, double wj, double r0) {
   const double srur = 1.4901161193847656e-08;
-  double r = fmax(srur * fabs(yj), r0 * ewtj);
+  double r = fmax(srur * fabs(yj), r0 / wj);
   return j == 0 ? r : fmax(r, srur * 1.0e-6);
 }
 
 // Integrate the reactor of one cell from 0 to tout (ITASK = 1: step past tout
 // and interpolate the Nordsieck history back to it).
 // Kinetics policies: rhs(rho, y, f) evaluates the reactor right-hand side;
-// jac(rho, y, f0, hl0, ewt, r0, P) writes P = -hl0 * J with a
+// jac(rho, y, f0, hl0, w, r0, P) writes P = -hl0 * J with a
 // finite-difference Jacobian (DVJAC with MF = 22, vode.f:3728-3770); both
 // leave y unchanged.
 struct KinTables {
@@ -288,13 +297,13 @@ struct KinTables {
     d_reactor_rhs(rho, y, f);
   }
   IZ_MHD inline void jac(double rho, const double *y, const double *f0, double hl0,
-                         const double *ewt, double r0, double (*P)[N_]) const {
+                         const double *w, double r0, double (*P)[N_]) const {
     for (int j = 0; j < N_; j++) {
       double yl[N_], ftmp[N_];
 #pragma unroll
       for (int n = 0; n < N_; n++) yl[n] = y[n];
       const double yj = yl[j];
-      const double r = d_fd_increment(j, yj, ewt[j], r0);
+      const double r = d_fd_increment(j, yj, w[j], r0);
       yl[j] = yj + r;
       d_reactor_rhs(rho, yl, ftmp);
       const double ri = 1.0 / r;
@@ -320,7 +329,7 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double rho,
   const int maxstep = 10000, maxcor = 3, msbp = 20;
   const double ccmax = 0.3, ccmxj = 0.2;
   BdfResult res{0, 0, 0, 1, 0};
-  double acor[N_], ewt[N_], savf[N_], yy[N_];
+  double acor[N_], ewt[N_], savf[N_], yy[N_];  // ewt: reciprocal weights
   double J[N_][N_], P[N_][N_];
   int piv[N_];
   // J is written by the first step's Jacobian before any use; cleared so
@@ -329,7 +338,7 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double rho,
 #pragma unroll
     for (int j = 0; j < N_; j++) J[i][j] = 0.0;
 #pragma unroll
-  for (int i = 0; i < N_; i++) { z(0, i) = y[i]; ewt[i] = rtol * fabs(y[i]) + atol; }
+  for (int i = 0; i < N_; i++) { z(0, i) = y[i]; ewt[i] = 1.0 / (rtol * fabs(y[i]) + atol); }
   kin.rhs(rho, y, savf);
   res.nfe++;
   // initial step (the LSODE-style estimate also used by DVHIN's first guess)
@@ -350,7 +359,7 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double rho,
     if (res.nst >= maxstep) { res.ok = 0; break; }
     // error weights at the start of the step
 #pragma unroll
-    for (int i = 0; i < N_; i++) ewt[i] = rtol * fabs(z(0, i)) + atol;
+    for (int i = 0; i < N_; i++) ewt[i] = 1.0 / (rtol * fabs(z(0, i)) + atol);
     // predictor: multiply the Nordsieck array by the Pascal triangle
     d_nordsieck_shift(z, q, 1.0);
     const double l0 = IZ_CBDF.el[q][0];
@@ -467,11 +476,11 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double rho,
         if (kflag <= -10) { res.ok = 0; break; }
         continue;
       }
-      double rh = 1.0 / (1.2 * pow(dsm, 1.0 / (q + 1)) + 1.2e-6);
+      double rh = 1.0 / (1.2 * d_powpos(dsm, IZ_CBDF.rexp[q + 1]) + 1.2e-6);
       int newq = q;
       if (q > 1) {
         double ddn = d_wrms_col(z, q, ewt) / IZ_CBDF.tesco[q][0];
-        double rhdn = 1.0 / (1.3 * pow(ddn, 1.0 / q) + 1.3e-6);
+        double rhdn = 1.0 / (1.3 * d_powpos(ddn, IZ_CBDF.rexp[q]) + 1.3e-6);
         if (rhdn > rh) { rh = rhdn; newq = q - 1; }
       }
       if (kflag <= -2) rh = fmin(rh, 0.2);
@@ -499,18 +508,18 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double rho,
     }
     if (ialth > 0) continue;
     // order and step selection (every q+1 steps)
-    double rhsm = 1.0 / (1.2 * pow(dsm, 1.0 / (q + 1)) + 1.2e-6);
+    double rhsm = 1.0 / (1.2 * d_powpos(dsm, IZ_CBDF.rexp[q + 1]) + 1.2e-6);
     double rhdn = 0.0, rhup = 0.0;
     if (q > 1) {
       double ddn = d_wrms_col(z, q, ewt) / IZ_CBDF.tesco[q][0];
-      rhdn = 1.0 / (1.3 * pow(ddn, 1.0 / q) + 1.3e-6);
+      rhdn = 1.0 / (1.3 * d_powpos(ddn, IZ_CBDF.rexp[q]) + 1.3e-6);
     }
     if (q < 5) {
       double dd[N_];
 #pragma unroll
       for (int n = 0; n < N_; n++) dd[n] = acor[n] - z(5, n);
       double dup = d_wrms(dd, ewt) / IZ_CBDF.tesco[q][2];
-      rhup = 1.0 / (1.4 * pow(dup, 1.0 / (q + 2)) + 1.4e-6);
+      rhup = 1.0 / (1.4 * d_powpos(dup, IZ_CBDF.rexp[q + 2]) + 1.4e-6);
     }
     int newq = q;
     double rh = rhsm;
@@ -559,12 +568,12 @@ struct KinGen {
   // perturbed temperature; the constants of the base state are built after
   // it, so that only one set of them is live at a time.
   IZ_MHD inline void jac(double rho, const double *y, const double *f0, double hl0,
-                         const double *ewt, double r0, double (*P)[N_]) const {
+                         const double *w, double r0, double (*P)[N_]) const {
     {
       double yl[N_], ftmp[N_], kt0[GEN_NKT];
 #pragma unroll
       for (int n = 0; n < N_; n++) yl[n] = y[n];
-      const double r = d_fd_increment(0, yl[0], ewt[0], r0);
+      const double r = d_fd_increment(0, yl[0], w[0], r0);
       yl[0] = yl[0] + r;
       gen_kT(yl[0], kt0);
       gen_rhs(rho, yl, kt0, ftmp);
@@ -579,7 +588,7 @@ struct KinGen {
 #pragma unroll
       for (int n = 0; n < N_; n++) yl[n] = y[n];
       const double yj = yl[j];
-      const double r = d_fd_increment(j, yj, ewt[j], r0);
+      const double r = d_fd_increment(j, yj, w[j], r0);
       yl[j] = yj + r;
       gen_rhs(rho, yl, kt, ftmp);
       const double ri = 1.0 / r;

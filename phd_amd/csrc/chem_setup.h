@@ -57,4 +57,5 @@ static inline void iz_bdf_tables(IzBdf &t) {
     t.tesco[q][1] = (q + 1) / t.el[q][0];
     t.tesco[q][2] = (q + 2) / t.el[q][0];
   }
+  for (int m = 1; m < 8; m++) t.rexp[m] = 1.0 / m;
 }

@@ -46,6 +46,7 @@ struct IzMech {
 struct IzBdf {
   double el[6][6];     // el[q][j], j = 0..q, el[q][1] = 1
   double tesco[6][3];  // [q][0] order-down, [1] same order, [2] order-up constants
+  double rexp[8];      // rexp[m] = 1 / m, the exponents of the step ratios
 };
 
 

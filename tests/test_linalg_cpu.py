@@ -6,7 +6,9 @@ the host, one cell at a time (the wave-wide pivot test IZ_ANY reduces to the
 lane's own predicate).  The factorisation follows LINPACK's dgefa/dgesl
 (chemkin/vode.f:7033, 7630): partial pivoting by the largest magnitude, the
 row interchange of step k applied to columns k..N-1 only, and the solve
-applying the interchanges interleaved with the forward elimination.
+applying the interchanges interleaved with the forward elimination.  The
+diagonal of the factors holds the reciprocal pivots, which the back
+substitution multiplies by.
 """
 import ctypes as C
 import os
@@ -110,10 +112,11 @@ def test_lu_solve_residual_and_lapack(lib):
         assert np.abs(x - x_ref).max() <= 1e-12 * cond * np.abs(x_ref).max()
         # same pivot rows and the same U as LAPACK's getrf (which also picks
         # the first largest magnitude); L is stored LINPACK-style, so only
-        # U is compared entry by entry
+        # U is compared entry by entry, its diagonal from the reciprocals
         lu_ref, piv_ref = sla.lu_factor(A)
         assert np.array_equal(piv, piv_ref)
-        U, U_ref = np.triu(lu), np.triu(lu_ref)
+        U, U_ref = np.triu(lu, 1), np.triu(lu_ref)
+        U[np.arange(N), np.arange(N)] = 1.0 / np.diag(lu)
         assert np.abs(U - U_ref).max() <= 1e-12 * np.abs(U_ref).max()
 
 
