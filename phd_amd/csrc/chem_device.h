@@ -289,15 +289,20 @@ IZ_HD inline double d_fd_increment(int j, double yj, double wj, double r0) {
 // Integrate the reactor of one cell from 0 to tout (ITASK = 1: step past tout
 // and interpolate the Nordsieck history back to it).
 // Kinetics policies: rhs(rho, y, f) evaluates the reactor right-hand side;
-// jac(rho, y, f0, hl0, w, r0, P) writes P = -hl0 * J with a
-// finite-difference Jacobian (DVJAC with MF = 22, vode.f:3728-3770); both
-// leave y unchanged.
+// rhs_c(rho, y, f, c) does the same and keeps in c what the Jacobian at the
+// same y can reuse; jac_c(rho, y, f0, c, hl0, w, r0, P), given f0 = f(y) and
+// the c of that evaluation, writes P = -hl0 * J with a finite-difference
+// Jacobian (DVJAC with MF = 22, vode.f:3728-3770).  None of them writes y.
 struct KinTables {
+  struct Cache {};
   IZ_MHD inline void rhs(double rho, const double *y, double *f) const {
     d_reactor_rhs(rho, y, f);
   }
-  IZ_MHD inline void jac(double rho, const double *y, const double *f0, double hl0,
-                         const double *w, double r0, double (*P)[N_]) const {
+  IZ_MHD inline void rhs_c(double rho, const double *y, double *f, Cache &) const {
+    d_reactor_rhs(rho, y, f);
+  }
+  IZ_MHD inline void jac_c(double rho, const double *y, const double *f0, const Cache &,
+                           double hl0, const double *w, double r0, double (*P)[N_]) const {
     for (int j = 0; j < N_; j++) {
       double yl[N_], ftmp[N_];
 #pragma unroll
@@ -369,7 +374,8 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double rho,
     // corrector (chord Newton on G(e) = h f(z0 + l0 e) - z1 - e)
 #pragma unroll
     for (int n = 0; n < N_; n++) { yy[n] = z(0, n); acor[n] = 0.0; }
-    kin.rhs(rho, yy, savf);
+    typename Kin::Cache kc;
+    kin.rhs_c(rho, yy, savf, kc);
     res.nfe++;
     bool jcur = false;
     // The iteration matrix is factored at every step: a lane that asked
@@ -388,7 +394,7 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double rho,
         double fac = d_wrms(savf, ewt);
         double r0 = 1000.0 * fabs(h) * uround * N_ * fac;
         if (r0 == 0.0) r0 = 1.0;
-        kin.jac(rho, yy, savf, -1.0, ewt, r0, J);  // J = (f(y + r e_j) - f(y)) / r
+        kin.jac_c(rho, yy, savf, kc, -1.0, ewt, r0, J);  // J = (f(y + r e_j) - f(y)) / r
         res.nfe += N_;
         res.nje++;
         nslj = res.nst;
@@ -555,34 +561,26 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double rho,
 // straight-line kinetics of the compiled-in mechanism; the T-only part kt is
 // shared by the Jacobian columns that perturb mass fractions
 struct KinGen {
+  struct Cache { double kt[GEN_NKT]; };  // rate constants at the evaluated T
   IZ_MHD inline void rhs(double rho, const double *y, double *f) const {
     double kt[GEN_NKT];
     gen_kT(y[0], kt);
     gen_rhs(rho, y, kt, f);
   }
-  // the species columns share kt (T is not perturbed).  Each column
-  // perturbs a local copy of the state: perturbing the caller's vector in
-  // place and restoring it gave wrong Jacobians in a GPU build of the
-  // unrolled loop (correct on the CPU), so the state is never written here
-  // The temperature column comes first, with the rate constants of the
-  // perturbed temperature; the constants of the base state are built after
-  // it, so that only one set of them is live at a time.
-  IZ_MHD inline void jac(double rho, const double *y, const double *f0, double hl0,
-                         const double *w, double r0, double (*P)[N_]) const {
-    {
-      double yl[N_], ftmp[N_], kt0[GEN_NKT];
-#pragma unroll
-      for (int n = 0; n < N_; n++) yl[n] = y[n];
-      const double r = d_fd_increment(0, yl[0], w[0], r0);
-      yl[0] = yl[0] + r;
-      gen_kT(yl[0], kt0);
-      gen_rhs(rho, yl, kt0, ftmp);
-      const double ri = 1.0 / r;
-#pragma unroll
-      for (int i = 0; i < N_; i++) P[i][0] = -hl0 * (ftmp[i] - f0[i]) * ri;
-    }
-    double kt[GEN_NKT];
-    gen_kT(y[0], kt);
+  IZ_MHD inline void rhs_c(double rho, const double *y, double *f, Cache &c) const {
+    gen_kT(y[0], c.kt);
+    gen_rhs(rho, y, c.kt, f);
+  }
+  // The species columns reuse the rate constants of the evaluation of f0 (T
+  // is not perturbed) and come first; the temperature column, with the
+  // constants of the perturbed temperature, comes last, so that only one
+  // set of constants is live at a time.  Each column perturbs a local copy
+  // of the state: perturbing the caller's vector in place and restoring it
+  // gave wrong Jacobians in a GPU build of the unrolled loop (correct on
+  // the CPU), so the state is never written here.
+  IZ_MHD inline void jac_c(double rho, const double *y, const double *f0, const Cache &c,
+                           double hl0, const double *w, double r0, double (*P)[N_]) const {
+    const double *kt = c.kt;
     for (int j = 1; j < N_; j++) {
       double yl[N_], ftmp[N_];
 #pragma unroll
@@ -595,6 +593,16 @@ struct KinGen {
 #pragma unroll
       for (int i = 0; i < N_; i++) P[i][j] = -hl0 * (ftmp[i] - f0[i]) * ri;
     }
+    double yl[N_], ftmp[N_], kt0[GEN_NKT];
+#pragma unroll
+    for (int n = 0; n < N_; n++) yl[n] = y[n];
+    const double r = d_fd_increment(0, yl[0], w[0], r0);
+    yl[0] = yl[0] + r;
+    gen_kT(yl[0], kt0);
+    gen_rhs(rho, yl, kt0, ftmp);
+    const double ri = 1.0 / r;
+#pragma unroll
+    for (int i = 0; i < N_; i++) P[i][0] = -hl0 * (ftmp[i] - f0[i]) * ri;
   }
 };
 #endif
