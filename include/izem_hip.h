@@ -163,8 +163,8 @@ int izem_chem_setup(izem_ctx *ctx, const double *mech, long nmech, double rtol, 
  * in the reference (DVODE runs on the CPU). */
 int izem_chem_tune(izem_ctx *ctx, int waves, int blocks_per_cu, int jac_age, int kinetics);
 /* work counters of the chemistry integrations since the previous call:
- * out[0] BDF steps, out[1] RHS evaluations, out[2] Jacobians (LU
- * factorisations), out[3] LU solves; synchronises and clears them.
+ * out[0] BDF steps, out[1] RHS evaluations, out[2] Jacobian evaluations,
+ * out[3] LU solves; synchronises and clears them.
  * Instrumentation without a reference counterpart (DVODE keeps the same
  * counts per call in IWORK(11..13), vode.f:4286). */
 int izem_chem_stats(izem_ctx *ctx, unsigned long long out[4]);

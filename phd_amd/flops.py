@@ -110,13 +110,16 @@ def solve_ops(n: int = NEQ) -> int:
 def chem_flops(stats, counts: dict | None = None) -> float:
     """Operations of a batch of integrations from its counters
     (steps, RHS evaluations, Jacobians, LU solves).  A Jacobian is one
-    temperature part and one full RHS per perturbed column, the species
-    columns sharing the temperature part, plus the difference quotients
-    (3 per entry, 6 per column for the increment) and the diagonal."""
+    concentration part per perturbed column and the temperature part of the
+    temperature column, plus the difference quotients (3 per entry, 6 per
+    column for the increment) and the diagonal: the species columns reuse
+    the temperature part of the evaluation of f(y) that precedes every
+    Jacobian (counted among the plain evaluations).  One factorisation is
+    counted per Jacobian, DVODE's minimum."""
     nst, nfe, nje, nsol = [float(x) for x in stats]
     c = counts or kinetics_counts()
     rhs = c["kT"] + c["conc"]
-    jac = 2 * c["kT"] + NEQ * c["conc"] + NEQ * (3 * NEQ + 6) + NEQ
+    jac = c["kT"] + NEQ * c["conc"] + NEQ * (3 * NEQ + 6) + NEQ
     plain_rhs = nfe - NEQ * nje  # evaluations outside the Jacobians
     return plain_rhs * rhs + nje * (jac + lu_ops()) + nsol * solve_ops()
 
