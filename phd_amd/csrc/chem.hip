@@ -80,18 +80,25 @@ __device__ inline void chem_stats_add(unsigned long long *stats, unsigned long l
 #define IZ_ZLDS_MAX_WAVES 1
 #endif
 
-// integrate one cell with the history storage of the variant
+// integrate one cell with the history storage of the variant; J is the
+// lane's saved-Jacobian storage, kept across its cells
 template <class Kin, int WAVES>
-__device__ IZ_FORCEINLINE BdfResult chem_integrate(double *zs, double rho, double *y, double dt,
-                                                   double rtol, double atol, int jac_age) {
+__device__ IZ_FORCEINLINE BdfResult chem_integrate(double *zs, double (&J)[N_][N_], double rho,
+                                                   double *y, double dt, double rtol,
+                                                   double atol, int jac_age) {
   if constexpr (WAVES <= IZ_ZLDS_MAX_WAVES) {
     ZLds z{zs + threadIdx.x};
-    return d_bdf_integrate(Kin(), z, rho, y, dt, rtol, atol, jac_age);
+    return d_bdf_integrate(Kin(), z, J, rho, y, dt, rtol, atol, jac_age);
   } else {
     ZArr z;
-    return d_bdf_integrate(Kin(), z, rho, y, dt, rtol, atol, jac_age);
+    return d_bdf_integrate(Kin(), z, J, rho, y, dt, rtol, atol, jac_age);
   }
 }
+// the saved-Jacobian storage of a lane, cleared once when the lane starts
+#define IZ_LANE_JACOBIAN(J)        \
+  double J[N_][N_];                \
+  for (int i_ = 0; i_ < N_; i_++)  \
+    for (int j_ = 0; j_ < N_; j_++) J[i_][j_] = 0.0
 #define IZ_ZLDS_DOUBLES(WAVES) ((WAVES) <= IZ_ZLDS_MAX_WAVES ? 6 * N_ * 64 : 1)
 
 // persistent work-queue kernel over independent cells: each lane takes the
@@ -103,6 +110,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                             int jac_age, int *__restrict__ counter, int *__restrict__ nsteps,
                             int *__restrict__ status, unsigned long long *__restrict__ stats) {
   __shared__ double zs[IZ_ZLDS_DOUBLES(WAVES)];
+  IZ_LANE_JACOBIAN(J);
   unsigned long long w_nst = 0, w_nfe = 0, w_nje = 0, w_nsol = 0;
   for (;;) {
     int cell = atomicAdd(counter, 1);
@@ -110,7 +118,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     double y[N_];
     y[0] = T[cell];
     for (int k = 0; k < N_ - 1; k++) y[1 + k] = Y[(long)cell * (N_ - 1) + k];
-    BdfResult r = chem_integrate<Kin, WAVES>(zs, rho[cell], y, dt, rtol, atol, jac_age);
+    BdfResult r = chem_integrate<Kin, WAVES>(zs, J, rho[cell], y, dt, rtol, atol, jac_age);
     T[cell] = y[0];
     for (int k = 0; k < N_ - 1; k++) Y[(long)cell * (N_ - 1) + k] = y[1 + k];
     if (nsteps) nsteps[cell] = r.nst;
@@ -139,6 +147,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
   const double dt_phys = (scale * (*dtp)) * time_ref;
   const double Tref_i = 1.0 / T_ref;
   __shared__ double zs[IZ_ZLDS_DOUBLES(WAVES)];
+  IZ_LANE_JACOBIAN(J);
   double dtmax = 0.0;
   unsigned long long w_nst = 0, w_nfe = 0, w_nje = 0, w_nsol = 0;
   for (;;) {
@@ -154,7 +163,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     double T0_phys = T[c] * T_ref;
     y[0] = T0_phys;
     for (int l = 0; l < N_ - 1; l++) y[1 + l] = v[(IZ_NIV + l) * d.nc + c] * rho_i;
-    BdfResult res = chem_integrate<Kin, WAVES>(zs, rho0 * rho_cgs, y, dt_phys, rtol, atol,
+    BdfResult res = chem_integrate<Kin, WAVES>(zs, J, rho0 * rho_cgs, y, dt_phys, rtol, atol,
                                                jac_age);
     if (!res.ok) atomicOr(status, IZ_ST_VODE_FAIL);
     w_nst += res.nst; w_nfe += res.nfe; w_nje += res.nje; w_nsol += res.nsol;

@@ -326,22 +326,21 @@ struct KinTables {
 // (DVODE's MSBJ, default 50) steps, or after a corrector failure that a
 // new h l0 cannot explain (DRC < CCMXJ = 0.2) or that a current J did not
 // prevent.
+// J is the caller's storage for the saved Jacobian.  The first step of
+// every integration evaluates a new one before any read, so the caller
+// initialises J once (the kernels once per lane, not once per cell) and
+// may pass the storage of a previous integration.
 template <class Kin, class Z>
-IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double rho, double *y,
-                                               double tout, double rtol, double atol,
-                                               int jac_age) {
+IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double (&J)[N_][N_],
+                                               double rho, double *y, double tout, double rtol,
+                                               double atol, int jac_age) {
   const double uround = 2.220446049250313e-16;
   const int maxstep = 10000, maxcor = 3, msbp = 20;
   const double ccmax = 0.3, ccmxj = 0.2;
   BdfResult res{0, 0, 0, 1, 0};
   double acor[N_], ewt[N_], savf[N_], yy[N_];  // ewt: reciprocal weights
-  double J[N_][N_], P[N_][N_];
+  double P[N_][N_];
   int piv[N_];
-  // J is written by the first step's Jacobian before any use; cleared so
-  // that no path of the loop reads memory the function has not written
-  for (int i = 0; i < N_; i++)
-#pragma unroll
-    for (int j = 0; j < N_; j++) J[i][j] = 0.0;
 #pragma unroll
   for (int i = 0; i < N_; i++) { z(0, i) = y[i]; ewt[i] = 1.0 / (rtol * fabs(y[i]) + atol); }
   kin.rhs(rho, y, savf);

@@ -31,9 +31,10 @@ extern "C" void host_psr(int kinetics, int n, double *T, const double *rho, doub
     y[0] = T[c];
     for (int k = 0; k < N_ - 1; k++) y[1 + k] = Y[c * (N_ - 1) + k];
     ZArr z;
+    double J[N_][N_] = {};
     BdfResult r = kinetics
-        ? d_bdf_integrate(KinGen(), z, rho[c], y, dt, rtol, atol, jac_age)
-        : d_bdf_integrate(KinTables(), z, rho[c], y, dt, rtol, atol, jac_age);
+        ? d_bdf_integrate(KinGen(), z, J, rho[c], y, dt, rtol, atol, jac_age)
+        : d_bdf_integrate(KinTables(), z, J, rho[c], y, dt, rtol, atol, jac_age);
     T[c] = y[0];
     for (int k = 0; k < N_ - 1; k++) Y[c * (N_ - 1) + k] = y[1 + k];
     nst[c] = r.nst;
