@@ -427,6 +427,29 @@ __device__ inline bool box_cell(const IzBox &b, long tid, int &i, int &j, int &k
   return true;
 }
 
+// The domains of one pass (interior and ghost slabs, domain_select) in one
+// launch: thread indices [first[q], first[q+1]) are the cells of box[q].
+// A slab alone is too small a grid to fill the device.
+struct IzDomSet {
+  IzBox box[7];
+  int dom[7];
+  long first[8];
+  int n;
+};
+__device__ inline bool domset_cell(const IzDomSet &s, long tid, int &dom, int &i, int &j,
+                                   int &k) {
+  if (tid >= s.first[s.n]) return false;
+  IzBox b = s.box[0];
+  int dm = s.dom[0];
+  long f0 = s.first[0];
+  // compile-time indices only, so the set stays in scalar registers
+#pragma unroll
+  for (int p = 1; p < 7; p++)
+    if (p < s.n && tid >= s.first[p]) { b = s.box[p]; dm = s.dom[p]; f0 = s.first[p]; }
+  dom = dm;
+  return box_cell(b, tid - f0, i, j, k);
+}
+
 // force_sumY (npv = 0): module_thermodynamics.f90:1709-1760
 __device__ inline void d_force_sumY(double *vc) {
   double rho = vc[0], rho_i = 1.0 / rho, sum = 0.0, Y;
@@ -658,11 +681,11 @@ __global__ void k_sensor(IzDims d, double max_rel_weight, const double *__restri
 // ===================================================================
 
 // velocities and mole fractions on a domain (molefrac :1669-1698)
-__global__ void k_visc_prim(IzDims d, IzBox b, const double *__restrict__ v,
+__global__ void k_visc_prim(IzDims d, IzDomSet ds, const double *__restrict__ v,
                             const double *__restrict__ W_i, double *__restrict__ prim) {
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int i, j, k;
-  if (!box_cell(b, tid, i, j, k)) return;
+  int i, j, k, dom;
+  if (!domset_cell(ds, tid, dom, i, j, k)) return;
   long c = iz_cidx(d, i, j, k);
   double r0 = v[c];
   for (int l = 0; l < IZ_NRV; l++)
@@ -752,15 +775,15 @@ __device__ inline void d_transport(double rho, double T, double W_i, const doubl
 // (into the species slots of fd) and the stresses and heat flux (sf) are
 // stored; the derivative slots of fd are not written.
 __global__ void __attribute__((amdgpu_flat_work_group_size(128, 128), amdgpu_waves_per_eu(2)))
-k_visc_fused(IzDims d, IzBox b, int dom, double sqgmr, double rvd0, double rlam0,
+k_visc_fused(IzDims d, IzDomSet ds, double sqgmr, double rvd0, double rlam0,
              const double *__restrict__ v, const double *__restrict__ T,
              const double *__restrict__ W_i, const double *__restrict__ ha,
              const double *__restrict__ prim, const double *__restrict__ dx_i,
              const double *__restrict__ dy_i, const double *__restrict__ dz_i,
              double *__restrict__ fd, double *__restrict__ sf) {
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int i, j, k;
-  if (!box_cell(b, tid, i, j, k)) return;
+  int i, j, k, dom;
+  if (!domset_cell(ds, tid, dom, i, j, k)) return;
   const long nc = d.nc;
   const long c = iz_cidx(d, i, j, k);
   // derivative location: the periodic image of a ghost-slab cell
@@ -1193,6 +1216,19 @@ static long box_count(const IzBox &b) {
   return (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
 }
 static long n_interior(const IzDims &d) { return (long)d.nx * d.ny * d.nz; }
+// the interior and the 2 * ndim ghost slabs, in domain_select's order
+static IzDomSet domain_set(const IzDims &d) {
+  IzDomSet s{};
+  s.n = 0;
+  s.first[0] = 0;
+  for (int dom = -d.ndim; dom <= d.ndim; dom++) {
+    s.box[s.n] = domain_box(d, dom);
+    s.dom[s.n] = dom;
+    s.first[s.n + 1] = s.first[s.n] + box_count(s.box[s.n]);
+    s.n++;
+  }
+  return s;
+}
 
 extern "C" int izem_create(izem_ctx **out, const izem_grid *g, const double *thd, long nthd,
                            const izem_opts *opts) {
@@ -1440,18 +1476,16 @@ static int visc_fluxes(izem_ctx *c, const double *dx_i, const double *dy_i, cons
                        const double *T, const double *W_i, const double *ha, const double *v) {
   const IzDims &d = c->d;
   int pk = iz_prof_begin(c, IZP_VISC);
-  for (int dom = -d.ndim; dom <= d.ndim; dom++) {
-    IzBox b = domain_box(d, dom);
-    hipLaunchKernelGGL(k_visc_prim, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0, c->stream,
-                       d, b, v, W_i, c->prim);
-  }
+  // every domain's primitives are written before any domain's fluxes read
+  // them (a slab cell's derivatives come from its interior image)
+  const IzDomSet ds = domain_set(d);
+  const long ncell = ds.first[ds.n];
+  hipLaunchKernelGGL(k_visc_prim, dim3(iz_blocks(ncell, 256)), dim3(256), 0, c->stream, d, ds, v,
+                     W_i, c->prim);
   const double rvd0 = c->o.sqgmr / c->o.Sc, rlam0 = c->o.sqgmr * c->o.ggmopr;
-  for (int dom = -d.ndim; dom <= d.ndim; dom++) {
-    IzBox b = domain_box(d, dom);
-    hipLaunchKernelGGL(k_visc_fused, dim3(iz_blocks(box_count(b), 128)), dim3(128), 0,
-                       c->stream, d, b, dom, c->o.sqgmr, rvd0, rlam0, v, T, W_i, ha, c->prim,
-                       dx_i, dy_i, dz_i, c->fd, c->sf);
-  }
+  hipLaunchKernelGGL(k_visc_fused, dim3(iz_blocks(ncell, 128)), dim3(128), 0, c->stream, d, ds,
+                     c->o.sqgmr, rvd0, rlam0, v, T, W_i, ha, c->prim, dx_i, dy_i, dz_i, c->fd,
+                     c->sf);
   iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
