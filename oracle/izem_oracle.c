@@ -1080,6 +1080,10 @@ void ora_ckwyr(const ora_mech *m, double rho, double T, const double *Y, double 
     rkrt[i] = 0.0;
     if (m->reversible[i]) rkrt[i] = rkft[i] / fmax(eqk[i], m->small_);
   }
+  /* reverse parameters given (REV): vode.f:1028-1032 */
+  for (int i = 0; i < ii; i++)
+    if (m->reversible[i] == 2)
+      rkrt[i] = m->rpar[i][0] * exp(m->rpar[i][1] * alogt - m->rpar[i][2] / T);
   for (int k = 0; k < kk; k++) c[k] = rho * Y[k] / m->wt[k];
   double ctb[ORA_IMAX];
   for (int i = 0; i < ii; i++) ctb[i] = 1.0;
@@ -1095,10 +1099,20 @@ void ora_ckwyr(const ora_mech *m, double rho, double T, const double *Y, double 
     if (!m->fal[i]) continue;
     const double *fp = m->fpar[i];
     double rklow = fp[0] * exp(fp[1] * alogt - fp[2] / T);
-    double pr = rklow * ctb[i] / rkft[i];
-    ctb[i] = 1.0;
+    double pr;
+    if (m->kfal[i] == 0) { /* vode.f:1193-1199 */
+      pr = rklow * ctb[i] / rkft[i];
+      ctb[i] = 1.0;
+    } else {
+      pr = rklow * c[m->kfal[i] - 1] / rkft[i];
+    }
     double pcor = pr / (1.0 + pr);
-    if (m->fal[i] > 1) {
+    if (m->fal[i] == 2) { /* 8-parameter SRI form, vode.f:1207-1216 */
+      double prlog = log10(fmax(pr, m->small_));
+      double xp = 1.0 / (1.0 + prlog * prlog);
+      double fc = pow(fp[3] * exp(-fp[4] / T) + exp(-T / fp[5]), xp) * fp[6] * pow(T, fp[7]);
+      pcor = fc * pcor;
+    } else if (m->fal[i] > 2) {
       double prlog = log10(fmax(pr, m->small_));
       double fcent = (1.0 - fp[3]) * exp(-T / fp[4]) + fp[3] * exp(-T / fp[5]);
       if (m->fal[i] == 4) fcent = fcent + exp(-fp[6] / T);

@@ -76,8 +76,10 @@ typedef struct {
   int ntb[ORA_IMAX];                   /* -1: no third body; else number of efficiencies */
   int tb_k[ORA_IMAX][ORA_TBMAX];       /* 1-based species */
   double tb_eff[ORA_IMAX][ORA_TBMAX];
-  int fal[ORA_IMAX];                   /* 0 none, 1 Lindemann, 3 Troe-3, 4 Troe-4 */
-  double fpar[ORA_IMAX][8];            /* A0, b0, E0/R, alpha, T***, T*, T** */
+  int fal[ORA_IMAX];                   /* 0 none, 1 Lindemann, 2 SRI, 3 Troe-3, 4 Troe-4 */
+  double fpar[ORA_IMAX][8];            /* A0, b0, E0/R, then alpha, T***, T*, T** or a..e */
+  int kfal[ORA_IMAX];                  /* falloff [M]: 0 third-body sum, k+1 species k */
+  double rpar[ORA_IMAX][3];            /* REV parameters (reversible == 2) */
   double ru, patm;                     /* erg/(mol K), dyn/cm^2 */
   double small_, big_, exparg;
 } ora_mech;

@@ -62,6 +62,7 @@ class Mech(C.Structure):
                 ("reversible", C.c_int * IMAX), ("ntb", C.c_int * IMAX),
                 ("tb_k", C.c_int * (IMAX * TBMAX)), ("tb_eff", C.c_double * (IMAX * TBMAX)),
                 ("fal", C.c_int * IMAX), ("fpar", C.c_double * (IMAX * 8)),
+                ("kfal", C.c_int * IMAX), ("rpar", C.c_double * (IMAX * 3)),
                 ("ru", C.c_double), ("patm", C.c_double), ("small_", C.c_double),
                 ("big_", C.c_double), ("exparg", C.c_double)]
 
@@ -103,11 +104,11 @@ def make_mech(mech) -> Mech:
     t = mech.tables()
     m = Mech()
     m.kk, m.ii = t["kk"], t["ii"]
-    for name in ("wt", "a", "tmid", "A", "b", "EoR", "tb_eff", "fpar"):
+    for name in ("wt", "a", "tmid", "A", "b", "EoR", "tb_eff", "fpar", "rpar"):
         arr = getattr(m, name)
         for i, x in enumerate(np.asarray(t[name], dtype=np.float64).ravel()):
             arr[i] = float(x)
-    for name in ("nu", "nunk", "reversible", "ntb", "tb_k", "fal"):
+    for name in ("nu", "nunk", "reversible", "ntb", "tb_k", "fal", "kfal"):
         arr = getattr(m, name)
         for i, x in enumerate(np.asarray(t[name]).ravel()):
             arr[i] = int(x)
@@ -152,18 +153,21 @@ class Oracle:
 class RefChem:
     """The reference CHEMKIN-II/DVODE built from its own sources (oracle/_ref).
 
-    The Fortran keeps its state in COMMON blocks, so calls are serialised."""
+    The Fortran keeps its state in COMMON blocks, so calls are serialised,
+    and each linking file has its own copy of the library (one per directory
+    of oracle/_ref: the bundled mechanism in _ref, the front-end test
+    mechanism in _ref/sri)."""
 
     _lock = threading.Lock()
-    _lib = None
+    _libs: dict = {}
 
     def __init__(self, ref_dir: str | None = None):
-        self.dir = ref_dir or os.path.join(HERE, "_ref")
+        self.dir = os.path.abspath(ref_dir or os.path.join(HERE, "_ref"))
         so = os.path.join(self.dir, "libpsrref.so")
         if not os.path.exists(so):
             raise FileNotFoundError(f"{so} missing: run `make -C oracle ref`")
         with RefChem._lock:
-            if RefChem._lib is None:
+            if self.dir not in RefChem._libs:
                 lib = C.CDLL(so)
                 cwd = os.getcwd()
                 try:
@@ -171,8 +175,8 @@ class RefChem:
                     lib.refinit_()
                 finally:
                     os.chdir(cwd)
-                RefChem._lib = lib
-        self.lib = RefChem._lib
+                RefChem._libs[self.dir] = lib
+        self.lib = RefChem._libs[self.dir]
         kk, neq = C.c_int(), C.c_int()
         self.lib.refkk_(C.byref(kk), C.byref(neq))
         self.kk = kk.value

@@ -314,6 +314,7 @@ class Mechanism:
             reversible=np.zeros(IMAX, np.int32), ntb=-np.ones(IMAX, np.int32),
             tb_k=np.zeros((IMAX, TBMAX), np.int32), tb_eff=np.zeros((IMAX, TBMAX)),
             fal=np.zeros(IMAX, np.int32), fpar=np.zeros((IMAX, 8)),
+            kfal=np.zeros(IMAX, np.int32), rpar=np.zeros((IMAX, 3)),
         )
         t["wt"][:kk] = self.wt
         t["a"][:kk] = self.a
@@ -324,7 +325,12 @@ class Mechanism:
                 t["nunk"][i, s], t["nu"][i, s] = k + 1, -nu
             for s, (k, nu) in enumerate(r["prod"]):
                 t["nunk"][i, 3 + s], t["nu"][i, 3 + s] = k + 1, nu
-            t["reversible"][i] = 1 if r["rev"] else 0
+            # 0 irreversible, 1 reverse from the equilibrium constant,
+            # 2 explicit reverse Arrhenius parameters (REV)
+            t["reversible"][i] = (2 if r.get("rpar") else 1) if r["rev"] else 0
+            if r.get("rpar"):
+                t["rpar"][i] = r["rpar"]
+            t["kfal"][i] = r.get("kfal", 0)
             if r["tb"]:
                 eff = r["eff"]
                 t["ntb"][i] = len(eff)
@@ -352,12 +358,14 @@ def _efac(units: str) -> float:
 
 
 def _parse_side(s: str, spidx: dict):
+    """Species and coefficients of one side; whether it has +M; the falloff
+    collider: None, "M" or the name of the species in (+name)."""
     s = s.strip()
     third = False
-    fall = False
-    m = re.search(r"\(\+M\)", s, re.I)
+    fall = None
+    m = re.search(r"\(\+([A-Za-z0-9]+)\)", s)
     if m:
-        fall = True
+        fall = m.group(1).upper()
         s = s[:m.start()] + s[m.end():]
     terms = [x for x in re.split(r"\+(?![^()]*\))", s) if x]
     out = []
@@ -383,8 +391,11 @@ def _parse_side(s: str, spidx: dict):
 
 def read_mechanism(inp_path: str | None = None, therm_path: str | None = None) -> Mechanism:
     """Parse a CHEMKIN-II mechanism (ELEMENTS / SPECIES / REACTIONS with +M,
-    (+M), LOW, TROE, DUPLICATE and third-body efficiencies) into the tables of
-    the kinetics kernels, with the unit conversions of ckinterp39.f."""
+    (+M) and (+species) falloff, LOW, TROE (3 or 4 values), SRI (3 or 5
+    values), REV, irreversible =>, DUPLICATE and third-body efficiencies)
+    into the tables of the kinetics kernels, with the unit conversions and
+    defaults of ckinterp39.f (SRI with 3 values: d = 1, e = 0, :1831-1834;
+    E of LOW and REV converted like the forward E, :2431-2439)."""
     inp_path = inp_path or os.path.join(DATA_DIR, "h2_air_li2004.inp")
     lines = []
     for raw in open(inp_path).read().splitlines():
@@ -449,8 +460,18 @@ def read_mechanism(inp_path: str | None = None, therm_path: str | None = None) -
             lhs, rhs = re.split(r"<=>|=>|=", eq, maxsplit=1)
             reac, tb1, f1 = _parse_side(lhs, spidx)
             prod, tb2, f2 = _parse_side(rhs, spidx)
+            if f1 != f2 or tb1 != tb2:
+                raise ValueError(f"third body or falloff collider differs across {eq}")
+            # (+M): third-body falloff; (+species): that species is [M]
+            # and the reaction has no third-body efficiencies (KFAL)
+            kfal = 0
+            if f1 is not None and f1 != "M":
+                if f1 not in spidx:
+                    raise ValueError(f"unknown falloff collider {f1} in {eq}")
+                kfal = spidx[f1] + 1
             cur = dict(eq=eq, A=A, b=b, EoR=E * efac, reac=reac, prod=prod, rev=rev,
-                       tb=tb1 or f1, fal=1 if f1 else 0, fpar=[], eff=[], dup=False)
+                       tb=tb1 or f1 == "M", fal=1 if f1 else 0, kfal=kfal, fpar=[],
+                       eff=[], dup=False, rpar=None)
             mech.reactions.append(cur)
             continue
         for kw, body in re.findall(r"([A-Z0-9()]+)\s*/([^/]*)/", u):
@@ -459,8 +480,23 @@ def read_mechanism(inp_path: str | None = None, therm_path: str | None = None) -
                 cur["fpar"] = [v[0], v[1], v[2] * efac]
             elif kw == "TROE":
                 v = [float(x) for x in body.split()]
+                if len(v) not in (3, 4) or not cur["fal"]:
+                    raise ValueError(f"bad TROE data for {cur['eq']}")
                 cur["fal"] = 3 if len(v) == 3 else 4
                 cur["fpar"] = cur["fpar"][:3] + v
+            elif kw == "SRI":
+                v = [float(x) for x in body.split()]
+                if len(v) == 3:
+                    v = v + [1.0, 0.0]
+                if len(v) != 5 or not cur["fal"]:
+                    raise ValueError(f"bad SRI data for {cur['eq']}")
+                cur["fal"] = 2
+                cur["fpar"] = cur["fpar"][:3] + v
+            elif kw == "REV":
+                v = [float(x) for x in body.split()]
+                if len(v) != 3 or not cur["rev"] or cur["fal"]:
+                    raise ValueError(f"REV not allowed for {cur['eq']}")
+                cur["rpar"] = [v[0], v[1], v[2] * efac]
             elif kw in spidx:
                 cur["eff"].append((spidx[kw], float(body)))
             else:

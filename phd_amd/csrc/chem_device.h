@@ -51,7 +51,11 @@ IZ_HD inline void d_ckwyr(double rho, double T, const double *Y, double *wdot) {
   for (int i = 0; i < ii; i++) {
     double kf = IZ_CMECH.A[i] * exp(IZ_CMECH.b[i] * alogt - IZ_CMECH.EoR[i] * Ti);
     double kr = 0.0;
-    if (IZ_CMECH.reversible[i]) {
+    if (IZ_CMECH.reversible[i] == 2) {
+      // explicit reverse parameters (CKRATT, vode.f:1028-1032)
+      const double *rp = IZ_CMECH.rpar[i];
+      kr = rp[0] * exp(rp[1] * alogt - rp[2] * Ti);
+    } else if (IZ_CMECH.reversible[i]) {
       double s = 0.0;
       int nus = 0;
       for (int n = 0; n < 6; n++) {
@@ -71,10 +75,18 @@ IZ_HD inline void d_ckwyr(double rho, double T, const double *Y, double *wdot) {
     if (IZ_CMECH.fal[i]) {
       const double *fp = IZ_CMECH.fpar[i];
       double rklow = fp[0] * exp(fp[1] * alogt - fp[2] * Ti);
-      double pr = rklow * ctb / kf;
+      // [M] is the third-body sum, or one species' concentration (KFAL)
+      const int kf_m = IZ_CMECH.kfal[i];
+      double pr = rklow * (kf_m ? c[kf_m - 1] : ctb) / kf;
       ctb = 1.0;
       double pcor = pr / (1.0 + pr);
-      if (IZ_CMECH.fal[i] > 1) {
+      if (IZ_CMECH.fal[i] == 2) {
+        // SRI form (CKRATX, vode.f:1211-1216)
+        double prlog = log10(fmax(pr, 1e-300));
+        double xp = 1.0 / (1.0 + prlog * prlog);
+        double fc = pow(fp[3] * exp(-fp[4] / T) + exp(-T / fp[5]), xp) * fp[6] * pow(T, fp[7]);
+        pcor *= fc;
+      } else if (IZ_CMECH.fal[i] > 2) {
         double prlog = log10(fmax(pr, 1e-300));
         double fcent = (1.0 - fp[3]) * exp(-T / fp[4]) + fp[3] * exp(-T / fp[5]);
         if (IZ_CMECH.fal[i] == 4) fcent += exp(-fp[6] * Ti);

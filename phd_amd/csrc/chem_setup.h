@@ -16,7 +16,8 @@
 static inline bool iz_mech_from_flat(const double *mf, long nmech, int kk_expected, IzMech &m) {
   memset(&m, 0, sizeof m);
   const long need = 2 + CK_KMAX + CK_KMAX * 14 + CK_KMAX + 3 * CK_IMAX + 12 * CK_IMAX +
-                    2 * CK_IMAX + 2 * CK_IMAX * CK_TBMAX + CK_IMAX + 8 * CK_IMAX;
+                    2 * CK_IMAX + 2 * CK_IMAX * CK_TBMAX + CK_IMAX + 8 * CK_IMAX + CK_IMAX +
+                    3 * CK_IMAX;
   if (!mf || nmech != need) return false;
   long p = 0;
   auto take = [&](long n) { long s = p; p += n; return s; };
@@ -39,6 +40,12 @@ static inline bool iz_mech_from_flat(const double *mf, long nmech, int kk_expect
   for (int i = 0; i < CK_IMAX * CK_TBMAX; i++) (&m.tb_eff[0][0])[i] = mf[o + i];
   o = take(CK_IMAX); for (int i = 0; i < CK_IMAX; i++) m.fal[i] = (int)mf[o + i];
   o = take(8 * CK_IMAX); memcpy(m.fpar, mf + o, sizeof(double) * 8 * CK_IMAX);
+  o = take(CK_IMAX); for (int i = 0; i < CK_IMAX; i++) m.kfal[i] = (int)mf[o + i];
+  o = take(3 * CK_IMAX); memcpy(m.rpar, mf + o, sizeof(double) * 3 * CK_IMAX);
+  for (int i = 0; i < m.ii; i++)
+    if (m.kfal[i] < 0 || m.kfal[i] > m.kk || m.fal[i] < 0 || m.fal[i] > 4 ||
+        m.reversible[i] < 0 || m.reversible[i] > 2)
+      return false;
   return true;
 }
 

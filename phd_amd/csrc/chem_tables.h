@@ -37,8 +37,10 @@ struct IzMech {
   int ntb[CK_IMAX];
   int tb_k[CK_IMAX][CK_TBMAX];
   double tb_eff[CK_IMAX][CK_TBMAX];
-  int fal[CK_IMAX];
-  double fpar[CK_IMAX][8];
+  int fal[CK_IMAX];            // 0 none, 1 Lindemann, 2 SRI, 3/4 Troe (CKINTP's IFOP)
+  double fpar[CK_IMAX][8];     // low-pressure A, b, E/R, then the SRI or Troe values
+  int kfal[CK_IMAX];           // falloff [M]: 0 the third-body sum, k + 1 species k
+  double rpar[CK_IMAX][3];     // REV: reverse A, b, E/R (reversible[i] == 2)
 };
 
 

@@ -210,7 +210,8 @@ def mech_flat(mech: cd.Mechanism) -> np.ndarray:
              t["A"], t["b"], t["EoR"], t["nu"].ravel().astype(np.float64),
              t["nunk"].ravel().astype(np.float64), t["reversible"].astype(np.float64),
              t["ntb"].astype(np.float64), t["tb_k"].ravel().astype(np.float64),
-             t["tb_eff"].ravel(), t["fal"].astype(np.float64), t["fpar"].ravel()]
+             t["tb_eff"].ravel(), t["fal"].astype(np.float64), t["fpar"].ravel(),
+             t["kfal"].astype(np.float64), t["rpar"].ravel()]
     return np.ascontiguousarray(np.concatenate(parts), dtype=np.float64)
 
 

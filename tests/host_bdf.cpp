@@ -41,3 +41,9 @@ extern "C" void host_psr(int kinetics, int n, double *T, const double *rho, doub
     ok[c] = r.ok;
   }
 }
+
+// the reactor right-hand side (FUN) of either kinetics form
+extern "C" void host_rhs(int kinetics, double rho, const double *z, double *zp) {
+  if (kinetics) KinGen().rhs(rho, z, zp);
+  else KinTables().rhs(rho, z, zp);
+}

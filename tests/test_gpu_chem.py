@@ -13,6 +13,7 @@ Bars:
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -65,6 +66,68 @@ def test_ckwyr_gpu_vs_reference(mech, thd):
         worst = max(worst, e)
     print("ckwyr worst per-cell rel err", worst)
     assert worst <= 1e-12
+
+
+SRI_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+                       "_ref", "sri")
+SRI_MECH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "h2_air_sri_rev.inp")
+needs_sri = pytest.mark.skipif(not os.path.exists(os.path.join(SRI_DIR, "chem.bin")),
+                               reason="oracle/_ref/sri not built")
+
+
+@needs_sri
+def test_front_end_mechanism_on_the_gpu(thd):
+    """A mechanism with SRI, 4-value Troe, a (+N2) collider, REV and an
+    irreversible reaction (tests/golden/h2_air_sri_rev.inp) loaded through
+    izem_chem_setup: the context falls back to the table-driven kinetics (the
+    straight-line code is for the bundled mechanism), izem_ckwyr matches the
+    reference CKWYR on the same file, and izem_psr_batch matches the
+    reference DVODE on igniting cells."""
+    import torch
+    from oracle.oracle import RefChem
+    from phd_amd import chemdata as cd, ics
+    m = cd.read_mechanism(SRI_MECH)
+    rc = RefChem(SRI_DIR)
+    s = _solver(m)
+    assert s.lib.izem_chem_tune(s.ctx, 1, 4, 50, 1) == -1  # no straight-line code for it
+    rng = np.random.default_rng(6)
+    n = 512
+    T = rng.uniform(300.0, 3200.0, n)
+    rho = 10 ** rng.uniform(-5.0, -2.5, n)
+    Y = rng.random((n, 9)) ** 3
+    Y /= Y.sum(axis=1, keepdims=True)
+    d = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
+    dT, drho, dY = d(T), d(rho), d(Y)
+    w = torch.zeros((n, 9), dtype=torch.float64, device="cuda:0")
+    assert s.lib.izem_ckwyr(s.ctx, n, C.c_void_p(drho.data_ptr()), C.c_void_p(dT.data_ptr()),
+                            C.c_void_p(dY.data_ptr()), C.c_void_p(w.data_ptr())) == 0
+    s.sync()
+    wg = w.cpu().numpy()
+    worst = max(np.max(np.abs(wg[c] - rc.wdot(rho[c], T[c], Y[c]))) /
+                np.max(np.abs(rc.wdot(rho[c], T[c], Y[c]))) for c in range(n))
+    print(f"front-end mechanism: ckwyr worst per-cell rel err {worst:.2e}")
+    assert worst <= 1e-12
+    nc = 64
+    Tc, pc, Yc = ics.config3_cells(thd, nc, seed=12)
+    Wmix = 1.0 / (Yc / (np.array(thd.W) * 1000.0)).sum(axis=1)
+    rhoc = (pc * 10.0) * Wmix / (8.314510e7 * Tc)
+    dTc, drc, dYc = d(Tc), d(rhoc), d(Yc)
+    nst = torch.zeros(nc, dtype=torch.int32, device="cuda:0")
+    assert s.lib.izem_psr_batch(s.ctx, nc, C.c_void_p(dTc.data_ptr()), C.c_void_p(drc.data_ptr()),
+                                C.c_void_p(dYc.data_ptr()), 1e-6, C.c_void_p(nst.data_ptr())) == 0
+    s.sync()
+    assert s.status() == 0
+    Tg, Yg = dTc.cpu().numpy(), dYc.cpu().numpy()
+    errT, errY = 0.0, 0.0
+    for c in range(nc):
+        Tr, Yr, ist, ns, nfe, nje = rc.psr(Tc[c], rhoc[c], Yc[c], 1e-6, 1e-6, 1e-12)
+        assert ist == 2
+        errT = max(errT, abs(Tg[c] - Tr) / Tr)
+        errY = max(errY, np.max(np.abs(Yg[c] - Yr) / (2e-4 * np.abs(Yr) + 1e-8)))
+    print(f"front-end mechanism: psr max |dT|/T {errT:.2e}, Y err / bound {errY:.2f}")
+    assert errT <= 2e-4 and errY <= 1.0
+    assert np.max(np.abs(Tg - Tc)) > 1.0
 
 
 def test_element_conservation_of_rates(mech):

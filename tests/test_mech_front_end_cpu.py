@@ -1,0 +1,99 @@
+"""The CHEMKIN front end beyond the bundled mechanism: a mechanism with SRI
+falloff (5 values), 4-value Troe, a named falloff collider (+N2), explicit
+reverse parameters (REV) and an irreversible reaction
+(tests/golden/h2_air_sri_rev.inp), parsed by phd_amd.chemdata, turned into
+straight-line code by phd_amd.codegen_mech and into device tables by
+izem_chem_setup's flat image.  Both kinetics forms are compiled for the host
+from the kernels' own headers and checked against the reference's
+interpreter + CKWYR/FUN/DVODE built from its sources (oracle/_ref/sri)."""
+import ctypes as C
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRI_MECH = os.path.join(HERE, "golden", "h2_air_sri_rev.inp")
+SRI_DIR = os.path.join(ROOT, "oracle", "_ref", "sri")
+pytestmark = pytest.mark.skipif(not os.path.exists(os.path.join(SRI_DIR, "chem.bin")),
+                                reason="oracle/_ref/sri not built")
+
+
+@pytest.fixture(scope="module")
+def sri_host(tmp_path_factory):
+    """Host build of the integrator with the straight-line kinetics generated
+    for the test mechanism (its mech_gen.h first on the include path)."""
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    from phd_amd import chemdata as cd
+    from phd_amd.codegen_mech import generate
+    from phd_amd.solver import mech_flat
+    d = tmp_path_factory.mktemp("sri")
+    mech = cd.read_mechanism(SRI_MECH)
+    (d / "mech_gen.h").write_text(generate(mech))
+    so = d / "libsri.so"
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-shared", "-fPIC", "-I", str(d), "-I",
+                    os.path.join(ROOT, "phd_amd", "csrc"), "-o", str(so),
+                    os.path.join(HERE, "host_bdf.cpp")], check=True)
+    L = C.CDLL(str(so))
+    L.host_setup.argtypes = [C.c_void_p, C.c_long]
+    L.host_rhs.argtypes = [C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+    L.host_psr.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                           C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
+    mf = mech_flat(mech)
+    assert L.host_setup(mf.ctypes.data, mf.size) == 0
+    return L
+
+
+@pytest.mark.parametrize("kinetics", [0, 1])
+def test_front_end_rhs_vs_reference_fun(sri_host, kinetics):
+    """Reactor RHS (T and Y rates) of the table-driven (0) and straight-line
+    (1) kinetics against the reference FUN at 300 random states."""
+    from oracle.oracle import RefChem
+    rc = RefChem(SRI_DIR)
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for _ in range(300):
+        z = np.concatenate([[rng.uniform(400.0, 3000.0)], rng.random(9) ** 3])
+        z[1:] /= z[1:].sum()
+        rho = 10 ** rng.uniform(-6.0, -3.0)
+        zp = np.zeros(10)
+        sri_host.host_rhs(kinetics, rho, z.ctypes.data, zp.ctypes.data)
+        zr = rc.rhs(rho, z)
+        worst = max(worst, np.max(np.abs(zp - zr) / (np.abs(zr) + 1e-30 * np.max(np.abs(zr)))))
+    print(f"kinetics {kinetics}: worst relative RHS difference {worst:.2e}")
+    assert worst <= 1e-10
+
+
+@pytest.mark.parametrize("kinetics", [0, 1])
+def test_front_end_integration_vs_reference_dvode(sri_host, thd, kinetics):
+    """48 ignition cells (the C3 recipe) over 1e-6 s with the test mechanism:
+    the integrator with either kinetics form against the reference DVODE,
+    with the bars of the bundled mechanism's test."""
+    from oracle.oracle import RefChem
+    from phd_amd import ics
+    rc = RefChem(SRI_DIR)
+    n = 48
+    T, p, Y = ics.config3_cells(thd, n, seed=12)
+    Wmix = 1.0 / (Y / (np.array(thd.W) * 1000.0)).sum(axis=1)
+    rho = (p * 10.0) * Wmix / (8.314510e7 * T)
+    dt, rtol, atol = 1e-6, 1e-6, 1e-12
+    Th, Yh = T.copy(), np.ascontiguousarray(Y.copy())
+    nst, ok = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    sri_host.host_psr(kinetics, n, Th.ctypes.data, rho.ctypes.data, Yh.ctypes.data, dt, rtol,
+                      atol, 50, nst.ctypes.data, ok.ctypes.data)
+    assert ok.all()
+    errT, errY = 0.0, 0.0
+    for c in range(n):
+        Tr, Yr, ist, ns, nfe, nje = rc.psr(T[c], rho[c], Y[c], dt, rtol, atol)
+        assert ist == 2
+        errT = max(errT, abs(Th[c] - Tr) / Tr)
+        errY = max(errY, np.max(np.abs(Yh[c] - Yr) / (2e-4 * np.abs(Yr) + 1e-8)))
+    print(f"kinetics {kinetics}: max |dT|/T {errT:.2e}, Y err / bound {errY:.2f}, "
+          f"max |T - T0| {np.max(np.abs(Th - T)):.1f} K")
+    assert errT <= 2e-4
+    assert errY <= 1.0
+    assert np.max(np.abs(Th - T)) > 1.0  # the cells react

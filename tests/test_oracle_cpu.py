@@ -50,6 +50,45 @@ def test_oracle_ckwyr_matches_reference_chemkin(oracle, mech):
     assert worst <= 1e-13
 
 
+SRI_DIR = os.path.join(os.path.dirname(REF), "sri")
+SRI_MECH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "h2_air_sri_rev.inp")
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(SRI_DIR, "chem.bin")),
+                    reason="oracle/_ref/sri not built")
+def test_front_end_mechanism_matches_reference_chemkin(oracle, mech):
+    """The mechanism front end and the restated CKWYR on a mechanism that
+    uses SRI (5 values), 4-value Troe, a named falloff collider (+N2), REV
+    and an irreversible reaction (tests/golden/h2_air_sri_rev.inp), against
+    the reference's interpreter and CKWYR on the same file.  The bundled
+    mechanism's rates differ from it at these states, so the comparison is
+    not with the wrong linking file."""
+    from oracle.oracle import RefChem, make_mech
+    from phd_amd import chemdata as cd
+    m_sri = cd.read_mechanism(SRI_MECH)
+    assert [r["fal"] for r in m_sri.reactions if r["fal"]] == [1, 2, 4]
+    assert sum(1 for r in m_sri.reactions if r["rpar"]) == 1
+    assert sum(1 for r in m_sri.reactions if not r["rev"]) == 1
+    rc = RefChem(SRI_DIR)
+    rc0 = RefChem()
+    m = make_mech(m_sri)
+    rng = np.random.default_rng(4)
+    worst, differs = 0.0, 0.0
+    for _ in range(300):
+        T = rng.uniform(300.0, 3200.0)
+        rho = 10 ** rng.uniform(-6.0, -2.0)
+        Y = rng.random(9) ** 3
+        Y /= Y.sum()
+        w = np.zeros(9)
+        oracle.lib.ora_ckwyr(C.byref(m), rho, T, _dp(Y), _dp(w))
+        wr = rc.wdot(rho, T, Y)
+        scale = np.max(np.abs(wr))
+        worst = max(worst, np.max(np.abs(w - wr)) / scale)
+        differs = max(differs, np.max(np.abs(rc0.wdot(rho, T, Y) - wr)) / scale)
+    assert worst <= 1e-13
+    assert differs > 1e-3
+
+
 @needs_ref
 def test_oracle_reactor_rhs_matches_reference_fun(oracle, mech):
     from oracle.oracle import RefChem, make_mech
