@@ -414,6 +414,8 @@ def read_mechanism(inp_path: str | None = None, therm_path: str | None = None) -
         if u.startswith("REAC"):
             sec = "R"
             for w in u.split()[1:]:
+                if w.startswith("MOLEC"):
+                    raise ValueError("MOLECULES units for the pre-exponential factors are not read")
                 if w.startswith("KELV"): eunits = "KELV"
                 elif w.startswith("KCAL"): eunits = "KCAL"
                 elif w.startswith("KJOU"): eunits = "KJOU"

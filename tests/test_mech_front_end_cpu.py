@@ -18,8 +18,27 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRI_MECH = os.path.join(HERE, "golden", "h2_air_sri_rev.inp")
 SRI_DIR = os.path.join(ROOT, "oracle", "_ref", "sri")
-pytestmark = pytest.mark.skipif(not os.path.exists(os.path.join(SRI_DIR, "chem.bin")),
-                                reason="oracle/_ref/sri not built")
+needs_ref = pytest.mark.skipif(not os.path.exists(os.path.join(SRI_DIR, "chem.bin")),
+                               reason="oracle/_ref/sri not built")
+
+
+@pytest.mark.parametrize("line,aux", [
+    ("REACTIONS MOLECULES", ""),
+    ("REACTIONS", "   PLOG/1.0 1.0E+13 0.0 0.0/"),
+    ("REACTIONS", "   LT/1.0 2.0/"),
+    ("REACTIONS", "   FORD/H2 1.5/"),
+])
+def test_front_end_refuses_unread_forms(tmp_path, line, aux):
+    """Forms the kinetics do not evaluate are refused, not dropped."""
+    from phd_amd import chemdata as cd
+    text = open(SRI_MECH).read().replace("\nREACTIONS\n", "\n" + line + "\n")
+    if aux:
+        text = text.replace("O+H2=H+OH                 5.080E+04   2.670  6.290E+03\n",
+                            "O+H2=H+OH                 5.080E+04   2.670  6.290E+03\n" + aux + "\n")
+    p = tmp_path / "m.inp"
+    p.write_text(text)
+    with pytest.raises(ValueError):
+        cd.read_mechanism(str(p))
 
 
 @pytest.fixture(scope="module")
@@ -48,6 +67,7 @@ def sri_host(tmp_path_factory):
     return L
 
 
+@needs_ref
 @pytest.mark.parametrize("kinetics", [0, 1])
 def test_front_end_rhs_vs_reference_fun(sri_host, kinetics):
     """Reactor RHS (T and Y rates) of the table-driven (0) and straight-line
@@ -68,6 +88,7 @@ def test_front_end_rhs_vs_reference_fun(sri_host, kinetics):
     assert worst <= 1e-10
 
 
+@needs_ref
 @pytest.mark.parametrize("kinetics", [0, 1])
 def test_front_end_integration_vs_reference_dvode(sri_host, thd, kinetics):
     """48 ignition cells (the C3 recipe) over 1e-6 s with the test mechanism:
