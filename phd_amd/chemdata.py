@@ -244,9 +244,12 @@ class Thd:
         return np.ascontiguousarray(np.concatenate(parts), dtype=np.float64)
 
 
-def build_thd(adi: Adim, therm_path=None, tranfit_path=None) -> Thd:
-    """thd_init (thd_tools_perfectgas.f90:12-248)."""
-    tr = read_tranfit(tranfit_path)
+def build_thd(adi: Adim, therm_path=None, tranfit_path=None,
+              fits: "TransportFits | None" = None) -> Thd:
+    """thd_init (thd_tools_perfectgas.f90:12-248).  The transport fits come
+    from a tranfit.out file (the bundled one by default), or are given
+    (phd_amd.tranfit.fit_transport)."""
+    tr = fits if fits is not None else read_tranfit(tranfit_path)
     nasa, _ = read_therm(therm_path)
     n = len(tr.names)
     heat = np.zeros((n, 2, 7))
