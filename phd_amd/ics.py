@@ -28,6 +28,9 @@ class Box:
     ng: int
     ndim: int
     L: tuple  # physical box sides [m]
+    # nondimensional 1/dx arrays (x, y, z; ghosts included) of a grid read
+    # from a file (phd_amd.grid); None: the uniform grid of the sides L
+    metrics: tuple | None = None
 
     @property
     def shape(self):
@@ -48,6 +51,8 @@ class Box:
         grid: dx_i = L_ref / dx (metrics + adimensionalize,
         module_tools.f90:223-442, :160-212).  Directions beyond ndim get
         L_ref * 1 as the reference's metrics leave them at 1/m."""
+        if self.metrics is not None:
+            return [np.array(m, dtype=np.float64) for m in self.metrics]
         out = []
         for d, n in enumerate((self.nx, self.ny, self.nz)):
             a = np.empty(n + 2 * self.ng)

@@ -18,7 +18,21 @@ def P(a):
 
 def small_state(adi, thd, kind: str, n: int, seed: int = 7):
     """kind '2d': config-1 recipe on n x n; '3d': config-2 recipe on n^3,
-    both with strong (hot-spot / turbulence) structure."""
+    both with strong (hot-spot / turbulence) structure; '2ds': the 2-D state
+    on a stretched grid, its metric arrays computed from the coordinates
+    by phd_amd.grid as the reference computes them from grid.dat."""
+    if kind == "2ds":
+        import dataclasses
+        from phd_amd.grid import metric
+        box, v, T = ics.config1_state(adi, thd, n=n, seed=seed)
+        i = np.arange(n, dtype=np.float64)
+        mets = []
+        for d in range(2):
+            h = box.L[d] / n
+            x = h * (i + 0.8 * n / (2 * np.pi) * np.sin(2 * np.pi * i / n) * (0.3 + 0.1 * d))
+            mets.append(metric(x) * adi.L_ref)
+        mets.append(np.ones(box.nz + 2 * box.ng) * adi.L_ref)
+        return dataclasses.replace(box, metrics=tuple(mets)), v, T
     if kind == "2d":
         box, v, T = ics.config1_state(adi, thd, n=n, seed=seed)
     else:

@@ -46,7 +46,8 @@ def _relerr(a, b, box):
     return np.array(out)
 
 
-CASES = [("2d", 24), ("3d", 12), ("2d", 7), ("3d", 8)]  # the last two: minimum and small boxes
+# the third and fourth: minimum and small boxes; the last: a stretched grid
+CASES = [("2d", 24), ("3d", 12), ("2d", 7), ("3d", 8), ("2ds", 16)]
 
 
 @pytest.mark.parametrize("kind,n", CASES)
