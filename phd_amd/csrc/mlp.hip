@@ -418,7 +418,7 @@ extern "C" int izem_mlp_mse(izem_ctx *ctx, int Bn, int nout, int npad, const flo
 // -------------------------------------------------------------------
 #define FX 264  // LDS row stride of an activation tile (bf16): 528 B, 16-byte aligned
 
-template <int K, int N, bool LAST>
+template <int K, int N, bool LAST, int RB>
 __device__ inline void fused_layer(u16 (*X)[FX], const u16 *__restrict__ Wt,
                                    const float *__restrict__ b, float *__restrict__ pred,
                                    int row0, int M) {
@@ -426,9 +426,9 @@ __device__ inline void fused_layer(u16 (*X)[FX], const u16 *__restrict__ Wt,
   constexpr int CB = NW / 16;      // 16-column blocks per wave
   constexpr int KS = K / 32;       // 32-deep slices
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  f32x4 acc[4][CB];
+  f32x4 acc[RB][CB];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < RB; i++)
 #pragma unroll
     for (int j = 0; j < CB; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
   const u16 *wrow[CB];
@@ -449,7 +449,7 @@ __device__ inline void fused_layer(u16 (*X)[FX], const u16 *__restrict__ Wt,
         bnext[j] = __builtin_bit_cast(bf16x8, *(const uint4 *)(wrow[j] + k0 + 32));
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < RB; i++) {
       const uint4 q = *(const uint4 *)&X[i * 16 + (lane & 15)][k0 + (lane >> 4) * 8];
       const bf16x8 af = __builtin_bit_cast(bf16x8, q);
 #pragma unroll
@@ -463,7 +463,7 @@ __device__ inline void fused_layer(u16 (*X)[FX], const u16 *__restrict__ Wt,
   }
   if (!LAST) __syncthreads();  // every wave has read the input tile
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < RB; i++)
 #pragma unroll
     for (int j = 0; j < CB; j++)
 #pragma unroll
@@ -482,7 +482,11 @@ __device__ inline void fused_layer(u16 (*X)[FX], const u16 *__restrict__ Wt,
 
 // resident 256-thread blocks per CU (one wave per SIMD each)
 #ifndef IZ_MLP_WAVES
-#define IZ_MLP_WAVES 3
+#define IZ_MLP_WAVES 2
+#endif
+// rows of an activation tile (16 per row block)
+#ifndef IZ_MLP_ROWS
+#define IZ_MLP_ROWS 128
 #endif
 
 __global__ void __attribute__((amdgpu_flat_work_group_size(256, 256),
@@ -492,22 +496,24 @@ k_mlp_fused(int M, const u16 *__restrict__ X, const u16 *__restrict__ W0,
             const float *__restrict__ b0, const float *__restrict__ b1,
             const float *__restrict__ b2, const float *__restrict__ b3,
             float *__restrict__ pred) {
-  __shared__ __attribute__((aligned(16))) u16 A[64][FX];
-  const int ntiles = (M + 63) / 64;
+  constexpr int R = IZ_MLP_ROWS, RB = R / 16;
+  __shared__ __attribute__((aligned(16))) u16 A[R][FX];
+  const int ntiles = (M + R - 1) / R;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int row0 = tile * 64;
-    // input tile: 64 rows x 32 features (16 B per thread)
-    {
-      const int t = threadIdx.x, r = t >> 2, c = (t & 3) * 8;
+    const int row0 = tile * R;
+    // input tile: R rows x 32 features (16 B per thread and pass)
+#pragma unroll
+    for (int p = 0; p < R / 64; p++) {
+      const int t = threadIdx.x, r = p * 64 + (t >> 2), c = (t & 3) * 8;
       uint4 q = make_uint4(0, 0, 0, 0);
       if (row0 + r < M) q = *(const uint4 *)(X + (long)(row0 + r) * 32 + c);
       *(uint4 *)&A[r][c] = q;
     }
     __syncthreads();
-    fused_layer<32, 256, false>(A, W0, b0, nullptr, row0, M);
-    fused_layer<256, 256, false>(A, W1, b1, nullptr, row0, M);
-    fused_layer<256, 256, false>(A, W2, b2, nullptr, row0, M);
-    fused_layer<256, 64, true>(A, W3, b3, pred, row0, M);
+    fused_layer<32, 256, false, RB>(A, W0, b0, nullptr, row0, M);
+    fused_layer<256, 256, false, RB>(A, W1, b1, nullptr, row0, M);
+    fused_layer<256, 256, false, RB>(A, W2, b2, nullptr, row0, M);
+    fused_layer<256, 64, true, RB>(A, W3, b3, pred, row0, M);
     __syncthreads();  // the next tile's input overwrites A
   }
 }
@@ -518,7 +524,7 @@ extern "C" int izem_mlp_infer(izem_ctx *ctx, int M, const void *X, const void *c
   for (int l = 0; l < 4; l++)
     if (!Wt[l] || !b[l] || (((uintptr_t)Wt[l]) & 15)) return IZEM_E_ARG;
   if (((uintptr_t)X) & 15) return IZEM_E_ARG;
-  const long ntiles = (M + 63) / 64;
+  const long ntiles = (M + IZ_MLP_ROWS - 1) / IZ_MLP_ROWS;
   long g = (long)ctx->ncu * IZ_MLP_WAVES;
   if (g > ntiles) g = ntiles;
   int pk = iz_prof_begin(ctx, IZP_CHEM);
