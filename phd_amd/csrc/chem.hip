@@ -218,6 +218,84 @@ void iz_chem_destroy(izem_ctx *ctx) {
   ctx->chem = nullptr;
 }
 
+// Setup self-check of the straight-line kinetics' compiled integrator: 64
+// ignition cells (stoichiometric H2/air at 1 atm, 1000..1630 K, over
+// 1e-6 s) integrated by the one-wave kernel with the straight-line and with
+// the table-driven kinetics.  The two differ only in round-off, so end
+// temperatures agree to 1e-6 relative and the step totals to 3 %; a build
+// of the straight-line kernel that departs from its source (as some builds
+// of its heavily spilling variants did) fails this and the context keeps
+// to the tables.  The work counters and the status word are not touched.
+static int gen_bdf_check(izem_ctx *ctx, const IzMech &m, bool &agree) {
+  constexpr int n = 64, nsp = CK_NEQ - 1;
+  const double dt = 1.0e-6;
+  double T[2][n], rho[n], Y[2][n * nsp];
+  int nst[2][n];
+  for (int c = 0; c < n; c++) {
+    double y[nsp] = {0.0, 0.0283, 0.0, 0.2264, 0.0, 0.0, 0.0, 0.0, 0.0};
+    y[0] = 1.0e-6 * (1 + c % 4);           // a trace of H atoms
+    y[4] = 1.0e-6 * (1 + (c / 4) % 4);     // and of OH
+    double s = 0.0;
+    for (int k = 0; k < nsp - 1; k++) s += y[k];
+    y[nsp - 1] = 1.0 - s;                  // N2 closes the sum
+    double wi = 0.0;
+    for (int k = 0; k < nsp; k++) wi += y[k] / m.wt[k];
+    T[0][c] = T[1][c] = 1000.0 + 10.0 * c;
+    rho[c] = 1.01325e6 / (8.314510e7 * T[0][c] * wi);
+    for (int k = 0; k < nsp; k++) Y[0][c * nsp + k] = Y[1][c * nsp + k] = y[k];
+  }
+  double *dT = nullptr, *dr = nullptr, *dY = nullptr;
+  int *dn = nullptr, *dst = nullptr;
+  int rc = IZEM_OK;
+  IzChem *ch = ctx->chem;
+  if (hipMalloc(&dT, sizeof T[0]) != hipSuccess || hipMalloc(&dr, sizeof rho) != hipSuccess ||
+      hipMalloc(&dY, sizeof Y[0]) != hipSuccess || hipMalloc(&dn, sizeof nst[0]) != hipSuccess ||
+      hipMalloc(&dst, sizeof(int)) != hipSuccess) {
+    rc = IZEM_E_NOMEM;
+  } else {
+    int zero = 0;
+    for (int v = 0; v < 2 && rc == IZEM_OK; v++) {
+      if (hipMemcpyAsync(dT, T[v], sizeof T[0], hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+          hipMemcpyAsync(dr, rho, sizeof rho, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+          hipMemcpyAsync(dY, Y[v], sizeof Y[0], hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+          hipMemcpyAsync(dst, &zero, sizeof zero, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        rc = IZEM_E_HIP;
+        break;
+      }
+      hipLaunchKernelGGL(k_chem_prep, dim3(1), dim3(1), 0, ctx->stream, ch->counter, ctx->ts, 0);
+      if (v == 0)
+        hipLaunchKernelGGL((k_psr_queue<KinGen, 1>), dim3(1), dim3(64), 0, ctx->stream, n, dT, dr,
+                           dY, dt, ch->rtol, ch->atol, ch->jac_age, ch->counter, dn, dst, ch->stats);
+      else
+        hipLaunchKernelGGL((k_psr_queue<KinTables, 1>), dim3(1), dim3(64), 0, ctx->stream, n, dT,
+                           dr, dY, dt, ch->rtol, ch->atol, ch->jac_age, ch->counter, dn, dst,
+                           ch->stats);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(T[v], dT, sizeof T[0], hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+          hipMemcpyAsync(nst[v], dn, sizeof nst[0], hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(ctx->stream) != hipSuccess)
+        rc = IZEM_E_HIP;
+    }
+    hipLaunchKernelGGL(k_stats_zero, dim3(1), dim3(64), 0, ctx->stream, ch->stats);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = IZEM_E_HIP;
+  }
+  if (dT) (void)hipFree(dT);
+  if (dr) (void)hipFree(dr);
+  if (dY) (void)hipFree(dY);
+  if (dn) (void)hipFree(dn);
+  if (dst) (void)hipFree(dst);
+  if (rc != IZEM_OK) return rc;
+  long s0 = 0, s1 = 0;
+  agree = true;
+  for (int c = 0; c < n; c++) {
+    s0 += nst[0][c];
+    s1 += nst[1][c];
+    if (!(fabs(T[0][c] - T[1][c]) <= 1.0e-6 * T[1][c])) agree = false;
+  }
+  if (!(labs(s0 - s1) <= 0.03 * s1)) agree = false;
+  return IZEM_OK;
+}
+
 extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, double rtol,
                                double atol, int trim) {
   if (!ctx || !mf || rtol <= 0 || atol <= 0) return IZEM_E_ARG;
@@ -248,11 +326,17 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
                             ctx->stream));
     IZ_CHECK(hipStreamSynchronize(ctx->stream));
   }
-  ctx->chem->generated = same;
-  ctx->chem->use_gen = same;
   ctx->chem->rtol = rtol;
   ctx->chem->atol = atol;
   ctx->chem->trim = trim;
+  if (same) {
+    bool agree = false;
+    int rc = gen_bdf_check(ctx, m, agree);
+    if (rc != IZEM_OK) return rc;
+    if (!agree) same = 0;
+  }
+  ctx->chem->generated = same;
+  ctx->chem->use_gen = same;
   return IZEM_OK;
 }
 

@@ -295,6 +295,19 @@ def test_straight_line_kinetics_one_wave_only(mech):
     assert s.lib.izem_chem_tune(s.ctx, 1, 4, 50, 1) == 0
 
 
+def test_setup_self_check_is_clean(mech):
+    """izem_chem_setup integrates 64 ignition cells with both kinetics forms
+    to check the straight-line build against the table-driven one: on this
+    build they agree (the straight-line kinetics stay selectable), and the
+    check leaves no trace in the work counters or the status word."""
+    s = _solver(mech)
+    out = np.zeros(4, dtype=np.uint64)
+    assert s.lib.izem_chem_stats(s.ctx, out.ctypes.data) == 0
+    assert np.all(out == 0)
+    assert s.status() == 0
+    assert s.lib.izem_chem_tune(s.ctx, 1, 4, 50, 1) == 0
+
+
 @pytest.mark.parametrize("kinetics,waves", [(0, 1), (0, 2), (0, 4), (1, 1)])
 def test_variants_follow_host_build(mech, thd, host_bdf, kinetics, waves):
     """Every kernel variant integrates the same source as a g++ build of it;
