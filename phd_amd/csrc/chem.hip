@@ -26,6 +26,10 @@ __constant__ IzBdf c_bdf;
 #include "mech_gen.h"
 #include "chem_device.h"
 
+// bins of the stiffness key of izem_psr_batch: eight per octave of
+// max_k |df_k/dY_k| [1/s] over [1, 2^64)
+#define IZ_ORDER_BINS 512
+
 struct IzChem {
   double rtol, atol;
   int trim;
@@ -39,6 +43,11 @@ struct IzChem {
   // BDF steps, RHS evaluations, Jacobians, LU solves
   unsigned long long *stats;
   int use_gen;    // run the straight-line kinetics (generated && not disabled)
+  // stiffness order of izem_psr_batch: on/off, per-cell bin and queue
+  // permutation (capacity order_cap cells), bin counts then offsets
+  int order;
+  long order_cap;
+  int *order_bin, *order_perm, *order_hist;
 };
 
 // ------------------------------------------------------------ kernels
@@ -107,14 +116,16 @@ __device__ IZ_FORCEINLINE BdfResult chem_integrate(double *zs, double (&J)[N_][N
 template <class Kin, int WAVES>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(WAVES, WAVES))) k_psr_queue(int n, double *__restrict__ T, const double *__restrict__ rho,
                             double *__restrict__ Y, double dt, double rtol, double atol,
-                            int jac_age, int *__restrict__ counter, int *__restrict__ nsteps,
+                            int jac_age, int *__restrict__ counter,
+                            const int *__restrict__ order, int *__restrict__ nsteps,
                             int *__restrict__ status, unsigned long long *__restrict__ stats) {
   __shared__ double zs[IZ_ZLDS_DOUBLES(WAVES)];
   IZ_LANE_JACOBIAN(J);
   unsigned long long w_nst = 0, w_nfe = 0, w_nje = 0, w_nsol = 0;
   for (;;) {
-    int cell = atomicAdd(counter, 1);
-    if (cell >= n) break;
+    int q = atomicAdd(counter, 1);
+    if (q >= n) break;
+    const int cell = order ? order[q] : q;
     double y[N_];
     y[0] = T[cell];
     for (int k = 0; k < N_ - 1; k++) y[1 + k] = Y[(long)cell * (N_ - 1) + k];
@@ -214,6 +225,9 @@ void iz_chem_destroy(izem_ctx *ctx) {
   if (!ctx || !ctx->chem) return;
   if (ctx->chem->counter) (void)hipFree(ctx->chem->counter);
   if (ctx->chem->stats) (void)hipFree(ctx->chem->stats);
+  if (ctx->chem->order_hist) (void)hipFree(ctx->chem->order_hist);
+  if (ctx->chem->order_bin) (void)hipFree(ctx->chem->order_bin);
+  if (ctx->chem->order_perm) (void)hipFree(ctx->chem->order_perm);
   free(ctx->chem);
   ctx->chem = nullptr;
 }
@@ -265,11 +279,12 @@ static int gen_bdf_check(izem_ctx *ctx, const IzMech &m, bool &agree) {
       hipLaunchKernelGGL(k_chem_prep, dim3(1), dim3(1), 0, ctx->stream, ch->counter, ctx->ts, 0);
       if (v == 0)
         hipLaunchKernelGGL((k_psr_queue<KinGen, 1>), dim3(1), dim3(64), 0, ctx->stream, n, dT, dr,
-                           dY, dt, ch->rtol, ch->atol, ch->jac_age, ch->counter, dn, dst, ch->stats);
+                           dY, dt, ch->rtol, ch->atol, ch->jac_age, ch->counter, nullptr, dn, dst,
+                           ch->stats);
       else
         hipLaunchKernelGGL((k_psr_queue<KinTables, 1>), dim3(1), dim3(64), 0, ctx->stream, n, dT,
-                           dr, dY, dt, ch->rtol, ch->atol, ch->jac_age, ch->counter, dn, dst,
-                           ch->stats);
+                           dr, dY, dt, ch->rtol, ch->atol, ch->jac_age, ch->counter, nullptr,
+                           dn, dst, ch->stats);
       if (hipGetLastError() != hipSuccess ||
           hipMemcpyAsync(T[v], dT, sizeof T[0], hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
           hipMemcpyAsync(nst[v], dn, sizeof nst[0], hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
@@ -311,6 +326,9 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
     if (hipMalloc(&ctx->chem->counter, sizeof(int)) != hipSuccess) return IZEM_E_NOMEM;
     if (hipMalloc(&ctx->chem->stats, 4 * sizeof(unsigned long long)) != hipSuccess)
       return IZEM_E_NOMEM;
+    if (hipMalloc(&ctx->chem->order_hist, IZ_ORDER_BINS * sizeof(int)) != hipSuccess)
+      return IZEM_E_NOMEM;
+    ctx->chem->order = 1;
     hipLaunchKernelGGL(k_stats_zero, dim3(1), dim3(64), 0, ctx->stream, ctx->chem->stats);
     // defaults measured on MI355X (2^18 cells at dt = 1e-6 s and 2^20 cells
     // of a reacting HIT half step): one wave per SIMD with the whole
@@ -361,6 +379,130 @@ extern "C" int izem_chem_tune(izem_ctx *ctx, int waves, int blocks_per_cu, int j
   return IZEM_OK;
 }
 
+// ---------------------------------------------------- stiffness order
+// A 64-lane wave fetches its next cells only when its slowest lane is done,
+// so a wave costs the largest step count among its cells.  izem_psr_batch
+// hands cells out in decreasing order of a stiffness key, the largest
+// diagonal entry of the species block of the reactor Jacobian (the fastest
+// chemical rate), so each wave gets cells of like step counts and the
+// stiffest come first.  Each cell is still integrated on its own; only the
+// order in which lanes take cells changes.  Counting sort in three passes:
+// key and bin counts, exclusive scan, scatter with block-aggregated atomics.
+__global__ void k_order_zero(int *__restrict__ hist) {
+  for (int i = threadIdx.x; i < IZ_ORDER_BINS; i += blockDim.x) hist[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_order_key(int n, const double *__restrict__ T,
+                                                   const double *__restrict__ rho,
+                                                   const double *__restrict__ Y,
+                                                   int *__restrict__ bin,
+                                                   int *__restrict__ hist) {
+  __shared__ int h[IZ_ORDER_BINS];
+  for (int i = threadIdx.x; i < IZ_ORDER_BINS; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) {
+    double y[N_], f0[N_], yl[N_], f[N_];
+    y[0] = T[c];
+    for (int k = 0; k < N_ - 1; k++) y[1 + k] = Y[(long)c * (N_ - 1) + k];
+    const double r_c = rho[c];
+    d_reactor_rhs(r_c, y, f0);
+    double s = 0.0;
+    for (int j = 1; j < N_; j++) {
+      for (int k = 0; k < N_; k++) yl[k] = y[k];
+      const double dy = 1.4901161193847656e-08 * fmax(fabs(y[j]), 1.0e-6);
+      yl[j] = y[j] + dy;
+      d_reactor_rhs(r_c, yl, f);
+      s = fmax(s, fabs((f[j] - f0[j]) / dy));
+    }
+    // decreasing key -> increasing slot; NaN keys go last, infinite first
+    const double l = log2(s) * 8.0;
+    const int b = (l >= 1.0) ? (l < IZ_ORDER_BINS - 1 ? (int)l : IZ_ORDER_BINS - 1) : 0;
+    const int slot = IZ_ORDER_BINS - 1 - b;
+    bin[c] = slot;
+    atomicAdd(&h[slot], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < IZ_ORDER_BINS; i += blockDim.x)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// counts -> exclusive offsets, in place (one 64-lane block)
+__global__ void k_order_scan(int *__restrict__ hist) {
+  __shared__ int part[64];
+  constexpr int per = IZ_ORDER_BINS / 64;
+  const int t = threadIdx.x;
+  int loc[per], sum = 0;
+  for (int i = 0; i < per; i++) {
+    loc[i] = sum;
+    sum += hist[t * per + i];
+  }
+  part[t] = sum;
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int i = 0; i < 64; i++) {
+      const int v = part[i];
+      part[i] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  for (int i = 0; i < per; i++) hist[t * per + i] = part[t] + loc[i];
+}
+
+__global__ void __launch_bounds__(256) k_order_scatter(int n, const int *__restrict__ bin,
+                                                       int *__restrict__ off,
+                                                       int *__restrict__ perm) {
+  __shared__ int h[IZ_ORDER_BINS];
+  for (int i = threadIdx.x; i < IZ_ORDER_BINS; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int slot = 0, rank = 0;
+  if (c < n) {
+    slot = bin[c];
+    rank = atomicAdd(&h[slot], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < IZ_ORDER_BINS; i += blockDim.x)
+    if (h[i]) h[i] = atomicAdd(&off[i], h[i]);
+  __syncthreads();
+  if (c < n) perm[h[slot] + rank] = c;
+}
+
+// builds the queue permutation of n cells; IZEM_E_NOMEM if the buffers
+// cannot grow
+static int order_cells(izem_ctx *ctx, int n, const double *T, const double *rho,
+                       const double *Y) {
+  IzChem *ch = ctx->chem;
+  if (n > ch->order_cap) {
+    IZ_CHECK(hipStreamSynchronize(ctx->stream));
+    if (ch->order_bin) (void)hipFree(ch->order_bin);
+    if (ch->order_perm) (void)hipFree(ch->order_perm);
+    ch->order_bin = ch->order_perm = nullptr;
+    ch->order_cap = 0;
+    if (hipMalloc(&ch->order_bin, (size_t)n * sizeof(int)) != hipSuccess ||
+        hipMalloc(&ch->order_perm, (size_t)n * sizeof(int)) != hipSuccess)
+      return IZEM_E_NOMEM;
+    ch->order_cap = n;
+  }
+  const int nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_order_zero, dim3(1), dim3(256), 0, ctx->stream, ch->order_hist);
+  hipLaunchKernelGGL(k_order_key, dim3(nb), dim3(256), 0, ctx->stream, n, T, rho, Y,
+                     ch->order_bin, ch->order_hist);
+  hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, ctx->stream, ch->order_hist);
+  hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(256), 0, ctx->stream, n, ch->order_bin,
+                     ch->order_hist, ch->order_perm);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+extern "C" int izem_chem_order(izem_ctx *ctx, int on) {
+  if (!ctx || !ctx->chem || on < 0 || on > 1) return IZEM_E_ARG;
+  ctx->chem->order = on;
+  return IZEM_OK;
+}
+
 static int persistent_grid(izem_ctx *ctx, long n) {
   int dev = 0, ncu = 256;
   if (hipGetDevice(&dev) == hipSuccess)
@@ -386,12 +528,21 @@ extern "C" int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho
   if (!ctx || !ctx->chem || n < 0 || !T || !rho || !Y || !(dt > 0)) return IZEM_E_ARG;
   if (n == 0) return IZEM_OK;
   IzChem *ch = ctx->chem;
-  hipLaunchKernelGGL(k_chem_prep, dim3(1), dim3(1), 0, ctx->stream, ch->counter, ctx->ts, 0);
   int pk = iz_prof_begin(ctx, IZP_CHEM);
+  const int *order = nullptr;
+  if (ch->order) {
+    int rc = order_cells(ctx, n, T, rho, Y);
+    if (rc != IZEM_OK) {
+      iz_prof_end(ctx, pk);
+      return rc;
+    }
+    order = ch->order_perm;
+  }
+  hipLaunchKernelGGL(k_chem_prep, dim3(1), dim3(1), 0, ctx->stream, ch->counter, ctx->ts, 0);
   dim3 grid(persistent_grid(ctx, n)), blk(64);
 #define IZ_PSR(K, W)                                                                        \
   hipLaunchKernelGGL((k_psr_queue<K, W>), grid, blk, 0, ctx->stream, n, T, rho, Y, dt, ch->rtol, \
-                     ch->atol, ch->jac_age, ch->counter, nsteps, ctx->status, ch->stats)
+                     ch->atol, ch->jac_age, ch->counter, order, nsteps, ctx->status, ch->stats)
   if (ch->use_gen) {
     IZ_PSR(KinGen, 1);
   } else {

@@ -44,7 +44,8 @@ module izem_hip_mod
             izem_euler_llf_z, izem_viscflux, izem_force_sumY, izem_prim_inv_var, &
             izem_upd_prim_var_woT, izem_comm_cons, izem_upd_prim_var_ghost, &
             izem_shock_sensor, izem_rk3, izem_rhs, izem_timestep, izem_chem_setup, &
-            izem_chem_tune, izem_chem_stats, izem_ckwyr, izem_psr_batch, izem_reaction_dvode
+            izem_chem_tune, izem_chem_stats, izem_ckwyr, izem_psr_batch, izem_reaction_dvode, &
+            izem_chem_order
   public :: izem_surr_features, izem_mlp_infer, izem_surr_apply, izem_surr_fallback
   public :: izem_check, hip_rk3, hip_timestep, hip_euler_llf, hip_viscflux, hip_reaction_selec
 
@@ -184,6 +185,11 @@ module izem_hip_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: waves, blocks_per_cu, jac_age, kinetics
+    end function
+    integer(c_int) function izem_chem_order(ctx, on) bind(C, name="izem_chem_order")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
     end function
     integer(c_int) function izem_chem_stats(ctx, out) bind(C, name="izem_chem_stats")
       import :: c_int, c_ptr, c_long_long

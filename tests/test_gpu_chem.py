@@ -435,3 +435,33 @@ def test_psr_batch_sizes(mech, thd, host_bdf, n):
     host_bdf.host_psr(1, n, Th.ctypes.data, rho.ctypes.data, Yh.ctypes.data, 1e-6, 1e-6, 1e-12,
                       50, nh.ctypes.data, ok.ctypes.data)
     assert np.all(np.abs(dT.cpu().numpy() - Th) / Th <= 1e-6)
+
+
+def test_stiffness_order_changes_no_result(mech, thd):
+    """izem_psr_batch hands cells out stiffest first by default; the end
+    states and per-cell step counts are bit-identical to index order, since
+    every lane integrates its cell on its own.  Cells of an ignition
+    ensemble (step counts from a few tens to several hundred) at a ragged
+    count; an out-of-range switch is refused."""
+    import torch
+    s = _solver(mech)
+    n = 4096 + 37
+    T, rho, Y = _cells(thd, n, seed=5)
+    d = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
+    out = []
+    for on in (1, 0):
+        assert s.lib.izem_chem_order(s.ctx, on) == 0
+        dT, drho, dY = d(T), d(rho), d(Y)
+        nst = torch.full((n,), -1, dtype=torch.int32, device="cuda:0")
+        assert s.lib.izem_psr_batch(s.ctx, n, C.c_void_p(dT.data_ptr()),
+                                    C.c_void_p(drho.data_ptr()), C.c_void_p(dY.data_ptr()), 1e-6,
+                                    C.c_void_p(nst.data_ptr())) == 0
+        s.sync()
+        assert s.status() == 0
+        out.append((dT.cpu().numpy(), dY.cpu().numpy(), nst.cpu().numpy()))
+    (T1, Y1, n1), (T0, Y0, n0) = out
+    assert np.all(n1 > 0) and n1.max() > 4 * n1.min()
+    assert np.array_equal(n1, n0)
+    assert np.array_equal(T1.view(np.uint64), T0.view(np.uint64))
+    assert np.array_equal(Y1.view(np.uint64), Y0.view(np.uint64))
+    assert s.lib.izem_chem_order(s.ctx, 2) == -1
