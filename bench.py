@@ -120,17 +120,27 @@ def chem_vode_rate(s, thd, ncell: int):
     Wmix = 1.0 / (Y / (np.array(thd.W) * 1000.0)).sum(axis=1)
     rho = (p * 10.0) * Wmix / (8.314510e7 * T)
     d = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
-    dT, drho, dY = d(T), d(rho), d(Y)
+    drho = d(rho)
     nst = torch.zeros(ncell, dtype=torch.int32, device="cuda:0")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rc = s.lib.izem_psr_batch(s.ctx, ncell, C.c_void_p(dT.data_ptr()), C.c_void_p(drho.data_ptr()),
-                              C.c_void_p(dY.data_ptr()), 1.0e-6, C.c_void_p(nst.data_ptr()))
-    s.sync()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    el, rc = {}, 0
+    # index order, then stiffest first (izem_chem_order): same end states,
+    # the waves of the second hold cells of like step counts
+    for order in (0, 1):
+        dT, dY = d(T), d(Y)
+        rc |= s.lib.izem_chem_order(s.ctx, order)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rc |= s.lib.izem_psr_batch(s.ctx, ncell, C.c_void_p(dT.data_ptr()),
+                                   C.c_void_p(drho.data_ptr()), C.c_void_p(dY.data_ptr()), 1.0e-6,
+                                   C.c_void_p(nst.data_ptr()))
+        s.sync()
+        torch.cuda.synchronize()
+        el[order] = time.perf_counter() - t0
+    rc |= s.lib.izem_chem_order(s.ctx, 0)
     st = nst.cpu().numpy()
-    return {"cells": ncell, "dt_s": 1e-6, "vode_cells_per_s": ncell / el, "seconds": el,
+    return {"cells": ncell, "dt_s": 1e-6, "queue_order": "stiffness",
+            "vode_cells_per_s": ncell / el[1], "seconds": el[1],
+            "vode_cells_per_s_index_order": ncell / el[0],
             "bdf_steps_mean": float(st.mean()), "bdf_steps_max": int(st.max()), "rc": rc}
 
 
@@ -226,7 +236,8 @@ def main():
         Ts, rs, Ys = autotune.sample_reacting_cells(s, 1 << 20)
         tuned = autotune.tune(s, Ts, rs, Ys, dt_half, key=f"reacting-hit-{n}", use_cache=False)
         log(f"chemistry kernels tuned: waves {tuned[0]}, blocks/CU {tuned[1]}, "
-            f"kinetics {'straight-line' if tuned[3] else 'tables'} ({tuned[4]:.3e} cells/s)")
+            f"kinetics {'straight-line' if tuned[3] else 'tables'}, "
+            f"{'stiffness' if tuned[5] else 'index'} order ({tuned[4]:.3e} cells/s)")
         del Ts, rs, Ys
     log("state resident on the GPU; warm-up")
     for _ in range(args.warmup):

@@ -177,12 +177,14 @@ int izem_ckwyr(izem_ctx *ctx, int n, const double *rho, const double *T, const d
  * nsteps (may be NULL) receives the BDF step count of each cell */
 int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho, double *Y, double dt,
                    int *nsteps);
-/* order in which izem_psr_batch hands cells to lanes: 1 (the default)
- * stiffest first -- a key pass evaluates max_k |df_k/dY_k| of each cell
- * and counting-sorts the cells by it, so the cells sharing a 64-lane wave
- * have like step counts; 0 in index order.  Results are the same either
- * way; only the time differs.  No counterpart in the reference (PSR_CHEMKIN
- * integrates one cell per call). */
+/* order in which izem_psr_batch hands cells to lanes: 0 (the default) in
+ * index order; 1 stiffest first -- a key pass evaluates max_k |df_k/dY_k|
+ * of each cell (ten RHS evaluations) and counting-sorts the cells by it, so
+ * the cells sharing a 64-lane wave have like step counts.  Pays where step
+ * counts differ widely (ignition ensembles); where every cell takes a few
+ * steps the key pass costs more than it saves.  Results are the same
+ * either way.  No counterpart in the reference (PSR_CHEMKIN integrates one
+ * cell per call). */
 int izem_chem_order(izem_ctx *ctx, int on);
 /* reaction_selec with reac_selec = DVODE (module_reaction.f90:47, 102):
  * integrates every cell over scale*dt (Strang half step: scale = 0.5),

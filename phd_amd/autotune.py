@@ -7,7 +7,8 @@ launched with a chosen number of resident 64-lane blocks per CU.  Which combinat
 the cells' stiffness (steps per cell), the device and the mechanism.  The
 tuner times each candidate with izem_psr_batch on a sample of the cells the
 caller will integrate, keeps the fastest and applies it with
-izem_chem_tune.  Every candidate integrates the same BDF method with the
+izem_chem_tune; with that variant it then times izem_psr_batch's stiffness
+order (izem_chem_order) against index order and keeps the faster.  Every candidate integrates the same BDF method with the
 same tolerances and Jacobian reuse ages (DVODE's MSBP = 20, MSBJ = 50), so the choice
 changes speed only; the two kinetics forms differ at round-off level.
 
@@ -81,21 +82,20 @@ def tune(solver, T_K: torch.Tensor, rho: torch.Tensor, Y: torch.Tensor, dt: floa
          key: str = "", candidates=None, use_cache: bool = True, verbose: bool = False):
     """Time the candidates on (T_K, rho, Y) advanced by dt seconds, apply the
     fastest to the solver's context and return (waves, blocks_per_cu,
-    jac_age, kinetics, cells_per_s)."""
+    jac_age, kinetics, cells_per_s, order)."""
     lib, ctx = solver.lib, solver.ctx
     dev = torch.cuda.get_device_properties(T_K.device).name
     mech_hash = mechanism_hash(solver.mech)
     ck = f"{dev}|{mech_hash}|{key}|n={T_K.numel()}"
     cache = _load_cache() if use_cache else {}
     if ck in cache:
-        w, b, j, k, rate = cache[ck]
-        if lib.izem_chem_tune(ctx, w, b, j, k) == 0:
-            return (w, b, j, k, rate)
+        w, b, j, k, rate, o = (cache[ck] + [0])[:6]
+        if lib.izem_chem_tune(ctx, w, b, j, k) == 0 and lib.izem_chem_order(ctx, o) == 0:
+            return (w, b, j, k, rate, o)
     n = T_K.numel()
     best = None
-    for (w, b, k) in (candidates or CANDIDATES):
-        if lib.izem_chem_tune(ctx, w, b, JAC_AGE, k) != 0:
-            continue  # e.g. straight-line kinetics for another mechanism
+
+    def rate_of(label):
         el = []
         for _ in range(2):
             t, r, y = T_K.clone(), rho.clone(), Y.clone()
@@ -106,18 +106,28 @@ def tune(solver, T_K: torch.Tensor, rho: torch.Tensor, Y: torch.Tensor, dt: floa
             solver.sync()
             el.append(time.perf_counter() - t0)
             if rc != 0:
-                break
-        if rc != 0 or solver.status() != 0:
-            continue
-        rate = n / min(el)
+                return None
+        if solver.status() != 0:
+            return None
         if verbose:
-            print(f"[autotune] waves {w} blocks/CU {b} kinetics {k}: {rate:.3e} cells/s",
-                  flush=True)
-        if best is None or rate > best[4]:
-            best = (w, b, JAC_AGE, k, rate)
+            print(f"[autotune] {label}: {n / min(el):.3e} cells/s", flush=True)
+        return n / min(el)
+
+    _lib.check(lib.izem_chem_order(ctx, 0), "izem_chem_order")
+    for (w, b, k) in (candidates or CANDIDATES):
+        if lib.izem_chem_tune(ctx, w, b, JAC_AGE, k) != 0:
+            continue  # e.g. straight-line kinetics for another mechanism
+        rate = rate_of(f"waves {w} blocks/CU {b} kinetics {k}")
+        if rate is not None and (best is None or rate > best[4]):
+            best = (w, b, JAC_AGE, k, rate, 0)
     if best is None:
         raise _lib.IzemError("no chemistry kernel variant ran on the sample")
     _lib.check(lib.izem_chem_tune(ctx, *best[:4]), "izem_chem_tune")
+    _lib.check(lib.izem_chem_order(ctx, 1), "izem_chem_order")
+    rate = rate_of("stiffness order")
+    if rate is not None and rate > best[4]:
+        best = best[:4] + (rate, 1)
+    _lib.check(lib.izem_chem_order(ctx, best[5]), "izem_chem_order")
     if use_cache:
         cache[ck] = list(best)
         _store_cache(cache)

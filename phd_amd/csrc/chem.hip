@@ -328,7 +328,7 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
       return IZEM_E_NOMEM;
     if (hipMalloc(&ctx->chem->order_hist, IZ_ORDER_BINS * sizeof(int)) != hipSuccess)
       return IZEM_E_NOMEM;
-    ctx->chem->order = 1;
+    ctx->chem->order = 0;
     hipLaunchKernelGGL(k_stats_zero, dim3(1), dim3(64), 0, ctx->stream, ctx->chem->stats);
     // defaults measured on MI355X (2^18 cells at dt = 1e-6 s and 2^20 cells
     // of a reacting HIT half step): one wave per SIMD with the whole

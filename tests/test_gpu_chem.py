@@ -437,15 +437,15 @@ def test_psr_batch_sizes(mech, thd, host_bdf, n):
     assert np.all(np.abs(dT.cpu().numpy() - Th) / Th <= 1e-6)
 
 
-def test_stiffness_order_changes_no_result(mech, thd):
-    """izem_psr_batch hands cells out stiffest first by default; the end
-    states and per-cell step counts are bit-identical to index order, since
-    every lane integrates its cell on its own.  Cells of an ignition
-    ensemble (step counts from a few tens to several hundred) at a ragged
-    count; an out-of-range switch is refused."""
+@pytest.mark.parametrize("n", [1, 65, 4096 + 37])
+def test_stiffness_order_changes_no_result(mech, thd, n):
+    """With izem_chem_order on, izem_psr_batch hands cells out stiffest
+    first; the end states and per-cell step counts are bit-identical to index
+    order, since every lane integrates its cell on its own.  Cells of an
+    ignition ensemble (step counts from a few tens to several hundred) at
+    single, ragged and larger counts; an out-of-range switch is refused."""
     import torch
     s = _solver(mech)
-    n = 4096 + 37
     T, rho, Y = _cells(thd, n, seed=5)
     d = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
     out = []
@@ -460,7 +460,9 @@ def test_stiffness_order_changes_no_result(mech, thd):
         assert s.status() == 0
         out.append((dT.cpu().numpy(), dY.cpu().numpy(), nst.cpu().numpy()))
     (T1, Y1, n1), (T0, Y0, n0) = out
-    assert np.all(n1 > 0) and n1.max() > 4 * n1.min()
+    assert np.all(n1 > 0)
+    if n > 1000:
+        assert n1.max() > 4 * n1.min()
     assert np.array_equal(n1, n0)
     assert np.array_equal(T1.view(np.uint64), T0.view(np.uint64))
     assert np.array_equal(Y1.view(np.uint64), Y0.view(np.uint64))
