@@ -177,8 +177,9 @@ int izem_ckwyr(izem_ctx *ctx, int n, const double *rho, const double *T, const d
  * nsteps (may be NULL) receives the BDF step count of each cell */
 int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho, double *Y, double dt,
                    int *nsteps);
-/* order in which izem_psr_batch hands cells to lanes: 0 (the default) in
- * index order; 1 stiffest first -- a key pass evaluates max_k |df_k/dY_k|
+/* order in which izem_psr_batch and izem_reaction_dvode hand cells to
+ * lanes: 0 (the default) in index order; 1 stiffest first -- a key pass
+ * evaluates max_k |df_k/dY_k|
  * of each cell (ten RHS evaluations) and counting-sorts the cells by it, so
  * the cells sharing a 64-lane wave have like step counts.  Pays where step
  * counts differ widely (ignition ensembles); where every cell takes a few

@@ -13,6 +13,7 @@
 // integration tolerance, not bit for bit.
 #include "izem_ctx.h"
 
+#include <limits.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -150,7 +151,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                                  const double *__restrict__ dtp, double scale, double time_ref,
                                  double T_ref, double rho_cgs, double rtol, double atol,
                                  int jac_age, double *__restrict__ T, double *__restrict__ v,
-                                 int *__restrict__ counter, double *__restrict__ ts,
+                                 int *__restrict__ counter, const int *__restrict__ order,
+                                 double *__restrict__ ts,
                                  int *__restrict__ status,
                                  unsigned long long *__restrict__ stats) {
   const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
@@ -162,8 +164,9 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
   double dtmax = 0.0;
   unsigned long long w_nst = 0, w_nfe = 0, w_nje = 0, w_nsol = 0;
   for (;;) {
-    long cell = atomicAdd(counter, 1);
-    if (cell >= n) break;
+    const long q = atomicAdd(counter, 1);
+    if (q >= n) break;
+    const long cell = order ? (long)order[q] : q;
     int i = i0 + (int)(cell % ni);
     long r = cell / ni;
     int j = j0 + (int)(r % nj), k = k0 + (int)(r / nj);
@@ -392,6 +395,24 @@ __global__ void k_order_zero(int *__restrict__ hist) {
   for (int i = threadIdx.x; i < IZ_ORDER_BINS; i += blockDim.x) hist[i] = 0;
 }
 
+// queue slot of a cell of density rho [g/cm3] and state y = (T [K], Y_k):
+// decreasing key -> increasing slot; NaN keys go last, infinite first
+__device__ inline int order_slot(double rho, const double *y) {
+  double f0[N_], yl[N_], f[N_];
+  d_reactor_rhs(rho, y, f0);
+  double s = 0.0;
+  for (int j = 1; j < N_; j++) {
+    for (int k = 0; k < N_; k++) yl[k] = y[k];
+    const double dy = 1.4901161193847656e-08 * fmax(fabs(y[j]), 1.0e-6);
+    yl[j] = y[j] + dy;
+    d_reactor_rhs(rho, yl, f);
+    s = fmax(s, fabs((f[j] - f0[j]) / dy));
+  }
+  const double l = log2(s) * 8.0;
+  const int b = (l >= 1.0) ? (l < IZ_ORDER_BINS - 1 ? (int)l : IZ_ORDER_BINS - 1) : 0;
+  return IZ_ORDER_BINS - 1 - b;
+}
+
 __global__ void __launch_bounds__(256) k_order_key(int n, const double *__restrict__ T,
                                                    const double *__restrict__ rho,
                                                    const double *__restrict__ Y,
@@ -402,24 +423,41 @@ __global__ void __launch_bounds__(256) k_order_key(int n, const double *__restri
   __syncthreads();
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < n) {
-    double y[N_], f0[N_], yl[N_], f[N_];
+    double y[N_];
     y[0] = T[c];
     for (int k = 0; k < N_ - 1; k++) y[1 + k] = Y[(long)c * (N_ - 1) + k];
-    const double r_c = rho[c];
-    d_reactor_rhs(r_c, y, f0);
-    double s = 0.0;
-    for (int j = 1; j < N_; j++) {
-      for (int k = 0; k < N_; k++) yl[k] = y[k];
-      const double dy = 1.4901161193847656e-08 * fmax(fabs(y[j]), 1.0e-6);
-      yl[j] = y[j] + dy;
-      d_reactor_rhs(r_c, yl, f);
-      s = fmax(s, fabs((f[j] - f0[j]) / dy));
-    }
-    // decreasing key -> increasing slot; NaN keys go last, infinite first
-    const double l = log2(s) * 8.0;
-    const int b = (l >= 1.0) ? (l < IZ_ORDER_BINS - 1 ? (int)l : IZ_ORDER_BINS - 1) : 0;
-    const int slot = IZ_ORDER_BINS - 1 - b;
+    const int slot = order_slot(rho[c], y);
     bin[c] = slot;
+    atomicAdd(&h[slot], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < IZ_ORDER_BINS; i += blockDim.x)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// the same key on the cells of the reacting box of the flow state, in the
+// queue numbering of k_reaction_queue (i fastest) and its unit conversion
+__global__ void __launch_bounds__(256) k_order_key_flow(IzDims d, int i0, int j0, int k0, int ni,
+                                                        int nj, int n, double T_ref,
+                                                        double rho_cgs,
+                                                        const double *__restrict__ T,
+                                                        const double *__restrict__ v,
+                                                        int *__restrict__ bin,
+                                                        int *__restrict__ hist) {
+  __shared__ int h[IZ_ORDER_BINS];
+  for (int i = threadIdx.x; i < IZ_ORDER_BINS; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) {
+    const int i = i0 + q % ni;
+    const int r = q / ni;
+    const long c = iz_cidx(d, i, j0 + r % nj, k0 + r / nj);
+    const double rho0 = v[c];
+    double y[N_];
+    y[0] = T[c] * T_ref;
+    for (int l = 0; l < N_ - 1; l++) y[1 + l] = v[(IZ_NIV + l) * d.nc + c] / rho0;
+    const int slot = order_slot(rho0 * rho_cgs, y);
+    bin[q] = slot;
     atomicAdd(&h[slot], 1);
   }
   __syncthreads();
@@ -470,10 +508,10 @@ __global__ void __launch_bounds__(256) k_order_scatter(int n, const int *__restr
   if (c < n) perm[h[slot] + rank] = c;
 }
 
-// builds the queue permutation of n cells; IZEM_E_NOMEM if the buffers
-// cannot grow
-static int order_cells(izem_ctx *ctx, int n, const double *T, const double *rho,
-                       const double *Y) {
+// queue permutation of n cells: order_begin sizes the buffers and clears
+// the bin counts (IZEM_E_NOMEM if the buffers cannot grow), a key kernel
+// writes the bins and counts, order_end scans and scatters
+static int order_begin(izem_ctx *ctx, int n) {
   IzChem *ch = ctx->chem;
   if (n > ch->order_cap) {
     IZ_CHECK(hipStreamSynchronize(ctx->stream));
@@ -486,10 +524,13 @@ static int order_cells(izem_ctx *ctx, int n, const double *T, const double *rho,
       return IZEM_E_NOMEM;
     ch->order_cap = n;
   }
-  const int nb = (n + 255) / 256;
   hipLaunchKernelGGL(k_order_zero, dim3(1), dim3(256), 0, ctx->stream, ch->order_hist);
-  hipLaunchKernelGGL(k_order_key, dim3(nb), dim3(256), 0, ctx->stream, n, T, rho, Y,
-                     ch->order_bin, ch->order_hist);
+  return IZEM_OK;
+}
+
+static int order_end(izem_ctx *ctx, int n) {
+  IzChem *ch = ctx->chem;
+  const int nb = (n + 255) / 256;
   hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, ctx->stream, ch->order_hist);
   hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(256), 0, ctx->stream, n, ch->order_bin,
                      ch->order_hist, ch->order_perm);
@@ -531,7 +572,12 @@ extern "C" int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho
   int pk = iz_prof_begin(ctx, IZP_CHEM);
   const int *order = nullptr;
   if (ch->order) {
-    int rc = order_cells(ctx, n, T, rho, Y);
+    int rc = order_begin(ctx, n);
+    if (rc == IZEM_OK) {
+      hipLaunchKernelGGL(k_order_key, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, T, rho,
+                         Y, ch->order_bin, ch->order_hist);
+      rc = order_end(ctx, n);
+    }
     if (rc != IZEM_OK) {
       iz_prof_end(ctx, pk);
       return rc;
@@ -579,11 +625,27 @@ extern "C" int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale
   if (n > 0) {
     double rho_cgs = ctx->o.rho_ref * 1.0e-3;
     int pk = iz_prof_begin(ctx, IZP_CHEM);
+    const int *order = nullptr;
+    if (ch->order && n <= INT_MAX) {
+      const int ni = i1 - i0 + 1, nj = j1 - j0 + 1;
+      int rc = order_begin(ctx, (int)n);
+      if (rc == IZEM_OK) {
+        hipLaunchKernelGGL(k_order_key_flow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           ctx->stream, d, i0, j0, k0, ni, nj, (int)n, ctx->o.T_ref, rho_cgs, T,
+                           v, ch->order_bin, ch->order_hist);
+        rc = order_end(ctx, (int)n);
+      }
+      if (rc != IZEM_OK) {
+        iz_prof_end(ctx, pk);
+        return rc;
+      }
+      order = ch->order_perm;
+    }
     dim3 grid(persistent_grid(ctx, n)), blk(64);
 #define IZ_RQ(K, W)                                                                            \
   hipLaunchKernelGGL((k_reaction_queue<K, W>), grid, blk, 0, ctx->stream, d, i0, i1, j0, j1, k0, \
                      k1, dt, scale, ctx->o.time_ref, ctx->o.T_ref, rho_cgs, ch->rtol, ch->atol,   \
-                     ch->jac_age, T, v, ch->counter, ctx->ts, ctx->status, ch->stats)
+                     ch->jac_age, T, v, ch->counter, order, ctx->ts, ctx->status, ch->stats)
     if (ch->use_gen) {
       IZ_RQ(KinGen, 1);
     } else {

@@ -467,3 +467,34 @@ def test_stiffness_order_changes_no_result(mech, thd, n):
     assert np.array_equal(T1.view(np.uint64), T0.view(np.uint64))
     assert np.array_equal(Y1.view(np.uint64), Y0.view(np.uint64))
     assert s.lib.izem_chem_order(s.ctx, 2) == -1
+
+
+@pytest.mark.parametrize("kinetics", [0, 1])
+def test_stiffness_order_flow_state(adi, thd, mech, kinetics):
+    """izem_reaction_dvode with izem_chem_order on takes the cells of the
+    reacting box stiffest first; the conserved state, temperature, DTmax and
+    work counters are bit-identical to index order (2-D hot-spot state)."""
+    import torch
+    from phd_amd.solver import Options, Solver
+    from tests.flowcase import small_state
+    box, v, T = small_state(adi, thd, "2d", 32)
+    s = Solver(box, Options(rtol=1e-6, atol=1e-12), adi=adi, thd=thd, mech=mech)
+    assert s.lib.izem_chem_tune(s.ctx, 1, 4, 50, kinetics) == 0
+    dt = torch.tensor([2.0e-7 / adi.time_ref], dtype=torch.float64, device="cuda:0")
+    out = []
+    for on in (1, 0):
+        s.load_state(v, T)
+        assert s.lib.izem_chem_order(s.ctx, on) == 0
+        st = np.zeros(4, dtype=np.uint64)
+        s.lib.izem_chem_stats(s.ctx, st.ctypes.data)
+        s.reaction_selec(dt.data_ptr(), 0.5)
+        s.sync()
+        assert s.status() == 0
+        s.lib.izem_chem_stats(s.ctx, st.ctypes.data)
+        out.append((s.v.cpu().numpy().copy(), s.T.cpu().numpy().copy(), s.time_state()[5],
+                    st.copy()))
+    (v1, T1, d1, s1), (v0, T0, d0, s0) = out
+    assert d1 > 0.0 and d1 == d0
+    assert np.array_equal(s1, s0)
+    assert np.array_equal(v1.view(np.uint64), v0.view(np.uint64))
+    assert np.array_equal(T1.view(np.uint64), T0.view(np.uint64))
