@@ -184,8 +184,10 @@ int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho, double *Y
  * the cells sharing a 64-lane wave have like step counts.  Pays where step
  * counts differ widely (ignition ensembles); where every cell takes a few
  * steps the key pass costs more than it saves.  Results are the same
- * either way.  No counterpart in the reference (PSR_CHEMKIN integrates one
- * cell per call). */
+ * either way.  The context keeps two int buffers of the largest cell count
+ * ordered so far; a call that grows them synchronises the context stream
+ * and allocates.  No counterpart in the reference (PSR_CHEMKIN integrates
+ * one cell per call). */
 int izem_chem_order(izem_ctx *ctx, int on);
 /* reaction_selec with reac_selec = DVODE (module_reaction.f90:47, 102):
  * integrates every cell over scale*dt (Strang half step: scale = 0.5),
