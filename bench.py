@@ -329,7 +329,8 @@ def main():
         "chem_tuning": (None if tuned is None else
                         {"waves_per_simd": tuned[0], "blocks_per_cu": tuned[1],
                          "jac_age": tuned[2], "kinetics": "straight-line" if tuned[3] else
-                         "tables", "sample_cells_per_s": tuned[4]}),
+                         "tables", "queue_order": "stiffness" if tuned[5] else "index",
+                         "sample_cells_per_s": tuned[4]}),
         "chemistry": chem,
         "chem_surrogate": surr,
         "cpu_baseline": cpu,

@@ -183,8 +183,10 @@ int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho, double *Y
  * of each cell (ten RHS evaluations) and counting-sorts the cells by it, so
  * the cells sharing a 64-lane wave have like step counts.  Pays where step
  * counts differ widely (ignition ensembles); where every cell takes a few
- * steps the key pass costs more than it saves.  Results are the same
- * either way.  The context keeps two int buffers of the largest cell count
+ * steps the key pass costs more than it saves.  Applied only to calls with
+ * more cells than resident lanes (64 x CUs x blocks per CU): with fewer,
+ * each lane takes one cell in any order.  Results are the same either
+ * way.  The context keeps two int buffers of the largest cell count
  * ordered so far; a call that grows them synchronises the context stream
  * and allocates.  No counterpart in the reference (PSR_CHEMKIN integrates
  * one cell per call). */

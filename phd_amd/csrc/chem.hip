@@ -570,8 +570,11 @@ extern "C" int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho
   if (n == 0) return IZEM_OK;
   IzChem *ch = ctx->chem;
   int pk = iz_prof_begin(ctx, IZP_CHEM);
+  dim3 grid(persistent_grid(ctx, n)), blk(64);
   const int *order = nullptr;
-  if (ch->order) {
+  // with no more cells than resident lanes each lane takes one cell and the
+  // batch lasts as long as its stiffest cell in any order
+  if (ch->order && (long)n > 64L * grid.x) {
     int rc = order_begin(ctx, n);
     if (rc == IZEM_OK) {
       hipLaunchKernelGGL(k_order_key, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, T, rho,
@@ -585,7 +588,6 @@ extern "C" int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho
     order = ch->order_perm;
   }
   hipLaunchKernelGGL(k_chem_prep, dim3(1), dim3(1), 0, ctx->stream, ch->counter, ctx->ts, 0);
-  dim3 grid(persistent_grid(ctx, n)), blk(64);
 #define IZ_PSR(K, W)                                                                        \
   hipLaunchKernelGGL((k_psr_queue<K, W>), grid, blk, 0, ctx->stream, n, T, rho, Y, dt, ch->rtol, \
                      ch->atol, ch->jac_age, ch->counter, order, nsteps, ctx->status, ch->stats)
@@ -625,8 +627,9 @@ extern "C" int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale
   if (n > 0) {
     double rho_cgs = ctx->o.rho_ref * 1.0e-3;
     int pk = iz_prof_begin(ctx, IZP_CHEM);
+    dim3 grid(persistent_grid(ctx, n)), blk(64);
     const int *order = nullptr;
-    if (ch->order && n <= INT_MAX) {
+    if (ch->order && n <= INT_MAX && n > 64L * grid.x) {
       const int ni = i1 - i0 + 1, nj = j1 - j0 + 1;
       int rc = order_begin(ctx, (int)n);
       if (rc == IZEM_OK) {
@@ -641,7 +644,6 @@ extern "C" int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale
       }
       order = ch->order_perm;
     }
-    dim3 grid(persistent_grid(ctx, n)), blk(64);
 #define IZ_RQ(K, W)                                                                            \
   hipLaunchKernelGGL((k_reaction_queue<K, W>), grid, blk, 0, ctx->stream, d, i0, i1, j0, j1, k0, \
                      k1, dt, scale, ctx->o.time_ref, ctx->o.T_ref, rho_cgs, ch->rtol, ch->atol,   \

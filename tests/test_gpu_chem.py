@@ -437,15 +437,18 @@ def test_psr_batch_sizes(mech, thd, host_bdf, n):
     assert np.all(np.abs(dT.cpu().numpy() - Th) / Th <= 1e-6)
 
 
-@pytest.mark.parametrize("n", [1, 65, 4096 + 37])
+@pytest.mark.parametrize("n", [1, 65, 4096 + 37, (1 << 17) + 37])
 def test_stiffness_order_changes_no_result(mech, thd, n):
     """With izem_chem_order on, izem_psr_batch hands cells out stiffest
-    first; the end states and per-cell step counts are bit-identical to index
-    order, since every lane integrates its cell on its own.  Cells of an
-    ignition ensemble (step counts from a few tens to several hundred) at
-    single, ragged and larger counts; an out-of-range switch is refused."""
+    first once the cells outnumber the resident lanes (65536 at one wave
+    per SIMD, four blocks per CU: the last size); the end states and per-
+    cell step counts are bit-identical to index order, since every lane
+    integrates its cell on its own.  Cells of an ignition ensemble (step
+    counts from a few tens to several hundred) at single, ragged and larger
+    counts; an out-of-range switch is refused."""
     import torch
     s = _solver(mech)
+    assert s.lib.izem_chem_tune(s.ctx, 1, 4, 50, -1) == 0
     T, rho, Y = _cells(thd, n, seed=5)
     d = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
     out = []
@@ -473,11 +476,12 @@ def test_stiffness_order_changes_no_result(mech, thd, n):
 def test_stiffness_order_flow_state(adi, thd, mech, kinetics):
     """izem_reaction_dvode with izem_chem_order on takes the cells of the
     reacting box stiffest first; the conserved state, temperature, DTmax and
-    work counters are bit-identical to index order (2-D hot-spot state)."""
+    work counters are bit-identical to index order (2-D hot-spot state of
+    288^2 cells, more reacting cells than the 65536 resident lanes)."""
     import torch
     from phd_amd.solver import Options, Solver
     from tests.flowcase import small_state
-    box, v, T = small_state(adi, thd, "2d", 32)
+    box, v, T = small_state(adi, thd, "2d", 288)
     s = Solver(box, Options(rtol=1e-6, atol=1e-12), adi=adi, thd=thd, mech=mech)
     assert s.lib.izem_chem_tune(s.ctx, 1, 4, 50, kinetics) == 0
     dt = torch.tensor([2.0e-7 / adi.time_ref], dtype=torch.float64, device="cuda:0")
