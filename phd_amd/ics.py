@@ -152,15 +152,16 @@ def conserved_from_primitive(box: Box, adi: cd.Adim, thd: cd.Thd, T_K: np.ndarra
     return v, T
 
 
-def config1_state(adi, thd, n: int = 128, seed: int = 1):
+def config1_state(adi, thd, n: int = 128, seed: int = 1, T_peak: float = 1500.0):
     """2-D periodic 1 cm box, stoichiometric H2/air at 1 atm, T = 900 K plus a
-    Gaussian hot spot (peak 1500 K, sigma 1 mm), u_rms = 10 m/s."""
+    Gaussian hot spot (peak T_peak, 1500 K in the workload, sigma 1 mm),
+    u_rms = 10 m/s."""
     L = 0.01
     box = Box(n, n, 1, 3, 2, (L, L, L))
     x = (np.arange(n) + 0.0) * (L / n)
     X, Yy = np.meshgrid(x, x, indexing="xy")
     r2 = (X - L / 2) ** 2 + (Yy - L / 2) ** 2
-    T = 900.0 + 600.0 * np.exp(-r2 / (2 * 0.001 ** 2))
+    T = 900.0 + (T_peak - 900.0) * np.exp(-r2 / (2 * 0.001 ** 2))
     T = T[None]
     Ymix = cd.mass_fractions_premix(thd.names, 1.0)
     Y = np.broadcast_to(Ymix[:, None, None, None], (9, 1, n, n)).copy()

@@ -71,5 +71,48 @@ class OracleRun:
             assert s == 0
         return it.value
 
-    def opts(self, vis=1, reaction=0, weno_avg=1, mrw=-1.0, forcesum=1):
-        return OOpts(weno_avg, mrw, vis, 1, reaction, forcesum, 0.5, 0.1, 0.1, 20.0)
+    def opts(self, vis=1, reaction=0, weno_avg=1, mrw=-1.0, forcesum=1, CPR=20.0):
+        return OOpts(weno_avg, mrw, vis, 1, reaction, forcesum, 0.5, 0.1, 0.1, CPR)
+
+
+def oracle_reaction_dvode(run: OracleRun, rc, adi, dt_nondim: float, rtol: float, atol: float,
+                          trim: bool = True) -> float:
+    """reaction_dvode (devel-izem/src/module_reaction.f90:102-304) on the
+    oracle state with the reference's own compiled PSR_CHEMKIN/DVODE per
+    cell: the reacting box trimmed by ng+1 planes at the upper faces
+    (:138-150), CGS conversion, T updated from the reactor but not rho e
+    (:278), Y clipped with the last species closing the sum (:285-293);
+    then upd_prim_var_woT, comm_cons and the ghost Newton (:73-83).
+    Returns DTmax [K]."""
+    box = run.box
+    ng, nx, ny, nz = box.ng, box.nx, box.ny, box.nz
+    dt_phys = dt_nondim * adi.time_ref
+    rho_cgs = adi.rho_ref * 1.0e-3
+    DTmax = 0.0
+    jr = range(ng, ny - ng - 1) if trim else range(0, ny)
+    ir = range(ng, nx - ng - 1) if trim else range(0, nx)
+    if box.ndim == 3:
+        kr = range(ng, min(ny - 1, nz - ng - 2) + 1) if trim else range(0, nz)
+    else:
+        kr = range(0, 1)
+    for k in kr:
+        for j in jr:
+            for i in ir:
+                c = (k + ng, j + ng, i + ng)
+                rho0 = run.v[(0,) + c]
+                Ya = run.v[(slice(5, 14),) + c] / rho0
+                T0 = run.T[c] * adi.T_ref
+                Tn, Yn, ist, *_ = rc.psr(T0, rho0 * rho_cgs, Ya, dt_phys, rtol, atol)
+                assert ist == 2
+                run.T[c] = Tn * (1.0 / adi.T_ref)
+                DTmax = max(DTmax, abs(Tn - T0))
+                Yc = np.clip(Yn[:8], 0.0, 1.0)
+                run.v[(slice(5, 13),) + c] = rho0 * Yc
+                run.v[(13,) + c] = rho0 * (1.0 - Yc.sum())
+    run.o.lib.ora_upd_prim_var_woT(0, C.byref(run.g), C.byref(run.t), P(run.v), P(run.T),
+                                   P(run.W_i), P(run.cp), P(run.ha))
+    run.o.lib.ora_comm_cons(C.byref(run.g), P(run.v))
+    st = run.o.lib.ora_upd_prim_var_ghost(C.byref(run.g), C.byref(run.t), 1, P(run.v),
+                                          P(run.W_i), P(run.T), P(run.cp), P(run.ha))
+    assert st == 0
+    return DTmax
