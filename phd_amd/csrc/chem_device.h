@@ -357,16 +357,51 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double (&J)
   for (int i = 0; i < N_; i++) { z(0, i) = y[i]; ewt[i] = 1.0 / (rtol * fabs(y[i]) + atol); }
   kin.rhs(rho, y, savf);
   res.nfe++;
-  // initial step (the LSODE-style estimate also used by DVHIN's first guess)
-  double tol = fmax(rtol, 100.0 * uround);
-  double fn = d_wrms(savf, ewt);
-  double sum = 1.0 / (tol * tout * tout) + tol * fn * fn;
-  double h = 1.0 / sqrt(sum);
-  h = fmin(h, tout);
+  // initial step by DVHIN's rule (vode.f:3338-3460): between 100 uround tout
+  // and a tenth of tout, and no component moving by more than a tenth of
+  // itself plus atol in one step; within that, h ~ sqrt(2 / ||y''||) with
+  // y'' from a difference of derivatives, refined at most four times, then
+  // halved
+  double h;
+  {
+    const double hlb = 100.0 * uround * tout;
+    double hub = 0.1 * tout;
+#pragma unroll
+    for (int i = 0; i < N_; i++) {
+      const double dely = 0.1 * fabs(y[i]) + atol, af = fabs(savf[i]);
+      if (af * hub > dely) hub = dely / af;
+    }
+    double hg = sqrt(hlb * hub);
+    if (hub < hlb) {
+      h = hg;
+    } else {
+      double hnew = hg;
+      for (int it = 1; it <= 4; it++) {
+        double fy[N_];
+#pragma unroll
+        for (int i = 0; i < N_; i++) yy[i] = y[i] + hg * savf[i];
+        kin.rhs(rho, yy, fy);
+        res.nfe++;
+        const double rhg = 1.0 / hg;
+#pragma unroll
+        for (int i = 0; i < N_; i++) fy[i] = (fy[i] - savf[i]) * rhg;
+        const double ydd = d_wrms(fy, ewt);
+        hnew = (ydd * hub * hub > 2.0) ? sqrt(2.0 / ydd) : sqrt(hg * hub);
+        if (it == 4) break;
+        const double hrat = hnew / hg;
+        if (hrat > 0.5 && hrat < 2.0) break;
+        if (it >= 2 && hnew > 2.0 * hg) { hnew = hg; break; }
+        hg = hnew;
+      }
+      h = fmin(fmax(0.5 * hnew, hlb), hub);
+    }
+  }
 #pragma unroll
   for (int i = 0; i < N_; i++) z(1, i) = h * savf[i];
-  int q = 1, ialth = 2, ncf = 0, kflag = 0;
-  double t = 0.0, rc = 0.0, crate = 0.7, hl0 = 0.0, rmax = 1.0e4;
+  int q = 1, nqwait = 2, ncf = 0, kflag = 0;
+  // etamax: largest step growth (DVSTEP's ETAMX1 = 1e4 after the first step,
+  // then 10; 1 right after a failure)
+  double t = 0.0, rc = 0.0, crate = 1.0, hl0 = 0.0, etamax = 1.0e4, hsave = 1.0;
   bool need_p = true;   // DVODE's IPUP
   bool p_ok = true;     // the last factorisation found a pivot in every column
   int nslp = 0, nslj = 0;
@@ -423,7 +458,7 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double (&J)
         rc = hl0;
         nslp = res.nst;
         need_p = false;
-        crate = 0.7;
+        crate = 1.0;
       }
     }
     bool conv = false;
@@ -444,8 +479,11 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double (&J)
       del = d_wrms(d, ewt);
 #pragma unroll
       for (int n = 0; n < N_; n++) { acor[n] += d[n]; yy[n] = z(0, n) + l0 * acor[n]; }
-      if (m > 0) crate = fmax(0.2 * crate, del / delp);
-      double dcon = del * fmin(1.0, 1.5 * crate) / (IZ_CBDF.tesco[q][1] * (0.5 / (q + 2)));
+      // DVNLSD's convergence test: the rate estimate decays by CRDOWN = 0.3,
+      // and the correction must be below CORTES = 0.1 of the error-test
+      // tolerance
+      if (m > 0) crate = fmax(0.3 * crate, del / delp);
+      double dcon = del * fmin(1.0, crate) / (0.1 * IZ_CBDF.tesco[q][1]);
       if (dcon <= 1.0) { conv = true; break; }
       if (m >= 1 && del > 2.0 * delp) break;
       delp = del;
@@ -454,17 +492,18 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double (&J)
     }
     if (!conv) {
       // undo the prediction; retry with a new iteration matrix, and with a
-      // smaller step when the Jacobian was already current
+      // quarter of the step (DVSTEP's ETACF) when the Jacobian was already
+      // current; the step after a failure keeps its size (ETAMAX = 1)
       d_nordsieck_shift(z, q, -1.0);
       ncf++;
       need_p = true;
+      etamax = 1.0;
       const bool hard = jcur || !p_ok;
       icf = hard ? 2 : 1;
       if (hard || ncf > 10) {
         if (ncf > 10) { res.ok = 0; break; }
         d_nordsieck_rescale(z, q, 0.25);
         h *= 0.25;
-        rmax = 2.0;
       }
       continue;
     }
@@ -476,36 +515,37 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double (&J)
            (int)jcur, z(0, 0));
 #endif
     if (dsm > 1.0) {
+      // DVSTEP's policy after an error-test failure: eta = 1 / ((6 dsm)^(1/L)
+      // + 1e-6), at least 0.1 and at most 0.2 after the second failure in a
+      // row; after the third, one order lower at a tenth of the step, or at
+      // order 1 a restart from a fresh derivative; the seventh gives up
       d_nordsieck_shift(z, q, -1.0);
       kflag--;
-      if (kflag <= -3) {
-        // repeated failures: restart at order 1 from a fresh derivative
-        h *= 0.1;
-#pragma unroll
-        for (int n = 0; n < N_; n++) yy[n] = z(0, n);
-        kin.rhs(rho, yy, savf);
-        res.nfe++;
-#pragma unroll
-        for (int n = 0; n < N_; n++) z(1, n) = h * savf[n];
-        q = 1;
-        ialth = 5;
-        need_p = true;
-        if (kflag <= -10) { res.ok = 0; break; }
+      etamax = 1.0;
+      if (kflag <= -7) { res.ok = 0; break; }
+      if (kflag > -3) {
+        double eta = fmax(1.0 / (d_powpos(6.0 * dsm, IZ_CBDF.rexp[q + 1]) + 1.0e-6), 0.1);
+        if (kflag <= -2) eta = fmin(eta, 0.2);
+        d_nordsieck_rescale(z, q, eta);
+        h *= eta;
         continue;
       }
-      double rh = 1.0 / (1.2 * d_powpos(dsm, IZ_CBDF.rexp[q + 1]) + 1.2e-6);
-      int newq = q;
       if (q > 1) {
-        double ddn = d_wrms_col(z, q, ewt) / IZ_CBDF.tesco[q][0];
-        double rhdn = 1.0 / (1.3 * d_powpos(ddn, IZ_CBDF.rexp[q]) + 1.3e-6);
-        if (rhdn > rh) { rh = rhdn; newq = q - 1; }
+        q--;
+        d_nordsieck_rescale(z, q, 0.1);
+        h *= 0.1;
+        nqwait = q + 1;
+        continue;
       }
-      if (kflag <= -2) rh = fmin(rh, 0.2);
-      q = newq;
-      d_nordsieck_rescale(z, q, rh);
-      h *= rh;
-      ialth = q + 1;
-      rmax = 2.0;
+      h *= 0.1;
+#pragma unroll
+      for (int n = 0; n < N_; n++) yy[n] = z(0, n);
+      kin.rhs(rho, yy, savf);
+      res.nfe++;
+#pragma unroll
+      for (int n = 0; n < N_; n++) z(1, n) = h * savf[n];
+      nqwait = 10;
+      need_p = true;
       continue;
     }
     // accept
@@ -518,43 +558,59 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double (&J)
 #pragma unroll
       for (int n = 0; n < N_; n++) z(j, n) += e * acor[n];
     }
-    ialth--;
-    if (ialth == 1 && q < 5) {
+    nqwait--;
+    if (nqwait == 1 && q < 5) {
 #pragma unroll
       for (int n = 0; n < N_; n++) z(5, n) = acor[n];  // kept for the order-up estimate
+      hsave = h;
     }
-    if (ialth > 0) continue;
-    // order and step selection (every q+1 steps)
-    double rhsm = 1.0 / (1.2 * d_powpos(dsm, IZ_CBDF.rexp[q + 1]) + 1.2e-6);
-    double rhdn = 0.0, rhup = 0.0;
-    if (q > 1) {
-      double ddn = d_wrms_col(z, q, ewt) / IZ_CBDF.tesco[q][0];
-      rhdn = 1.0 / (1.3 * d_powpos(ddn, IZ_CBDF.rexp[q]) + 1.3e-6);
+    if (etamax == 1.0) {
+      // the step after a failure keeps h and q
+      if (nqwait < 2) nqwait = 2;
+      etamax = 10.0;
+      continue;
     }
-    if (q < 5) {
-      double dd[N_];
-#pragma unroll
-      for (int n = 0; n < N_; n++) dd[n] = acor[n] - z(5, n);
-      double dup = d_wrms(dd, ewt) / IZ_CBDF.tesco[q][2];
-      rhup = 1.0 / (1.4 * d_powpos(dup, IZ_CBDF.rexp[q + 2]) + 1.4e-6);
-    }
+    // step size every step, order every q+1 steps (DVSTEP: ETAQ, and
+    // ETAQM1 / ETAQP1 when NQWAIT reaches 0; biases 6, 6 and 10)
+    double eta = 1.0 / (d_powpos(6.0 * dsm, IZ_CBDF.rexp[q + 1]) + 1.0e-6);
     int newq = q;
-    double rh = rhsm;
-    if (rhup > rh && rhup >= rhdn) { newq = q + 1; rh = rhup; }
-    else if (rhdn > rh) { newq = q - 1; rh = rhdn; }
-    if (rh < 1.1) { ialth = 3; continue; }
-    rh = fmin(rh, rmax);
-    rmax = 10.0;
+    if (nqwait == 0) {
+      nqwait = 2;
+      double etam = 0.0, etap = 0.0;
+      if (q > 1) {
+        double ddn = d_wrms_col(z, q, ewt) / IZ_CBDF.tesco[q][0];
+        etam = 1.0 / (d_powpos(6.0 * ddn, IZ_CBDF.rexp[q]) + 1.0e-6);
+      }
+      if (q < 5) {
+        // the saved correction, scaled to the current step (acor ~ h^(q+1))
+        const double cq = d_powpos(h / hsave, (double)(q + 1));
+        double dd[N_];
+#pragma unroll
+        for (int n = 0; n < N_; n++) dd[n] = acor[n] - cq * z(5, n);
+        double dup = d_wrms(dd, ewt) / IZ_CBDF.tesco[q][2];
+        etap = 1.0 / (d_powpos(10.0 * dup, IZ_CBDF.rexp[q + 2]) + 1.0e-6);
+      }
+      if (eta >= etap) {
+        if (eta < etam) { eta = etam; newq = q - 1; }
+      } else if (etap > etam) {
+        eta = etap; newq = q + 1;
+      } else {
+        eta = etam; newq = q - 1;
+      }
+    }
+    if (eta < 1.5) { etamax = 10.0; continue; }  // THRESH: keep h and q
+    eta = fmin(eta, etamax);
+    etamax = 10.0;
     if (newq > q) {
       // new highest-order column from the last correction
       double e = IZ_CBDF.el[q][q] / (q + 1);
 #pragma unroll
       for (int n = 0; n < N_; n++) z(newq, n) = acor[n] * e;
     }
+    if (newq != q) nqwait = newq + 1;
     q = newq;
-    d_nordsieck_rescale(z, q, rh);
-    h *= rh;
-    ialth = q + 1;
+    d_nordsieck_rescale(z, q, eta);
+    h *= eta;
   }
   // interpolate back to tout: y = sum_j z(j) s^j
   double s = (tout - t) / h;

@@ -44,25 +44,36 @@ def test_host_integrator_vs_reference_dvode(host_bdf, thd, kinetics):
 
 @pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built")
 def test_host_integrator_global_error_like_dvode(host_bdf, thd):
-    """At the same rtol/atol the integrator's global error is no larger than
-    DVODE's: both measured against DVODE run at rtol 1e-11, atol 1e-17."""
-    from oracle.oracle import RefChem
-    from phd_amd import ics
+    """At the same rtol/atol the integrator's global error is like DVODE's:
+    both measured against DVODE run at rtol 1e-11, atol 1e-17, in DVODE's
+    error norm (tests/chembars.py), on C3 cells at two substeps and on the
+    cells of a 3-D decaying-HIT state over an igniting 2e-4 s substep."""
+    from oracle.oracle import Oracle, RefChem
+    from phd_amd import chemdata as cd, ics
+    from tests import chembars
+    from tests.flowcase import OracleRun, small_state
     rc = RefChem()
-    n = 32
-    for seed, dt in ((11, 1e-6), (7, 1e-7)):
+    cases = []
+    for seed, n, dt in ((11, 64, 1e-6), (7, 48, 1e-7), (5, 48, 1e-5)):
         T, p, Y = ics.config3_cells(thd, n, seed=seed)
         Wmix = 1.0 / (Y / (np.array(thd.W) * 1000.0)).sum(axis=1)
-        rho = (p * 10.0) * Wmix / (8.314510e7 * T)
-        Tt = np.array([rc.psr(T[c], rho[c], Y[c], dt, 1e-11, 1e-17)[0] for c in range(n)])
-        Td = np.array([rc.psr(T[c], rho[c], Y[c], dt, 1e-6, 1e-12)[0] for c in range(n)])
+        cases.append((T, (p * 10.0) * Wmix / (8.314510e7 * T), Y, dt))
+    adi = cd.Adim()
+    box, v, T = small_state(adi, thd, "3d", 10)
+    run = OracleRun(Oracle(), adi, thd, box, v, T)
+    run.prim_all()
+    I = box.interior()
+    rho = run.v[(0,) + I].ravel()
+    cases.append((run.T[I].ravel() * adi.T_ref, rho * adi.rho_ref * 1e-3,
+                  np.ascontiguousarray((run.v[(slice(5, 14),) + I] / run.v[(0,) + I]).reshape(9, -1).T),
+                  2e-4))
+    for T, rho, Y, dt in cases:
+        n = len(T)
         Th, Yh = T.copy(), np.ascontiguousarray(Y.copy())
         nst, ok = np.zeros(n, np.int32), np.zeros(n, np.int32)
-        host_bdf.host_psr(1, n, Th.ctypes.data, rho.ctypes.data, Yh.ctypes.data, dt, 1e-6, 1e-12,
-                          50, nst.ctypes.data, ok.ctypes.data)
+        host_bdf.host_psr(1, n, Th.ctypes.data, np.ascontiguousarray(rho).ctypes.data,
+                          Yh.ctypes.data, dt, 1e-6, 1e-12, 50, nst.ctypes.data, ok.ctypes.data)
         assert ok.all()
-        eD, eB = np.abs(Td - Tt) / Tt, np.abs(Th - Tt) / Tt
-        print(f"dt {dt:g}: global |dT|/T max {eB.max():.2e} mean {eB.mean():.2e} "
-              f"(DVODE {eD.max():.2e} {eD.mean():.2e})")
-        assert eB.max() <= 2.0 * eD.max()
-        assert eB.mean() <= 2.0 * eD.mean()
+        e = chembars.dvode_errors(rc, T, rho, Y, dt, Th, Yh)
+        print(f"{n} cells, dt {dt:g}: {chembars.summary(e)}")
+        chembars.assert_like_dvode(e)
