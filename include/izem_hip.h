@@ -191,6 +191,13 @@ int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho, double *Y
  * and allocates.  No counterpart in the reference (PSR_CHEMKIN integrates
  * one cell per call). */
 int izem_chem_order(izem_ctx *ctx, int on);
+/* the last stiffness sort of the context: *n = its cell count (0 when the
+ * last izem_psr_batch / izem_reaction_dvode call ran unsorted), perm[q] =
+ * cell at queue position q, bin[c] = stiffness slot of cell c (smaller =
+ * stiffer) for the first *n entries (host arrays of at least cap entries;
+ * IZEM_E_ARG if *n > cap).  Synchronises the context stream.  A diagnostic
+ * with no counterpart in the reference. */
+int izem_chem_order_read(izem_ctx *ctx, long cap, long *n, int *perm, int *bin);
 /* reaction_selec with reac_selec = DVODE (module_reaction.f90:47, 102):
  * integrates every cell over scale*dt (Strang half step: scale = 0.5),
  * refreshes W_i/cp/ha, fills ghosts, ghost Newton; DTmax into the time state */

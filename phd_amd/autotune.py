@@ -31,6 +31,7 @@ from .codegen_mech import mechanism_hash
 CANDIDATES = ([(1, b, 1) for b in (4, 2)] +
               [(w, b, 0) for w in (4, 2, 1) for b in (4 * w, 2 * w)])
 JAC_AGE = 50  # DVODE MSBJ
+ORDER_MARGIN = 1.05  # the stiffness order must be at least 5 % faster to be kept
 
 
 def _cache_path() -> str:
@@ -123,10 +124,17 @@ def tune(solver, T_K: torch.Tensor, rho: torch.Tensor, Y: torch.Tensor, dt: floa
     if best is None:
         raise _lib.IzemError("no chemistry kernel variant ran on the sample")
     _lib.check(lib.izem_chem_tune(ctx, *best[:4]), "izem_chem_tune")
-    _lib.check(lib.izem_chem_order(ctx, 1), "izem_chem_order")
-    rate = rate_of("stiffness order")
-    if rate is not None and rate > best[4]:
-        best = best[:4] + (rate, 1)
+    # The library sorts only batches with more cells than resident lanes
+    # (64 x CUs x blocks per CU); on a smaller sample both timings would run
+    # the same code and the choice would be noise, so index order is kept.
+    # The order must also win by a clear margin: the key pass it adds is
+    # about as expensive as a short integration.
+    ncu = torch.cuda.get_device_properties(T_K.device).multi_processor_count
+    if n > 64 * ncu * best[1]:
+        _lib.check(lib.izem_chem_order(ctx, 1), "izem_chem_order")
+        rate = rate_of("stiffness order")
+        if rate is not None and rate > ORDER_MARGIN * best[4]:
+            best = best[:4] + (rate, 1)
     _lib.check(lib.izem_chem_order(ctx, best[5]), "izem_chem_order")
     if use_cache:
         cache[ck] = list(best)

@@ -49,6 +49,7 @@ struct IzChem {
   int order;
   long order_cap;
   int *order_bin, *order_perm, *order_hist;
+  long order_n;   // cells of the last sorted batch, 0 when the last call ran unsorted
 };
 
 // ------------------------------------------------------------ kernels
@@ -263,13 +264,16 @@ static int gen_bdf_check(izem_ctx *ctx, const IzMech &m, bool &agree) {
   }
   double *dT = nullptr, *dr = nullptr, *dY = nullptr;
   int *dn = nullptr, *dst = nullptr;
+  unsigned long long *dstats = nullptr;  // the check's own work counters
   int rc = IZEM_OK;
   IzChem *ch = ctx->chem;
   if (hipMalloc(&dT, sizeof T[0]) != hipSuccess || hipMalloc(&dr, sizeof rho) != hipSuccess ||
       hipMalloc(&dY, sizeof Y[0]) != hipSuccess || hipMalloc(&dn, sizeof nst[0]) != hipSuccess ||
-      hipMalloc(&dst, sizeof(int)) != hipSuccess) {
+      hipMalloc(&dst, sizeof(int)) != hipSuccess ||
+      hipMalloc(&dstats, 4 * sizeof(unsigned long long)) != hipSuccess) {
     rc = IZEM_E_NOMEM;
   } else {
+    hipLaunchKernelGGL(k_stats_zero, dim3(1), dim3(64), 0, ctx->stream, dstats);
     int zero = 0;
     for (int v = 0; v < 2 && rc == IZEM_OK; v++) {
       if (hipMemcpyAsync(dT, T[v], sizeof T[0], hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
@@ -283,20 +287,20 @@ static int gen_bdf_check(izem_ctx *ctx, const IzMech &m, bool &agree) {
       if (v == 0)
         hipLaunchKernelGGL((k_psr_queue<KinGen, 1>), dim3(1), dim3(64), 0, ctx->stream, n, dT, dr,
                            dY, dt, ch->rtol, ch->atol, ch->jac_age, ch->counter, nullptr, dn, dst,
-                           ch->stats);
+                           dstats);
       else
         hipLaunchKernelGGL((k_psr_queue<KinTables, 1>), dim3(1), dim3(64), 0, ctx->stream, n, dT,
                            dr, dY, dt, ch->rtol, ch->atol, ch->jac_age, ch->counter, nullptr,
-                           dn, dst, ch->stats);
+                           dn, dst, dstats);
       if (hipGetLastError() != hipSuccess ||
           hipMemcpyAsync(T[v], dT, sizeof T[0], hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
           hipMemcpyAsync(nst[v], dn, sizeof nst[0], hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
           hipStreamSynchronize(ctx->stream) != hipSuccess)
         rc = IZEM_E_HIP;
     }
-    hipLaunchKernelGGL(k_stats_zero, dim3(1), dim3(64), 0, ctx->stream, ch->stats);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = IZEM_E_HIP;
   }
+  if (dstats) (void)hipFree(dstats);
   if (dT) (void)hipFree(dT);
   if (dr) (void)hipFree(dr);
   if (dY) (void)hipFree(dY);
@@ -326,11 +330,12 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
   if (!ctx->chem) {
     ctx->chem = (IzChem *)calloc(1, sizeof(IzChem));
     if (!ctx->chem) return IZEM_E_NOMEM;
-    if (hipMalloc(&ctx->chem->counter, sizeof(int)) != hipSuccess) return IZEM_E_NOMEM;
-    if (hipMalloc(&ctx->chem->stats, 4 * sizeof(unsigned long long)) != hipSuccess)
+    if (hipMalloc(&ctx->chem->counter, sizeof(int)) != hipSuccess ||
+        hipMalloc(&ctx->chem->stats, 4 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&ctx->chem->order_hist, IZ_ORDER_BINS * sizeof(int)) != hipSuccess) {
+      iz_chem_destroy(ctx);  // frees what was allocated, ctx->chem = nullptr
       return IZEM_E_NOMEM;
-    if (hipMalloc(&ctx->chem->order_hist, IZ_ORDER_BINS * sizeof(int)) != hipSuccess)
-      return IZEM_E_NOMEM;
+    }
     ctx->chem->order = 0;
     hipLaunchKernelGGL(k_stats_zero, dim3(1), dim3(64), 0, ctx->stream, ctx->chem->stats);
     // defaults measured on MI355X (2^18 cells at dt = 1e-6 s and 2^20 cells
@@ -528,12 +533,10 @@ static int order_begin(izem_ctx *ctx, int n) {
   return IZEM_OK;
 }
 
-static int order_end(izem_ctx *ctx, int n) {
-  IzChem *ch = ctx->chem;
+static int order_end(izem_ctx *ctx, int n, const int *bin, int *hist, int *perm) {
   const int nb = (n + 255) / 256;
-  hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, ctx->stream, ch->order_hist);
-  hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(256), 0, ctx->stream, n, ch->order_bin,
-                     ch->order_hist, ch->order_perm);
+  hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, ctx->stream, hist);
+  hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(256), 0, ctx->stream, n, bin, hist, perm);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
 }
@@ -541,6 +544,18 @@ static int order_end(izem_ctx *ctx, int n) {
 extern "C" int izem_chem_order(izem_ctx *ctx, int on) {
   if (!ctx || !ctx->chem || on < 0 || on > 1) return IZEM_E_ARG;
   ctx->chem->order = on;
+  return IZEM_OK;
+}
+
+extern "C" int izem_chem_order_read(izem_ctx *ctx, long cap, long *n, int *perm, int *bin) {
+  if (!ctx || !ctx->chem || !n || cap < 0) return IZEM_E_ARG;
+  IzChem *ch = ctx->chem;
+  IZ_CHECK(hipStreamSynchronize(ctx->stream));
+  *n = ch->order_n;
+  if (ch->order_n == 0) return IZEM_OK;
+  if (ch->order_n > cap || !perm || !bin) return IZEM_E_ARG;
+  IZ_CHECK(hipMemcpy(perm, ch->order_perm, ch->order_n * sizeof(int), hipMemcpyDeviceToHost));
+  IZ_CHECK(hipMemcpy(bin, ch->order_bin, ch->order_n * sizeof(int), hipMemcpyDeviceToHost));
   return IZEM_OK;
 }
 
@@ -564,33 +579,16 @@ extern "C" int izem_ckwyr(izem_ctx *ctx, int n, const double *rho, const double 
   return IZEM_OK;
 }
 
-extern "C" int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho, double *Y,
-                              double dt, int *nsteps) {
-  if (!ctx || !ctx->chem || n < 0 || !T || !rho || !Y || !(dt > 0)) return IZEM_E_ARG;
-  if (n == 0) return IZEM_OK;
+// reset the queue head and run the persistent reactor kernel of the tuned
+// variant over n cells (queue position q takes cell order[q], or q)
+static void psr_launch(izem_ctx *ctx, dim3 grid, int n, double *T, const double *rho, double *Y,
+                       double dt, int *counter, const int *order, int *nsteps) {
   IzChem *ch = ctx->chem;
-  int pk = iz_prof_begin(ctx, IZP_CHEM);
-  dim3 grid(persistent_grid(ctx, n)), blk(64);
-  const int *order = nullptr;
-  // with no more cells than resident lanes each lane takes one cell and the
-  // batch lasts as long as its stiffest cell in any order
-  if (ch->order && (long)n > 64L * grid.x) {
-    int rc = order_begin(ctx, n);
-    if (rc == IZEM_OK) {
-      hipLaunchKernelGGL(k_order_key, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, T, rho,
-                         Y, ch->order_bin, ch->order_hist);
-      rc = order_end(ctx, n);
-    }
-    if (rc != IZEM_OK) {
-      iz_prof_end(ctx, pk);
-      return rc;
-    }
-    order = ch->order_perm;
-  }
-  hipLaunchKernelGGL(k_chem_prep, dim3(1), dim3(1), 0, ctx->stream, ch->counter, ctx->ts, 0);
-#define IZ_PSR(K, W)                                                                        \
-  hipLaunchKernelGGL((k_psr_queue<K, W>), grid, blk, 0, ctx->stream, n, T, rho, Y, dt, ch->rtol, \
-                     ch->atol, ch->jac_age, ch->counter, order, nsteps, ctx->status, ch->stats)
+  hipLaunchKernelGGL(k_chem_prep, dim3(1), dim3(1), 0, ctx->stream, counter, ctx->ts, 0);
+#define IZ_PSR(K, W)                                                                            \
+  hipLaunchKernelGGL((k_psr_queue<K, W>), grid, dim3(64), 0, ctx->stream, n, T, rho, Y, dt,      \
+                     ch->rtol, ch->atol, ch->jac_age, counter, order, nsteps, ctx->status,      \
+                     ch->stats)
   if (ch->use_gen) {
     IZ_PSR(KinGen, 1);
   } else {
@@ -599,6 +597,33 @@ extern "C" int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho
     else IZ_PSR(KinTables, 4);
   }
 #undef IZ_PSR
+}
+
+extern "C" int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho, double *Y,
+                              double dt, int *nsteps) {
+  if (!ctx || !ctx->chem || n < 0 || !T || !rho || !Y || !(dt > 0)) return IZEM_E_ARG;
+  if (n == 0) return IZEM_OK;
+  IzChem *ch = ctx->chem;
+  int pk = iz_prof_begin(ctx, IZP_CHEM);
+  dim3 grid(persistent_grid(ctx, n));
+  const int *order = nullptr;
+  // with no more cells than resident lanes each lane takes one cell and the
+  // batch lasts as long as its stiffest cell in any order
+  if (ch->order && (long)n > 64L * grid.x) {
+    int rc = order_begin(ctx, n);
+    if (rc == IZEM_OK) {
+      hipLaunchKernelGGL(k_order_key, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, T, rho,
+                         Y, ch->order_bin, ch->order_hist);
+      rc = order_end(ctx, n, ch->order_bin, ch->order_hist, ch->order_perm);
+    }
+    if (rc != IZEM_OK) {
+      iz_prof_end(ctx, pk);
+      return rc;
+    }
+    order = ch->order_perm;
+  }
+  ch->order_n = order ? n : 0;
+  psr_launch(ctx, grid, n, T, rho, Y, dt, ch->counter, order, nsteps);
   iz_prof_end(ctx, pk);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
@@ -636,7 +661,7 @@ extern "C" int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale
         hipLaunchKernelGGL(k_order_key_flow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                            ctx->stream, d, i0, j0, k0, ni, nj, (int)n, ctx->o.T_ref, rho_cgs, T,
                            v, ch->order_bin, ch->order_hist);
-        rc = order_end(ctx, (int)n);
+        rc = order_end(ctx, (int)n, ch->order_bin, ch->order_hist, ch->order_perm);
       }
       if (rc != IZEM_OK) {
         iz_prof_end(ctx, pk);
@@ -644,6 +669,7 @@ extern "C" int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale
       }
       order = ch->order_perm;
     }
+    ch->order_n = order ? n : 0;
 #define IZ_RQ(K, W)                                                                            \
   hipLaunchKernelGGL((k_reaction_queue<K, W>), grid, blk, 0, ctx->stream, d, i0, i1, j0, j1, k0, \
                      k1, dt, scale, ctx->o.time_ref, ctx->o.T_ref, rho_cgs, ch->rtol, ch->atol,   \

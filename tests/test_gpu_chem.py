@@ -445,6 +445,8 @@ def test_stiffness_order_changes_no_result(mech, thd, n):
         s.sync()
         assert s.status() == 0
         out.append((dT.cpu().numpy(), dY.cpu().numpy(), nst.cpu().numpy()))
+        if on:
+            _check_order_ran(s, n, n > 65536, nst=out[-1][2])
     (T1, Y1, n1), (T0, Y0, n0) = out
     assert np.all(n1 > 0)
     if n > 1000:
@@ -453,6 +455,51 @@ def test_stiffness_order_changes_no_result(mech, thd, n):
     assert np.array_equal(T1.view(np.uint64), T0.view(np.uint64))
     assert np.array_equal(Y1.view(np.uint64), Y0.view(np.uint64))
     assert s.lib.izem_chem_order(s.ctx, 2) == -1
+
+
+def _check_order_ran(s, n, sorted_, nst=None):
+    """The context's last stiffness sort (izem_chem_order_read): with more
+    cells than resident lanes it covers all n cells, its permutation is one
+    of 0..n-1 and the cells' slots are non-decreasing along it (stiffest
+    first); otherwise no sort ran."""
+    cap = max(n, 1)
+    nn = C.c_long(-1)
+    perm = np.zeros(cap, np.int32)
+    bins = np.zeros(cap, np.int32)
+    assert s.lib.izem_chem_order_read(s.ctx, cap, C.byref(nn), perm.ctypes.data,
+                                      bins.ctypes.data) == 0
+    if not sorted_:
+        assert nn.value == 0
+        return
+    assert nn.value == n
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    b = bins[perm]
+    assert np.all(np.diff(b) >= 0) and b[0] < b[-1]
+    if nst is not None:
+        # the stiffest tenth of the queue takes more steps than the least stiff
+        k = max(n // 10, 1)
+        assert nst[perm[:k]].mean() > nst[perm[-k:]].mean()
+
+
+def test_resetup_keeps_work_counters(mech, thd):
+    """A second izem_chem_setup on a live context (new tolerances) runs its
+    self-check on counters of its own: the work counters of the integrations
+    so far survive it."""
+    import torch
+    s = _solver(mech)
+    T, rho, Y = _cells(thd, 256, seed=9)
+    d = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
+    dT, drho, dY = d(T), d(rho), d(Y)
+    assert s.lib.izem_psr_batch(s.ctx, 256, C.c_void_p(dT.data_ptr()), C.c_void_p(drho.data_ptr()),
+                                C.c_void_p(dY.data_ptr()), 1e-6, None) == 0
+    s.sync()
+    from phd_amd.solver import mech_flat
+    mf = mech_flat(mech)
+    assert s.lib.izem_chem_setup(s.ctx, mf.ctypes.data, mf.size, 1e-7, 1e-13, 1) == 0
+    st = np.zeros(4, dtype=np.uint64)
+    assert s.lib.izem_chem_stats(s.ctx, st.ctypes.data) == 0
+    assert st[0] > 256 and st[1] > st[0] and st[2] >= 256, st
+    assert s.status() == 0
 
 
 @pytest.mark.parametrize("kinetics", [0, 1])
@@ -480,6 +527,9 @@ def test_stiffness_order_flow_state(adi, thd, mech, kinetics):
         s.lib.izem_chem_stats(s.ctx, st.ctypes.data)
         out.append((s.v.cpu().numpy().copy(), s.T.cpu().numpy().copy(), s.time_state()[5],
                     st.copy()))
+        if on:
+            nr = (box.nx - 7) * (box.ny - 7)
+            _check_order_ran(s, nr, True)
     (v1, T1, d1, s1), (v0, T0, d0, s0) = out
     assert d1 > 0.0 and d1 == d0
     assert np.array_equal(s1, s0)
