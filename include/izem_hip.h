@@ -177,6 +177,18 @@ int izem_ckwyr(izem_ctx *ctx, int n, const double *rho, const double *T, const d
  * nsteps (may be NULL) receives the BDF step count of each cell */
 int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho, double *Y, double dt,
                    int *nsteps);
+/* izem_psr_batch that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The work
+ * queue head and the stiffness order's buffers live in a static device
+ * array of the library, one copy per GPU held until the process ends (about
+ * 8 MB), located by izem_chem_setup; n may be at most IZEM_NOALLOC_MAX_CELLS
+ * (IZEM_E_ARG above it).  Every call on a GPU shares that GPU's array, so
+ * calls that overlap on one GPU (two streams, threads or contexts) race.
+ * Results, step counts and work counters are those of izem_psr_batch. */
+#define IZEM_NOALLOC_MAX_CELLS (1 << 20)
+int izem_psr_batch_noalloc(izem_ctx *ctx, int n, double *T, const double *rho, double *Y,
+                           double dt, int *nsteps);
 /* order in which izem_psr_batch and izem_reaction_dvode hand cells to
  * lanes: 0 (the default) in index order; 1 stiffest first -- a key pass
  * evaluates max_k |df_k/dY_k|

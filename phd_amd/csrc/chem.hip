@@ -225,6 +225,33 @@ __global__ void k_gen_check(int *ok) {
   *ok = good;
 }
 
+// Scratch of izem_psr_batch_noalloc, one copy per GPU for the life of the
+// process: the work-queue head, the stiffness order's bin counts, then the
+// cells' bins and the queue permutation of up to IZEM_NOALLOC_MAX_CELLS
+// cells: 4 (2 IZEM_NOALLOC_MAX_CELLS + 576) bytes, about 8 MB.  Host code
+// uses the address noalloc_scratch returns, never the symbol itself.
+#define IZ_NOALLOC_HIST 64
+#define IZ_NOALLOC_BIN (IZ_NOALLOC_HIST + IZ_ORDER_BINS)
+#define IZ_NOALLOC_PERM (IZ_NOALLOC_BIN + IZEM_NOALLOC_MAX_CELLS)
+static __device__ __align__(256) int scratch[IZ_NOALLOC_PERM + IZEM_NOALLOC_MAX_CELLS];
+
+#define IZ_MAX_DEVICES 64
+static int *g_scratch[IZ_MAX_DEVICES];
+
+// device address of the current GPU's scratch, looked up once per device
+static int noalloc_scratch(int **base) {
+  int dev = 0;
+  IZ_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= IZ_MAX_DEVICES) return IZEM_E_STATE;
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    IZ_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = (int *)p;
+  }
+  *base = g_scratch[dev];
+  return IZEM_OK;
+}
+
 void iz_chem_destroy(izem_ctx *ctx) {
   if (!ctx || !ctx->chem) return;
   if (ctx->chem->counter) (void)hipFree(ctx->chem->counter);
@@ -351,6 +378,12 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
     IZ_CHECK(hipMemcpyAsync(&same, ctx->chem->counter, sizeof(int), hipMemcpyDeviceToHost,
                             ctx->stream));
     IZ_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+  {
+    // locate this GPU's scratch of izem_psr_batch_noalloc at set-up
+    int *base = nullptr;
+    const int rs = noalloc_scratch(&base);
+    if (rs != IZEM_OK) return rs;
   }
   ctx->chem->rtol = rtol;
   ctx->chem->atol = atol;
@@ -624,6 +657,37 @@ extern "C" int izem_psr_batch(izem_ctx *ctx, int n, double *T, const double *rho
   }
   ch->order_n = order ? n : 0;
   psr_launch(ctx, grid, n, T, rho, Y, dt, ch->counter, order, nsteps);
+  iz_prof_end(ctx, pk);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+extern "C" int izem_psr_batch_noalloc(izem_ctx *ctx, int n, double *T, const double *rho,
+                                      double *Y, double dt, int *nsteps) {
+  if (!ctx || !ctx->chem || n < 0 || !T || !rho || !Y || !(dt > 0)) return IZEM_E_ARG;
+  if (n > IZEM_NOALLOC_MAX_CELLS) return IZEM_E_ARG;
+  if (n == 0) return IZEM_OK;
+  int *base = nullptr;
+  const int rs = noalloc_scratch(&base);
+  if (rs != IZEM_OK) return rs;
+  int *counter = base, *hist = base + IZ_NOALLOC_HIST, *bin = base + IZ_NOALLOC_BIN,
+      *perm = base + IZ_NOALLOC_PERM;
+  IzChem *ch = ctx->chem;
+  int pk = iz_prof_begin(ctx, IZP_CHEM);
+  dim3 grid(persistent_grid(ctx, n));
+  const int *order = nullptr;
+  if (ch->order && (long)n > 64L * grid.x) {
+    hipLaunchKernelGGL(k_order_zero, dim3(1), dim3(256), 0, ctx->stream, hist);
+    hipLaunchKernelGGL(k_order_key, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, T, rho,
+                       Y, bin, hist);
+    const int rc = order_end(ctx, n, bin, hist, perm);
+    if (rc != IZEM_OK) {
+      iz_prof_end(ctx, pk);
+      return rc;
+    }
+    order = perm;
+  }
+  psr_launch(ctx, grid, n, T, rho, Y, dt, counter, order, nsteps);
   iz_prof_end(ctx, pk);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
