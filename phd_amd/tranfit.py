@@ -1,67 +1,84 @@
-"""Transport property fits from Lennard-Jones parameters: the project's own
-restatement of CHEMKIN's TRANFIT as the reference builds it
-(devel-izem/src/tranfit/tranfit.f: TRANFIT :3-378, LAMFIT36 :379-500,
-DIFFIT36 :501-612, D1236 :734-788, PARKER36 :789-812, INTERP36 :813-839,
-OMEG1236 :840-1005, OMEGXX36 :1006-1039, CKCPML36 :1978-2038).
+"""Transport-property fits from Lennard-Jones parameters, the input the
+reference's solver reads as tranfit.out (devel-izem/src/tranfit/tranfit.f
+makes that file; thd/thd_tools_perfectgas.f90:478-638 reads it).
 
 For every species the viscosity and the thermal conductivity, and for every
 pair the binary diffusion coefficient at 1 atm, are evaluated at 50
 temperatures evenly spaced over the range that the thermodynamic data of all
-species cover, and ln(property) is fitted by a cubic in ln T.  The result
-has the layout of the reference's tranfit.out (phd_amd.chemdata.TransportFits),
-so that build_thd can take fits made here or read from a file.
+species cover; ln(property) is then fitted by a least-squares cubic in ln T.
+The methods are the published ones that CHEMKIN's transport package uses:
 
-* Viscosity: Chapman-Enskog with the reduced collision integral
-  Omega(2,2)*(T*, delta*) of the Stockmayer potential; self-diffusion with
-  Omega(1,1)*.
-* Conductivity: translational, rotational and vibrational parts (Warnatz),
-  with the rotational relaxation number scaled from 298 K by Parker's
-  temperature dependence (Brau and Jonkman 1970).
-* Binary diffusion: combination rules with the polar/non-polar induction
-  correction, and the Marrero-Mason second-order correction factor (1972).
-* Collision integrals: the Monchick-Mason tables (J. Chem. Phys. 35, 1676,
-  1961) of Omega(1,1)* and Omega(2,2)* over 37 reduced temperatures and 8
-  reduced dipole moments, with TRANFIT's three-point interpolation in delta*
-  and then in T*, and its closed forms above T* = 75.
+* dilute-gas kinetic theory (Chapman-Enskog; Hirschfelder, Curtiss and Bird,
+  "Molecular Theory of Gases and Liquids", 1954): viscosity with the reduced
+  collision integral Omega(2,2)*, self- and binary diffusion with
+  Omega(1,1)*, both for the Stockmayer potential (Lennard-Jones plus
+  point dipoles);
+* Omega* from the tables of Monchick and Mason (J. Chem. Phys. 35, 1676,
+  1961) over reduced temperature T* and reduced dipole moment delta*,
+  interpolated through three neighbouring nodes, first across delta* and
+  then across T*, and by cubics in T* above the tables;
+* pure-species conductivity split into translational, rotational and
+  vibrational parts (Warnatz, 1978; Kee et al., "Chemically Reacting
+  Flow", 2003, sec. 12.4), the rotational relaxation number scaled from its
+  298 K value by Parker's temperature dependence (Parker 1959; Brau and
+  Jonkman 1970);
+* binary pairs: the combination rules for sigma and epsilon, with the
+  induction correction of a polar / non-polar pair, and the second
+  Chapman-Enskog approximation to D_12 (Marrero and Mason, J. Phys. Chem.
+  Ref. Data 1, 3, 1972).
 
-TRANFIT writes its constants as default-REAL (single precision) literals:
-pi, the unit factors, the viscosity and diffusion prefactors, Avogadro's
-number, the table entries and the fit constants of the A*, B*, C* ratios
-and of Parker's function.  They are rounded to single precision here in the
-same places, so that the fits agree with the reference program's to
-rounding.  The cubic is an ordinary least-squares fit (TRANFIT calls
-SLATEC's DPOLFT with uniform weights and DPCOEF to expand it in powers of
-ln T); the two agree to the conditioning of the cubic in ln T.
+The reference program writes its constants as single-precision literals; the
+values here are rounded to single precision where it rounds them, so that
+the fits match its tranfit.out to rounding of the double-precision
+arithmetic.  The least-squares cubic is numpy's (the reference uses SLATEC's
+orthogonal-polynomial fit; the two agree to the conditioning of a cubic in
+ln T).
 """
 from __future__ import annotations
 
 import os
+from dataclasses import dataclass
 
 import numpy as np
 
 from . import chemdata as cd
 from .chemdata import _f32
 
-NT = 50     # fit temperatures (tranfit.f:46)
-NO = 4      # polynomial coefficients (cubic)
+NT = 50     # fit temperatures
+NO = 4      # coefficients of the cubic in ln T
 
-# constants of TRANFIT (tranfit.f:71-76, 140) and of the CHEMKIN-II 3.6
-# linking file it reads (RU, PATM: tranfit.f:1153), single-precision literals
+# cgs constants, single precision as the reference program holds them
 PI = _f32(3.1415926535)
-FDTCGS = _f32(1.0e-18)     # Debye -> cgs
-FATCM = _f32(1.0e8)        # Angstrom -> cm
-DIPMIN = _f32(1.0e-20)
-RU = _f32(8.314e7)         # erg/(mol K)
-PATM = _f32(1.01325e6)     # dyn/cm^2
-BOLTZ = RU / _f32(6.02217e23)
+R_CGS = _f32(8.314e7)                      # erg/(mol K)
+P_ATM = _f32(1.01325e6)                    # dyn/cm^2
+K_BOLTZ = R_CGS / _f32(6.02217e23)         # erg/K
+DEBYE = _f32(1.0e-18)                      # esu cm
+ANGSTROM = _f32(1.0e8)                     # 1/cm
+POLAR_EPS = _f32(1.0e-20)                  # dipole [D] below which a species is non-polar
+# Chapman-Enskog prefactors: eta = C_ETA sqrt(W T) / (sigma^2 Omega22*) [g/cm/s],
+# D = C_D sqrt(T^3 / W_red) / (p[atm] sigma^2 Omega11*) [cm^2/s]
+C_ETA = _f32(2.6693e-5)
+C_D = _f32(2.6280e-3)
+C_D12 = _f32(0.002628)
+# Parker's rotational-relaxation function: 1 + a x^(1/2) + b x + c x^(3/2),
+# x = eps / kT, with a = pi^(3/2) / 2, b = pi^2 / 4 + 2, c = pi^(3/2) to
+# four decimals
+PARKER = (_f32(2.7842), _f32(4.4674), _f32(5.5683))
+# Marrero-Mason factors 6/5, 8/5, 12/5 and the ratios B*(T*), C*(T*) of
+# collision integrals, cubic in T* up to 5 and constant above
+MM_6_5, MM_8_5, MM_12_5 = _f32(1.2), _f32(1.6), _f32(2.4)
+B_STAR = ((_f32(1.36), -_f32(0.223), _f32(0.0613), -_f32(0.00554)), _f32(1.095))
+C_STAR = ((_f32(0.823), _f32(0.0128), _f32(0.0112), -_f32(0.00193)), _f32(0.9483))
+# cubics in T* for non-polar pairs above the tables (T* > 75): Omega11*, Omega22*
+HIGH_T = {1: (_f32(0.623), -_f32(0.136e-2), _f32(0.346e-5), -_f32(0.343e-8)),
+          2: (_f32(0.703), -_f32(0.146e-2), _f32(0.357e-5), -_f32(0.343e-8))}
 
-_TSTERN = [_f32(x) for x in (.1, .2, .3, .4, .5, .6, .7, .8, .9, 1., 1.2, 1.4, 1.6, 1.8, 2., 2.5,
-                             3., 3.5, 4., 5., 6., 7., 8., 9., 10., 12., 14., 16., 18., 20., 25.,
-                             30., 35., 40., 50., 75., 100.)]
-_DELTA = [_f32(x) for x in (0., .25, .5, .75, 1., 1.5, 2., 2.5)]
+# Monchick-Mason reduced collision integrals: rows T*, columns delta*
+T_STAR = np.array([_f32(x) for x in (
+    .1, .2, .3, .4, .5, .6, .7, .8, .9, 1., 1.2, 1.4, 1.6, 1.8, 2., 2.5, 3., 3.5, 4., 5., 6., 7.,
+    8., 9., 10., 12., 14., 16., 18., 20., 25., 30., 35., 40., 50., 75., 100.)])
+DELTA_STAR = np.array([_f32(x) for x in (0., .25, .5, .75, 1., 1.5, 2., 2.5)])
 
-# Monchick-Mason reduced collision integrals, rows T* (above), columns
-# delta* (above): Omega(1,1)* and Omega(2,2)*
 _OMEGA11 = """
   4.008   4.002   4.655    5.52   6.454   8.214   9.824   11.31
   3.130   3.164   3.355   3.721   4.198    5.23   6.225   7.160
@@ -142,115 +159,85 @@ _OMEGA22 = """
 """
 
 
-def _table(text):
-    return [[_f32(float(x)) for x in row.split()] for row in text.strip().splitlines()]
 
 
-_O11 = _table(_OMEGA11)
-_O22 = _table(_OMEGA22)
+def _parse_table(text: str) -> np.ndarray:
+    return np.array([[_f32(float(x)) for x in row.split()] for row in text.strip().splitlines()])
 
 
-def _interp3(arg, x, y):
-    """TRANFIT's quadratic interpolation through three points (INTERP36)."""
-    val1 = y[0] + (arg - x[0]) * (y[1] - y[0]) / (x[1] - x[0])
-    val2 = y[1] + (arg - x[1]) * (y[2] - y[1]) / (x[2] - x[1])
-    fac1 = (arg - x[0]) / (x[1] - x[0]) / 2.0
-    fac2 = (x[2] - arg) / (x[2] - x[1]) / 2.0
-    if arg >= x[1]:
-        return (val1 * fac2 + val2) / (1.0 + fac2)
-    return (val1 + val2 * fac1) / (1.0 + fac1)
+TABLES = {1: _parse_table(_OMEGA11), 2: _parse_table(_OMEGA22)}
 
 
-def omega(n: int, tr: float, dr: float) -> float:
-    """Reduced collision integral Omega(1,1)* (n = 1) or Omega(2,2)* (n = 2)
-    at reduced temperature tr and reduced dipole moment dr (OMEG1236)."""
-    if dr < -1.0e-5 or dr > 2.5 or tr < 0.09 or tr > 500.0 or (abs(dr) > 1.0e-5 and tr > 75.0):
-        raise ValueError(f"collision integral undefined at T* = {tr}, delta* = {dr}")
-    if tr > 75.0:
-        if n == 1:
-            return (_f32(0.623) - _f32(0.136e-2) * tr + _f32(0.346e-5) * tr * tr
-                    - _f32(0.343e-8) * tr * tr * tr)
-        return (_f32(0.703) - _f32(0.146e-2) * tr + _f32(0.357e-5) * tr * tr
-                - _f32(0.343e-8) * tr * tr * tr)
-    tab = _O11 if n == 1 else _O22
-    if tr <= 0.2:
-        ii = 2
+def _three_point(x, x3, y3):
+    """Value at x of three samples (x3, y3), x3 increasing: the secant through
+    the first two and the one through the last two, blended with a weight on
+    the farther secant that falls off linearly from one half at the middle
+    node -- a smooth curve through the three points that reduces to the
+    nearer secant at the outer nodes."""
+    x0, x1, x2 = x3
+    s_lo = y3[0] + (x - x0) * (y3[1] - y3[0]) / (x1 - x0)
+    s_hi = y3[1] + (x - x1) * (y3[2] - y3[1]) / (x2 - x1)
+    if x >= x1:
+        w = (x2 - x) / (x2 - x1) / 2.0
+        return (s_lo * w + s_hi) / (1.0 + w)
+    w = (x - x0) / (x1 - x0) / 2.0
+    return (s_lo + s_hi * w) / (1.0 + w)
+
+
+def _window(nodes: np.ndarray, x: float, lowest: float) -> int:
+    """First index of the three table nodes around x: the triple whose middle
+    node is the first node at or above x, clamped to the table."""
+    if x <= lowest:
+        return 0
+    m = int(np.searchsorted(nodes, x, side="left"))
+    return min(max(m - 1, 0), len(nodes) - 3)
+
+
+def collision_integral(l: int, tstar: float, dstar: float) -> float:
+    """Reduced collision integral Omega(l,l)* (l = 1 or 2) of the Stockmayer
+    potential at reduced temperature tstar = kT/eps and reduced dipole
+    moment dstar (Monchick-Mason tables; cubics in T* above T* = 75 for
+    non-polar pairs)."""
+    polar = abs(dstar) > 1.0e-5
+    if dstar < -1.0e-5 or dstar > 2.5 or tstar < 0.09 or tstar > 500.0 or (polar and tstar > 75.0):
+        raise ValueError(f"collision integral undefined at T* = {tstar}, delta* = {dstar}")
+    if tstar > 75.0:
+        c = HIGH_T[l]
+        return c[0] + c[1] * tstar + c[2] * tstar * tstar + c[3] * tstar * tstar * tstar
+    tab = TABLES[l]
+    i = _window(T_STAR, tstar, 0.2)
+    rows = range(i, i + 3)
+    if polar:
+        j = _window(DELTA_STAR, dstar, 0.25)
+        col = [_three_point(dstar, DELTA_STAR[j:j + 3], tab[r, j:j + 3]) for r in rows]
     else:
-        ii = 37
-        for i in range(2, 37):
-            if _TSTERN[i - 2] < tr and _TSTERN[i - 1] >= tr:
-                ii = i
-    targ = [_TSTERN[ii - 2 + i] for i in range(3)]
-    if abs(dr) >= 1.0e-5:
-        if dr <= 0.25:
-            kk = 2
-        else:
-            kk = 7
-            for k in range(2, 8):
-                if _DELTA[k - 2] < dr and _DELTA[k - 1] >= dr:
-                    kk = k
-        darg = [_DELTA[kk - 2 + k] for k in range(3)]
-        vert = [_interp3(dr, darg, [tab[ii - 2 + i][kk - 2 + k] for k in range(3)])
-                for i in range(3)]
-        return _interp3(tr, targ, vert)
-    return _interp3(tr, targ, [tab[ii - 2 + i][0] for i in range(3)])
+        col = [tab[r, 0] for r in rows]
+    return _three_point(tstar, T_STAR[i:i + 3], col)
 
 
-def _omegxx(n: int, tr: float, dst: float) -> float:
-    """Omega(1,1)*, Omega(2,2)* and the ratios B*, C* (OMEGXX36)."""
-    if n in (1, 2):
-        return omega(n, tr, dst)
-    if n == 3:
-        if tr <= 5.0:
-            return _f32(1.36) - _f32(0.223) * tr + _f32(0.0613) * tr ** 2 - _f32(0.00554) * tr ** 3
-        return _f32(1.095)
-    if tr <= 5.0:
-        return _f32(0.823) + _f32(0.0128) * tr + _f32(0.0112) * tr ** 2 - _f32(0.00193) * tr ** 3
-    return _f32(0.9483)
+def _ratio_fit(fit, tstar: float) -> float:
+    (c0, c1, c2, c3), high = fit
+    return c0 + c1 * tstar + c2 * tstar ** 2 + c3 * tstar ** 3 if tstar <= 5.0 else high
 
 
-def _d12_correction(w1, w2, t, eps1, eps2, sig1, sig2, dst):
-    """Second-order correction of the binary diffusion coefficient (Marrero
-    and Mason, J. Phys. Chem. Ref. Data 1, 3, 1972; D1236)."""
-    sumw = w1 + w2
-    sig12 = 0.5 * (sig1 + sig2)
-    tr11, tr22, tr12 = t / eps1, t / eps2, t / np.sqrt(eps1 * eps2)
-    om2f11 = _omegxx(2, tr11, dst)
-    om2f22 = _omegxx(2, tr22, dst)
-    om1f12 = _omegxx(1, tr12, dst)
-    om2f12 = _omegxx(2, tr12, dst)
-    bst12 = _omegxx(3, tr12, dst)
-    cst12 = _omegxx(4, tr12, dst)
-    ast12 = om2f12 / om1f12
-    c12, c16, c24 = _f32(1.2), _f32(1.6), _f32(2.4)
-    h = (sig1 / sig2) ** 2 * np.sqrt(2.0 * w2 / sumw) * 2.0 * w1 ** 2 / (sumw * w2)
-    p1 = h * om2f11 / om1f12
-    h = (sig2 / sig1) ** 2 * np.sqrt(2.0 * w1 / sumw) * 2.0 * w2 ** 2 / (sumw * w1)
-    p2 = h * om2f22 / om1f12
-    p12 = 15.0 * ((w1 - w2) / sumw) ** 2 + 8.0 * w1 * w2 * ast12 / sumw ** 2
-    h = 2.0 / (w2 * sumw) * np.sqrt(2.0 * w2 / sumw) * om2f11 / om1f12 * (sig1 / sig12) ** 2
-    q1 = ((2.5 - c12 * bst12) * w1 ** 2 + 3.0 * w2 ** 2 + c16 * w1 * w2 * ast12) * h
-    h = 2.0 / (w1 * sumw) * np.sqrt(2.0 * w1 / sumw) * om2f22 / om1f12 * (sig2 / sig12) ** 2
-    q2 = ((2.5 - c12 * bst12) * w2 ** 2 + 3.0 * w1 ** 2 + c16 * w1 * w2 * ast12) * h
-    h = (((w1 - w2) / sumw) ** 2 * (2.5 - c12 * bst12) * 15.0
-         + 4.0 * w1 * w2 * ast12 / sumw ** 2 * (11.0 - c24 * bst12))
-    q12 = h + c16 * sumw * om2f11 * om2f22 / (np.sqrt(w1 * w2) * om1f12 ** 2) * \
-        sig1 ** 2 * sig2 ** 2 / sig12 ** 4
-    return ((6.0 * cst12 - 5.0) ** 2 / 10.0) * (p1 + p2 + p12) / (q1 + q2 + q12) + 1.0
+@dataclass
+class LJSpecies:
+    """Molecular parameters of a CHEMKIN transport database entry."""
+    geometry: int      # 0 atom, 1 linear, 2 non-linear
+    eps_k: float       # well depth over Boltzmann's constant [K]
+    sigma: float       # collision diameter [A]
+    dipole: float      # [Debye]
+    polarizability: float  # [A^3]
+    zrot298: float     # rotational relaxation collision number at 298 K
 
-
-def _parker(t, eps):
-    """Temperature dependence of the rotational collision number (PARKER36)."""
-    a, b, c = _f32(2.7842), _f32(4.4674), _f32(5.5683)
-    d, dr = eps / t, eps / 298.0
-    return ((1.0 + a * np.sqrt(dr) + b * dr + c * dr ** 1.5)
-            / (1.0 + a * np.sqrt(d) + b * d + c * d ** 1.5))
+    def reduced_dipole_sq(self) -> float:
+        """mu*^2 = mu^2 / (eps sigma^3) in cgs."""
+        return (self.dipole * DEBYE) ** 2 / (self.eps_k * K_BOLTZ * (self.sigma / ANGSTROM) ** 3)
 
 
 def read_tran_dat(species, path: str | None = None) -> dict:
     """Lennard-Jones parameters of the given species from a CHEMKIN transport
-    database: name, geometry (0 atom, 1 linear, 2 nonlinear), eps/k [K],
-    sigma [A], dipole [Debye], polarizability [A^3], Z_rot(298 K)."""
+    database (name, geometry, eps/k, sigma, dipole, polarizability, Z_rot)."""
     path = path or os.path.join(cd.DATA_DIR, "tran.dat")
     want = {s.upper() for s in species}
     out = {}
@@ -258,110 +245,154 @@ def read_tran_dat(species, path: str | None = None) -> dict:
         ln = raw.split("!")[0].split()
         if len(ln) < 7 or ln[0].upper() not in want:
             continue
-        v = [float(x) for x in ln[1:7]]
-        out[ln[0].upper()] = dict(nlin=int(v[0]), eps=v[1], sig=v[2], dip=v[3], pol=v[4],
-                                  zrot=v[5])
+        g, e, s, d, a, z = (float(x) for x in ln[1:7])
+        out[ln[0].upper()] = LJSpecies(int(g), e, s, d, a, z)
     missing = [s for s in species if s.upper() not in out]
     if missing:
         raise ValueError(f"transport data not given for {missing}")
     return out
 
 
-def _cv(mech, k, t):
-    """Molar heat capacity at constant volume (CKCPML36 minus R)."""
-    a = mech.a[k][1] if t > mech.tmid[k] else mech.a[k][0]
-    tn = (1.0, t, t ** 2, t ** 3, t ** 4)
+def _zrot_scale(sp: LJSpecies, T: float) -> float:
+    """Z_rot(T) / Z_rot(298 K) by Parker's function of eps / kT."""
+    a, b, c = PARKER
+
+    def F(x):
+        return 1.0 + a * np.sqrt(x) + b * x + c * x ** 1.5
+
+    return F(sp.eps_k / 298.0) / F(sp.eps_k / T)
+
+
+def _cv_molar(mech, k: int, T: float) -> float:
+    """c_v of species k [erg/(mol K)] from its NASA-7 c_p/R polynomial."""
+    a = mech.a[k][1] if T > mech.tmid[k] else mech.a[k][0]
     cp = 0.0
-    for n in range(5):
-        cp = cp + RU * tn[n] * a[n]
-    return cp - RU
+    for n, tn in enumerate((1.0, T, T ** 2, T ** 3, T ** 4)):
+        cp = cp + R_CGS * tn * a[n]
+    return cp - R_CGS
 
 
-def _cubic(x, y):
-    # coefficients of ln T^0..3, least squares
-    return np.polynomial.polynomial.polyfit(np.asarray(x), np.asarray(y), NO - 1)
+def pure_properties(sp: LJSpecies, W: float, T: float, cv: float, p: float):
+    """Viscosity [g/(cm s)] and thermal conductivity [erg/(cm s K)] of one
+    species at T; cv is its molar c_v [erg/(mol K)]."""
+    dstar = 0.5 * sp.reduced_dipole_sq()
+    tstar = T / sp.eps_k
+    eta = C_ETA * np.sqrt(W) / sp.sigma ** 2 * np.sqrt(T) / collision_integral(2, tstar, dstar)
+    D = C_D * (P_ATM / p) / (np.sqrt(W) * sp.sigma ** 2) * np.sqrt(T ** 3) / \
+        collision_integral(1, tstar, dstar)
+    rhoD_eta = D * W * (p / (R_CGS * T)) / eta
+    cv_tr = 1.5 * R_CGS
+    if sp.geometry == 0:
+        return eta, eta / W * (2.5 * cv_tr)
+    # rotational part: c_v,rot = R (linear) or 3/2 R (non-linear)
+    if sp.geometry == 1:
+        cv_rot, rot_over_tr, five_thirds_cvrot = R_CGS, _f32(2.0 / 3.0), _f32(5.0 / 3.0)
+    else:
+        cv_rot, rot_over_tr, five_thirds_cvrot = 1.5 * R_CGS, 1.0, 2.5
+    A = 2.5 - rhoD_eta
+    B = sp.zrot298 * _zrot_scale(sp, T) + 2.0 * (five_thirds_cvrot + rhoD_eta) / PI
+    f_tr = 2.5 * (1.0 - 2.0 * rot_over_tr * A / (B * PI))
+    f_rot = rhoD_eta * (1.0 + 2.0 * A / (B * PI))
+    f_vib = rhoD_eta
+    cv_vib = cv - cv_tr - cv_rot
+    if sp.geometry == 1:
+        c = (f_tr * 1.5 + f_rot) * R_CGS + f_vib * cv_vib
+    else:
+        c = (f_tr + f_rot) * 1.5 * R_CGS + f_vib * cv_vib
+    return eta, eta / W * c
+
+
+def pair_potential(a: LJSpecies, b: LJSpecies):
+    """eps/k [K], sigma [A] and delta* of the a-b interaction: arithmetic
+    sigma and geometric eps, a polar / non-polar pair corrected by the
+    induced dipole (xi = 1 + alpha*_n mu*_p^2 sqrt(eps_p / eps_n) / 4, sigma
+    times xi^(-1/6), eps times xi^2; delta* = 0 for it), two polar species
+    combined as delta* = mu_a mu_b / (2 eps sigma^3)."""
+    sigma = 0.5 * (a.sigma + b.sigma)
+    eps = np.sqrt(a.eps_k * b.eps_k)
+    pa, pb = a.dipole > POLAR_EPS, b.dipole > POLAR_EPS
+    if pa != pb:
+        p, n = (a, b) if pa else (b, a)
+        xi = 1.0 + 0.25 * (n.polarizability / n.sigma ** 3) * p.reduced_dipole_sq() * \
+            np.sqrt(p.eps_k / n.eps_k)
+        return eps * xi ** 2, sigma * xi ** _f32(-1.0 / 6.0), 0.0
+    dstar = 0.5 * (a.dipole * DEBYE) * (b.dipole * DEBYE) / \
+        (eps * K_BOLTZ * (sigma / ANGSTROM) ** 3)
+    return eps, sigma, dstar
+
+
+def second_order_factor(Wa, Wb, T, a: LJSpecies, b: LJSpecies, dstar) -> float:
+    """Marrero-Mason's second Chapman-Enskog approximation to a binary
+    diffusion coefficient, f = 1 + (6 C* - 5)^2 / 10 (P_a + P_b + P_ab) /
+    (Q_a + Q_b + Q_ab), written with the pair's mass fractions m = W / (Wa +
+    Wb) and the collision integrals of the like and unlike interactions."""
+    ma, mb = Wa / (Wa + Wb), Wb / (Wa + Wb)
+    sab = 0.5 * (a.sigma + b.sigma)
+    t_ab = T / np.sqrt(a.eps_k * b.eps_k)
+    o22a = collision_integral(2, T / a.eps_k, dstar)
+    o22b = collision_integral(2, T / b.eps_k, dstar)
+    o11ab = collision_integral(1, t_ab, dstar)
+    A = collision_integral(2, t_ab, dstar) / o11ab
+    Bs = _ratio_fit(B_STAR, t_ab)
+    Cs = _ratio_fit(C_STAR, t_ab)
+    ra, rb = o22a / o11ab, o22b / o11ab
+
+    def P(mi, mj, si, sj, r):
+        return (si / sj) ** 2 * np.sqrt(2.0 * mj) * 2.0 * mi * mi / mj * r
+
+    def Q(mi, mj, si, r):
+        return ((2.5 - MM_6_5 * Bs) * mi * mi + 3.0 * mj * mj + MM_8_5 * mi * mj * A) * \
+            2.0 / mj * np.sqrt(2.0 * mj) * r * (si / sab) ** 2
+
+    dm2 = (ma - mb) ** 2
+    p_ab = 15.0 * dm2 + 8.0 * ma * mb * A
+    q_ab = dm2 * (2.5 - MM_6_5 * Bs) * 15.0 + 4.0 * ma * mb * A * (11.0 - MM_12_5 * Bs) + \
+        MM_8_5 * o22a * o22b / (np.sqrt(ma * mb) * o11ab ** 2) * a.sigma ** 2 * b.sigma ** 2 / sab ** 4
+    num = P(ma, mb, a.sigma, b.sigma, ra) + P(mb, ma, b.sigma, a.sigma, rb) + p_ab
+    den = Q(ma, mb, a.sigma, ra) + Q(mb, ma, b.sigma, rb) + q_ab
+    return 1.0 + ((6.0 * Cs - 5.0) ** 2 / 10.0) * num / den
+
+
+def binary_diffusion(a: LJSpecies, b: LJSpecies, Wa, Wb, T, p) -> float:
+    """D_ab [cm^2/s] at T and p [dyn/cm^2]."""
+    eps, sigma, dstar = pair_potential(a, b)
+    d1 = C_D12 * np.sqrt((Wa + Wb) / (2.0 * Wa * Wb) * T ** 3) * P_ATM / \
+        (collision_integral(1, T / eps, dstar) * sigma ** 2 * p)
+    return d1 * second_order_factor(Wa, Wb, T, a, b, dstar)
+
+
+def fit_temperatures(mech, therm_path=None) -> np.ndarray:
+    """50 temperatures over the range every species' thermodynamic data cover."""
+    nasa, defaults = cd.read_therm(therm_path)
+    lo = max((nasa[s.upper()].tlo or defaults[0]) for s in mech.species)
+    hi = min((nasa[s.upper()].thi or defaults[2]) for s in mech.species)
+    return np.array([lo + n * ((hi - lo) / (NT - 1)) for n in range(NT)])
+
+
+def _lsq_cubic(lnT, y) -> np.ndarray:
+    return np.polynomial.polynomial.polyfit(lnT, np.asarray(y), NO - 1)
 
 
 def fit_transport(mech: "cd.Mechanism", tran_path: str | None = None,
-                  therm_path: str | None = None, p: float = PATM) -> "cd.TransportFits":
-    """Viscosity, conductivity and binary diffusion fits of the mechanism's
-    species (TRANFIT): the content of the reference's tranfit.out."""
+                  therm_path: str | None = None, p: float = P_ATM) -> "cd.TransportFits":
+    """ln(viscosity), ln(conductivity) and ln(binary diffusion) of the
+    mechanism's species as cubics in ln T: the content of tranfit.out."""
+    lj = read_tran_dat(mech.species, tran_path)
+    sp = [lj[s.upper()] for s in mech.species]
+    W = [float(w) for w in mech.wt]
+    T = fit_temperatures(mech, therm_path)
+    lnT = np.log(T)
     kk = mech.kk
-    tp = read_tran_dat(mech.species, tran_path)
-    lj = [tp[s.upper()] for s in mech.species]
-    nasa, defaults = cd.read_therm(therm_path)
-    tlow = max((nasa[s.upper()].tlo or defaults[0]) for s in mech.species)
-    thigh = min((nasa[s.upper()].thi or defaults[2]) for s in mech.species)
-    dt = (thigh - tlow) / (NT - 1)
-    temps = [tlow + n * dt for n in range(NT)]
-    alogt = [np.log(t) for t in temps]
-    wt = [float(w) for w in mech.wt]
-    fcon = 0.5 * FDTCGS ** 2 * FATCM ** 3 / BOLTZ
-    pfac = PATM / p
-    coflam = np.zeros((kk, NO))
-    cofeta = np.zeros((kk, NO))
+    cofeta, coflam = np.zeros((kk, NO)), np.zeros((kk, NO))
     for k in range(kk):
-        s = lj[k]
-        dst = fcon * s["dip"] ** 2 / (s["eps"] * s["sig"] ** 3)
-        helpe = _f32(2.6693e-5) * np.sqrt(wt[k]) / s["sig"] ** 2
-        helpd = _f32(2.6280e-3) * pfac / (np.sqrt(wt[k]) * s["sig"] ** 2)
-        xla, xeta = [], []
-        for t in temps:
-            prut = p / (RU * t)
-            tr = t / s["eps"]
-            dii = np.sqrt(t ** 3) * helpd / omega(1, tr, dst)
-            eta = np.sqrt(t) * helpe / omega(2, tr, dst)
-            rodet = dii * wt[k] * prut / eta
-            aa = 2.5 - rodet
-            park = _parker(t, s["eps"])
-            if s["nlin"] == 2:
-                bb = s["zrot"] * park + 2.0 * (5.0 / 2.0 + rodet) / PI
-                croct = 1.0
-            else:
-                bb = s["zrot"] * park + 2.0 * (_f32(5.0 / 3.0) + rodet) / PI
-                croct = _f32(2.0 / 3.0)
-            ftra = 2.5 * (1.0 - 2.0 * croct * aa / (bb * PI))
-            frot = rodet * (1.0 + 2.0 * aa / (bb * PI))
-            fvib = rodet
-            if s["nlin"] == 0:
-                faktor = 5.0 / 2.0 * 1.5 * RU
-            elif s["nlin"] == 1:
-                faktor = (ftra * 1.5 + frot) * RU + fvib * (_cv(mech, k, t) - 2.5 * RU)
-            else:
-                faktor = (ftra + frot) * 1.5 * RU + fvib * (_cv(mech, k, t) - 3.0 * RU)
-            xla.append(np.log(eta / wt[k] * faktor))
-            xeta.append(np.log(eta))
-        coflam[k] = _cubic(alogt, xla)
-        cofeta[k] = _cubic(alogt, xeta)
+        props = [pure_properties(sp[k], W[k], t, _cv_molar(mech, k, t), p) for t in T]
+        cofeta[k] = _lsq_cubic(lnT, [np.log(e) for e, _ in props])
+        coflam[k] = _lsq_cubic(lnT, [np.log(l) for _, l in props])
     cofd = np.zeros((kk, kk, NO))
     for j in range(kk):
         for k in range(j + 1):
-            sj, sk = lj[j], lj[k]
-            sigm = 0.5 * (sj["sig"] + sk["sig"])
-            epsm = np.sqrt(sj["eps"] * sk["eps"])
-            if sj["dip"] < DIPMIN and sk["dip"] > DIPMIN:
-                dst = 0.0   # k polar, j not
-                xi = 1.0 + sj["pol"] * fcon * sk["dip"] ** 2 * np.sqrt(sk["eps"] / sj["eps"]) / \
-                    (2.0 * sj["sig"] ** 3 * sk["eps"] * sk["sig"] ** 3)
-            elif sj["dip"] > DIPMIN and sk["dip"] < DIPMIN:
-                dst = 0.0   # j polar, k not
-                xi = 1.0 + sk["pol"] * fcon * sj["dip"] ** 2 * np.sqrt(sj["eps"] / sk["eps"]) / \
-                    (2.0 * sk["sig"] ** 3 * sj["eps"] * sj["sig"] ** 3)
-            else:
-                dst = fcon * sj["dip"] * sk["dip"] / (epsm * sigm ** 3)
-                xi = 1.0
-            sigm = sigm * xi ** _f32(-1.0 / 6.0)
-            epsm = epsm * xi ** 2
-            help1 = (wt[j] + wt[k]) / (2.0 * wt[j] * wt[k])
-            xd = []
-            for t in temps:
-                tr = t / epsm
-                help2 = _f32(0.002628) * np.sqrt(help1 * t ** 3)
-                d = help2 * PATM / (omega(1, tr, dst) * sigm ** 2 * p)
-                d = d * _d12_correction(wt[j], wt[k], t, sj["eps"], sk["eps"], sj["sig"],
-                                        sk["sig"], dst)
-                xd.append(np.log(d))
-            cofd[j, k] = cofd[k, j] = _cubic(alogt, xd)
+            d = [np.log(binary_diffusion(sp[j], sp[k], W[j], W[k], t, p)) for t in T]
+            cofd[j, k] = cofd[k, j] = _lsq_cubic(lnT, d)
     kncf = np.asarray(mech.ncf, dtype=np.int64).T.copy()
     awt = np.array([_f32(cd._AWT[e]) for e in mech.elements]) / 1000.0
     return cd.TransportFits(list(mech.species), cofd, coflam, cofeta,
@@ -369,32 +400,30 @@ def fit_transport(mech: "cd.Mechanism", tran_path: str | None = None,
 
 
 def write_tranfit(fits: "cd.TransportFits", path: str, nreac: int = 0) -> None:
-    """The fits in the record order of the reference's tranfit.out
-    (tranfit.f:262-318), readable by thd_cofd_read and read_tranfit."""
+    """The fits in the record layout of tranfit.out as the reference's
+    reader (thd_tools_perfectgas.f90:478-638) and chemdata.read_tranfit take
+    it: counts; the lower triangle of D fits per species; conductivity,
+    viscosity, molar masses, atomic weights, element counts."""
     kk = len(fits.names)
     nele = fits.awt.size
-    L = [str(nreac), str(kk), str(NO), str(nele)]
+    out = [str(nreac), str(kk), str(NO), str(nele)]
+    num = lambda x: repr(float(x))
     for j in range(kk):
-        L += [str(j + 1), fits.names[j]]
+        out += [str(j + 1), fits.names[j]]
         for k in range(j + 1):
-            L.append(fits.names[k])
-            L += [repr(float(x)) for x in fits.cofd[j, k]]
-    L.append("-2")
+            out += [fits.names[k]] + [num(x) for x in fits.cofd[j, k]]
+    for block in (fits.coflam, fits.cofeta):
+        out.append("-2")
+        for k in range(kk):
+            out += [fits.names[k]] + [num(x) for x in block[k]]
+    out.append("-2")
     for k in range(kk):
-        L.append(fits.names[k])
-        L += [repr(float(x)) for x in fits.coflam[k]]
-    L.append("-2")
+        out += [fits.names[k], num(fits.W[k] * 1000.0)]
+    out.append("-2")
+    out += [num(a * 1000.0) for a in fits.awt]
+    out.append("-2")
     for k in range(kk):
-        L.append(fits.names[k])
-        L += [repr(float(x)) for x in fits.cofeta[k]]
-    L.append("-2")
-    for k in range(kk):
-        L += [fits.names[k], repr(float(fits.W[k] * 1000.0))]
-    L.append("-2")
-    L += [repr(float(a * 1000.0)) for a in fits.awt]
-    L.append("-2")
-    for k in range(kk):
-        L += [str(int(fits.kncf[n, k])) for n in range(nele)]
-    L.append("-1")
+        out += [str(int(fits.kncf[n, k])) for n in range(nele)]
+    out.append("-1")
     with open(path, "w") as f:
-        f.write("\n".join(L) + "\n")
+        f.write("\n".join(out) + "\n")

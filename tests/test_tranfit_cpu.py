@@ -1,5 +1,6 @@
-"""The transport-fit front end (phd_amd/tranfit.py, a restatement of the
-reference's TRANFIT: devel-izem/src/tranfit/tranfit.f) against
+"""The transport-fit front end (phd_amd/tranfit.py: kinetic-theory fits of
+the kind the reference's TRANFIT, devel-izem/src/tranfit/tranfit.f, writes
+to tranfit.out) against
 
 * the reference's own fixture devel-izem/testcases/shockedshearlayer/
   tranfit.out (phd_amd/data/tranfit.out), made by a working build of that
@@ -74,18 +75,25 @@ def test_fits_vs_reference_program(fits):
 def test_collision_integrals_at_table_nodes():
     """At a tabulated reduced temperature and dipole moment the interpolation
     returns the Monchick-Mason table entry; above T* = 75 (non-polar only)
-    the closed forms take over, and a polar pair beyond it is refused."""
+    the cubics take over continuously, and a polar pair beyond it is
+    refused."""
     from phd_amd import tranfit as tf
-    assert abs(tf.omega(1, tf._TSTERN[9], 0.0) - tf._O11[9][0]) <= 1e-15
-    assert abs(tf.omega(2, tf._TSTERN[19], 0.0) - tf._O22[19][0]) <= 1e-15
-    assert abs(tf.omega(2, tf._TSTERN[14], tf._DELTA[4]) - tf._O22[14][4]) <= 1e-14
+    Om = tf.collision_integral
+    assert abs(Om(1, tf.T_STAR[9], 0.0) - tf.TABLES[1][9, 0]) <= 1e-15
+    assert abs(Om(2, tf.T_STAR[19], 0.0) - tf.TABLES[2][19, 0]) <= 1e-15
+    assert abs(Om(2, tf.T_STAR[14], tf.DELTA_STAR[4]) - tf.TABLES[2][14, 4]) <= 1e-14
+    # between nodes the value lies between its neighbours (monotone columns)
+    for i in (3, 12, 25):
+        x = 0.5 * (tf.T_STAR[i] + tf.T_STAR[i + 1])
+        lo, hi = sorted((tf.TABLES[1][i, 0], tf.TABLES[1][i + 1, 0]))
+        assert lo <= Om(1, x, 0.0) <= hi
     # Omega(2,2)* / Omega(1,1)* = A* lies near 1.1 for the LJ potential
     for tr in (0.5, 1.0, 3.0, 10.0, 50.0):
-        a = tf.omega(2, tr, 0.0) / tf.omega(1, tr, 0.0)
+        a = Om(2, tr, 0.0) / Om(1, tr, 0.0)
         assert 1.0 < a < 1.2
-    assert abs(tf.omega(1, 80.0, 0.0) - tf.omega(1, 75.0, 0.0)) < 0.02
+    assert abs(Om(1, 80.0, 0.0) - Om(1, 75.0, 0.0)) < 0.02
     with pytest.raises(ValueError):
-        tf.omega(1, 100.0, 0.5)
+        Om(1, 100.0, 0.5)
 
 
 def test_fits_round_trip_and_thd(fits, tmp_path):
