@@ -215,6 +215,15 @@ int izem_chem_order_read(izem_ctx *ctx, long cap, long *n, int *perm, int *bin);
  * refreshes W_i/cp/ha, fills ghosts, ghost Newton; DTmax into the time state */
 int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale, double *T, double *W_i,
                         double *cp, double *ha, double *v);
+/* one iteration of the main loop (izem.f90:242-389): timestep, then with
+ * chemistry reaction_selec(dt/2) -> rk3(dt) -> reaction_selec(dt/2)
+ * (:254-277), dt kept in the time state.  use_graph = 1 replays the
+ * iteration from a HIP graph captured at the first call (and again whenever
+ * the arrays or the chemistry settings change): same results, no launch
+ * gaps.  IZEM_E_STATE with use_graph = 1 and the stiffness order on. */
+int izem_step(izem_ctx *ctx, const double *dx_i, const double *dy_i, const double *dz_i,
+              double *T, double *W_i, double *cp, double *ha, uint8_t *sensor, double *v,
+              int use_graph);
 
 /* ---- chemistry surrogate (no reference counterpart) -------------------
  * A BF16 MLP (12 -> 256 -> 256 -> 256 -> 10, GELU) stands in for the

@@ -27,6 +27,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall",
 SOURCES = {
     "flow.hip": ["-ffp-contract=off"],
     "chem.hip": ["-ffp-contract=off"],
+    "step.hip": ["-ffp-contract=off"],
     "mlp.hip": ["-ffp-contract=fast"],
 }
 

@@ -574,6 +574,20 @@ static int order_end(izem_ctx *ctx, int n, const int *bin, int *hist, int *perm)
   return IZEM_OK;
 }
 
+// the chemistry settings a captured step bakes into its kernel arguments;
+// IZEM_E_STATE with the stiffness order on (its buffers may grow, which
+// allocates and synchronises)
+int iz_chem_signature(izem_ctx *ctx, double sig[IZ_CHEM_SIG]) {
+  IzChem *ch = ctx->chem;
+  if (!ch) return IZEM_E_STATE;
+  if (ch->order) return IZEM_E_STATE;
+  const double s[IZ_CHEM_SIG] = {(double)ch->use_gen, (double)ch->waves,
+                                 (double)ch->blocks_per_cu, (double)ch->jac_age, ch->rtol,
+                                 ch->atol, (double)ch->trim, 0.0};
+  for (int i = 0; i < IZ_CHEM_SIG; i++) sig[i] = s[i];
+  return IZEM_OK;
+}
+
 extern "C" int izem_chem_order(izem_ctx *ctx, int on) {
   if (!ctx || !ctx->chem || on < 0 || on > 1) return IZEM_E_ARG;
   ctx->chem->order = on;

@@ -1330,6 +1330,7 @@ extern "C" int izem_destroy(izem_ctx *c) {
   if (!c) return IZEM_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   izem_profile(c, 0);
+  iz_graph_destroy(c);
   iz_chem_destroy(c);
   for (int q = 0; q < 3; q++) if (c->F[q]) (void)hipFree(c->F[q]);
   if (c->prim) (void)hipFree(c->prim);
@@ -1456,12 +1457,14 @@ extern "C" int izem_euler_llf_z(izem_ctx *c, const double *dz_i, const double *T
 
 int iz_comm_cons(izem_ctx *c, double *a, int nf) {
   const IzDims &d = c->d;
+  int pk = iz_prof_begin(c, IZP_OTHER);
   for (int dir = 0; dir < d.ndim; dir++) {
     long n0 = (dir == 0) ? d.ny : d.nx, n1 = (dir == 2) ? d.ny : d.nz;
     long tot = n0 * n1 * 2 * d.ng * nf;
     hipLaunchKernelGGL(k_ghost_fill, dim3(iz_blocks(tot, 256)), dim3(256), 0, c->stream, d, a, nf,
                        dir);
   }
+  iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
 }
@@ -1519,6 +1522,7 @@ extern "C" int izem_force_sumY(izem_ctx *c, int dom, double *v) {
 static int newton_domains(izem_ctx *c, const int *doms, int nd, int forcesum, double *v,
                           double *W_i, double *T, double *cp, double *ha) {
   const IzDims &d = c->d;
+  int pk = iz_prof_begin(c, IZP_NEWTON);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->masks, 7);
   for (int q = 0; q < nd; q++) {
     IzBox b = domain_box(d, doms[q]);
@@ -1531,6 +1535,7 @@ static int newton_domains(izem_ctx *c, const int *doms, int nd, int forcesum, do
                        c->stream, d, b, v, T, W_i, cp, ha, c->masks + (doms[q] + 3), 200,
                        c->status, c->iters + (doms[q] + 3));
   }
+  iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
 }
@@ -1557,8 +1562,10 @@ extern "C" int izem_upd_prim_var_woT(izem_ctx *c, int dom, const double *v, cons
 int iz_upd_prim_woT(izem_ctx *c, int dom, const double *v, const double *T, double *W_i,
                     double *cp, double *ha) {
   IzBox b = domain_box(c->d, dom);
+  int pk = iz_prof_begin(c, IZP_NEWTON);
   hipLaunchKernelGGL(k_upd_prim_woT, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0, c->stream,
                      c->d, b, v, T, W_i, cp, ha);
+  iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
 }
@@ -1577,8 +1584,10 @@ extern "C" int izem_upd_prim_var_ghost(izem_ctx *c, double *T, double *W_i, doub
 extern "C" int izem_shock_sensor(izem_ctx *c, const double *v, const double *T,
                                  const double *W_i, uint8_t *sensor) {
   if (!c || !v || !T || !W_i || !sensor) return IZEM_E_ARG;
+  int pk = iz_prof_begin(c, IZP_OTHER);
   hipLaunchKernelGGL(k_sensor, dim3(iz_blocks(n_interior(c->d), 256)), dim3(256), 0, c->stream,
                      c->d, c->o.max_rel_weight, v, T, W_i, sensor);
+  iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
 }
@@ -1623,7 +1632,9 @@ extern "C" int izem_rk3(izem_ctx *c, const double *dt, const double *dx_i, const
   static const double ck3[3] = {1.00, 0.25, 2.00 / 3.00};
   const IzDims &d = c->d;
   long ntot = (long)IZ_NV * d.nc;
+  int pc = iz_prof_begin(c, IZP_OTHER);
   hipLaunchKernelGGL(k_copy, dim3(iz_blocks(ntot, 256)), dim3(256), 0, c->stream, v, c->v0, ntot);
+  iz_prof_end(c, pc);
   for (int ik = 0; ik < 3; ik++) {
     int r = stage_fluxes(c, dx_i, dy_i, dz_i, T, W_i, cp, ha, sensor, v);
     if (r) return r;
@@ -1655,6 +1666,7 @@ extern "C" int izem_timestep(izem_ctx *c, const double *dx_i, const double *dy_i
   if (!c || !dx_i || !dy_i || !dz_i || !cp || !W_i || !T || !v) return IZEM_E_ARG;
   const IzDims &d = c->d;
   unsigned nb = iz_blocks_capped(c, n_interior(d), 256, 16);
+  int pk = iz_prof_begin(c, IZP_OTHER);
   hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(64), 0, c->stream, c->red);
   hipLaunchKernelGGL(k_dt_conv, dim3(nb), dim3(256), 0, c->stream, d, v, T, W_i, cp, dx_i, dy_i,
                      dz_i, c->red);
@@ -1666,6 +1678,7 @@ extern "C" int izem_timestep(izem_ctx *c, const double *dx_i, const double *dy_i
   double rvd0_i = c->o.Sc / c->o.sqgmr, rlamcp0_i = c->o.Pr / c->o.sqgmr;
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, c->stream, c->red, c->o.CFL, c->o.CFLmin,
                      c->o.Fo, c->o.CPR, visdt, c->o.reaction, rvd0_i, rlamcp0_i, c->ts);
+  iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
 }

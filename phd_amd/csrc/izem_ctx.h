@@ -32,6 +32,17 @@ struct IzProf {
   int *cls;
 };
 
+// a captured main-loop iteration (izem_step) and what it was captured with
+#define IZ_GRAPH_PTRS 9
+#define IZ_CHEM_SIG 8
+struct IzGraph {
+  hipGraph_t g;
+  hipGraphExec_t exec;
+  const void *ptr[IZ_GRAPH_PTRS];
+  int react;
+  double chem[IZ_CHEM_SIG];
+};
+
 struct izem_ctx {
   IzDims d;
   izem_opts o;
@@ -53,6 +64,7 @@ struct izem_ctx {
   double *ts;   // dt_4[4], dtmin, DTmax
   IzChem *chem;
   IzProf prof;
+  IzGraph graph;
   int ncu;  // compute units of the device (grid caps of reducing kernels)
 };
 
@@ -84,3 +96,4 @@ int iz_comm_cons(izem_ctx *ctx, double *a, int nf);
 int iz_upd_prim_woT(izem_ctx *ctx, int domain, const double *v, const double *T, double *W_i,
                     double *cp, double *ha);
 void iz_chem_destroy(izem_ctx *ctx);
+void iz_graph_destroy(izem_ctx *ctx);

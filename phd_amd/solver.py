@@ -188,18 +188,13 @@ class Solver:
                                                 _ptr(self.T), _ptr(self.W_i), _ptr(self.cp),
                                                 _ptr(self.ha), _ptr(self.v)), "reaction_dvode")
 
-    def step(self):
+    def step(self, graph: bool = False):
         """timestep -> reaction(dt/2) -> rk3(dt) -> reaction(dt/2)
-        (izem.f90:254-277), with dt kept on the device."""
-        self.timestep()
-        dtp = self.lib.izem_time_state(self.ctx) + 4 * 8
-        if self.opts.reaction and self.mech is not None:
-            self.reaction_selec(dtp, 0.5)
-        _lib.check(self.lib.izem_rk3(self.ctx, C.c_void_p(dtp), *[_ptr(x) for x in self.d_i],
-                                     _ptr(self.T), _ptr(self.W_i), _ptr(self.cp), _ptr(self.ha),
-                                     _ptr(self.sensor), _ptr(self.v)), "rk3")
-        if self.opts.reaction and self.mech is not None:
-            self.reaction_selec(dtp, 0.5)
+        (izem.f90:254-277), with dt kept on the device (izem_step); graph
+        replays the iteration from a captured HIP graph."""
+        _lib.check(self.lib.izem_step(self.ctx, *[_ptr(x) for x in self.d_i], _ptr(self.T),
+                                      _ptr(self.W_i), _ptr(self.cp), _ptr(self.ha),
+                                      _ptr(self.sensor), _ptr(self.v), int(graph)), "izem_step")
 
 
 def mech_flat(mech: cd.Mechanism) -> np.ndarray:
