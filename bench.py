@@ -199,6 +199,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--chem-cells", type=int, default=1 << 22)
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--flux-mode", type=int, default=1,
+                    help="Euler face kernels: 0 reference operation order (bit parity), "
+                         "1 structured projections (within 1e-12 of the reference)")
     ap.add_argument("--chem-tune", type=str, default=None,
                     help="waves,blocks_per_cu,jac_age,kinetics for the chemistry kernels "
                          "(default: autotuned on a sample of the reacting cells)")
@@ -223,7 +226,8 @@ def main():
     log(f"building the {n}^3 reacting HIT state")
     box, v, T = ics.config2_state(adi, thd, n=n, seed=2)
     tune = tuple(int(x) for x in args.chem_tune.split(",")) if args.chem_tune else None
-    s = Solver(box, Options(chem_tune=tune), adi=adi, thd=thd, mech=mech)
+    s = Solver(box, Options(chem_tune=tune, flux_mode=args.flux_mode), adi=adi, thd=thd,
+               mech=mech)
     s.load_state(v, T)
     del v, T
     tuned = None
@@ -339,6 +343,7 @@ def main():
                                f"WENO5-JS LLF + viscous + mixture-averaged diffusion, SSP-RK3 "
                                f"with Strang BDF chemistry (rtol 1e-6, atol 1e-12)",
                    "cells": ncell, "dt_s": float(ts[4] * adi.time_ref),
+                   "flux_mode": args.flux_mode,
                    "status_bits": status},
         "roofline": dominant,
         "roofline_kernels": {"k_reaction_queue": chem_roof, "k_euler_faces": euler_roof},

@@ -221,6 +221,12 @@ int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale, double *T
  * iteration from a HIP graph captured at the first call (and again whenever
  * the arrays or the chemistry settings change): same results, no launch
  * gaps.  IZEM_E_STATE with use_graph = 1 and the stiffness order on. */
+/* operation order of the Euler face kernels: 0 (the default) the
+ * reference's dense characteristic projections in its summation order (bit
+ * parity with it); 1 the projections through the structure of the left
+ * eigenvectors (a shared scalar per stencil point; a tenth of the work,
+ * results within rounding of mode 0).  No counterpart in the reference. */
+int izem_flux_mode(izem_ctx *ctx, int mode);
 int izem_step(izem_ctx *ctx, const double *dx_i, const double *dy_i, const double *dz_i,
               double *T, double *W_i, double *cp, double *ha, uint8_t *sensor, double *v,
               int use_graph);

@@ -18,7 +18,7 @@ EXPORTS = [
     "izem_wenorec", "izem_euler_llf_x", "izem_euler_llf_y", "izem_euler_llf_z",
     "izem_viscflux", "izem_force_sumY", "izem_prim_inv_var", "izem_upd_prim_var_woT",
     "izem_comm_cons", "izem_upd_prim_var_ghost", "izem_shock_sensor", "izem_rk3", "izem_rhs",
-    "izem_timestep", "izem_chem_setup", "izem_chem_tune", "izem_chem_stats", "izem_ckwyr", "izem_psr_batch", "izem_psr_batch_noalloc", "izem_chem_order", "izem_chem_order_read", "izem_reaction_dvode", "izem_step",
+    "izem_timestep", "izem_chem_setup", "izem_chem_tune", "izem_chem_stats", "izem_ckwyr", "izem_psr_batch", "izem_psr_batch_noalloc", "izem_chem_order", "izem_chem_order_read", "izem_reaction_dvode", "izem_step", "izem_flux_mode",
     "izem_mlp_gemm", "izem_mlp_gemm_splitk", "izem_mlp_colsum", "izem_mlp_adam", "izem_mlp_to_bf16",
     "izem_mlp_mse", "izem_mlp_infer", "izem_surr_features", "izem_surr_apply", "izem_surr_fallback",
 ]
@@ -103,6 +103,7 @@ def load(path: str | None = None):
     lib.izem_chem_order_read.argtypes = [vp, C.c_long, C.POINTER(C.c_long), vp, vp]
     lib.izem_reaction_dvode.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp]
     lib.izem_step.argtypes = [vp] + [vp] * 9 + [C.c_int]
+    lib.izem_flux_mode.argtypes = [vp, C.c_int]
     _lib = lib
     return lib
 

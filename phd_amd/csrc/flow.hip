@@ -64,7 +64,16 @@ __device__ inline void eu_stage_cell(const double *__restrict__ v, const double 
 // F^ of one face from its six staged stencil cells: Q(q, st) is staged
 // quantity q of stencil point st (st = 2 is i_l, st = 3 is i_r); cL, cR are
 // the HBM indices of i_l, i_r (for h_k); F^ goes to F[m * nfaces + fidx].
-template <int DIR, int QS, int SS>
+// FAST = false keeps the reference's dense projections (w = el . U, g =
+// el . F over all 14 columns, in its summation order: bit parity with the
+// oracle).  FAST = true uses the structure of el: with
+//   S = -b1 U_0 + b2 (u . m) - b2 E - b2 sum_l psi_l U_l   (and the same of F)
+// every row but the tangential ones is S-plus-few-terms: row 1 is U_0 + S,
+// rows 0 and 2 are -S/2 -+ (u_n U_0 - U_n) / (2 c), row 5 is
+// -(1 + xi) / b2 S - sum_l psi_l U_l, and species row 6 + s is U_{5+s} + Y_s S.
+// The characteristic projections then cost about a tenth of the dense ones;
+// results differ from the dense order by rounding only.
+template <int DIR, int QS, int SS, bool FAST>
 __device__ inline void eu_face(const double *tile, int off, bool js,
                                const double *__restrict__ ha, long nc, long cL, long cR,
                                double *__restrict__ F, long nfaces, long fidx) {
@@ -133,6 +142,33 @@ __device__ inline void eu_face(const double *tile, int off, bool js,
   const double un_c = uc[cn - 1];
   const double hb2 = 0.5 * b2;
 
+  // FAST: per stencil point, S of U and of F, sum_l psi_l U_l and its F
+  // counterpart, and u_n U_0 - U_n with its F counterpart
+  double sw[6], sg[6], pw[6], pg[6], dw[6], dg[6];
+  if constexpr (FAST) {
+#pragma unroll
+    for (int st = 0; st < 6; st++) {
+      const double u = Q(14, st), p = Q(15, st);
+      const double u0 = Q(0, st), un = Q(cn, st), u4 = Q(4, st);
+      double aw = -b1 * u0 - b2 * u4, ag = -b1 * un - b2 * (u * (u4 + p));
+#pragma unroll
+      for (int c = 1; c <= 3; c++) {
+        const double vc = Q(c, st);
+        aw = aw + b2 * uc[c - 1] * vc;
+        ag = ag + b2 * uc[c - 1] * ((c == cn) ? vc * u + p : vc * u);
+      }
+      double ps = 0.0;
+#pragma unroll
+      for (int l = 0; l < IZ_NRV; l++) ps = ps + psi[l] * Q(IZ_NIV + l, st);
+      pw[st] = ps;
+      pg[st] = ps * u;
+      sw[st] = aw - b2 * ps;
+      sg[st] = ag - b2 * pg[st];
+      dw[st] = un_c * u0 - un;
+      dg[st] = un_c * un - (un * u + p);
+    }
+  }
+
   double ghat[IZ_NV];
 #pragma unroll 1
   for (int m = 0; m < IZ_NV; m++) {
@@ -145,7 +181,9 @@ __device__ inline void eu_face(const double *tile, int off, bool js,
     // row m of the left eigenvector matrix (module_weno.f90:775-874)
     double e[IZ_NV];
     double lam;
-    if (m == 0) {
+    if (FAST) {
+      lam = (m == 0) ? lmax0 : (m == 2 ? lmax2 : lmax1);
+    } else if (m == 0) {
       e[0] = 0.5 * (b1 + un_c * cs_c_i);
 #pragma unroll
       for (int c = 1; c <= 3; c++)
@@ -199,7 +237,27 @@ __device__ inline void eu_face(const double *tile, int off, bool js,
     for (int st = 0; st < 6; st++) {
       const double u = Qm(14, st), p = Qm(15, st);
       double wc, gc;
-      if (m == 3 || m == 4) {
+      if (FAST && m != 3 && m != 4) {
+        const double hc = 0.5 * cs_c_i;
+        if (m == 0) {
+          wc = -0.5 * sw[st] + hc * dw[st];
+          gc = -0.5 * sg[st] + hc * dg[st];
+        } else if (m == 1) {
+          wc = Qm(0, st) + sw[st];
+          gc = Qm(cn, st) + sg[st];
+        } else if (m == 2) {
+          wc = -0.5 * sw[st] - hc * dw[st];
+          gc = -0.5 * sg[st] - hc * dg[st];
+        } else if (m == 5) {
+          const double k = (1.0 + xi) / b2;
+          wc = -k * sw[st] - pw[st];
+          gc = -k * sg[st] - pg[st];
+        } else {
+          const double y = Yc[m - 6], vs = Qm(IZ_NIV + m - 6, st);
+          wc = vs + y * sw[st];
+          gc = vs * u + y * sg[st];
+        }
+      } else if (m == 3 || m == 4) {
         const int tc = (m == 3) ? t1 : t2;
         const double et0 = -uc[tc - 1];
         const double v0 = Qm(0, st), vt = Qm(tc, st);
@@ -289,7 +347,7 @@ static inline long eu_ntiles(const IzDims &d, int dir) {
   return ntf * ntx * (dir == 1 ? d.nz : d.ny);
 }
 
-template <int DIR>
+template <int DIR, bool FAST>
 __global__ void __attribute__((amdgpu_flat_work_group_size(IZ_EU_BLOCK, IZ_EU_BLOCK),
                                amdgpu_waves_per_eu(IZ_EULER_WAVES, IZ_EULER_WAVES)))
 k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
@@ -334,7 +392,7 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
       int shk = 0;
 #pragma unroll
       for (int st = 0; st < 6; st++) shk |= tsn[o + st];
-      eu_face<0, 1, IZ_EU_NQ>(lds, (wave * tile0 + o) * IZ_EU_NQ, shk && weno_avg, ha, d.nc,
+      eu_face<0, 1, IZ_EU_NQ, FAST>(lds, (wave * tile0 + o) * IZ_EU_NQ, shk && weno_avg, ha, d.nc,
                               cL, cL + 1, F, nfaces, (long)f);
     }
   } else {
@@ -380,7 +438,7 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
         cR = cL + d.NX * d.NY;
         fidx = x + (long)d.nx * (oth + (long)d.ny * (f + 1));
       }
-      eu_face<DIR, FS, IZ_EU_TX>(lds, fl * IZ_EU_TX + xl, shk && weno_avg, ha, d.nc, cL, cR, F,
+      eu_face<DIR, FS, IZ_EU_TX, FAST>(lds, fl * IZ_EU_TX + xl, shk && weno_avg, ha, d.nc, cL, cR, F,
                                  nfaces, fidx);
     }
   }
@@ -1183,15 +1241,30 @@ __global__ void k_wenorec_batch(int n, int js, const double *__restrict__ vp,
 }
 
 // explicit instantiations used by the API
-template __global__ void k_euler_faces<0>(IzDims, const double *, const double *, const double *,
-                                          const double *, const double *, const uint8_t *, int,
-                                          double *, long, int *, int, long);
-template __global__ void k_euler_faces<1>(IzDims, const double *, const double *, const double *,
-                                          const double *, const double *, const uint8_t *, int,
-                                          double *, long, int *, int, long);
-template __global__ void k_euler_faces<2>(IzDims, const double *, const double *, const double *,
-                                          const double *, const double *, const uint8_t *, int,
-                                          double *, long, int *, int, long);
+template __global__ void k_euler_faces<0, false>(IzDims, const double *, const double *,
+                                                 const double *, const double *, const double *,
+                                                 const uint8_t *, int, double *, long, int *, int,
+                                                 long);
+template __global__ void k_euler_faces<0, true>(IzDims, const double *, const double *,
+                                                 const double *, const double *, const double *,
+                                                 const uint8_t *, int, double *, long, int *, int,
+                                                 long);
+template __global__ void k_euler_faces<1, false>(IzDims, const double *, const double *,
+                                                 const double *, const double *, const double *,
+                                                 const uint8_t *, int, double *, long, int *, int,
+                                                 long);
+template __global__ void k_euler_faces<1, true>(IzDims, const double *, const double *,
+                                                 const double *, const double *, const double *,
+                                                 const uint8_t *, int, double *, long, int *, int,
+                                                 long);
+template __global__ void k_euler_faces<2, false>(IzDims, const double *, const double *,
+                                                 const double *, const double *, const double *,
+                                                 const uint8_t *, int, double *, long, int *, int,
+                                                 long);
+template __global__ void k_euler_faces<2, true>(IzDims, const double *, const double *,
+                                                 const double *, const double *, const double *,
+                                                 const uint8_t *, int, double *, long, int *, int,
+                                                 long);
 template __global__ void k_euler_apply<0>(IzDims, const double *, long, const double *, int, double *);
 template __global__ void k_euler_apply<1>(IzDims, const double *, long, const double *, int, double *);
 template __global__ void k_euler_apply<2>(IzDims, const double *, long, const double *, int, double *);
@@ -1347,6 +1420,12 @@ extern "C" int izem_destroy(izem_ctx *c) {
   return IZEM_OK;
 }
 
+extern "C" int izem_flux_mode(izem_ctx *c, int mode) {
+  if (!c || mode < 0 || mode > 1) return IZEM_E_ARG;
+  c->flux_mode = mode;
+  return IZEM_OK;
+}
+
 extern "C" int izem_synchronize(izem_ctx *c) {
   if (!c) return IZEM_E_ARG;
   IZ_CHECK(hipStreamSynchronize(c->stream));
@@ -1410,15 +1489,19 @@ static int euler_dir(izem_ctx *c, int dir, const double *d_i, const double *T,
   // multiple of the 8 XCDs (blocks past the last tile only pass the barrier)
   const unsigned nb = dir == 0 ? iz_blocks(nf, IZ_EU_BLOCK) : (unsigned)(((ntiles + 7) / 8) * 8);
   int pk = iz_prof_begin(c, IZP_EULER);
-  if (dir == 0)
-    hipLaunchKernelGGL(k_euler_faces<0>, dim3(nb), dim3(IZ_EU_BLOCK), lds, c->stream, c->d, v,
-                       T, W_i, cp, ha, sensor, wa, c->F[0], nf, c->status, tile0, ntiles);
-  else if (dir == 1)
-    hipLaunchKernelGGL(k_euler_faces<1>, dim3(nb), dim3(IZ_EU_BLOCK), lds, c->stream, c->d, v,
-                       T, W_i, cp, ha, sensor, wa, c->F[1], nf, c->status, tile0, ntiles);
-  else
-    hipLaunchKernelGGL(k_euler_faces<2>, dim3(nb), dim3(IZ_EU_BLOCK), lds, c->stream, c->d, v,
-                       T, W_i, cp, ha, sensor, wa, c->F[2], nf, c->status, tile0, ntiles);
+#define IZ_EU(D, FAST)                                                                     \
+  hipLaunchKernelGGL((k_euler_faces<D, FAST>), dim3(nb), dim3(IZ_EU_BLOCK), lds, c->stream, \
+                     c->d, v, T, W_i, cp, ha, sensor, wa, c->F[D], nf, c->status, tile0, ntiles)
+  if (c->flux_mode) {
+    if (dir == 0) IZ_EU(0, true);
+    else if (dir == 1) IZ_EU(1, true);
+    else IZ_EU(2, true);
+  } else {
+    if (dir == 0) IZ_EU(0, false);
+    else if (dir == 1) IZ_EU(1, false);
+    else IZ_EU(2, false);
+  }
+#undef IZ_EU
   iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
   if (apply) {

@@ -39,7 +39,7 @@ struct IzGraph {
   hipGraph_t g;
   hipGraphExec_t exec;
   const void *ptr[IZ_GRAPH_PTRS];
-  int react;
+  int react, flux_mode;
   double chem[IZ_CHEM_SIG];
 };
 
@@ -65,6 +65,7 @@ struct izem_ctx {
   IzChem *chem;
   IzProf prof;
   IzGraph graph;
+  int flux_mode;  // 0: reference operation order (bit parity), 1: structured projections
   int ncu;  // compute units of the device (grid caps of reducing kernels)
 };
 

@@ -48,13 +48,14 @@ extern "C" int izem_step(izem_ctx *c, const double *dx_i, const double *dy_i, co
   const void *ptrs[IZ_GRAPH_PTRS] = {dx_i, dy_i, dz_i, T, W_i, cp, ha, sensor, v};
   memcpy(key.ptr, ptrs, sizeof ptrs);
   key.react = react;
+  key.flux_mode = c->flux_mode;
   if (react) {
     int r = iz_chem_signature(c, key.chem);
     if (r) return r;  // e.g. the stiffness order, whose buffers may grow
   }
   IzGraph &g = c->graph;
   const bool same = g.exec && !memcmp(key.ptr, g.ptr, sizeof key.ptr) && key.react == g.react &&
-                    !memcmp(key.chem, g.chem, sizeof key.chem);
+                    key.flux_mode == g.flux_mode && !memcmp(key.chem, g.chem, sizeof key.chem);
   if (!same) {
     iz_graph_destroy(c);
     // capture on the context's own stream (never the null stream), with the

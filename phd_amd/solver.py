@@ -45,6 +45,9 @@ class Options:
     rtol: float = 1.0e-6
     atol: float = 1.0e-12
     trim_box: int = 1   # reaction_dvode skips ng+1 planes at global faces
+    # Euler face kernels: 0 the reference's operation order (bit parity),
+    # 1 projections through the eigenvectors' structure (izem_flux_mode)
+    flux_mode: int = 0
     # (waves, blocks_per_cu, jac_age, kinetics) for izem_chem_tune; None keeps
     # the library defaults
     chem_tune: tuple | None = None
@@ -79,6 +82,7 @@ class Solver:
         _lib.check(self.lib.izem_create(C.byref(ctx), C.byref(g), flat.ctypes.data, flat.size,
                                         C.byref(op)), "izem_create")
         self.ctx = ctx
+        _lib.check(self.lib.izem_flux_mode(ctx, o.flux_mode), "izem_flux_mode")
         # run on torch's current stream so that library and torch work is ordered
         _lib.check(self.lib.izem_set_stream(ctx, C.c_void_p(
             torch.cuda.current_stream(self.dev).cuda_stream)), "izem_set_stream")

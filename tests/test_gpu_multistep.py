@@ -63,14 +63,16 @@ def _oracle_rk3(run, opts, dt):
     assert st == 0
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("kind,n", [("2d", 32), ("3d", 16)])
-def test_flow_multistep_parity(oracle, adi, thd, kind, n):
+def test_flow_multistep_parity(oracle, adi, thd, kind, n, mode):
     from phd_amd.solver import Options, Solver
     steps = 100
     box, v, T = small_state(adi, thd, kind, n)
     run = OracleRun(oracle, adi, thd, box, v, T)
     run.prim_all()
     s = Solver(box, Options(reaction=0), adi=adi, thd=thd)
+    assert s.lib.izem_flux_mode(s.ctx, mode) == 0
     _upload(s, run)
     s.set_time_state([1.0, 1.0, 1.0, 1.0, 1.0, 0.0])
     opts = run.opts(vis=1)
@@ -87,7 +89,7 @@ def test_flow_multistep_parity(oracle, adi, thd, kind, n):
             e = _relerr(s.v.cpu().numpy(), run.v, box)
             I = box.interior()
             eT = np.max(np.abs(s.T.cpu().numpy()[I] - run.T[I]) / run.T[I])
-            print(f"{kind} {n}: step {it}: state max rel err {e.max():.2e}, T {eT:.2e}, "
+            print(f"{kind} {n} mode {mode}: step {it}: state max rel err {e.max():.2e}, T {eT:.2e}, "
                   f"dt {worst_dt:.2e}")
             assert e.max() <= TOL and eT <= TOL
             assert np.array_equal(s.sensor.cpu().numpy(), run.sensor)
