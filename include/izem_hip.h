@@ -57,6 +57,10 @@ typedef struct {
   double CFL, CFLmin, Fo, CPR;
   /* nondimensional constants (adi_type, module_tools.f90:453-562) */
   double sqgmr, Sc, Pr, ggmopr, time_ref, T_ref, rho_ref, L_ref;
+  /* WENO%SHOCK_SENSOR_TYPE: 0 ADAMSSHARIFF, 1 DUCROS (module_solver.f90:758-762),
+   * and WENO%DUCROS_TREDUC */
+  int sensor_type;
+  double ducros_treduc;
 } izem_opts;
 
 /* ---- context ---------------------------------------------------------- */
@@ -130,6 +134,13 @@ int izem_upd_prim_var_ghost(izem_ctx *ctx, double *T, double *W_i, double *cp, d
 /* shock_sensor_modified_adams_shariff (module_tools.f90:2733), sticky */
 int izem_shock_sensor(izem_ctx *ctx, const double *v, const double *T, const double *W_i,
                       uint8_t *sensor);
+/* shock_sensor_ducros with compute_duixj (module_tools.f90:2665-2719,
+ * 2521-2651): flags an interior cell (sticky) where (div u)^2 / ((div u)^2 +
+ * |curl u|^2 + 1e-30) > ducros_treduc; velocity gradients by the sixth-order
+ * central stencil (also in z, where the reference's eighth-order operator
+ * reads beyond an ng = 3 halo).  Selected in rk3 by sensor_type = 1. */
+int izem_shock_sensor_ducros(izem_ctx *ctx, const double *dx_i, const double *dy_i,
+                             const double *dz_i, const double *v, uint8_t *sensor);
 
 /* ---- module_solver ---------------------------------------------------- */
 /* rk3 (module_solver.f90:694): one SSP-RK3 step with the time step read

@@ -647,6 +647,62 @@ void ora_sensor_adams_shariff(const ora_grid *g, double max_rel_weight, const do
   }
 }
 
+/* shock_sensor_ducros: src/module_tools.f90:2665-2719, with the velocity
+ * gradients of compute_duixj (:2521-2651: velocities on the interior and
+ * the face ghost slabs, then d?_fixed1).  z uses alpha6 as x and y do (the
+ * reference's z operator is the 8-point alph8, which reads k+-4 of an
+ * ng = 3 halo).  Sticky: flags are only ever set. */
+static inline double deriv7(const double *a, long c, long stride, double dinv);
+
+void ora_sensor_ducros(const ora_grid *g, double treduc, const double *dx_i, const double *dy_i,
+                       const double *dz_i, const double *v, uint8_t *sensor) {
+  const long nc = ncell(g);
+  const int ng = g->ng, nd = g->ndim;
+  double *vel = calloc((size_t)3 * nc, sizeof(double));
+  int i0, i1, j0, j1, k0, k1;
+  for (int d = -nd; d <= nd; d++) {
+    domain_select(g, d, &i0, &i1, &j0, &j1, &k0, &k1);
+    for (int k = k0; k <= k1; k++)
+      for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+          long c = cidx(g, i, j, k);
+          double rho_i = 1.0 / v[c];
+          for (int q = 0; q < nd; q++) vel[q * nc + c] = v[(1 + q) * nc + c] * rho_i;
+        }
+  }
+  const long strd[3] = {1, NXg(g), NXg(g) * NYg(g)};
+  for (int k = 0; k < g->nz; k++)
+    for (int j = 0; j < g->ny; j++)
+      for (int i = 0; i < g->nx; i++) {
+        long c = cidx(g, i, j, k);
+        const double dinv[3] = {dx_i[i + ng], dy_i[j + ng], dz_i[k + ng]};
+        double du[3][3]; /* du[a][b] = d u_a / d x_b: duixj(3 a + b + 1) */
+        for (int a = 0; a < nd; a++)
+          for (int b = 0; b < nd; b++) du[a][b] = deriv7(vel + a * nc, c, strd[b], dinv[b]);
+        double div, vort, wrk;
+        if (nd == 2) {
+          div = du[0][0] + du[1][1];
+          div = div * div;
+          wrk = du[1][0] - du[0][1];
+          vort = wrk * wrk;
+        } else {
+          div = du[0][0] + du[1][1] + du[2][2];
+          div = div * div;
+          wrk = du[2][1] - du[1][2];
+          vort = wrk * wrk;
+          wrk = du[0][2] - du[2][0];
+          wrk = wrk * wrk;
+          vort = vort + wrk;
+          wrk = du[1][0] - du[0][1];
+          wrk = wrk * wrk;
+          vort = vort + wrk;
+        }
+        double ducloc = div / (div + vort + 1.0e-30);
+        if (ducloc > treduc) sensor[c] = 1;
+      }
+  free(vel);
+}
+
 /* --------------------------------------------------------- transport */
 
 /* prim_vis_vars for one cell: src/module_thermodynamics.f90:989-1230 */
@@ -925,7 +981,9 @@ int ora_rk3(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const ora
   memcpy(v0, v, sizeof(double) * NV * nc);
   int status = 0;
   for (int ik = 0; ik < 3; ik++) {
-    ora_sensor_adams_shariff(g, op->max_rel_weight, v, T, W_i, sensor);
+    /* module_solver.f90:758-762 */
+    if (op->sensor_type == 1) ora_sensor_ducros(g, op->ducros_treduc, dx_i, dy_i, dz_i, v, sensor);
+    else ora_sensor_adams_shariff(g, op->max_rel_weight, v, T, W_i, sensor);
     int s;
     const double *dd[3] = {dx_i, dy_i, dz_i};
     for (int d = 0; d < g->ndim; d++) {

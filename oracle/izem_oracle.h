@@ -58,6 +58,8 @@ typedef struct {
   double max_rel_weight; /* WENO%PERCENT_WEIGHT */
   int vis, vis_dt, reaction, forcesum;
   double CFL, CFLmin, Fo, CPR;
+  int sensor_type;      /* WENO%SHOCK_SENSOR_TYPE: 0 ADAMSSHARIFF, 1 DUCROS */
+  double ducros_treduc; /* WENO%DUCROS_TREDUC */
 } ora_opts;
 
 /* CHEMKIN-II mechanism tables (the content of the linking file, see
@@ -117,6 +119,8 @@ int ora_upd_prim_var_ghost(const ora_grid *g, const ora_thd *thd, int forcesum, 
                            double *W_i, double *T, double *cp, double *ha);
 void ora_sensor_adams_shariff(const ora_grid *g, double max_rel_weight, const double *v,
                               const double *T, const double *W_i, uint8_t *sensor);
+void ora_sensor_ducros(const ora_grid *g, double treduc, const double *dx_i, const double *dy_i,
+                       const double *dz_i, const double *v, uint8_t *sensor);
 void ora_transport_cell(const ora_thd *thd, double rho, double T, double W_i, const double *Xa,
                         double *dm, double *mu, double *ct);
 int ora_viscflux(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const double *dx_i,

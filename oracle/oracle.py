@@ -51,7 +51,7 @@ class Opts(C.Structure):
     _fields_ = [("weno_avg", C.c_int), ("max_rel_weight", C.c_double), ("vis", C.c_int),
                 ("vis_dt", C.c_int), ("reaction", C.c_int), ("forcesum", C.c_int),
                 ("CFL", C.c_double), ("CFLmin", C.c_double), ("Fo", C.c_double),
-                ("CPR", C.c_double)]
+                ("CPR", C.c_double), ("sensor_type", C.c_int), ("ducros_treduc", C.c_double)]
 
 
 class Mech(C.Structure):

@@ -17,7 +17,7 @@ EXPORTS = [
     "izem_profile_read",
     "izem_wenorec", "izem_euler_llf_x", "izem_euler_llf_y", "izem_euler_llf_z",
     "izem_viscflux", "izem_force_sumY", "izem_prim_inv_var", "izem_upd_prim_var_woT",
-    "izem_comm_cons", "izem_upd_prim_var_ghost", "izem_shock_sensor", "izem_rk3", "izem_rhs",
+    "izem_comm_cons", "izem_upd_prim_var_ghost", "izem_shock_sensor", "izem_shock_sensor_ducros", "izem_rk3", "izem_rhs",
     "izem_timestep", "izem_chem_setup", "izem_chem_tune", "izem_chem_stats", "izem_ckwyr", "izem_psr_batch", "izem_psr_batch_noalloc", "izem_chem_order", "izem_chem_order_read", "izem_reaction_dvode", "izem_step", "izem_flux_mode",
     "izem_mlp_gemm", "izem_mlp_gemm_splitk", "izem_mlp_colsum", "izem_mlp_adam", "izem_mlp_to_bf16",
     "izem_mlp_mse", "izem_mlp_infer", "izem_surr_features", "izem_surr_apply", "izem_surr_fallback",
@@ -35,7 +35,8 @@ class Opts(C.Structure):
                 ("CFL", C.c_double), ("CFLmin", C.c_double), ("Fo", C.c_double),
                 ("CPR", C.c_double), ("sqgmr", C.c_double), ("Sc", C.c_double),
                 ("Pr", C.c_double), ("ggmopr", C.c_double), ("time_ref", C.c_double),
-                ("T_ref", C.c_double), ("rho_ref", C.c_double), ("L_ref", C.c_double)]
+                ("T_ref", C.c_double), ("rho_ref", C.c_double), ("L_ref", C.c_double),
+                ("sensor_type", C.c_int), ("ducros_treduc", C.c_double)]
 
 
 class IzemError(RuntimeError):
@@ -79,6 +80,7 @@ def load(path: str | None = None):
     lib.izem_comm_cons.argtypes = [vp, vp]
     lib.izem_upd_prim_var_ghost.argtypes = [vp] * 6
     lib.izem_shock_sensor.argtypes = [vp] * 5
+    lib.izem_shock_sensor_ducros.argtypes = [vp] * 6
     lib.izem_rk3.argtypes = [vp] * 11
     lib.izem_rhs.argtypes = [vp] * 11
     lib.izem_timestep.argtypes = [vp] * 8

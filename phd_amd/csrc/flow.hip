@@ -733,6 +733,67 @@ __global__ void k_sensor(IzDims d, double max_rel_weight, const double *__restri
   if (flag) sensor[c] = 1;
 }
 
+// shock_sensor_ducros (sticky) with compute_duixj's velocity gradients:
+// module_tools.f90:2665-2719, 2521-2651.  The gradients are d?_fixed1's
+// sixth-order central differences (alpha6, in z as well: the reference's z
+// operator reads k +- 4 of an ng = 3 halo); velocities m / rho in the
+// reference's form m * (1 / rho).
+__global__ void k_sensor_ducros(IzDims d, double treduc, const double *__restrict__ dx_i,
+                                const double *__restrict__ dy_i, const double *__restrict__ dz_i,
+                                const double *__restrict__ v, uint8_t *__restrict__ sensor) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ncin = (long)d.nx * d.ny * d.nz;
+  if (tid >= ncin) return;
+  int i = (int)(tid % d.nx);
+  long r = tid / d.nx;
+  int j = (int)(r % d.ny), k = (int)(r / d.ny);
+  const long c = iz_cidx(d, i, j, k);
+  const long st[3] = {1, d.NX, (long)d.NX * d.NY};
+  const double dinv[3] = {dx_i[i + d.ng], dy_i[j + d.ng], dz_i[k + d.ng]};
+  // g[a][b] = d u_a / d x_b
+  double g[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      if (a >= d.ndim || b >= d.ndim) { g[a][b] = 0.0; continue; }
+      const long s = st[b];
+      auto u = [&](int l) {
+        const long cc = c + l * s;
+        const double rho_i = 1.0 / v[cc];
+        return v[(long)(1 + a) * d.nc + cc] * rho_i;
+      };
+      double dd = 0.0;
+      dd = dd + (-1.0 / 60.0) * u(-3);
+      dd = dd + (9.0 / 60.0) * u(-2);
+      dd = dd + (-45.0 / 60.0) * u(-1);
+      dd = dd + (45.0 / 60.0) * u(1);
+      dd = dd + (-9.0 / 60.0) * u(2);
+      dd = dd + (1.0 / 60.0) * u(3);
+      g[a][b] = dd * dinv[b];
+    }
+  double div, vort, wrk;
+  if (d.ndim == 2) {
+    div = g[0][0] + g[1][1];
+    div = div * div;
+    wrk = g[1][0] - g[0][1];
+    vort = wrk * wrk;
+  } else {
+    div = g[0][0] + g[1][1] + g[2][2];
+    div = div * div;
+    wrk = g[2][1] - g[1][2];
+    vort = wrk * wrk;
+    wrk = g[0][2] - g[2][0];
+    wrk = wrk * wrk;
+    vort = vort + wrk;
+    wrk = g[1][0] - g[0][1];
+    wrk = wrk * wrk;
+    vort = vort + wrk;
+  }
+  const double ducloc = div / (div + vort + 1.0e-30);
+  if (ducloc > treduc) sensor[c] = 1;
+}
+
 // ===================================================================
 // Viscous and species-diffusion fluxes: viscflux_xy / viscflux_xyz
 // (module_viscflux.f90:721-1135, 1660-2189), periodic faces.
@@ -1675,11 +1736,24 @@ extern "C" int izem_shock_sensor(izem_ctx *c, const double *v, const double *T,
   return IZEM_OK;
 }
 
+extern "C" int izem_shock_sensor_ducros(izem_ctx *c, const double *dx_i, const double *dy_i,
+                                        const double *dz_i, const double *v, uint8_t *sensor) {
+  if (!c || !dx_i || !dy_i || !dz_i || !v || !sensor || c->d.ndim < 2) return IZEM_E_ARG;
+  int pk = iz_prof_begin(c, IZP_OTHER);
+  hipLaunchKernelGGL(k_sensor_ducros, dim3(iz_blocks(n_interior(c->d), 256)), dim3(256), 0,
+                     c->stream, c->d, c->o.ducros_treduc, dx_i, dy_i, dz_i, v, sensor);
+  iz_prof_end(c, pk);
+  IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
 // sensor + fluxes of one stage into the context scratch
 static int stage_fluxes(izem_ctx *c, const double *dx_i, const double *dy_i,
                         const double *dz_i, const double *T, const double *W_i,
                         const double *cp, const double *ha, uint8_t *sensor, const double *v) {
-  int r = izem_shock_sensor(c, v, T, W_i, sensor);
+  // module_solver.f90:758-762
+  int r = c->o.sensor_type == 1 ? izem_shock_sensor_ducros(c, dx_i, dy_i, dz_i, v, sensor)
+                                : izem_shock_sensor(c, v, T, W_i, sensor);
   if (r) return r;
   const double *di[3] = {dx_i, dy_i, dz_i};
   for (int q = 0; q < c->d.ndim; q++) {

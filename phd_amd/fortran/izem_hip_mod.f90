@@ -36,6 +36,9 @@ module izem_hip_mod
     integer(c_int) :: vis, vis_dt, reaction, forcesum
     real(c_double) :: CFL, CFLmin, Fo, CPR
     real(c_double) :: sqgmr, Sc, Pr, ggmopr, time_ref, T_ref, rho_ref, L_ref
+    ! WENO%SHOCK_SENSOR_TYPE (0 ADAMSSHARIFF, 1 DUCROS), WENO%DUCROS_TREDUC
+    integer(c_int) :: sensor_type
+    real(c_double) :: ducros_treduc
   end type izem_opts
 
   public :: izem_create, izem_destroy, izem_synchronize, izem_set_stream, izem_status, &

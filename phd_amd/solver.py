@@ -48,6 +48,9 @@ class Options:
     # Euler face kernels: 0 the reference's operation order (bit parity),
     # 1 projections through the eigenvectors' structure (izem_flux_mode)
     flux_mode: int = 0
+    # WENO%SHOCK_SENSOR_TYPE (0 ADAMSSHARIFF, 1 DUCROS) and WENO%DUCROS_TREDUC
+    sensor_type: int = 0
+    ducros_treduc: float = 0.5
     # (waves, blocks_per_cu, jac_age, kinetics) for izem_chem_tune; None keeps
     # the library defaults
     chem_tune: tuple | None = None
@@ -75,7 +78,7 @@ class Solver:
         g = _lib.Grid(box.nx, box.ny, box.nz, box.ng, box.ndim)
         op = _lib.Opts(o.weno_avg, o.max_rel_weight, o.vis, o.vis_dt, o.reaction, o.forcesum,
                        o.CFL, o.CFLmin, o.Fo, o.CPR, a.sqgmr, a.Sc, a.pr, a.ggmopr, a.time_ref,
-                       a.T_ref, a.rho_ref, a.L_ref)
+                       a.T_ref, a.rho_ref, a.L_ref, o.sensor_type, o.ducros_treduc)
         flat = self.thd.flat()
         ctx = C.c_void_p()
         torch.cuda.set_device(self.dev)
@@ -166,6 +169,10 @@ class Solver:
         _lib.check(self.lib.izem_prim_inv_var(self.ctx, domain, _ptr(self.v), _ptr(self.W_i),
                                               _ptr(self.T), _ptr(self.cp), _ptr(self.ha), None),
                    "prim_inv_var")
+
+    def shock_sensor_ducros(self):
+        _lib.check(self.lib.izem_shock_sensor_ducros(self.ctx, *[_ptr(x) for x in self.d_i],
+                                                     _ptr(self.v), _ptr(self.sensor)), "ducros")
 
     def shock_sensor(self):
         _lib.check(self.lib.izem_shock_sensor(self.ctx, _ptr(self.v), _ptr(self.T),

@@ -71,8 +71,10 @@ class OracleRun:
             assert s == 0
         return it.value
 
-    def opts(self, vis=1, reaction=0, weno_avg=1, mrw=-1.0, forcesum=1, CPR=20.0):
-        return OOpts(weno_avg, mrw, vis, 1, reaction, forcesum, 0.5, 0.1, 0.1, CPR)
+    def opts(self, vis=1, reaction=0, weno_avg=1, mrw=-1.0, forcesum=1, CPR=20.0,
+             sensor_type=0, ducros_treduc=0.5):
+        return OOpts(weno_avg, mrw, vis, 1, reaction, forcesum, 0.5, 0.1, 0.1, CPR, sensor_type,
+                     ducros_treduc)
 
 
 def oracle_reaction_dvode(run: OracleRun, rc, adi, dt_nondim: float, rtol: float, atol: float,

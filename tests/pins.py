@@ -758,7 +758,53 @@ def check_sensor_threshold(oracle, adi, thd):
     assert np.array_equal(got, want), np.argwhere(got != want)
 
 
+def ducros_case(adi, thd, ndim, n=32):
+    """Uniform T, p, Y with u = A sin kx + B sin ky, v = D sin kx + E sin ky
+    (and w = F sin kz in 3-D): div u and the vorticity in closed form."""
+    L = 0.004
+    nz = n if ndim == 3 else 1
+    box = ics.Box(n, n, nz, 3, ndim, (L, L, L))
+    sh = (nz, n, n)
+    k = 2 * np.pi / L
+    x = np.arange(n) * (L / n)
+    Z, Yg, X = np.meshgrid(np.arange(nz) * (L / n), x, x, indexing="ij")
+    A, B, D, E, F = 40.0, 25.0, -30.0, 15.0, 20.0
+    u = np.zeros((3,) + sh)
+    u[0] = A * np.sin(k * X) + B * np.sin(k * Yg)
+    u[1] = D * np.sin(k * X) + E * np.sin(k * Yg)
+    if ndim == 3:
+        u[2] = F * np.sin(k * Z)
+    div = k * (A * np.cos(k * X) + E * np.cos(k * Yg) + (F * np.cos(k * Z) if ndim == 3 else 0.0))
+    wz = k * (D * np.cos(k * X) - B * np.cos(k * Yg))
+    mag = div ** 2 + wz ** 2
+    duc = div ** 2 / (mag + 1e-300)
+    T = np.full(sh, 1100.0)
+    Y0 = _Y_blend(np.array(0.4))
+    Y = np.broadcast_to(Y0[:, None, None, None], (9,) + sh).copy()
+    v, Tn = ics.conserved_from_primitive(box, adi, thd, T, 101325.0, Y, u)
+    return box, v, Tn, duc, mag / mag.max()
+
+
+def check_ducros_sensor(oracle, adi, thd):
+    """shock_sensor_ducros flags exactly the cells where (div u)^2 /
+    ((div u)^2 + |curl u|^2) exceeds the threshold (cells within 0.02 of it
+    are left out, and so are cells where div and curl both nearly vanish:
+    the sixth-order gradients are not exact)."""
+    thr = 0.5
+    for ndim in (2, 3):
+        box, v, Tn, duc, mag = ducros_case(adi, thd, ndim, n=32 if ndim == 2 else 16)
+        run = OracleRun(oracle, adi, thd, box, v, Tn)
+        sens = np.zeros(box.shape, np.uint8)
+        oracle.lib.ora_sensor_ducros(C.byref(run.g), C.c_double(thr), P(run.d_i[0]),
+                                     P(run.d_i[1]), P(run.d_i[2]), P(run.v), _u8(sens))
+        got = sens[box.interior()].astype(bool)
+        clear = (np.abs(duc - thr) > 0.02) & (mag > 1e-3)
+        want = duc > thr
+        assert clear.mean() > 0.8 and want[clear].any() and (~want[clear]).any()
+        assert np.array_equal(got[clear], want[clear]), (ndim, np.argwhere(got != want)[:5])
+
+
 ALL_CHECKS = [check_euler_convergence, check_euler_linear_upwind, check_euler_riemann_face,
               check_euler_axis_permutation, check_visc_stress, check_visc_heat,
               check_visc_binary_fick, check_visc_species_sum, check_species_split_invariance, check_rk3_third_order,
-              check_timestep_closed_form, check_sensor_threshold]
+              check_timestep_closed_form, check_sensor_threshold, check_ducros_sensor]

@@ -230,3 +230,32 @@ def test_rhs_and_rk3_parity(oracle, adi, thd, kind, n, mode):
     Tg = s.T.cpu().numpy()
     I = box.interior()
     assert np.max(np.abs(Tg[I] - run.T[I]) / run.T[I]) <= TOL
+
+
+def test_viscflux_with_fits_from_tranfit(oracle, adi, mech):
+    """The GPU viscous fluxes with transport fits made by phd_amd.tranfit
+    from the Lennard-Jones database (not read from tranfit.out), against the
+    oracle with the same fits; the fits differ from the bundled file's in
+    the c_v-dependent conductivities, so this runs other tables."""
+    import torch
+    from phd_amd import chemdata as cd
+    from phd_amd.solver import Options, Solver
+    from phd_amd.tranfit import fit_transport
+    fits = fit_transport(mech)
+    thd = cd.build_thd(adi, fits=fits)
+    thd0 = cd.build_thd(adi)
+    assert np.max(np.abs(thd.coflam - thd0.coflam)) > 1e-3
+    box, v, T = small_state(adi, thd, "3d", 12)
+    run = OracleRun(oracle, adi, thd, box, v, T)
+    run.prim_all()
+    s = Solver(box, Options(reaction=0), device="cuda:0", adi=adi, thd=thd)
+    _upload(s, run)
+    f_ref = np.zeros((14,) + box.shape)
+    f_gpu = torch.zeros((14,) + box.shape, dtype=torch.float64, device="cuda:0")
+    oracle.lib.ora_viscflux(C.byref(run.g), C.byref(run.t), C.byref(run.a), P(run.d_i[0]),
+                            P(run.d_i[1]), P(run.d_i[2]), P(run.T), P(run.W_i), P(run.ha),
+                            P(run.v), P(f_ref))
+    s.viscflux(f_gpu)
+    s.sync()
+    err = _relerr(f_gpu.cpu().numpy(), f_ref, box)
+    assert err.max() <= TOL, err

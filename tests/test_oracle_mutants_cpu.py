@@ -73,6 +73,11 @@ MUTANTS = [
      "double b = d2 / (ctall[q] / cp[c] + adi->Pr * 0.0);", [P.check_timestep_closed_form]),
     ("timestep_sound_speed", "double csound = sqrt(gamma * P * rho_i);\n        double dtcx_i",
      "double csound = sqrt(P * rho_i);\n        double dtcx_i", [P.check_timestep_closed_form]),
+    ("ducros_vorticity_sign", "          wrk = du[1][0] - du[0][1];\n          vort = wrk * wrk;\n        } else {",
+     "          wrk = du[1][0] + du[0][1];\n          vort = wrk * wrk;\n        } else {",
+     [P.check_ducros_sensor]),
+    ("ducros_divergence_3d", "          div = du[0][0] + du[1][1] + du[2][2];",
+     "          div = du[0][0] + du[1][1] + du[2][1];", [P.check_ducros_sensor]),
     ("sensor_density_measure", "double drho = fabs(v[c2] - v[c]) * fmin(ri, ri2);",
      "double drho = fabs(v[c2] - v[c]) * fmax(ri, ri2);", [P.check_sensor_threshold]),
 ]
