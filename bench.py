@@ -229,6 +229,89 @@ def surrogate_c3(s, thd, n_train_cells: int = 1 << 22, n_eval: int = 1 << 22,
     return res
 
 
+def coupled_shear_layer(adi, thd, mech, n: int, steps: int, tune, flux_mode: int):
+    """The coupled learned-chemistry workload: the periodic double shear
+    layer on n^3 cells (fuel 50 % H2 / 50 % N2 at 300 K and +100 m/s, air
+    at 1200 K and -100 m/s, vorticity thickness 0.5 mm, 8 mm box), the
+    surrogate trained on the run's own cells (2^20 cells aged by 0, 10, 100
+    and 1000 half steps, each labelled over 0.5, 1 and 2 half steps: 12.6 M
+    states; Adam lr 3e-4, batch 65536, 8 epochs) and calibrated for 2 K;
+    then the same steps from the same state with reaction_selec on the BDF
+    path and on the surrogate, each timed over `steps` graph-replayed steps
+    after one eager and one graph warm-up step, with one event-profiled
+    eager step for the chemistry time.  Peak T and heat-release rate of the
+    two end states are compared."""
+    import numpy as np
+    import torch
+
+    from phd_amd import _lib, coupled, ics
+    from phd_amd.solver import Options, Solver, profile_read, profile_start
+
+    t0 = time.perf_counter()
+    box, v, T = ics.config5_state(adi, thd, n=n)
+    s = Solver(box, Options(chem_tune=tune, flux_mode=flux_mode), adi=adi, thd=thd, mech=mech)
+    s.load_state(v, T)
+    del v, T
+    t_state = time.perf_counter() - t0
+    ts0 = s.time_state()
+    s.timestep()
+    dt_half = 0.5 * float(s.time_state()[4]) * adi.time_ref
+    s.set_time_state(ts0)
+    log(f"shear layer {n}^3 resident ({t_state:.0f} s); training the surrogate on its cells")
+    sur, info = coupled.train_flow_surrogate(s, dt_half, n_cells=1 << 20,
+                                             ages=(0, 10, 100, 1000), epochs=8, batch=65536,
+                                             lr=3e-4, n_cal=1 << 18)
+    s.attach_surrogate(sur)
+    snap = [x.clone() for x in (s.T, s.v, s.W_i, s.cp, s.ha)]
+    out = {"workload": f"3-D periodic double shear layer {n}^3, H2/N2 (300 K, +100 m/s) "
+                       f"against air (1200 K, -100 m/s), vorticity thickness 0.5 mm, 8 mm box, "
+                       f"2 m/s broadband perturbation in the layers",
+           "cells": n ** 3, "dt_s": 2.0 * dt_half, "surrogate_training": info}
+    for which in ("dvode", "surrogate"):
+        for dst, src in zip((s.T, s.v, s.W_i, s.cp, s.ha), snap):
+            dst.copy_(src)
+        s.sensor.zero_()
+        s.set_time_state(ts0)
+        s.reac_selec(_lib.REAC_SURROGATE if which == "surrogate" else _lib.REAC_DVODE)
+        s.step(graph=False)
+        s.step(graph=True)
+        s.sync()
+        if which == "surrogate":
+            s.surrogate_counts()  # clear
+        log(f"shear layer: timing {steps} steps with reaction_selec = {which}")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            s.step(graph=True)
+        s.sync()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t1
+        fall = None
+        if which == "surrogate":
+            cells, nf = s.surrogate_counts()
+            fall = nf / max(cells, 1)
+        profile_start(s)
+        s.step(graph=False)
+        s.sync()
+        prof = profile_read(s)
+        if which == "surrogate":
+            s.surrogate_counts()
+        st = coupled.flame_stats(s)
+        out[which] = {"ms_per_step": 1e3 * el / steps, "cell_steps_per_s": n ** 3 * steps / el,
+                      "chemistry_ms_per_step": prof["chemistry"][0],
+                      "kernel_ms_per_step": {k: v[0] for k, v in prof.items()},
+                      "fallback_rate": fall, "status_bits": s.status(),
+                      "steps_from_initial_state": steps + 3, **st}
+    d, g = out["dvode"], out["surrogate"]
+    out["chemistry_speedup"] = d["chemistry_ms_per_step"] / g["chemistry_ms_per_step"]
+    out["T_peak_diff_K"] = g["T_peak_K"] - d["T_peak_K"]
+    out["hrr_peak_rel_diff"] = (g["hrr_peak_W_m3"] - d["hrr_peak_W_m3"]) / abs(d["hrr_peak_W_m3"])
+    out["hrr_mean_rel_diff"] = (g["hrr_mean_W_m3"] - d["hrr_mean_W_m3"]) / abs(d["hrr_mean_W_m3"])
+    s.attach_surrogate(None)
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,6 +322,9 @@ def main():
     ap.add_argument("--chem-cells", type=int, default=1 << 22)
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-surrogate-c3", action="store_true")
+    ap.add_argument("--shear-n", type=int, default=384,
+                    help="cells per side of the coupled learned-chemistry shear layer (0: skip)")
+    ap.add_argument("--shear-steps", type=int, default=5)
     ap.add_argument("--flux-mode", type=int, default=1,
                     help="Euler face kernels: 0 reference operation order (bit parity), "
                          "1 structured projections (within 1e-12 of the reference)")
@@ -374,6 +460,18 @@ def main():
         log(f"batched VODE-path chemistry on {args.chem_cells} independent cells")
         chem = chem_vode_rate(s, thd, args.chem_cells)
     s.close()
+    del s
+    torch.cuda.empty_cache()
+    shear = None
+    if args.shear_n > 0:
+        shear_tune = tune if tune is not None else (tuned[:4] if tuned else None)
+        shear = coupled_shear_layer(adi, thd, mech, args.shear_n, args.shear_steps, shear_tune,
+                                    args.flux_mode)
+        log(f"shear layer {args.shear_n}^3: {shear['dvode']['ms_per_step']:.0f} ms/step with "
+            f"the BDF path, {shear['surrogate']['ms_per_step']:.0f} with the surrogate "
+            f"(chemistry x{shear['chemistry_speedup']:.2f}, fallback "
+            f"{100 * shear['surrogate']['fallback_rate']:.1f} %), peak T difference "
+            f"{shear['T_peak_diff_K']:.3f} K, peak heat release {100 * shear['hrr_peak_rel_diff']:.2f} %")
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle flow + reference DVODE, 1 core)")
@@ -404,6 +502,7 @@ def main():
         "chemistry": chem,
         "chem_surrogate": surr,
         "chem_surrogate_c3": surr_c3,
+        "coupled_shear_layer": shear,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

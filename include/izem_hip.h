@@ -226,18 +226,26 @@ int izem_chem_order_read(izem_ctx *ctx, long cap, long *n, int *perm, int *bin);
  * refreshes W_i/cp/ha, fills ghosts, ghost Newton; DTmax into the time state */
 int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale, double *T, double *W_i,
                         double *cp, double *ha, double *v);
-/* one iteration of the main loop (izem.f90:242-389): timestep, then with
- * chemistry reaction_selec(dt/2) -> rk3(dt) -> reaction_selec(dt/2)
- * (:254-277), dt kept in the time state.  use_graph = 1 replays the
- * iteration from a HIP graph captured at the first call (and again whenever
- * the arrays or the chemistry settings change): same results, no launch
- * gaps.  IZEM_E_STATE with use_graph = 1 and the stiffness order on. */
+/* reaction_selec (module_reaction.f90:47-90): the chemistry substep of the
+ * solver chosen by izem_reac_selec (the reference's inp%reac_selec), then
+ * the W_i/cp/ha refresh, ghost fill and ghost Newton of :79-83 */
+#define IZEM_REAC_DVODE 0     /* reaction_dvode */
+#define IZEM_REAC_SURROGATE 1 /* the attached surrogate, BDF on its rejected cells */
+int izem_reac_selec(izem_ctx *ctx, int solver);
+int izem_reaction_selec(izem_ctx *ctx, const double *dt, double scale, double *T, double *W_i,
+                        double *cp, double *ha, double *v);
 /* operation order of the Euler face kernels: 0 (the default) the
  * reference's dense characteristic projections in its summation order (bit
  * parity with it); 1 the projections through the structure of the left
  * eigenvectors (a shared scalar per stencil point; a tenth of the work,
  * results within rounding of mode 0).  No counterpart in the reference. */
 int izem_flux_mode(izem_ctx *ctx, int mode);
+/* one iteration of the main loop (izem.f90:242-389): timestep, then with
+ * chemistry reaction_selec(dt/2) -> rk3(dt) -> reaction_selec(dt/2)
+ * (:254-277), dt kept in the time state.  use_graph = 1 replays the
+ * iteration from a HIP graph captured at the first call (and again whenever
+ * the arrays or the chemistry settings change): same results, no launch
+ * gaps.  IZEM_E_STATE with use_graph = 1 and the stiffness order on. */
 int izem_step(izem_ctx *ctx, const double *dx_i, const double *dy_i, const double *dz_i,
               double *T, double *W_i, double *cp, double *ha, uint8_t *sensor, double *v,
               int use_graph);
@@ -300,6 +308,34 @@ int izem_surr_apply(izem_ctx *ctx, int n, double *T, double *Y, const float *pre
 int izem_surr_fallback(izem_ctx *ctx, int n, double *T, const double *rho, double *Y, double dt,
                        const int *flag, int *counter, int *idx, double *Tw, double *rw,
                        double *Yw, int *nfall);
+
+/* ---- the surrogate inside the flow's reaction step (no reference
+ * counterpart; the dispatch it joins is reaction_selec) ----------------
+ * The caller's trained network, all device arrays, read at every reaction
+ * call (they must outlive the attachment): */
+typedef struct {
+  const void *Wt[4];     /* bf16 weights of layer l stored [N][K], 16-byte aligned */
+  const float *b[4];     /* fp32 biases (N = 256, 256, 256, 64) */
+  const float *in_mean, *in_istd;            /* feature normalisation, 12 each */
+  const float *out_mean, *out_std, *out_c;   /* output scaling, 10 each */
+  const double *proj;    /* 9 x 9 element-conservation projector */
+  float feat_lim;        /* indicator thresholds as izem_surr_apply */
+  double tol_T, dT_max;  /* dT_max < 0 sends every cell to the BDF path */
+} izem_surrogate;
+/* attach s (copied; NULL detaches and selects DVODE) with a workspace for
+ * chunk cells per network pass (128 <= chunk <= 2^26; 320 bytes each)
+ * plus a 4-byte fallback-list entry per reacting cell */
+int izem_surr_attach(izem_ctx *ctx, const izem_surrogate *s, long chunk);
+/* reaction_selec's substep with the surrogate: on every cell of the
+ * reacting box the network's substep over scale*dt where the indicator
+ * accepts it, the reaction_dvode reactor kernel on the others; then the
+ * tail of reaction_selec.  DTmax over both sets.  IZEM_E_STATE when no
+ * surrogate is attached. */
+int izem_reaction_surrogate(izem_ctx *ctx, const double *dt, double scale, double *T,
+                            double *W_i, double *cp, double *ha, double *v);
+/* cells through izem_reaction_surrogate and how many fell back to the BDF
+ * since the previous call (synchronises the stream, then clears them) */
+int izem_surr_counts(izem_ctx *ctx, unsigned long long out[2]);
 
 #ifdef __cplusplus
 }

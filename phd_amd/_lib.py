@@ -21,6 +21,8 @@ EXPORTS = [
     "izem_timestep", "izem_chem_setup", "izem_chem_tune", "izem_chem_stats", "izem_ckwyr", "izem_psr_batch", "izem_psr_batch_noalloc", "izem_chem_order", "izem_chem_order_read", "izem_reaction_dvode", "izem_step", "izem_flux_mode",
     "izem_mlp_gemm", "izem_mlp_gemm_splitk", "izem_mlp_colsum", "izem_mlp_adam", "izem_mlp_to_bf16",
     "izem_mlp_mse", "izem_mlp_infer", "izem_surr_features", "izem_surr_apply", "izem_surr_fallback",
+    "izem_reac_selec", "izem_reaction_selec", "izem_surr_attach", "izem_reaction_surrogate",
+    "izem_surr_counts",
 ]
 
 
@@ -37,6 +39,17 @@ class Opts(C.Structure):
                 ("Pr", C.c_double), ("ggmopr", C.c_double), ("time_ref", C.c_double),
                 ("T_ref", C.c_double), ("rho_ref", C.c_double), ("L_ref", C.c_double),
                 ("sensor_type", C.c_int), ("ducros_treduc", C.c_double)]
+
+
+class SurrogateDesc(C.Structure):
+    """izem_surrogate: the device arrays of a trained surrogate."""
+    _fields_ = [("Wt", C.c_void_p * 4), ("b", C.c_void_p * 4), ("in_mean", C.c_void_p),
+                ("in_istd", C.c_void_p), ("out_mean", C.c_void_p), ("out_std", C.c_void_p),
+                ("out_c", C.c_void_p), ("proj", C.c_void_p), ("feat_lim", C.c_float),
+                ("tol_T", C.c_double), ("dT_max", C.c_double)]
+
+
+REAC_DVODE, REAC_SURROGATE = 0, 1
 
 
 class IzemError(RuntimeError):
@@ -106,6 +119,11 @@ def load(path: str | None = None):
     lib.izem_reaction_dvode.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp]
     lib.izem_step.argtypes = [vp] + [vp] * 9 + [C.c_int]
     lib.izem_flux_mode.argtypes = [vp, C.c_int]
+    lib.izem_reac_selec.argtypes = [vp, C.c_int]
+    lib.izem_reaction_selec.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp]
+    lib.izem_reaction_surrogate.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp]
+    lib.izem_surr_attach.argtypes = [vp, C.POINTER(SurrogateDesc), C.c_long]
+    lib.izem_surr_counts.argtypes = [vp, vp]
     _lib = lib
     return lib
 

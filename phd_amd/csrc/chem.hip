@@ -50,7 +50,36 @@ struct IzChem {
   long order_cap;
   int *order_bin, *order_perm, *order_hist;
   long order_n;   // cells of the last sorted batch, 0 when the last call ran unsorted
+  // reaction_selec's solver (0 BDF, 1 surrogate with BDF fallback), the
+  // attached surrogate, and a generation count of both for the step graph
+  int reac_selec;
+  struct IzSurr *surr;
+  int sel_gen;
 };
+
+// a surrogate attached to the flow's reaction step: the caller's network
+// and its workspace (feature rows and outputs of one chunk of cells, the
+// list of fallback cells of the whole reacting box)
+struct IzSurr {
+  izem_surrogate d;
+  long chunk;
+  unsigned short *X;   // [chunk][32] bf16
+  float *pred;         // [chunk][64]
+  int *list;           // [list_cap] box-cell indices
+  long list_cap;
+  int *nfall;          // [1] entries of list
+  unsigned long long *counts;  // [2] cells, fallback cells since izem_surr_counts
+};
+
+static void surr_free(IzSurr *u) {
+  if (!u) return;
+  if (u->X) (void)hipFree(u->X);
+  if (u->pred) (void)hipFree(u->pred);
+  if (u->list) (void)hipFree(u->list);
+  if (u->nfall) (void)hipFree(u->nfall);
+  if (u->counts) (void)hipFree(u->counts);
+  free(u);
+}
 
 // ------------------------------------------------------------ kernels
 __global__ void k_ckwyr(int n, const double *__restrict__ rho, const double *__restrict__ T,
@@ -153,11 +182,13 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                                  double T_ref, double rho_cgs, double rtol, double atol,
                                  int jac_age, double *__restrict__ T, double *__restrict__ v,
                                  int *__restrict__ counter, const int *__restrict__ order,
-                                 double *__restrict__ ts,
+                                 const int *__restrict__ list_n, double *__restrict__ ts,
                                  int *__restrict__ status,
                                  unsigned long long *__restrict__ stats) {
   const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
-  const long n = (long)ni * nj * nk;
+  // list_n: only the first *list_n cells of the list `order` (the
+  // surrogate's fallback cells), else every cell of the box
+  const long n = list_n ? (long)*list_n : (long)ni * nj * nk;
   const double dt_phys = (scale * (*dtp)) * time_ref;
   const double Tref_i = 1.0 / T_ref;
   __shared__ double zs[IZ_ZLDS_DOUBLES(WAVES)];
@@ -259,6 +290,7 @@ void iz_chem_destroy(izem_ctx *ctx) {
   if (ctx->chem->order_hist) (void)hipFree(ctx->chem->order_hist);
   if (ctx->chem->order_bin) (void)hipFree(ctx->chem->order_bin);
   if (ctx->chem->order_perm) (void)hipFree(ctx->chem->order_perm);
+  surr_free(ctx->chem->surr);
   free(ctx->chem);
   ctx->chem = nullptr;
 }
@@ -583,7 +615,7 @@ int iz_chem_signature(izem_ctx *ctx, double sig[IZ_CHEM_SIG]) {
   if (ch->order) return IZEM_E_STATE;
   const double s[IZ_CHEM_SIG] = {(double)ch->use_gen, (double)ch->waves,
                                  (double)ch->blocks_per_cu, (double)ch->jac_age, ch->rtol,
-                                 ch->atol, (double)ch->trim, 0.0};
+                                 ch->atol, (double)ch->trim, (double)ch->sel_gen};
   for (int i = 0; i < IZ_CHEM_SIG; i++) sig[i] = s[i];
   return IZEM_OK;
 }
@@ -707,38 +739,84 @@ extern "C" int izem_psr_batch_noalloc(izem_ctx *ctx, int n, double *T, const dou
   return IZEM_OK;
 }
 
+// reacting box: module_reaction.f90:138-150 (trimmed by ng+1 at the global
+// faces unless trim == 0); in 2-D the z range is the whole box
+struct IzBox {
+  int i0, i1, j0, j1, k0, k1;
+  long n;
+};
+static IzBox react_box(const izem_ctx *ctx) {
+  const IzDims &d = ctx->d;
+  IzBox b{0, d.nx - 1, 0, d.ny - 1, 0, d.nz - 1, 0};
+  if (ctx->chem->trim) {
+    b.i0 = d.ng; b.i1 = d.nx - d.ng - 2;
+    b.j0 = d.ng; b.j1 = d.ny - d.ng - 2;
+    if (d.ndim == 3) {
+      // fz = min(ey, ntz-ng-1): the reference bounds z by the y extent
+      b.k0 = d.ng;
+      b.k1 = d.nz - d.ng - 2;
+      if (b.k1 > d.ny - 1) b.k1 = d.ny - 1;
+    }
+  }
+  if (b.i1 < b.i0 || b.j1 < b.j0 || b.k1 < b.k0) return b;
+  b.n = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
+  return b;
+}
+
+// the persistent reactor kernel of the tuned variant over the box's cells
+// (queue position q takes box cell order[q], or q; with list_n the first
+// *list_n entries of order only)
+static void reaction_queue(izem_ctx *ctx, const IzBox &b, dim3 grid, const double *dt,
+                           double scale, double *T, double *v, const int *order,
+                           const int *list_n) {
+  IzChem *ch = ctx->chem;
+  const double rho_cgs = ctx->o.rho_ref * 1.0e-3;
+#define IZ_RQ(K, W)                                                                             \
+  hipLaunchKernelGGL((k_reaction_queue<K, W>), grid, dim3(64), 0, ctx->stream, ctx->d, b.i0,    \
+                     b.i1, b.j0, b.j1, b.k0, b.k1, dt, scale, ctx->o.time_ref, ctx->o.T_ref,    \
+                     rho_cgs, ch->rtol, ch->atol, ch->jac_age, T, v, ch->counter, order, list_n, \
+                     ctx->ts, ctx->status, ch->stats)
+  if (ch->use_gen) {
+    IZ_RQ(KinGen, 1);
+  } else {
+    if (ch->waves == 1) IZ_RQ(KinTables, 1);
+    else if (ch->waves == 2) IZ_RQ(KinTables, 2);
+    else IZ_RQ(KinTables, 4);
+  }
+#undef IZ_RQ
+}
+
+// the tail of reaction_selec (module_reaction.f90:79-83): W_i, cp, ha
+// from the new T and Y, ghost fill, ghost-slab primitives
+static int reaction_tail(izem_ctx *ctx, double *T, double *W_i, double *cp, double *ha,
+                         double *v) {
+  int r = iz_upd_prim_woT(ctx, 0, v, T, W_i, cp, ha);
+  if (r) return r;
+  r = iz_comm_cons(ctx, v, IZ_NV);
+  if (r) return r;
+  return iz_ghost_newton(ctx, T, W_i, cp, ha, v);
+}
+
 extern "C" int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale, double *T,
                                    double *W_i, double *cp, double *ha, double *v) {
   if (!ctx || !ctx->chem || !dt || !T || !W_i || !cp || !ha || !v) return IZEM_E_ARG;
   const IzDims &d = ctx->d;
   IzChem *ch = ctx->chem;
-  // reacting box: module_reaction.f90:138-150 (trimmed by ng+1 at the
-  // global faces unless trim == 0); in 2-D the z range is the whole box
-  int i0 = 0, i1 = d.nx - 1, j0 = 0, j1 = d.ny - 1, k0 = 0, k1 = d.nz - 1;
-  if (ch->trim) {
-    i0 = d.ng; i1 = d.nx - d.ng - 2;
-    j0 = d.ng; j1 = d.ny - d.ng - 2;
-    if (d.ndim == 3) {
-      // fz = min(ey, ntz-ng-1): the reference bounds z by the y extent
-      k0 = d.ng;
-      k1 = d.nz - d.ng - 2;
-      if (k1 > d.ny - 1) k1 = d.ny - 1;
-    }
-  }
-  long n = (long)(i1 - i0 + 1) * (j1 - j0 + 1) * (k1 - k0 + 1);
+  const IzBox b = react_box(ctx);
+  const long n = b.n;
   hipLaunchKernelGGL(k_chem_prep, dim3(1), dim3(1), 0, ctx->stream, ch->counter, ctx->ts, 1);
   if (n > 0) {
     double rho_cgs = ctx->o.rho_ref * 1.0e-3;
     int pk = iz_prof_begin(ctx, IZP_CHEM);
-    dim3 grid(persistent_grid(ctx, n)), blk(64);
+    dim3 grid(persistent_grid(ctx, n));
     const int *order = nullptr;
     if (ch->order && n <= INT_MAX && n > 64L * grid.x) {
-      const int ni = i1 - i0 + 1, nj = j1 - j0 + 1;
+      const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
       int rc = order_begin(ctx, (int)n);
       if (rc == IZEM_OK) {
         hipLaunchKernelGGL(k_order_key_flow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                           ctx->stream, d, i0, j0, k0, ni, nj, (int)n, ctx->o.T_ref, rho_cgs, T,
-                           v, ch->order_bin, ch->order_hist);
+                           ctx->stream, d, b.i0, b.j0, b.k0, ni, nj, (int)n, ctx->o.T_ref,
+                           rho_cgs, T, v, ch->order_bin, ch->order_hist);
         rc = order_end(ctx, (int)n, ch->order_bin, ch->order_hist, ch->order_perm);
       }
       if (rc != IZEM_OK) {
@@ -748,26 +826,11 @@ extern "C" int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale
       order = ch->order_perm;
     }
     ch->order_n = order ? n : 0;
-#define IZ_RQ(K, W)                                                                            \
-  hipLaunchKernelGGL((k_reaction_queue<K, W>), grid, blk, 0, ctx->stream, d, i0, i1, j0, j1, k0, \
-                     k1, dt, scale, ctx->o.time_ref, ctx->o.T_ref, rho_cgs, ch->rtol, ch->atol,   \
-                     ch->jac_age, T, v, ch->counter, order, ctx->ts, ctx->status, ch->stats)
-    if (ch->use_gen) {
-      IZ_RQ(KinGen, 1);
-    } else {
-      if (ch->waves == 1) IZ_RQ(KinTables, 1);
-      else if (ch->waves == 2) IZ_RQ(KinTables, 2);
-      else IZ_RQ(KinTables, 4);
-    }
-#undef IZ_RQ
+    reaction_queue(ctx, b, grid, dt, scale, T, v, order, nullptr);
     iz_prof_end(ctx, pk);
     IZ_CHECK(hipGetLastError());
   }
-  int r = iz_upd_prim_woT(ctx, 0, v, T, W_i, cp, ha);
-  if (r) return r;
-  r = iz_comm_cons(ctx, v, IZ_NV);
-  if (r) return r;
-  return iz_ghost_newton(ctx, T, W_i, cp, ha, v);
+  return reaction_tail(ctx, T, W_i, cp, ha, v);
 }
 
 // work counters of the chemistry integrations since the previous call
@@ -816,57 +879,62 @@ __device__ inline double d_u_mix(double T, const double *Y, double &cv) {
   return u;
 }
 
+// one normalised bf16 feature row (ln p, T, Y_k, ln dt) of a cell in
+// CHEMKIN units (K, g/cm^3)
+__device__ inline void surr_row(double T, double rho, const double *y, double lndt,
+                                const float *__restrict__ mean, const float *__restrict__ istd,
+                                unsigned short *__restrict__ row) {
+  double wi = 0.0;
+  float f[IZ_SURR_NF];
+  for (int k = 0; k < CK_NEQ - 1; k++) {
+    wi += y[k] / IZ_CMECH.wt[k];
+    f[2 + k] = (float)y[k];
+  }
+  const double p = rho * 8.314510e7 * T * wi;  // dyn/cm^2
+  f[0] = (float)log(p);
+  f[1] = (float)T;
+  f[11] = (float)lndt;
+  for (int j = 0; j < IZ_SURR_NF; j++) row[j] = surr_f2bf((f[j] - mean[j]) * istd[j]);
+  for (int j = IZ_SURR_NF; j < IZ_SURR_XLD; j++) row[j] = 0;
+}
+
 __global__ void k_surr_features(int n, const double *__restrict__ T, const double *__restrict__ rho,
                                 const double *__restrict__ Y, double lndt,
                                 const float *__restrict__ mean, const float *__restrict__ istd,
                                 unsigned short *__restrict__ X) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  double wi = 0.0;
-  float f[IZ_SURR_NF];
-  for (int k = 0; k < CK_NEQ - 1; k++) {
-    const double y = Y[(long)c * (CK_NEQ - 1) + k];
-    wi += y / IZ_CMECH.wt[k];
-    f[2 + k] = (float)y;
-  }
-  const double p = rho[c] * 8.314510e7 * T[c] * wi;  // dyn/cm^2
-  f[0] = (float)log(p);
-  f[1] = (float)T[c];
-  f[11] = (float)lndt;
-  unsigned short *row = X + (long)c * IZ_SURR_XLD;
-  for (int j = 0; j < IZ_SURR_NF; j++) row[j] = surr_f2bf((f[j] - mean[j]) * istd[j]);
-  for (int j = IZ_SURR_NF; j < IZ_SURR_XLD; j++) row[j] = 0;
+  double y[CK_NEQ - 1];
+  for (int k = 0; k < CK_NEQ - 1; k++) y[k] = Y[(long)c * (CK_NEQ - 1) + k];
+  surr_row(T[c], rho[c], y, lndt, mean, istd, X + (long)c * IZ_SURR_XLD);
 }
 
-// accept or reject the network's substep for each cell.  pred holds the
-// scaled outputs (dT, dY_k); proj is the 9 x 9 projector onto element
-// (hence mass) conservation; on acceptance Y += proj.dY and T += dT.  The
-// temperature that keeps the mixture internal energy of the constant-
-// volume reactor with the projected Y is the indicator's check: the
-// network's T and the one its species imply must agree within tol_T.
-// flag: 0 accepted, 1 fallback (T, Y untouched).
-__global__ void k_surr_apply(int n, double *__restrict__ T, double *__restrict__ Y,
-                             const float *__restrict__ pred, long ldp,
-                             const float *__restrict__ omean, const float *__restrict__ ostd,
-                             const float *__restrict__ oc,
-                             const double *__restrict__ proj, const unsigned short *__restrict__ X,
-                             float feat_lim, double tol_T, double dT_max,
-                             int *__restrict__ flag, double *__restrict__ T_energy) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+// the network's substep for one cell: decode the scaled outputs (dT, dY_k)
+// of pred, project dY onto element (hence mass) conservation, and run the
+// error indicator.  Returns 1 when the cell must fall back to the BDF path:
+// a normalised feature beyond feat_lim (outside the training range), a
+// projected mass fraction below -1e-7, or the temperature that keeps the
+// mixture internal energy of the constant-volume reactor with the projected
+// Y (Newton on u(T, Y1) = u(T0, Y0)) differing from T0 + dT by more than
+// tol_T, or |dT| > dT_max.  y1 and dT receive the substep, Te that energy
+// temperature.
+__device__ inline int surr_decide(const double *y0, double T0, const float *__restrict__ pr,
+                                  const float *__restrict__ omean,
+                                  const float *__restrict__ ostd, const float *__restrict__ oc,
+                                  const double *__restrict__ proj,
+                                  const unsigned short *__restrict__ xrow, float feat_lim,
+                                  double tol_T, double dT_max, double *y1, double &dT,
+                                  double &Te) {
   const int K = CK_NEQ - 1;
-  double y0[CK_NEQ - 1], dy[CK_NEQ - 1], y1[CK_NEQ - 1];
-  for (int k = 0; k < K; k++) y0[k] = Y[(long)c * K + k];
-  const float *pr = pred + (long)c * ldp;
+  double dy[CK_NEQ - 1];
   // outputs are standardised asinh(change / oc): the transform resolves
   // small and large changes with the same relative accuracy
-  const double dT = oc[0] * sinh((double)(pr[0] * ostd[0] + omean[0]));
+  dT = oc[0] * sinh((double)(pr[0] * ostd[0] + omean[0]));
   for (int k = 0; k < K; k++)
     dy[k] = oc[1 + k] * sinh((double)(pr[1 + k] * ostd[1 + k] + omean[1 + k]));
   int bad = 0;
-  // inputs outside the range the network was trained on
   for (int j = 0; j < IZ_SURR_NF; j++) {
-    const float x = __uint_as_float(((unsigned)X[(long)c * IZ_SURR_XLD + j]) << 16);
+    const float x = __uint_as_float(((unsigned)xrow[j]) << 16);
     if (!(fabsf(x) <= feat_lim)) bad = 1;
   }
   double ymin = 1.0;
@@ -880,18 +948,40 @@ __global__ void k_surr_apply(int n, double *__restrict__ T, double *__restrict__
   // undershoots (absent radicals) are kept, which keeps the element
   // masses exact
   if (!(ymin >= -1.0e-7)) bad = 1;
-  // energy-consistent temperature (Newton on u(T, Y1) = u(T0, Y0))
-  const double T0 = T[c];
   double cv;
   const double u0 = d_u_mix(T0, y0, cv);
-  double Te = T0 + dT;
-  for (int it = 0; it < 8; it++) {
+  // converged to 1e-12 relative (about 1e-9 K), far below the smallest
+  // tol_T a calibration on slowly reacting cells picks
+  Te = T0 + dT;
+  for (int it = 0; it < 10; it++) {
     const double u = d_u_mix(Te, y1, cv);
     const double d = (u - u0) / cv;
     Te -= d;
-    if (fabs(d) < 1.0e-10 * Te) break;
+    if (fabs(d) < 1.0e-12 * Te) break;
   }
   if (!(fabs(Te - (T0 + dT)) <= tol_T) || !(fabs(dT) <= dT_max) || !(Te > 0.0)) bad = 1;
+  return bad;
+}
+
+// accept or reject the network's substep for each of n packed cells; on
+// acceptance Y += proj.dY and T += dT.  flag: 0 accepted, 1 fallback (T, Y
+// untouched).
+__global__ void k_surr_apply(int n, double *__restrict__ T, double *__restrict__ Y,
+                             const float *__restrict__ pred, long ldp,
+                             const float *__restrict__ omean, const float *__restrict__ ostd,
+                             const float *__restrict__ oc,
+                             const double *__restrict__ proj, const unsigned short *__restrict__ X,
+                             float feat_lim, double tol_T, double dT_max,
+                             int *__restrict__ flag, double *__restrict__ T_energy) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int K = CK_NEQ - 1;
+  double y0[CK_NEQ - 1], y1[CK_NEQ - 1];
+  for (int k = 0; k < K; k++) y0[k] = Y[(long)c * K + k];
+  const double T0 = T[c];
+  double dT, Te;
+  const int bad = surr_decide(y0, T0, pred + (long)c * ldp, omean, ostd, oc, proj,
+                              X + (long)c * IZ_SURR_XLD, feat_lim, tol_T, dT_max, y1, dT, Te);
   if (T_energy) T_energy[c] = Te;
   flag[c] = bad;
   if (!bad) {
@@ -980,5 +1070,207 @@ extern "C" int izem_surr_fallback(izem_ctx *ctx, int n, double *T, const double 
   hipLaunchKernelGGL(k_surr_scatter, dim3(iz_blocks(m, 256)), dim3(256), 0, ctx->stream, m, idx,
                      Tw, Yw, T, Y);
   IZ_CHECK(hipGetLastError());
+  return IZEM_OK;
+}
+
+// ===================================================================
+// The surrogate inside the flow's reaction step: reaction_selec with
+// the learned substep on the cells the indicator accepts and the BDF
+// reactor kernel on the others.  The dispatch it joins is
+// reaction_selec (module_reaction.f90:47-90: DVODE or MIL by
+// inp%reac_selec); the surrogate itself has no reference counterpart.
+// ===================================================================
+
+// box cell -> flow-array cell (the mapping of k_reaction_queue)
+__device__ inline long box_cell(const IzDims &d, const IzBox &b, long cell) {
+  const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
+  const int i = b.i0 + (int)(cell % ni);
+  const long r = cell / ni;
+  return iz_cidx(d, i, b.j0 + (int)(r % nj), b.k0 + (int)(r / nj));
+}
+
+// feature rows of box cells c0 .. c0 + m - 1 (CHEMKIN units, as the
+// reactor kernel converts them)
+__global__ void __launch_bounds__(256) k_surr_flow_features(
+    IzDims d, IzBox b, long c0, int m, const double *__restrict__ dtp, double scale,
+    double time_ref, double T_ref, double rho_cgs, const double *__restrict__ T,
+    const double *__restrict__ v, const float *__restrict__ mean, const float *__restrict__ istd,
+    unsigned short *__restrict__ X) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const long c = box_cell(d, b, c0 + j);
+  const double rho0 = v[c], rho_i = 1.0 / rho0;
+  double y[CK_NEQ - 1];
+  for (int l = 0; l < CK_NEQ - 1; l++) y[l] = v[(IZ_NIV + l) * d.nc + c] * rho_i;
+  const double dt_phys = (scale * (*dtp)) * time_ref;
+  surr_row(T[c] * T_ref, rho0 * rho_cgs, y, log(dt_phys), mean, istd,
+           X + (long)j * IZ_SURR_XLD);
+}
+
+// the network's substep on box cells c0 .. c0 + m - 1: accepted cells get
+// T and rho Y_k as the reactor kernel writes them (clipped Y_k, the last
+// species closing the sum) and enter DTmax; rejected cells are appended
+// to the fallback list, untouched
+__global__ void __launch_bounds__(256) k_surr_flow_apply(
+    IzDims d, IzBox b, long c0, int m, const float *__restrict__ pred,
+    const float *__restrict__ omean, const float *__restrict__ ostd, const float *__restrict__ oc,
+    const double *__restrict__ proj, const unsigned short *__restrict__ X, float feat_lim,
+    double tol_T, double dT_max, double T_ref, double *__restrict__ T, double *__restrict__ v,
+    int *__restrict__ nfall, int *__restrict__ list, double *__restrict__ ts) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int reject = 0;
+  double dtabs = 0.0;
+  if (j < m) {
+    const long c = box_cell(d, b, c0 + j);
+    const double rho0 = v[c], rho_i = 1.0 / rho0;
+    double y0[CK_NEQ - 1], y1[CK_NEQ - 1];
+    for (int l = 0; l < CK_NEQ - 1; l++) y0[l] = v[(IZ_NIV + l) * d.nc + c] * rho_i;
+    const double T0 = T[c] * T_ref;
+    double dT, Te;
+    reject = surr_decide(y0, T0, pred + (long)j * 64, omean, ostd, oc, proj,
+                         X + (long)j * IZ_SURR_XLD, feat_lim, tol_T, dT_max, y1, dT, Te);
+    if (!reject) {
+      const double T1 = T0 + dT;
+      T[c] = T1 * (1.0 / T_ref);
+      dtabs = fabs(T1 - T0);
+      double sumY = 0.0;
+      for (int l = 0; l < CK_NEQ - 2; l++) {
+        const double Ya = fmin(fmax(y1[l], 0.0), 1.0);
+        v[(IZ_NIV + l) * d.nc + c] = rho0 * Ya;
+        sumY = sumY + Ya;
+      }
+      v[(IZ_NIV + CK_NEQ - 2) * d.nc + c] = rho0 * (1.0 - sumY);
+    }
+  }
+  // one list reservation per wave
+  const unsigned long long mask = __ballot(reject);
+  int base = 0;
+  if (lane == 0 && mask) base = atomicAdd(nfall, __popcll(mask));
+  base = __shfl(base, 0, 64);
+  if (reject) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int)(c0 + j);
+  for (int off = 32; off > 0; off >>= 1) dtabs = fmax(dtabs, __shfl_xor(dtabs, off, 64));
+  if (lane == 0) atomic_max_posd(&ts[5], dtabs);
+}
+
+__global__ void k_surr_count(unsigned long long *__restrict__ counts, long n,
+                             const int *__restrict__ nfall) {
+  counts[0] += (unsigned long long)n;
+  counts[1] += (unsigned long long)*nfall;
+}
+
+extern "C" int izem_surr_attach(izem_ctx *ctx, const izem_surrogate *s, long chunk) {
+  if (!ctx || !ctx->chem) return IZEM_E_ARG;
+  IzChem *ch = ctx->chem;
+  if (!s) {
+    IZ_CHECK(hipStreamSynchronize(ctx->stream));
+    surr_free(ch->surr);
+    ch->surr = nullptr;
+    ch->reac_selec = IZEM_REAC_DVODE;
+    ch->sel_gen++;
+    return IZEM_OK;
+  }
+  if (chunk < 128 || chunk > (1L << 26)) return IZEM_E_ARG;
+  for (int l = 0; l < 4; l++)
+    if (!s->Wt[l] || !s->b[l] || (((uintptr_t)s->Wt[l]) & 15)) return IZEM_E_ARG;
+  if (!s->in_mean || !s->in_istd || !s->out_mean || !s->out_std || !s->out_c || !s->proj ||
+      !(s->feat_lim > 0) || !(s->tol_T >= 0) || s->dT_max != s->dT_max)
+    return IZEM_E_ARG;
+  const IzBox b = react_box(ctx);
+  if (b.n > INT_MAX) return IZEM_E_ARG;
+  IZ_CHECK(hipStreamSynchronize(ctx->stream));
+  IzSurr *u = ch->surr;
+  if (u && (u->chunk != chunk || u->list_cap < b.n)) {
+    surr_free(u);
+    u = ch->surr = nullptr;
+  }
+  if (!u) {
+    u = (IzSurr *)calloc(1, sizeof(IzSurr));
+    if (!u) return IZEM_E_NOMEM;
+    u->chunk = chunk;
+    u->list_cap = b.n > 0 ? b.n : 1;
+    if (hipMalloc(&u->X, chunk * IZ_SURR_XLD * sizeof(unsigned short)) != hipSuccess ||
+        hipMalloc(&u->pred, chunk * 64 * sizeof(float)) != hipSuccess ||
+        hipMalloc(&u->list, u->list_cap * sizeof(int)) != hipSuccess ||
+        hipMalloc(&u->nfall, sizeof(int)) != hipSuccess ||
+        hipMalloc(&u->counts, 2 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(u->counts, 0, 2 * sizeof(unsigned long long)) != hipSuccess) {
+      surr_free(u);
+      return IZEM_E_NOMEM;
+    }
+    ch->surr = u;
+  }
+  u->d = *s;
+  ch->sel_gen++;
+  return IZEM_OK;
+}
+
+extern "C" int izem_reaction_surrogate(izem_ctx *ctx, const double *dt, double scale, double *T,
+                                       double *W_i, double *cp, double *ha, double *v) {
+  if (!ctx || !ctx->chem || !dt || !T || !W_i || !cp || !ha || !v) return IZEM_E_ARG;
+  IzChem *ch = ctx->chem;
+  IzSurr *u = ch->surr;
+  if (!u) return IZEM_E_STATE;
+  const IzBox b = react_box(ctx);
+  const long n = b.n;
+  if (n > u->list_cap) return IZEM_E_STATE;
+  hipLaunchKernelGGL(k_chem_prep, dim3(1), dim3(1), 0, ctx->stream, ch->counter, ctx->ts, 1);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, ctx->stream, u->nfall);
+  if (n > 0) {
+    const double rho_cgs = ctx->o.rho_ref * 1.0e-3;
+    const izem_surrogate &sd = u->d;
+    for (long c0 = 0; c0 < n; c0 += u->chunk) {
+      const int m = (int)((n - c0 < u->chunk) ? n - c0 : u->chunk);
+      int pk = iz_prof_begin(ctx, IZP_CHEM);
+      hipLaunchKernelGGL(k_surr_flow_features, dim3(iz_blocks(m, 256)), dim3(256), 0,
+                         ctx->stream, ctx->d, b, c0, m, dt, scale, ctx->o.time_ref, ctx->o.T_ref,
+                         rho_cgs, T, v, sd.in_mean, sd.in_istd, u->X);
+      iz_prof_end(ctx, pk);
+      int r = izem_mlp_infer(ctx, m, u->X, sd.Wt, sd.b, u->pred);
+      if (r) return r;
+      pk = iz_prof_begin(ctx, IZP_CHEM);
+      hipLaunchKernelGGL(k_surr_flow_apply, dim3(iz_blocks(m, 256)), dim3(256), 0, ctx->stream,
+                         ctx->d, b, c0, m, u->pred, sd.out_mean, sd.out_std, sd.out_c, sd.proj,
+                         u->X, sd.feat_lim, sd.tol_T, sd.dT_max, ctx->o.T_ref, T, v, u->nfall,
+                         u->list, ctx->ts);
+      iz_prof_end(ctx, pk);
+    }
+    // the fallback cells through the reactor kernel (a queue as long as
+    // the list, read on the device)
+    int pk = iz_prof_begin(ctx, IZP_CHEM);
+    reaction_queue(ctx, b, dim3(persistent_grid(ctx, n)), dt, scale, T, v, u->list, u->nfall);
+    iz_prof_end(ctx, pk);
+    ch->order_n = 0;
+    hipLaunchKernelGGL(k_surr_count, dim3(1), dim3(1), 0, ctx->stream, u->counts, n, u->nfall);
+    IZ_CHECK(hipGetLastError());
+  }
+  return reaction_tail(ctx, T, W_i, cp, ha, v);
+}
+
+extern "C" int izem_reac_selec(izem_ctx *ctx, int solver) {
+  if (!ctx || !ctx->chem || solver < IZEM_REAC_DVODE || solver > IZEM_REAC_SURROGATE)
+    return IZEM_E_ARG;
+  if (solver == IZEM_REAC_SURROGATE && !ctx->chem->surr) return IZEM_E_STATE;
+  ctx->chem->reac_selec = solver;
+  ctx->chem->sel_gen++;
+  return IZEM_OK;
+}
+
+extern "C" int izem_reaction_selec(izem_ctx *ctx, const double *dt, double scale, double *T,
+                                   double *W_i, double *cp, double *ha, double *v) {
+  if (!ctx || !ctx->chem) return IZEM_E_ARG;
+  if (ctx->chem->reac_selec == IZEM_REAC_SURROGATE)
+    return izem_reaction_surrogate(ctx, dt, scale, T, W_i, cp, ha, v);
+  return izem_reaction_dvode(ctx, dt, scale, T, W_i, cp, ha, v);
+}
+
+extern "C" int izem_surr_counts(izem_ctx *ctx, unsigned long long out[2]) {
+  if (!ctx || !ctx->chem || !out) return IZEM_E_ARG;
+  IzSurr *u = ctx->chem->surr;
+  if (!u) return IZEM_E_STATE;
+  IZ_CHECK(hipMemcpyAsync(out, u->counts, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  IZ_CHECK(hipStreamSynchronize(ctx->stream));
+  IZ_CHECK(hipMemset(u->counts, 0, 2 * sizeof(unsigned long long)));
   return IZEM_OK;
 }

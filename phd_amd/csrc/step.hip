@@ -21,9 +21,9 @@ static int step_body(izem_ctx *c, const double *dx_i, const double *dy_i, const 
   int r = izem_timestep(c, dx_i, dy_i, dz_i, cp, W_i, T, v);
   if (r) return r;
   const double *dt = c->ts + 4;  // dtmin of the time state
-  if (react && (r = izem_reaction_dvode(c, dt, 0.5, T, W_i, cp, ha, v))) return r;
+  if (react && (r = izem_reaction_selec(c, dt, 0.5, T, W_i, cp, ha, v))) return r;
   if ((r = izem_rk3(c, dt, dx_i, dy_i, dz_i, T, W_i, cp, ha, sensor, v))) return r;
-  if (react && (r = izem_reaction_dvode(c, dt, 0.5, T, W_i, cp, ha, v))) return r;
+  if (react && (r = izem_reaction_selec(c, dt, 0.5, T, W_i, cp, ha, v))) return r;
   return IZEM_OK;
 }
 

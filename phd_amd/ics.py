@@ -126,7 +126,8 @@ def _gaussian_random_field(n: tuple, ndim: int, corr_len_cells: float, rms: floa
 def conserved_from_primitive(box: Box, adi: cd.Adim, thd: cd.Thd, T_K: np.ndarray, p_Pa,
                              Y: np.ndarray, u_ms: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     """Interior arrays (physical units) -> nondimensional conserved v with
-    ghosts and T.  T_K (nz,ny,nx), Y (9,nz,ny,nx), u_ms (3,nz,ny,nx)."""
+    ghosts and T.  T_K (nz,ny,nx) or any shape that broadcasts to it,
+    Y (9,nz,ny,nx), u_ms (3,nz,ny,nx)."""
     from .thermo_host import ha_scalar_vec
 
     nv = cd.NV
@@ -213,3 +214,51 @@ def config3_cells(thd, ncell: int, seed: int = 3):
     Yp /= Yp.sum()
     Y[:] = (1.0 - prod)[:, None] * Ym + prod[:, None] * Yp[None, :]
     return T, p, Y
+
+
+def _mole_to_mass(thd, X: dict) -> np.ndarray:
+    idx = {nm.upper(): i for i, nm in enumerate(thd.names)}
+    x = np.zeros(len(thd.names))
+    for name, val in X.items():
+        x[idx[name]] = val
+    W = np.asarray(thd.W, dtype=np.float64)
+    y = x * W
+    return y / y.sum()
+
+
+def config5_state(adi, thd, n: int = 384, seed: int = 5, L: float = 0.008,
+                  delta_w: float = 5.0e-4, perturb_rms: float = 2.0):
+    """Periodic double shear layer of the coupled learned-chemistry workload
+    on an n^3 periodic box of side L: a fuel band (50 % H2 / 50 % N2 by mole,
+    300 K, +100 m/s in x) between y = L/4 and y = 3L/4 inside an oxidiser
+    (air, 1200 K, -100 m/s), p = 1 atm.  Across each layer a mixing
+    variable goes from 0 to 1 as 0.5 (1 + tanh(2 (y - y_l) / delta_w)), the
+    tanh profile of vorticity thickness delta_w (module_ICs.f90:1063-1207
+    uses the same form); u, Y and T are linear in it.  Two layers keep the
+    box periodic in y, so the flow path runs unchanged (the reference's
+    single layer needs the non-reflecting y boundaries instead).  A seeded
+    broadband solenoidal perturbation (k^4 exp(-2 (k/k0)^2), k0 = 8), weighted
+    by exp(-((y - y_l) / delta_w)^2) around each layer, has an rms of
+    perturb_rms m/s (2 % of the stream speed) where that weight exceeds 1/2."""
+    box = Box(n, n, n, 3, 3, (L, L, L))
+    y = (np.arange(n) + 0.5) * (L / n)
+    y1, y2 = 0.25 * L, 0.75 * L
+    Z = 0.5 * (np.tanh(2.0 * (y - y1) / delta_w) - np.tanh(2.0 * (y - y2) / delta_w))
+    Yf = _mole_to_mass(thd, {"H2": 0.5, "N2": 0.5})
+    Yo = _mole_to_mass(thd, {"O2": 0.21, "N2": 0.79})
+    # profiles along y (array axis 1 of [z][y][x])
+    Zc = Z[None, :, None]
+    # T varies along y only: the enthalpies are evaluated on the profile
+    T = 300.0 * Zc + 1200.0 * (1.0 - Zc)
+    Y = Yf[:, None, None, None] * Zc[None] + Yo[:, None, None, None] * (1.0 - Zc[None])
+    Y = np.broadcast_to(Y, (9, n, n, n))
+    u = np.zeros((3, n, n, n))
+    u[0] = 100.0 * Zc - 100.0 * (1.0 - Zc)
+    w = (np.exp(-((y - y1) / delta_w) ** 2) + np.exp(-((y - y2) / delta_w) ** 2))[None, :, None]
+    pert = _spectral_velocity((n, n, n), 3, 8.0, 1.0, seed) * w[None]
+    inside = np.broadcast_to(w >= 0.5, (n, n, n))
+    rms = np.sqrt(np.mean(pert[:, inside] ** 2))
+    u += pert * (perturb_rms / rms)
+    del pert
+    v, Tn = conserved_from_primitive(box, adi, thd, T, P_ATM, np.ascontiguousarray(Y), u)
+    return box, v, Tn

@@ -9,7 +9,8 @@ conservation and T advances by the predicted dT; a per-cell error
 indicator sends cells back to the BDF path:
 
   * a feature outside the training range (|normalised feature| > feat_lim,
-    6 standard deviations of the training data),
+    6 standard deviations of the training data; the spreads of ln p, T,
+    Y_k and ln dt have floors of 0.01, 1 K, 1e-8 and 0.05),
   * a mass fraction below -1e-7 after the projection,
   * |T(energy, projected Y) - (T0 + dT_net)| > tol_T: the network's
     temperature disagrees with the one its species imply through the
@@ -171,7 +172,12 @@ class Surrogate:
         F = np.concatenate(feats)
         G = np.concatenate(tgts)
         mean, std = F.mean(axis=0), F.std(axis=0)
-        std[std < 1e-12] = 1.0
+        # floors on the spreads: data at nearly one pressure, temperature
+        # or dt (an isobaric initial state, say) would otherwise make the
+        # feature-range test reject cells for physically negligible
+        # departures (a 1e-4 relative pressure fluctuation)
+        floor = np.array([0.01, 1.0] + [1.0e-8] * 9 + [0.05])
+        std = np.maximum(std, floor)
         self.in_mean = torch.tensor(mean, dtype=torch.float32, device=self.dev)
         self.in_istd = torch.tensor(1.0 / std, dtype=torch.float32, device=self.dev)
         # targets asinh(change / c): c = 0.5 K for T, 1e-6 for mass fractions
@@ -306,6 +312,18 @@ class Surrogate:
         return hist
 
     # ------------------------------------------------------------ inference
+    def descriptor(self) -> _lib.SurrogateDesc:
+        """The C-ABI description of the trained network (izem_surrogate)."""
+        d = _lib.SurrogateDesc()
+        for l in range(4):
+            d.Wt[l] = self.Wt[l].data_ptr()
+            d.b[l] = self.b[l].data_ptr()
+        d.in_mean, d.in_istd = self.in_mean.data_ptr(), self.in_istd.data_ptr()
+        d.out_mean, d.out_std = self.out_mean.data_ptr(), self.out_std.data_ptr()
+        d.out_c, d.proj = self.out_c.data_ptr(), self.proj.data_ptr()
+        d.feat_lim, d.tol_T, d.dT_max = self.feat_lim, self.tol_T, self.dT_max
+        return d
+
     def infer(self, X: torch.Tensor) -> torch.Tensor:
         """Scaled outputs [n][64] from the fused single-kernel network."""
         n = X.shape[0]
@@ -352,7 +370,8 @@ class Surrogate:
     def calibrate(self, L: Labelled, target_K: float = 2.0, margin: float = 0.75):
         """Pick (tol_T, dT_max) so that the accepted held-out cells stay
         within margin * target_K of the BDF temperature, accepting as many
-        cells as possible."""
+        cells as possible; when their largest error is below that budget the
+        thresholds are widened by the ratio (at most 10x)."""
         n = L.T.numel()
         T, Y = L.T.clone(), L.Y.clone()
         Te_d = torch.empty(n, dtype=torch.float64, device=self.dev)
@@ -385,13 +404,15 @@ class Surrogate:
                 if err[acc].max() <= margin * target_K and acc.mean() > best[0]:
                     best = (float(acc.mean()), float(tt), float(dm))
         tt, dm = best[1], best[2]
-        if best[0] == 1.0 and ok.all():
-            # every held-out cell passed with room to spare: the data bound
-            # the thresholds only from below, so widen them by the ratio of
-            # the error budget to the largest error seen (at most 10x)
-            room = margin * target_K / max(float(err.max()), 1e-300)
+        acc = ok & (gap <= tt) & (dT <= dm)
+        if acc.any():
+            # the accepted held-out cells bound the thresholds only from
+            # below when their largest error is under the budget: widen
+            # them by the ratio of the budget to that error (at most 10x)
+            room = margin * target_K / max(float(err[acc].max()), 1e-300)
             f = min(10.0, room)
-            tt, dm = tt * f, dm * f
+            if f > 1.0:
+                tt, dm = tt * f, dm * f
         self.tol_T, self.dT_max = tt, dm
         return {"accept_fraction": best[0], "tol_T": self.tol_T, "dT_max": self.dT_max,
                 **self.last_calibration}

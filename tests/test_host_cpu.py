@@ -66,6 +66,45 @@ def test_initial_conditions(adi, thd):
     assert np.max(np.abs(div)) <= 1e-9 * np.max(np.abs(KX * uh))
 
 
+def test_shear_layer_state(adi, thd):
+    """The double shear layer: streams at the band centre and edge, the
+    tanh mixing profile, p = 1 atm everywhere, element masses linear in the
+    mixing variable, and the layer-weighted perturbation rms."""
+    from phd_amd import ics
+    n, L, dw = 48, 0.008, 5.0e-4
+    box, v, T = ics.config5_state(adi, thd, n=n, L=L, delta_w=dw)
+    I = box.interior()
+    rho = v[0][I]
+    Y = v[5:14][(slice(None),) + I] / rho
+    assert np.max(np.abs(Y.sum(axis=0) - 1.0)) < 1e-13
+    y = (np.arange(n) + 0.5) * (L / n)
+    Z = 0.5 * (np.tanh(2 * (y - L / 4) / dw) - np.tanh(2 * (y - 3 * L / 4) / dw))
+    TK = T[I] * adi.T_ref
+    assert np.allclose(TK, (300.0 * Z + 1200.0 * (1 - Z))[None, :, None], rtol=1e-12)
+    names = [nm.upper() for nm in thd.names]
+    W = np.asarray(thd.W)
+    Xc = Y[:, 0, n // 2, 0] / W
+    Xc /= Xc.sum()
+    Xe = Y[:, 0, 0, 0] / W
+    Xe /= Xe.sum()
+    assert Xc[names.index("H2")] == pytest.approx(0.5, abs=1e-6)
+    assert Xe[names.index("O2")] == pytest.approx(0.21, abs=1e-6)
+    W_i = np.tensordot(thd.Wc_i, Y, axes=(0, 0))
+    p = rho * T[I] * W_i * adi.p_ref
+    assert np.allclose(p, 101325.0, rtol=1e-12)
+    kn = np.asarray(thd.kncf, float)
+    aw = np.asarray(thd.awt, float)
+    E = kn * aw[:, None] / (aw @ kn)[None, :]
+    e = np.tensordot(E, Y, axes=(1, 0))
+    e_lin = e[:, 0, n // 2, 0][:, None] * Z[None] + e[:, 0, 0, 0][:, None] * (1 - Z[None])
+    assert np.allclose(e[:, 0, :, 0], e_lin, atol=1e-7)
+    u = np.stack([v[1 + c][I] / rho for c in range(3)]) * adi.u_ref
+    u[0] -= (100.0 * Z - 100.0 * (1 - Z))[None, :, None]
+    wgt = np.exp(-((y - L / 4) / dw) ** 2) + np.exp(-((y - 3 * L / 4) / dw) ** 2)
+    inside = np.broadcast_to((wgt >= 0.5)[None, :, None], (n, n, n))
+    assert np.sqrt(np.mean(u[:, inside] ** 2)) == pytest.approx(2.0, rel=1e-9)
+
+
 def test_config3_cells(thd):
     from phd_amd import ics
     T, p, Y = ics.config3_cells(thd, 4096)
