@@ -13,10 +13,11 @@
  * arrays (see the ISO_C_BINDING module phd_amd/fortran/izem_hip_mod.f90).
  *
  * All functions enqueue on the context's HIP stream and return 0 on
- * success or a negative IZEM_E_* code; physical failures that the
- * reference aborts on (negative pressure, negative temperature, NaN,
- * Newton or VODE non-convergence) are accumulated in a status word read by
- * izem_status().  A context is not thread-safe; one context per host
+ * success or a negative IZEM_E_* code; the physical events that the
+ * reference aborts on (negative pressure, NaN, unrepairable negative
+ * temperature, VODE failure) or warns about (a repaired negative
+ * temperature, Newton non-convergence) are accumulated in a status word
+ * read by izem_status().  A context is not thread-safe; one context per host
  * thread.  Citations are to saras13015/Phd devel-izem/src/<file>:<line>.
  */
 #ifndef IZEM_HIP_H
@@ -36,10 +37,11 @@ extern "C" {
 
 /* status bits (izem_status) */
 #define IZEM_ST_NEG_PRESSURE 1  /* module_weno.f90:1027-1036 */
-#define IZEM_ST_NEG_TEMPERATURE 2 /* module_thermodynamics.f90:125-157 */
+#define IZEM_ST_NEG_TEMPERATURE 2 /* repaired (a warning): module_thermodynamics.f90:120-157 */
 #define IZEM_ST_NAN 4           /* module_thermodynamics.f90:180 */
 #define IZEM_ST_NEWTON_ITMAX 8  /* module_thermodynamics.f90:177 */
 #define IZEM_ST_VODE_FAIL 16    /* chemkin/reaction_v.f:59-62 */
+#define IZEM_ST_NEG_T_ABORT 32  /* repair impossible (>40000 cells or at the array edge): :131 */
 
 typedef struct izem_ctx izem_ctx;
 

@@ -516,45 +516,86 @@ static double newton_cell(const ora_thd *thd, const double *vc, double T, double
 
 /* prim_inv_var: src/module_thermodynamics.f90:79-186.  All cells of the
  * domain iterate together until sqrt(max DF^2) < 1e-5 or 200 iterations;
- * cp and ha keep the values of the last evaluation (pre-update T).  The
- * negative-temperature repair branch (:125-157) is reported as status -2
- * instead of being applied. */
+ * cp and ha keep the values of the last evaluation (pre-update T).  W_i is
+ * computed once, before the iterations (:102).
+ *
+ * Negative-temperature repair (:120-157): each iteration first evaluates
+ * the increments of every cell of the domain (newton, :116), then sweeps
+ * the cells in storage order adding them; a cell whose T turns negative is
+ * recorded, its conserved variables are replaced by the mean of its x
+ * neighbours (so a negative neighbour met earlier in the same sweep enters
+ * with its replaced values), and the iteration count restarts.  After the
+ * exit test, every cell recorded so far gets T = mean of its x neighbours'
+ * T, at every later iteration.  The report file of :141-153 is not
+ * written.  Returns 0, 1 (a repair was made: the reference's warning),
+ * -2 (more than 40000 repaired cells, where the reference aborts, or a
+ * repaired cell whose x neighbour lies outside the allocated array, where
+ * it reads out of bounds) or -3 (NaN). */
 int ora_prim_inv_var(int domain, const ora_grid *g, const ora_thd *thd, double *v, double *W_i,
                      double *T, double *cp, double *ha, int *iters) {
-  const int itmax = 200;
+  const int itmax = 200, nptnegT_max = 40000;
   const double eps_temp = 1.0e-5;
   int i0, i1, j0, j1, k0, k1;
   long nc = ncell(g);
+  const int NX = g->nx + 2 * g->ng;
   domain_select(g, domain, &i0, &i1, &j0, &j1, &k0, &k1);
   ora_wmix_i(domain, g, thd, v, W_i);
-  int it = 0, status = 0;
+  const long nd = (long)(i1 - i0 + 1) * (j1 - j0 + 1) * (k1 - k0 + 1);
+  double *DF = malloc(sizeof(double) * (nd > 0 ? nd : 1));
+  long *rec = NULL;
+  int it = 0, status = 0, ipt = 0, correct_T = 0;
   double error = 0.0;
   for (;;) {
     it = it + 1;
     if (it > itmax) break;
-    double maxdf = -INFINITY;
-    int nan_seen = 0;
+    long q = 0;
     for (int k = k0; k <= k1; k++)
       for (int j = j0; j <= j1; j++)
-        for (int i = i0; i <= i1; i++) {
+        for (int i = i0; i <= i1; i++, q++) {
           long c = cidx(g, i, j, k);
           double vc[NV], hc[NRV], cpc;
           for (int l = 0; l < NV; l++) vc[l] = v[l * nc + c];
-          double DF = newton_cell(thd, vc, T[c], &cpc, hc);
+          DF[q] = newton_cell(thd, vc, T[c], &cpc, hc);
           cp[c] = cpc;
           for (int l = 0; l < NRV; l++) ha[l * nc + c] = hc[l];
-          T[c] = T[c] + DF;
-          DF = DF * DF;
-          if (T[c] < 0.0) status = -2;
-          if (DF != DF) nan_seen = 1;
-          maxdf = fmax(maxdf, DF);
+        }
+    double maxdf = -INFINITY;
+    int nan_seen = 0;
+    q = 0;
+    for (int k = k0; k <= k1; k++)
+      for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++, q++) {
+          long c = cidx(g, i, j, k);
+          T[c] = T[c] + DF[q];
+          double d2 = DF[q] * DF[q];
+          if (T[c] < 0.0) {
+            ipt = ipt + 1;
+            const int ia = i + g->ng;  /* array index along x */
+            if (ipt > nptnegT_max || ia < 1 || ia > NX - 2) {
+              status = -2;
+              goto done;
+            }
+            if (!rec) rec = malloc(sizeof(long) * nptnegT_max);
+            rec[ipt - 1] = c;
+            it = 0;
+            correct_T = 1;
+            status = 1;
+            for (int l = 0; l < NV; l++)
+              v[l * nc + c] = 0.5 * (v[l * nc + c - 1] + v[l * nc + c + 1]);
+          }
+          if (d2 != d2) nan_seen = 1;
+          maxdf = fmax(maxdf, d2);
         }
     error = nan_seen ? NAN : sqrt(maxdf);
     if (error < eps_temp) break;
-    if (status) break;
+    if (correct_T)
+      for (int l = 0; l < ipt; l++) T[rec[l]] = 0.5 * (T[rec[l] - 1] + T[rec[l] + 1]);
   }
-  if (iters) *iters = it;
   if (error != error) status = -3;
+done:
+  if (iters) *iters = it;
+  free(DF);
+  free(rec);
   return status;
 }
 

@@ -525,15 +525,18 @@ __device__ inline void d_force_sumY(double *vc) {
 
 // Newton probe: iterate one cell and record in bit (it-1) whether its
 // increment blocks the global exit test sqrt(max DF^2) < eps at iteration it
-// (module_thermodynamics.f90:111-150).  Cells stop once their increment has
-// fallen far below eps: Newton's quadratic contraction keeps all later
-// increments below it.
-__device__ inline unsigned long long d_newton_probe(const double *vc, double T, double eps) {
+// (module_thermodynamics.f90:111-150), and in bit (it-1) of neg whether its
+// T is negative after iteration it (the repair branch, :120-139).  Cells
+// stop once their increment has fallen far below eps: Newton's quadratic
+// contraction keeps all later increments below it.
+__device__ inline unsigned long long d_newton_probe(const double *vc, double T, double eps,
+                                                    unsigned long long &neg) {
   unsigned long long mask = 0ull;
   for (int it = 1; it <= 64; it++) {
     double cpv, h[IZ_NRV];
     double DF = d_newton_cell(vc, T, cpv, h);
     T = T + DF;
+    if (T < 0.0) neg |= 1ull << (it - 1);
     double e = sqrt(DF * DF);
     if (!(e < eps)) mask |= 1ull << (it - 1);
     else if (e < eps * 1.0e-3) break;
@@ -556,6 +559,31 @@ __device__ inline void block_or_u64(unsigned long long m, unsigned long long *ds
     unsigned long long a = 0ull;
     for (int w = 0; w < (int)(blockDim.x >> 6); w++) a |= s_or[w];
     if (a) atomicOr(dst, a);
+  }
+}
+// the Newton probe's two masks: blocking iterations into dst[0], negative
+// temperatures into dst[IZ_NEG_MASKS]
+#define IZ_NEG_MASKS 8
+__device__ inline void block_or_probe(unsigned long long m, unsigned long long neg,
+                                      unsigned long long *dst) {
+  __shared__ unsigned long long s_or2[2][16];
+  for (int off = 32; off > 0; off >>= 1) {
+    m |= __shfl_xor(m, off, 64);
+    neg |= __shfl_xor(neg, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_or2[0][threadIdx.x >> 6] = m;
+    s_or2[1][threadIdx.x >> 6] = neg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0ull, b = 0ull;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+      a |= s_or2[0][w];
+      b |= s_or2[1][w];
+    }
+    if (a) atomicOr(dst, a);
+    if (b) atomicOr(dst + IZ_NEG_MASKS, b);
   }
 }
 
@@ -596,7 +624,7 @@ k_force_probe(IzDims d, IzBox b, int forcesum, double *__restrict__ v,
                               const double *__restrict__ T, double eps,
                               unsigned long long *__restrict__ mask) {
   int i, j, k;
-  unsigned long long m = 0ull;
+  unsigned long long m = 0ull, neg = 0ull;
   for (long tid = (long)blockIdx.x * blockDim.x + threadIdx.x; box_cell(b, tid, i, j, k);
        tid += (long)gridDim.x * blockDim.x) {
     long c = iz_cidx(d, i, j, k);
@@ -608,9 +636,9 @@ k_force_probe(IzDims d, IzBox b, int forcesum, double *__restrict__ v,
 #pragma unroll
       for (int l = IZ_NIV; l < IZ_NV; l++) v[l * d.nc + c] = vc[l];
     }
-    if (eps > 0.0) m |= d_newton_probe(vc, T[c], eps);
+    if (eps > 0.0) m |= d_newton_probe(vc, T[c], eps, neg);
   }
-  block_or_u64(m, mask);
+  block_or_probe(m, neg, mask);
 }
 
 // number of global iterations from the OR of all probes: the first
@@ -620,8 +648,17 @@ __device__ inline int d_newton_count(unsigned long long mask, int itmax) {
   return __builtin_ctzll(~mask) + 1;
 }
 
+// whether a cell's T turns negative before the global exit: the first
+// negative iteration (neg mask) is not after the exit iteration N
+__device__ inline bool d_newton_repairs(unsigned long long mask, unsigned long long neg,
+                                        int itmax) {
+  return neg && __builtin_ctzll(neg) + 1 <= d_newton_count(mask, itmax);
+}
+
 // Apply exactly N iterations, write T, W_i, cp (last evaluation, i.e. at the
 // pre-update T), ha (module_thermodynamics.f90:79-186, Wmix_i :700-728).
+// When a cell turns negative before the exit, nothing is written here and
+// k_newton_repair runs the domain's iterations instead.
 __global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(2, 2)))
 k_newton_apply(IzDims d, IzBox b, const double *__restrict__ v,
                                double *__restrict__ T, double *__restrict__ W_i,
@@ -631,6 +668,7 @@ k_newton_apply(IzDims d, IzBox b, const double *__restrict__ v,
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (!box_cell(b, tid, i, j, k)) return;
+  if (d_newton_repairs(*mask, mask[IZ_NEG_MASKS], itmax)) return;
   const int N = d_newton_count(*mask, itmax);
   if (tid == 0 && iters) *iters = N;
   long c = iz_cidx(d, i, j, k);
@@ -651,11 +689,160 @@ k_newton_apply(IzDims d, IzBox b, const double *__restrict__ v,
 #pragma unroll
   for (int l = 0; l < IZ_NRV; l++) ha[l * d.nc + c] = h[l];
   int st = 0;
-  if (Tc < 0.0) st |= IZ_ST_NEG_TEMPERATURE;
+  // negative only after iteration 64, beyond what the probe records
+  if (Tc < 0.0) st |= IZ_ST_NEG_T_ABORT;
   if (Tc != Tc) st |= IZ_ST_NAN;
   if (N >= itmax) st |= IZ_ST_NEWTON_ITMAX;
   if (st) atomicOr(status, st);
 }
+
+// The negative-temperature repair of prim_inv_var (module_thermodynamics.f90:
+// 79-186), run when the probe saw a cell of the domain turn negative before
+// the global exit: one block redoes the domain's iterations from the
+// caller's T in the reference's order.  Each iteration evaluates every
+// cell's increment (cp, ha at the pre-update T), adds it, and collects the
+// cells whose T is negative; those are then taken in storage order:
+// recorded, their conserved variables replaced by the mean of their x
+// neighbours (a negative neighbour met earlier in the sweep enters with its
+// replaced values), the iteration count restarted.  After the exit test
+// every recorded cell gets T = mean of its x neighbours' T.  W_i comes from
+// the conserved variables before any replacement (:102).  A rare, slow path
+// (one block; seconds on a 256^3 domain): status IZ_ST_NEG_TEMPERATURE
+// marks it; IZ_ST_NEG_T_ABORT marks a repair the reference aborts on (more
+// than IZ_NEGT_MAX cells, :131), one that would read outside the array (a
+// cell at its x edge) or more than IZ_NEGT_SWEEP negative cells in one
+// iteration.
+#define IZ_NEGT_MAX 40000
+#define IZ_NEGT_SWEEP 1024
+__global__ void __launch_bounds__(1024)
+k_newton_repair(IzDims d, IzBox b, double *__restrict__ v, double *__restrict__ T,
+                double *__restrict__ W_i, double *__restrict__ cp, double *__restrict__ ha,
+                const unsigned long long *__restrict__ mask, int itmax, double eps,
+                int *__restrict__ rec, int *__restrict__ status, int *__restrict__ iters) {
+  if (!d_newton_repairs(*mask, mask[IZ_NEG_MASKS], itmax)) return;
+  __shared__ int s_neg[IZ_NEGT_SWEEP];
+  __shared__ int s_nneg, s_it, s_ipt, s_correct, s_stop, s_bad;
+  __shared__ double s_max[16], s_err;
+  __shared__ int s_nan[16];
+  const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
+  const long n = (long)ni * nj * (b.k1 - b.k0 + 1);
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, w = tid >> 6;
+  for (long t = tid; t < n; t += nt) {
+    int i, j, k;
+    box_cell(b, t, i, j, k);
+    const long c = iz_cidx(d, i, j, k);
+    double wsum = 0.0;
+#pragma unroll
+    for (int l = 0; l < IZ_NRV; l++)
+      wsum = wsum + (v[(IZ_NIV + l) * d.nc + c] / v[c]) * c_thd.Wc_i[l];
+    W_i[c] = wsum;
+  }
+  if (tid == 0) {
+    s_it = 0;
+    s_ipt = 0;
+    s_correct = 0;
+    s_bad = 0;
+    s_err = 0.0;
+  }
+  __syncthreads();
+  for (;;) {
+    if (tid == 0) {
+      s_it = s_it + 1;
+      s_stop = s_it > itmax;
+      s_nneg = 0;
+    }
+    __syncthreads();
+    if (s_stop) break;
+    double mx = -INFINITY;
+    int nan = 0;
+    for (long t = tid; t < n; t += nt) {
+      int i, j, k;
+      box_cell(b, t, i, j, k);
+      const long c = iz_cidx(d, i, j, k);
+      double vc[IZ_NV];
+#pragma unroll
+      for (int l = 0; l < IZ_NV; l++) vc[l] = v[l * d.nc + c];
+      double cpv, h[IZ_NRV];
+      const double DF = d_newton_cell(vc, T[c], cpv, h);
+      cp[c] = cpv;
+#pragma unroll
+      for (int l = 0; l < IZ_NRV; l++) ha[l * d.nc + c] = h[l];
+      const double Tn = T[c] + DF;
+      T[c] = Tn;
+      const double d2 = DF * DF;
+      if (Tn < 0.0) {
+        const int q = atomicAdd(&s_nneg, 1);
+        if (q < IZ_NEGT_SWEEP) s_neg[q] = (int)t;
+      }
+      if (d2 != d2) nan = 1;
+      mx = fmax(mx, d2);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      mx = fmax(mx, __shfl_xor(mx, off, 64));
+      nan |= __shfl_xor(nan, off, 64);
+    }
+    if (lane == 0) {
+      s_max[w] = mx;
+      s_nan[w] = nan;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double m = s_max[0];
+      int nn = s_nan[0];
+      for (int q = 1; q < (nt >> 6); q++) {
+        m = fmax(m, s_max[q]);
+        nn |= s_nan[q];
+      }
+      const double error = nn ? NAN : sqrt(m);
+      s_err = error;
+      const int nneg = s_nneg;
+      if (nneg > IZ_NEGT_SWEEP) s_bad = 1;
+      // this sweep's negative cells in storage order
+      for (int a = 1; a < nneg && !s_bad; a++) {
+        const int x = s_neg[a];
+        int q = a - 1;
+        while (q >= 0 && s_neg[q] > x) {
+          s_neg[q + 1] = s_neg[q];
+          q--;
+        }
+        s_neg[q + 1] = x;
+      }
+      for (int q = 0; q < nneg && !s_bad; q++) {
+        int i, j, k;
+        box_cell(b, s_neg[q], i, j, k);
+        const long c = iz_cidx(d, i, j, k);
+        const int ia = i + d.ng;
+        s_ipt = s_ipt + 1;
+        if (s_ipt > IZ_NEGT_MAX || ia < 1 || ia > (int)d.NX - 2) {
+          s_bad = 1;
+          break;
+        }
+        rec[s_ipt - 1] = (int)c;
+        s_it = 0;
+        s_correct = 1;
+        for (int l = 0; l < IZ_NV; l++)
+          v[l * d.nc + c] = 0.5 * (v[l * d.nc + c - 1] + v[l * d.nc + c + 1]);
+      }
+      s_stop = s_bad || error < eps;
+      if (!s_stop && s_correct)
+        for (int l = 0; l < s_ipt; l++) {
+          const long c = rec[l];
+          T[c] = 0.5 * (T[c - 1] + T[c + 1]);
+        }
+    }
+    __syncthreads();
+    if (s_stop) break;
+  }
+  if (tid == 0) {
+    int st = IZ_ST_NEG_TEMPERATURE;
+    if (s_bad) st |= IZ_ST_NEG_T_ABORT;
+    if (s_err != s_err) st |= IZ_ST_NAN;
+    if (s_it > itmax) st |= IZ_ST_NEWTON_ITMAX;
+    atomicOr(status, st);
+    if (iters) *iters = s_it;
+  }
+}
+
 
 // upd_prim_var_woT: module_thermodynamics.f90:522-689
 __global__ void k_upd_prim_woT(IzDims d, IzBox b, const double *__restrict__ v,
@@ -1108,7 +1295,7 @@ k_rhs_update(IzDims d, int mode, const double *__restrict__ Fx, const double *__
   // one cell per thread (a grid-stride loop here was 2.7x slower: the
   // y/z face and viscous stencil neighbours lose their L2 reuse)
   const long ncin = (long)d.nx * d.ny * d.nz;
-  unsigned long long msk = 0ull;
+  unsigned long long msk = 0ull, neg = 0ull;
   const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid < ncin) {
     int i = (int)(tid % d.nx);
@@ -1155,10 +1342,10 @@ k_rhs_update(IzDims d, int mode, const double *__restrict__ Fx, const double *__
       if (forcesum) d_force_sumY(vc);
 #pragma unroll
       for (int m = 0; m < IZ_NV; m++) v[m * d.nc + c] = vc[m];
-      msk = d_newton_probe(vc, T[c], eps);
+      msk = d_newton_probe(vc, T[c], eps, neg);
     }
   }
-  if (mode != 0) block_or_u64(msk, mask);
+  if (mode != 0) block_or_probe(msk, neg, mask);
 }
 
 // ===================================================================
@@ -1404,7 +1591,8 @@ extern "C" int izem_create(izem_ctx **out, const izem_grid *g, const double *thd
   ok = ok && hipMalloc(&c->fd, sizeof(double) * c->nderiv * d.nc) == hipSuccess;
   ok = ok && hipMalloc(&c->sf, sizeof(double) * c->nsf * d.nc) == hipSuccess;
   ok = ok && hipMalloc(&c->v0, sizeof(double) * IZ_NV * d.nc) == hipSuccess;
-  ok = ok && hipMalloc(&c->masks, sizeof(unsigned long long) * 8) == hipSuccess;
+  ok = ok && hipMalloc(&c->masks, sizeof(unsigned long long) * 2 * IZ_NEG_MASKS) == hipSuccess;
+  ok = ok && hipMalloc(&c->negT_rec, sizeof(int) * IZ_NEGT_MAX) == hipSuccess;
   ok = ok && hipMalloc(&c->status, sizeof(int) * 2) == hipSuccess;
   ok = ok && hipMalloc(&c->iters, sizeof(int) * 8) == hipSuccess;
   ok = ok && hipMalloc(&c->red, sizeof(double) * 24) == hipSuccess;
@@ -1472,6 +1660,7 @@ extern "C" int izem_destroy(izem_ctx *c) {
   if (c->sf) (void)hipFree(c->sf);
   if (c->v0) (void)hipFree(c->v0);
   if (c->masks) (void)hipFree(c->masks);
+  if (c->negT_rec) (void)hipFree(c->negT_rec);
   if (c->status) (void)hipFree(c->status);
   if (c->iters) (void)hipFree(c->iters);
   if (c->red) (void)hipFree(c->red);
@@ -1667,7 +1856,7 @@ static int newton_domains(izem_ctx *c, const int *doms, int nd, int forcesum, do
                           double *W_i, double *T, double *cp, double *ha) {
   const IzDims &d = c->d;
   int pk = iz_prof_begin(c, IZP_NEWTON);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->masks, 7);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->masks, 2 * IZ_NEG_MASKS);
   for (int q = 0; q < nd; q++) {
     IzBox b = domain_box(d, doms[q]);
     hipLaunchKernelGGL(k_force_probe, dim3(iz_blocks_capped(c, box_count(b), 256, 16)), dim3(256), 0,
@@ -1678,6 +1867,9 @@ static int newton_domains(izem_ctx *c, const int *doms, int nd, int forcesum, do
     hipLaunchKernelGGL(k_newton_apply, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0,
                        c->stream, d, b, v, T, W_i, cp, ha, c->masks + (doms[q] + 3), 200,
                        c->status, c->iters + (doms[q] + 3));
+    hipLaunchKernelGGL(k_newton_repair, dim3(1), dim3(1024), 0, c->stream, d, b, v, T, W_i, cp,
+                       ha, c->masks + (doms[q] + 3), 200, 1.0e-5, c->negT_rec, c->status,
+                       c->iters + (doms[q] + 3));
   }
   iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
@@ -1795,7 +1987,7 @@ extern "C" int izem_rk3(izem_ctx *c, const double *dt, const double *dx_i, const
   for (int ik = 0; ik < 3; ik++) {
     int r = stage_fluxes(c, dx_i, dy_i, dz_i, T, W_i, cp, ha, sensor, v);
     if (r) return r;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->masks, 7);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->masks, 2 * IZ_NEG_MASKS);
     int pk = iz_prof_begin(c, IZP_RHS);
     hipLaunchKernelGGL(k_rhs_update, dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0,
                        c->stream, d, 1, c->F[0], c->F[1], c->F[2], c->o.vis, c->sf, c->fd, dx_i,
@@ -1807,6 +1999,8 @@ extern "C" int izem_rk3(izem_ctx *c, const double *dt, const double *dx_i, const
     hipLaunchKernelGGL(k_newton_apply, dim3(iz_blocks(box_count(b0), 256)), dim3(256), 0,
                        c->stream, d, b0, v, T, W_i, cp, ha, c->masks + 3, 200, c->status,
                        c->iters + 3);
+    hipLaunchKernelGGL(k_newton_repair, dim3(1), dim3(1024), 0, c->stream, d, b0, v, T, W_i, cp,
+                       ha, c->masks + 3, 200, 1.0e-5, c->negT_rec, c->status, c->iters + 3);
     iz_prof_end(c, pk);
     IZ_CHECK(hipGetLastError());
     r = iz_comm_cons(c, v, IZ_NV);

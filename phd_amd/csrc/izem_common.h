@@ -43,3 +43,4 @@ __host__ __device__ inline long iz_cidx(const IzDims &d, int i, int j, int k) {
 #define IZ_ST_NAN 4
 #define IZ_ST_NEWTON_ITMAX 8
 #define IZ_ST_VODE_FAIL 16
+#define IZ_ST_NEG_T_ABORT 32

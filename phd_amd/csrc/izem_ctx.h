@@ -56,8 +56,10 @@ struct izem_ctx {
   double *prim, *fd, *sf;
   int nderiv, nsf;
   double *v0;
-  // device words: Newton masks [7], status, iteration count
+  // device words: Newton probe masks (blocking iterations [0..6] and
+  // negative temperatures [8..14] per domain), status, iteration counts
   unsigned long long *masks;
+  int *negT_rec;  // cells recorded by the negative-temperature repair
   int *status;
   int *iters;
   double *red;  // timestep reductions [24]

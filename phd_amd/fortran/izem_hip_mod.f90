@@ -24,7 +24,7 @@ module izem_hip_mod
                                        IZEM_E_NOMEM = -3, IZEM_E_STATE = -4
   integer(c_int), parameter, public :: IZEM_ST_NEG_PRESSURE = 1, IZEM_ST_NEG_TEMPERATURE = 2, &
                                        IZEM_ST_NAN = 4, IZEM_ST_NEWTON_ITMAX = 8, &
-                                       IZEM_ST_VODE_FAIL = 16
+                                       IZEM_ST_VODE_FAIL = 16, IZEM_ST_NEG_T_ABORT = 32
 
   type, bind(C), public :: izem_grid
     integer(c_int) :: nx, ny, nz, ng, ndim
@@ -254,8 +254,10 @@ module izem_hip_mod
 contains
 
   ! ------------------------------------------------------------ part 2
-  ! A nonzero return or status word ends the run with the reference's
-  ! messages (abort_mpi in the reference; error stop here).
+  ! A nonzero return or an aborting status bit ends the run with the
+  ! reference's messages (abort_mpi in the reference; error stop here); a
+  ! repaired negative temperature and Newton non-convergence are warnings
+  ! there (module_thermodynamics.f90:126, 177) and here.
   subroutine izem_check(ctx, rc, where)
     type(c_ptr), intent(in) :: ctx
     integer(c_int), intent(in) :: rc
@@ -267,10 +269,15 @@ contains
     end if
     r2 = izem_status(ctx, bits)
     if (r2 /= IZEM_OK .or. bits == 0) return
+    if (iand(bits, IZEM_ST_NEG_TEMPERATURE) /= 0) &
+      write (*, '(a,a)') 'WARNING: negative temperature repaired after ', where
+    if (iand(bits, IZEM_ST_NEWTON_ITMAX) /= 0) &
+      write (*, '(a,a)') 'WARNING: failure in temperatures convergence after ', where
+    if (iand(bits, ior(ior(IZEM_ST_NEG_PRESSURE, IZEM_ST_NAN), &
+                       ior(IZEM_ST_VODE_FAIL, IZEM_ST_NEG_T_ABORT))) == 0) return
     if (iand(bits, IZEM_ST_NEG_PRESSURE) /= 0) write (*, '(a)') 'negative pressure'
-    if (iand(bits, IZEM_ST_NEG_TEMPERATURE) /= 0) write (*, '(a)') 'too many negative temperatures'
-    if (iand(bits, IZEM_ST_NAN) /= 0) write (*, '(a)') 'NaN in the Newton inversion'
-    if (iand(bits, IZEM_ST_NEWTON_ITMAX) /= 0) write (*, '(a)') 'Newton did not converge'
+    if (iand(bits, IZEM_ST_NEG_T_ABORT) /= 0) write (*, '(a)') 'too many points with negative temperature'
+    if (iand(bits, IZEM_ST_NAN) /= 0) write (*, '(a)') 'error is a NaN'
     if (iand(bits, IZEM_ST_VODE_FAIL) /= 0) write (*, '(a)') 'DVODE failure (ISTATE <= -2)'
     write (*, '(a,a)') 'izem_hip: abort after ', where
     error stop 2

@@ -118,3 +118,29 @@ def oracle_reaction_dvode(run: OracleRun, rc, adi, dt_nondim: float, rtol: float
                                           P(run.W_i), P(run.T), P(run.cp), P(run.ha))
     assert st == 0
     return DTmax
+
+
+def negative_T_case(adi, thd, cells, drop: float = 100.0, n: int = 16, twin: bool = False):
+    """The 2-D hot-spot state on n^2 with the listed interior cells (i, j)
+    spoiled so that their Newton iterate turns negative: N2 and H2 swapped
+    (so that their mixture molar mass differs from their neighbours') and
+    the energy lowered by drop * rho.  twin: the right x neighbour of each
+    cell is made a copy of the left one first (in both states), so that the
+    repaired cell equals its neighbours.  Returns box, the clean v, the
+    spoiled v, T and the array cells."""
+    box, v, T = small_state(adi, thd, "2d", n)
+    g = box.ng
+    if twin:
+        for (i, j) in cells:
+            v[:, g, j + g, i + g + 1] = v[:, g, j + g, i + g - 1]
+            T[g, j + g, i + g + 1] = T[g, j + g, i + g - 1]
+    vb = v.copy()
+    names = [nm.upper() for nm in thd.names]
+    h2, n2 = 5 + names.index("H2"), 5 + names.index("N2")
+    out = []
+    for (i, j) in cells:
+        c = (g, j + g, i + g)
+        vb[(h2,) + c], vb[(n2,) + c] = v[(n2,) + c], v[(h2,) + c]
+        vb[(4,) + c] -= vb[(0,) + c] * drop
+        out.append(c)
+    return box, v, vb, T, out

@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from tests.flowcase import OracleRun, P, small_state
+from tests.flowcase import OracleRun, P, negative_T_case, small_state
 
 pytestmark = pytest.mark.gpu
 TOL = 1.0e-12
@@ -229,6 +229,88 @@ def test_rhs_and_rk3_parity(oracle, adi, thd, kind, n, mode):
     assert err.max() <= TOL
     Tg = s.T.cpu().numpy()
     I = box.interior()
+    assert np.max(np.abs(Tg[I] - run.T[I]) / run.T[I]) <= TOL
+
+
+# negative-temperature repair (module_thermodynamics.f90:120-157): a cell
+# between identical neighbours (the loop exits), one between unequal
+# neighbours (runs to itmax), an adjacent pair, a cell of the x ghost slab,
+# and one at the array's x edge (refused)
+NEGT_CASES = [("converges", [(1, 1)], True, 0), ("itmax", [(1, 1)], False, 0),
+              ("pair", [(6, 8), (7, 8)], False, 0), ("ghost_slab", [(-2, 4)], False, -1),
+              ("edge", [(-3, 4)], False, -1)]
+
+
+@pytest.mark.parametrize("name,cells,twin,dom", NEGT_CASES, ids=[c[0] for c in NEGT_CASES])
+def test_negative_temperature_repair_parity(oracle, adi, thd, name, cells, twin, dom):
+    """izem_prim_inv_var on a domain with spoiled cells against the oracle's
+    prim_inv_var: the same conserved variables (bit for bit), iteration
+    count and status, T / cp / W_i / ha within the Newton parity bar."""
+    box, v, vb, T, cs = negative_T_case(adi, thd, cells, twin=twin)
+    run = OracleRun(oracle, adi, thd, box, vb, T)
+    s = _gpu_solver(box)
+    _upload(s, run)
+    it = C.c_int()
+    st = oracle.lib.ora_prim_inv_var(dom, C.byref(run.g), C.byref(run.t), P(run.v),
+                                     P(run.W_i), P(run.T), P(run.cp), P(run.ha), C.byref(it))
+    it_gpu = C.c_int()
+    assert s.lib.izem_prim_inv_var(s.ctx, dom, C.c_void_p(s.v.data_ptr()),
+                                   C.c_void_p(s.W_i.data_ptr()), C.c_void_p(s.T.data_ptr()),
+                                   C.c_void_p(s.cp.data_ptr()), C.c_void_p(s.ha.data_ptr()),
+                                   C.byref(it_gpu)) == 0
+    s.sync()
+    bits = s.status()
+    print(name, "oracle status", st, "iterations", it.value, "GPU bits", bits, it_gpu.value)
+    if name == "edge":
+        assert st == -2 and bits & 32
+        return
+    assert st == 1 and bits & 2 and not bits & 32
+    assert bool(bits & 8) == (it.value > 200)
+    assert it_gpu.value == it.value
+    assert np.array_equal(s.v.cpu().numpy(), run.v)
+    for name_, ref in (("T", run.T), ("cp", run.cp), ("W_i", run.W_i)):
+        g = getattr(s, name_).cpu().numpy()
+        sel = ref != 0
+        assert np.array_equal(g[~sel], ref[~sel]), name_
+        assert np.max(np.abs(g[sel] - ref[sel]) / np.abs(ref[sel])) <= 1e-14, name_
+    h = s.ha.cpu().numpy()
+    assert np.max(np.abs(h - run.ha)) <= 1e-14 * np.max(np.abs(run.ha))
+
+
+def test_rk3_with_negative_temperature_repair(oracle, adi, thd):
+    """One SSP-RK3 step from a state with a spoiled cell (its energy far
+    below the state's T): the first stage's Newton repairs it; state and T
+    against the oracle's rk3 at the flow bar."""
+    import torch
+    box, v, vb, T, cs = negative_T_case(adi, thd, [(6, 8)])
+    clean = OracleRun(oracle, adi, thd, box, v, T)
+    clean.prim_all()
+    run = OracleRun(oracle, adi, thd, box, vb, clean.T)
+    run.W_i[:], run.cp[:], run.ha[:] = clean.W_i, clean.cp, clean.ha
+    s = _gpu_solver(box, mode=1)
+    _upload(s, run)
+    opts = run.opts(vis=1)
+    ts = np.zeros(4)
+    dtmin = C.c_double(1.0)
+    oracle.lib.ora_timestep(C.byref(run.g), C.byref(run.t), C.byref(run.a), C.byref(opts),
+                            P(run.d_i[0]), P(run.d_i[1]), P(run.d_i[2]), P(run.cp), P(run.W_i),
+                            P(run.T), P(run.v), C.c_double(0.0), P(ts), C.byref(dtmin))
+    sens = np.zeros(box.shape, np.uint8)
+    st = oracle.lib.ora_rk3(C.byref(run.g), C.byref(run.t), C.byref(run.a), C.byref(opts),
+                            C.c_double(dtmin.value), P(run.d_i[0]), P(run.d_i[1]),
+                            P(run.d_i[2]), P(run.T), P(run.W_i), P(run.cp), P(run.ha),
+                            sens.ctypes.data_as(C.POINTER(C.c_uint8)), P(run.v), None)
+    assert st == 1
+    dt = torch.tensor([dtmin.value], dtype=torch.float64, device="cuda:0")
+    s.rk3(dt)
+    s.sync()
+    bits = s.status()
+    assert bits & 2 and not bits & (1 | 4 | 32)
+    err = _relerr(s.v.cpu().numpy(), run.v, box)
+    print("rk3 with a repaired cell: state max rel err", err.max(), "bits", bits)
+    assert err.max() <= TOL
+    I = box.interior()
+    Tg = s.T.cpu().numpy()
     assert np.max(np.abs(Tg[I] - run.T[I]) / run.T[I]) <= TOL
 
 

@@ -80,6 +80,16 @@ MUTANTS = [
      "          div = du[0][0] + du[1][1] + du[2][1];", [P.check_ducros_sensor]),
     ("sensor_density_measure", "double drho = fabs(v[c2] - v[c]) * fmin(ri, ri2);",
      "double drho = fabs(v[c2] - v[c]) * fmax(ri, ri2);", [P.check_sensor_threshold]),
+    ("negT_average_along_y", "v[l * nc + c] = 0.5 * (v[l * nc + c - 1] + v[l * nc + c + 1]);",
+     "v[l * nc + c] = 0.5 * (v[l * nc + c - NX] + v[l * nc + c + NX]);",
+     [P.check_negT_repair_converges, P.check_negT_repair_adjacent_pair]),
+    ("negT_no_restart", "            it = 0;\n            correct_T = 1;",
+     "            correct_T = 1;", [P.check_negT_repair_converges]),
+    ("negT_no_T_reset", "for (int l = 0; l < ipt; l++) T[rec[l]] = 0.5 * (T[rec[l] - 1] + T[rec[l] + 1]);",
+     "for (int l = 0; l < ipt; l++) (void)rec[l];", [P.check_negT_repair_runs_to_itmax]),
+    ("negT_wmix_after_repair", "  if (iters) *iters = it;\n  free(DF);",
+     "  ora_wmix_i(domain, g, thd, v, W_i);\n  if (iters) *iters = it;\n  free(DF);",
+     [P.check_negT_repair_converges]),
 ]
 
 
