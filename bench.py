@@ -44,10 +44,11 @@ def read_traffic(kernel: str, n: int):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(n: int = 48):
-    """The oracle (C restatement of the flow path, one core) plus the
-    reference's own compiled DVODE chemistry, one full reacting step on an
-    n^3 sample of the same workload recipe."""
+def _cpu_step_worker(arg):
+    """One full reacting step (timestep, chemistry(dt/2), SSP-RK3,
+    chemistry(dt/2)) on an n^3 box of the workload recipe: the oracle's C
+    restatement of the flow path and the reference's compiled
+    PSR_CHEMKIN/DVODE per cell.  Returns the step's seconds."""
     import ctypes as C
 
     import numpy as np
@@ -57,9 +58,10 @@ def cpu_baseline(n: int = 48):
     from phd_amd import ics
     from tests.flowcase import OracleRun, P
 
+    n, seed = arg
     adi = cd.Adim()
     thd = cd.build_thd(adi)
-    box, v, T = ics.config2_state(adi, thd, n=n, seed=2, L=0.01 * n / 256)
+    box, v, T = ics.config2_state(adi, thd, n=n, seed=seed, L=0.01 * n / 256)
     run = OracleRun(Oracle(), adi, thd, box, v, T)
     run.prim_all()
     rc = RefChem()
@@ -100,12 +102,65 @@ def cpu_baseline(n: int = 48):
                       P(run.W_i), P(run.cp), P(run.ha), sens.ctypes.data_as(C.POINTER(C.c_uint8)),
                       P(run.v), None)
     chem()
-    el = time.perf_counter() - t0
-    return {"value": n ** 3 / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"one full reacting step (timestep, 2 x chemistry, SSP-RK3) on a {n}^3 "
-                      f"box of the same recipe: oracle C restatement of the flow path "
-                      f"(1 core) + the reference's compiled PSR_CHEMKIN/DVODE per cell; "
-                      f"{el:.1f} s"}
+    return time.perf_counter() - t0
+
+
+def _cpu_c3_worker(arg):
+    """m independent cells of the batched-chemistry workload through the
+    reference's compiled PSR_CHEMKIN/DVODE over 1e-6 s (rtol 1e-6, atol
+    1e-12, the GPU path's tolerances).  Returns (seconds, BDF steps)."""
+    import numpy as np
+
+    from oracle.oracle import RefChem
+    from phd_amd import chemdata as cd
+    from phd_amd import ics
+
+    m, seed = arg
+    thd = cd.build_thd(cd.Adim())
+    T, p, Y = ics.config3_cells(thd, m, seed=seed)
+    Wmix = 1.0 / (Y / (np.array(thd.W) * 1000.0)).sum(axis=1)
+    rho = (p * 10.0) * Wmix / (8.314510e7 * T)
+    rc = RefChem()
+    nst = 0
+    t0 = time.perf_counter()
+    for c in range(m):
+        nst += rc.psr(T[c], rho[c], Y[c], 1.0e-6, 1.0e-6, 1.0e-12)[3]
+    return time.perf_counter() - t0, nst
+
+
+def cpu_baseline(n: int = 48, m: int = 4096):
+    """The CPU path on the host's cores (at most 16, the GPU's share of the
+    box): one process per core, each stepping its own n^3 box of the
+    workload recipe (processes run together; no halo exchange, so the sum
+    of their rates bounds an MPI run of the same cores from above), and
+    the same processes each integrating m batched-chemistry cells.  Must
+    run before the parent process touches the GPU (the workers are
+    forked)."""
+    import multiprocessing as mp
+
+    procs = max(1, min(16, len(os.sched_getaffinity(0))))
+    with mp.get_context("fork").Pool(procs) as pool:
+        t0 = time.perf_counter()
+        step_s = pool.map(_cpu_step_worker, [(n, 2 + r) for r in range(procs)])
+        wall_step = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        c3 = pool.map(_cpu_c3_worker, [(m, 300 + r) for r in range(procs)])
+        wall_c3 = time.perf_counter() - t0
+    rate = sum(n ** 3 / el for el in step_s)
+    c3_rate = sum(m / el for el, _ in c3)
+    return {"value": rate, "unit": "cell-steps/s", "cores": procs, "kind": "port",
+            "sample": f"{procs} processes, each one full reacting step (timestep, 2 x chemistry, "
+                      f"SSP-RK3) on its own {n}^3 box of the same recipe: the oracle's C "
+                      f"restatement of the flow path + the reference's compiled "
+                      f"PSR_CHEMKIN/DVODE per cell; step {min(step_s):.1f}-{max(step_s):.1f} s "
+                      f"per process, {wall_step:.1f} s wall with set-up",
+            "c3_chemistry": {"cells_per_s": c3_rate, "cells": procs * m, "dt_s": 1.0e-6,
+                             "rtol": 1.0e-6, "atol": 1.0e-12, "cores": procs,
+                             "kind": "reference",
+                             "bdf_steps_mean": sum(k for _, k in c3) / (procs * m),
+                             "sample": f"{procs} processes x {m} independent cells of the "
+                                       f"batched-chemistry recipe through the reference's "
+                                       f"compiled PSR_CHEMKIN/DVODE; {wall_c3:.1f} s wall"}}
 
 
 def chem_vode_rate(s, thd, ncell: int):
@@ -358,6 +413,13 @@ def main():
     from phd_amd.flops import EULER_FLOP_PER_FACE
     from phd_amd.solver import Options, Solver, profile_read, profile_start
 
+    cpu = None
+    if not args.no_cpu_baseline:
+        # before anything touches the GPU: the workers are forked
+        log("CPU baseline (oracle flow + reference DVODE, one process per core)")
+        cpu = cpu_baseline()
+        log(f"CPU baseline: {cpu['value']:.3e} cell-steps/s on {cpu['cores']} cores, "
+            f"batched chemistry {cpu['c3_chemistry']['cells_per_s']:.3e} cells/s")
     n = args.n
     adi = cd.Adim()
     thd = cd.build_thd(adi)
@@ -448,6 +510,25 @@ def main():
                                          "rhs_evals": float(chem_stats[1]) / c_cnt,
                                          "jacobians": float(chem_stats[2]) / c_cnt,
                                          "lu_solves": float(chem_stats[3]) / c_cnt}}
+    # the memory-bound flow classes against the HBM peak: algorithmic bytes
+    # per cell (phd_amd.flops) x cells per launch / event-timed launch time
+    from phd_amd.flops import BYTES_PER_CELL, STEP_BYTES_PER_CELL
+    hbm_roofs = {}
+    for cls, cells in (("rhs_update", ncell), ("viscous", ncell + 6 * 3 * n * n)):
+        ms, cnt = prof[cls]
+        if not cnt:
+            continue
+        avg_s = ms / cnt / 1e3  # one record per RK stage (viscous: prim + fused)
+        by = BYTES_PER_CELL[cls] * cells
+        hbm_roofs[cls] = {"kernel": "k_rhs_update" if cls == "rhs_update"
+                          else "k_visc_prim + k_visc_fused", "bound": "hbm",
+                          "achieved": by / avg_s / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": by / avg_s / 1e9 / PEAK_HBM_GBS,
+                          "traffic": read_traffic(cls, n), "avg_stage_ms": avg_s * 1e3,
+                          "bytes_per_cell": BYTES_PER_CELL[cls], "cells": cells}
+    step_hbm = {"bytes_per_cell_step": STEP_BYTES_PER_CELL,
+                "achieved": value * STEP_BYTES_PER_CELL / 1e9, "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": value * STEP_BYTES_PER_CELL / 1e9 / PEAK_HBM_GBS}
     # the roofline object describes the kernel with the largest share of
     # the step; the other one is reported beside it
     dominant = chem_roof if (chem_roof and c_ms > e_ms) else euler_roof
@@ -486,10 +567,6 @@ def main():
             f"{100 * shear['surrogate']['fallback_rate']:.1f} %), peak T difference "
             f"{shear['T_peak_diff_K']:.2e} K, max |dT| {shear['max_abs_dT_K']:.2e} K, "
             f"heat-release field {100 * shear['hrr_l2_rel_diff']:.2f} % (L2)")
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("CPU baseline (oracle flow + reference DVODE, 1 core)")
-        cpu = cpu_baseline()
     out = {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -506,7 +583,9 @@ def main():
                    "flux_mode": args.flux_mode,
                    "status_bits": status},
         "roofline": dominant,
-        "roofline_kernels": {"k_reaction_queue": chem_roof, "k_euler_faces": euler_roof},
+        "roofline_kernels": {"k_reaction_queue": chem_roof, "k_euler_faces": euler_roof,
+                             **{v["kernel"]: v for v in hbm_roofs.values()}},
+        "step_hbm_roofline": step_hbm,
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
         "chem_tuning": (None if tuned is None else
                         {"waves_per_simd": tuned[0], "blocks_per_cu": tuned[1],

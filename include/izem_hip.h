@@ -126,6 +126,12 @@ int izem_force_sumY(izem_ctx *ctx, int domain, double *v);
 int izem_prim_inv_var(izem_ctx *ctx, int domain, double *v, double *W_i, double *T,
                       double *cp, double *ha, int *iters);
 /* upd_prim_var_woT (module_thermodynamics.f90:522) */
+/* prim_inv_var_err_mach (module_thermodynamics.f90:201-260), the
+ * inversion of initialisation and restart: as izem_prim_inv_var to 1e-10
+ * and at most 100 iterations, with no repair (a negative temperature left
+ * at the end raises IZEM_ST_NEG_TEMPERATURE, the reference's warning) */
+int izem_prim_inv_var_mach(izem_ctx *ctx, int domain, double *v, double *W_i, double *T,
+                           double *cp, double *ha, int *iters);
 int izem_upd_prim_var_woT(izem_ctx *ctx, int domain, const double *v, const double *T,
                           double *W_i, double *cp, double *ha);
 /* comm_cons on one rank with periodic faces (module_parallel.f90:1715) */

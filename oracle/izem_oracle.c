@@ -599,6 +599,53 @@ done:
   return status;
 }
 
+/* prim_inv_var_err_mach: src/module_thermodynamics.f90:201-260, the
+ * inversion of initialisation and restart: the global loop of prim_inv_var
+ * to 1e-10 and at most 100 iterations, without the repair branch.  Returns
+ * 0, or the warnings it prints as bits: 2 a negative temperature left,
+ * 8 no convergence. */
+int ora_prim_inv_var_mach(int domain, const ora_grid *g, const ora_thd *thd, double *v,
+                          double *W_i, double *T, double *cp, double *ha, int *iters) {
+  const int itmax_mach = 100;
+  const double eps_temp_mach = 1.0e-10;
+  int i0, i1, j0, j1, k0, k1;
+  long nc = ncell(g);
+  domain_select(g, domain, &i0, &i1, &j0, &j1, &k0, &k1);
+  ora_wmix_i(domain, g, thd, v, W_i);
+  int it = 0;
+  double error = 0.0;
+  for (;;) {
+    it = it + 1;
+    if (it > itmax_mach) break;
+    double maxdf = -INFINITY;
+    int nan_seen = 0;
+    for (int k = k0; k <= k1; k++)
+      for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+          long c = cidx(g, i, j, k);
+          double vc[NV], hc[NRV], cpc;
+          for (int l = 0; l < NV; l++) vc[l] = v[l * nc + c];
+          double DF = newton_cell(thd, vc, T[c], &cpc, hc);
+          cp[c] = cpc;
+          for (int l = 0; l < NRV; l++) ha[l * nc + c] = hc[l];
+          T[c] = T[c] + DF;
+          DF = DF * DF;
+          if (DF != DF) nan_seen = 1;
+          maxdf = fmax(maxdf, DF);
+        }
+    error = nan_seen ? NAN : sqrt(maxdf);
+    if (error < eps_temp_mach) break;
+  }
+  int warn = 0;
+  for (int k = k0; k <= k1; k++)
+    for (int j = j0; j <= j1; j++)
+      for (int i = i0; i <= i1; i++)
+        if (T[cidx(g, i, j, k)] < 0.0) warn |= 2;
+  if (it > itmax_mach) warn |= 8;
+  if (iters) *iters = it;
+  return warn;
+}
+
 /* upd_prim_var_woT: src/module_thermodynamics.f90:522-689 */
 void ora_upd_prim_var_woT(int domain, const ora_grid *g, const ora_thd *thd, const double *v,
                           const double *T, double *W_i, double *cp, double *ha) {

@@ -112,6 +112,8 @@ void ora_wmix_i(int domain, const ora_grid *g, const ora_thd *thd, const double 
 void ora_force_sumY(int domain, const ora_grid *g, double *v);
 int ora_prim_inv_var(int domain, const ora_grid *g, const ora_thd *thd, double *v,
                      double *W_i, double *T, double *cp, double *ha, int *iters);
+int ora_prim_inv_var_mach(int domain, const ora_grid *g, const ora_thd *thd, double *v,
+                          double *W_i, double *T, double *cp, double *ha, int *iters);
 void ora_upd_prim_var_woT(int domain, const ora_grid *g, const ora_thd *thd, const double *v,
                           const double *T, double *W_i, double *cp, double *ha);
 void ora_comm_cons(const ora_grid *g, double *v);

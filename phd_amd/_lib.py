@@ -17,7 +17,7 @@ EXPORTS = [
     "izem_profile_read",
     "izem_wenorec", "izem_euler_llf_x", "izem_euler_llf_y", "izem_euler_llf_z",
     "izem_viscflux", "izem_force_sumY", "izem_prim_inv_var", "izem_upd_prim_var_woT",
-    "izem_comm_cons", "izem_upd_prim_var_ghost", "izem_shock_sensor", "izem_shock_sensor_ducros", "izem_rk3", "izem_rhs",
+    "izem_comm_cons", "izem_upd_prim_var_ghost", "izem_prim_inv_var_mach", "izem_shock_sensor", "izem_shock_sensor_ducros", "izem_rk3", "izem_rhs",
     "izem_timestep", "izem_chem_setup", "izem_chem_tune", "izem_chem_stats", "izem_ckwyr", "izem_psr_batch", "izem_psr_batch_noalloc", "izem_chem_order", "izem_chem_order_read", "izem_reaction_dvode", "izem_step", "izem_flux_mode",
     "izem_mlp_gemm", "izem_mlp_gemm_splitk", "izem_mlp_colsum", "izem_mlp_adam", "izem_mlp_to_bf16",
     "izem_mlp_mse", "izem_mlp_infer", "izem_surr_features", "izem_surr_apply", "izem_surr_fallback",
@@ -90,6 +90,7 @@ def load(path: str | None = None):
     lib.izem_force_sumY.argtypes = [vp, C.c_int, vp]
     lib.izem_prim_inv_var.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, C.POINTER(C.c_int)]
     lib.izem_upd_prim_var_woT.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+    lib.izem_prim_inv_var_mach.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, C.POINTER(C.c_int)]
     lib.izem_comm_cons.argtypes = [vp, vp]
     lib.izem_upd_prim_var_ghost.argtypes = [vp] * 6
     lib.izem_shock_sensor.argtypes = [vp] * 5

@@ -59,7 +59,32 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    build_io(verbose)
     return LIB
+
+
+HDF5 = os.environ.get("IZEM_HDF5_PREFIX", "/opt/conda")
+IO_LIB = os.path.join(HERE, "libizem_io.so")
+
+
+def build_io(verbose: bool = False) -> str | None:
+    """libizem_io.so (restart files, host code) with the host compiler
+    against the HDF5 C library; skipped with a message when HDF5 is absent."""
+    inc, lib = os.path.join(HDF5, "include"), os.path.join(HDF5, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"[build] HDF5 not found under {HDF5}: restart files unavailable", flush=True)
+        return None
+    tmp = IO_LIB + ".tmp"
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-Wall", "-I", inc, "-o", tmp,
+           os.path.join(CSRC, "restart_io.c"), "-L", lib, "-lhdf5", f"-Wl,-rpath,{lib}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("restart I/O library build failed")
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
 
 
 if __name__ == "__main__":

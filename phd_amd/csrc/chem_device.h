@@ -350,6 +350,8 @@ IZ_HD IZ_FORCEINLINE BdfResult d_bdf_integrate(const Kin &kin, Z &z, double (&J)
   const int maxstep = 10000, maxcor = 3, msbp = 20;
   const double ccmax = 0.3, ccmxj = 0.2;
   BdfResult res{0, 0, 0, 1, 0};
+  // a first call with tout = t returns at once, y untouched (vode.f:5475)
+  if (!(tout > 0.0)) return res;
   double acor[N_], ewt[N_], savf[N_], yy[N_];  // ewt: reciprocal weights
   double P[N_][N_];
   int piv[N_];

@@ -664,11 +664,11 @@ k_newton_apply(IzDims d, IzBox b, const double *__restrict__ v,
                                double *__restrict__ T, double *__restrict__ W_i,
                                double *__restrict__ cp, double *__restrict__ ha,
                                const unsigned long long *__restrict__ mask, int itmax,
-                               int *__restrict__ status, int *__restrict__ iters) {
+                               int repair, int *__restrict__ status, int *__restrict__ iters) {
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (!box_cell(b, tid, i, j, k)) return;
-  if (d_newton_repairs(*mask, mask[IZ_NEG_MASKS], itmax)) return;
+  if (repair && d_newton_repairs(*mask, mask[IZ_NEG_MASKS], itmax)) return;
   const int N = d_newton_count(*mask, itmax);
   if (tid == 0 && iters) *iters = N;
   long c = iz_cidx(d, i, j, k);
@@ -689,8 +689,9 @@ k_newton_apply(IzDims d, IzBox b, const double *__restrict__ v,
 #pragma unroll
   for (int l = 0; l < IZ_NRV; l++) ha[l * d.nc + c] = h[l];
   int st = 0;
-  // negative only after iteration 64, beyond what the probe records
-  if (Tc < 0.0) st |= IZ_ST_NEG_T_ABORT;
+  // with the repair: negative only after iteration 64, beyond what the
+  // probe records; without (prim_inv_var_err_mach): the reference's warning
+  if (Tc < 0.0) st |= repair ? IZ_ST_NEG_T_ABORT : IZ_ST_NEG_TEMPERATURE;
   if (Tc != Tc) st |= IZ_ST_NAN;
   if (N >= itmax) st |= IZ_ST_NEWTON_ITMAX;
   if (st) atomicOr(status, st);
@@ -1851,25 +1852,31 @@ extern "C" int izem_force_sumY(izem_ctx *c, int dom, double *v) {
   return IZEM_OK;
 }
 
-// force_sumY (optional) + probe + apply on a list of domains
+// force_sumY (optional) + probe + apply on a list of domains; prim_inv_var
+// (eps 1e-5, 200 iterations, negative-temperature repair) or, with mach,
+// prim_inv_var_err_mach (module_thermodynamics.f90:201-260: eps 1e-10, 100
+// iterations, a negative temperature only warned about)
 static int newton_domains(izem_ctx *c, const int *doms, int nd, int forcesum, double *v,
-                          double *W_i, double *T, double *cp, double *ha) {
+                          double *W_i, double *T, double *cp, double *ha, int mach = 0) {
+  const double eps = mach ? 1.0e-10 : 1.0e-5;
+  const int itmax = mach ? 100 : 200;
   const IzDims &d = c->d;
   int pk = iz_prof_begin(c, IZP_NEWTON);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->masks, 2 * IZ_NEG_MASKS);
   for (int q = 0; q < nd; q++) {
     IzBox b = domain_box(d, doms[q]);
     hipLaunchKernelGGL(k_force_probe, dim3(iz_blocks_capped(c, box_count(b), 256, 16)), dim3(256), 0,
-                       c->stream, d, b, forcesum, v, T, 1.0e-5, c->masks + (doms[q] + 3));
+                       c->stream, d, b, forcesum, v, T, eps, c->masks + (doms[q] + 3));
   }
   for (int q = 0; q < nd; q++) {
     IzBox b = domain_box(d, doms[q]);
     hipLaunchKernelGGL(k_newton_apply, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0,
-                       c->stream, d, b, v, T, W_i, cp, ha, c->masks + (doms[q] + 3), 200,
-                       c->status, c->iters + (doms[q] + 3));
-    hipLaunchKernelGGL(k_newton_repair, dim3(1), dim3(1024), 0, c->stream, d, b, v, T, W_i, cp,
-                       ha, c->masks + (doms[q] + 3), 200, 1.0e-5, c->negT_rec, c->status,
-                       c->iters + (doms[q] + 3));
+                       c->stream, d, b, v, T, W_i, cp, ha, c->masks + (doms[q] + 3), itmax,
+                       !mach, c->status, c->iters + (doms[q] + 3));
+    if (!mach)
+      hipLaunchKernelGGL(k_newton_repair, dim3(1), dim3(1024), 0, c->stream, d, b, v, T, W_i,
+                         cp, ha, c->masks + (doms[q] + 3), itmax, eps, c->negT_rec, c->status,
+                         c->iters + (doms[q] + 3));
   }
   iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
@@ -1880,6 +1887,19 @@ extern "C" int izem_prim_inv_var(izem_ctx *c, int dom, double *v, double *W_i, d
                                  double *cp, double *ha, int *iters) {
   if (!c || !v || !W_i || !T || !cp || !ha || !dom_ok(c->d, dom)) return IZEM_E_ARG;
   int r = newton_domains(c, &dom, 1, 0, v, W_i, T, cp, ha);
+  if (r) return r;
+  if (iters) {
+    IZ_CHECK(hipMemcpyAsync(iters, c->iters + dom + 3, sizeof(int), hipMemcpyDeviceToHost,
+                            c->stream));
+    IZ_CHECK(hipStreamSynchronize(c->stream));
+  }
+  return IZEM_OK;
+}
+
+extern "C" int izem_prim_inv_var_mach(izem_ctx *c, int dom, double *v, double *W_i, double *T,
+                                      double *cp, double *ha, int *iters) {
+  if (!c || !v || !W_i || !T || !cp || !ha || !dom_ok(c->d, dom)) return IZEM_E_ARG;
+  int r = newton_domains(c, &dom, 1, 0, v, W_i, T, cp, ha, 1);
   if (r) return r;
   if (iters) {
     IZ_CHECK(hipMemcpyAsync(iters, c->iters + dom + 3, sizeof(int), hipMemcpyDeviceToHost,
@@ -1997,7 +2017,7 @@ extern "C" int izem_rk3(izem_ctx *c, const double *dt, const double *dx_i, const
     IzBox b0 = domain_box(d, 0);
     pk = iz_prof_begin(c, IZP_NEWTON);
     hipLaunchKernelGGL(k_newton_apply, dim3(iz_blocks(box_count(b0), 256)), dim3(256), 0,
-                       c->stream, d, b0, v, T, W_i, cp, ha, c->masks + 3, 200, c->status,
+                       c->stream, d, b0, v, T, W_i, cp, ha, c->masks + 3, 200, 1, c->status,
                        c->iters + 3);
     hipLaunchKernelGGL(k_newton_repair, dim3(1), dim3(1024), 0, c->stream, d, b0, v, T, W_i, cp,
                        ha, c->masks + 3, 200, 1.0e-5, c->negT_rec, c->status, c->iters + 3);

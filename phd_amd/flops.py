@@ -96,6 +96,26 @@ EULER_FACE_SECTIONS = {
 EULER_FLOP_PER_FACE = sum(EULER_FACE_SECTIONS.values())
 
 
+# Algorithmic HBM bytes per cell (3-D, FP64, 9 species) of the memory-bound
+# flow kernels: every field a kernel must read or write, once (stencil
+# neighbours are the same arrays and count once).
+# k_rhs_update (flux difference + viscous divergence + RK stage update +
+# force_sumY + Newton probe): face fluxes 3 x 14, stresses and heat flux 9,
+# species diffusion fluxes 27, U^0 14, U 14 read and 14 written, T 1.
+RHS_UPDATE_FIELDS = 3 * 14 + 9 + 27 + 14 + 14 + 14 + 1
+# viscous pass: k_visc_prim (reads rho, rho u_i, rho Y_k, W_i; writes
+# u_i, X_k) then k_visc_fused (reads u_i, X_k, rho, rho u_i, rho Y_k, T, W_i,
+# h_k; writes the stresses and heat flux 9 and the diffusion fluxes 27)
+VISC_PRIM_FIELDS = (1 + 3 + 9 + 1) + (3 + 9)
+VISC_FUSED_FIELDS = (3 + 9) + (1 + 3 + 9) + 1 + 1 + 9 + 9 + 27
+BYTES_PER_CELL = {"rhs_update": 8 * RHS_UPDATE_FIELDS,
+                  "viscous": 8 * (VISC_PRIM_FIELDS + VISC_FUSED_FIELDS)}
+# the whole reacting step's compulsory traffic: RHS + SSP-RK3 944 B (stage
+# 1 reads U and writes U^(1) with T; stages 2 and 3 also read U^0) plus the
+# two Strang chemistry substeps 2 x 168 B
+STEP_BYTES_PER_CELL = 944 + 336
+
+
 def lu_ops(n: int = NEQ) -> int:
     """LU with partial pivoting of an n x n matrix: per column one reciprocal,
     per row below one multiplier and 2 operations per updated entry."""

@@ -1,0 +1,148 @@
+"""Run an input deck: the reference's main program (devel-izem/src/izem.f90:
+180-389) on one GPU.
+
+    python -m phd_amd.run DECK [--n N] [--dir DIR]
+
+reads the deck (phd_amd.deck), builds the initial state named by
+GENERAL%INIT or reads the restart MONITORING%NAME_RESTART_<NUMBER_RESTART>.h5
+(READ_RESTART = T), and iterates: time step, Strang-split chemistry around
+SSP-RK3 when REACTING%REACTION = T.  As in the reference, the step that
+would pass the next statistics time (TIME_OFFSET + k DTIME seconds, NSTAT of
+them) is shortened to land on it; there the iteration, time and the four dt
+limits are appended to times_stat.out and restart_<k>.h5 is written
+(STAT = T); the run ends after ITMAX iterations (ITLIM = T) or at the last
+statistics time.  ITSHOW sets how often the progress line is printed.
+
+Initial states (GENERAL%INIT; the reference's own initialisations are
+not rebuilt): HOTSPOT (2-D periodic 1 cm box, stoichiometric H2/air, 900 K
+with a 1500 K Gaussian hot spot, u_rms 10 m/s), HIT (3-D periodic 1 cm box,
+phi = 0.8, 1000 K +- 50 K, u_rms 20 m/s) and SHEARLAYER2 (3-D periodic double
+H2/N2 - air shear layer, 8 mm box).
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time as _time
+
+import numpy as np
+
+# GENERAL%INIT -> (dimension, box side [m]); a restart file holds no grid,
+# so a run read from one takes the side of its deck's INIT
+INITS = {"HOTSPOT": (2, 0.01), "HIT": (3, 0.01), "SHEARLAYER2": (3, 0.008)}
+
+
+def initial_state(deck, adi, thd, n: int):
+    from . import ics
+    init = _init(deck)
+    if init == "HOTSPOT":
+        return ics.config1_state(adi, thd, n=n)
+    if init == "HIT":
+        return ics.config2_state(adi, thd, n=n, L=INITS[init][1])
+    return ics.config5_state(adi, thd, n=n, L=INITS[init][1])
+
+
+def _init(deck) -> str:
+    init = deck.get("GENERAL%INIT").upper()
+    if init not in INITS:
+        raise SystemExit(f"GENERAL%INIT = {init}: not one of {', '.join(INITS)}")
+    if deck.get("GENERAL%DIM") != INITS[init][0]:
+        raise SystemExit(f"GENERAL%DIM = {deck.get('GENERAL%DIM')} but INIT = {init} is "
+                         f"{INITS[init][0]}-D")
+    return init
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("deck")
+    ap.add_argument("--n", type=int, default=64, help="cells per side of the initial state")
+    ap.add_argument("--dir", default=None, help="output directory (default: the deck's)")
+    ap.add_argument("--flux-mode", type=int, default=1)
+    args = ap.parse_args(argv)
+    from . import chemdata as cd
+    from .deck import Deck, options
+    from .ics import Box
+    from .restart import restart_name
+    from .solver import Solver
+
+    deck = Deck.read(args.deck)
+    opts = options(deck, flux_mode=args.flux_mode)
+    out = args.dir or os.path.dirname(os.path.abspath(args.deck))
+    os.makedirs(out, exist_ok=True)
+    adi = cd.Adim()
+    thd = cd.build_thd(adi)
+    mech = cd.read_mechanism() if opts.reaction else None
+    g = deck.get
+    nstat, dtime, offset = g("MONITORING%NSTAT"), g("MONITORING%DTIME"), g("MONITORING%TIME_OFFSET")
+    if g("MONITORING%READ_RESTART"):
+        nb = g("MONITORING%NUMBER_RESTART")
+        path = os.path.join(out, restart_name(g("MONITORING%NAME_RESTART"), nb))
+        from . import restart
+        (nx, ny, nz), _ = restart.dims(path)
+        dim, side = INITS[_init(deck)]
+        box = Box(nx, ny, nz, 3, dim, (side, side, side))
+        s = Solver(box, opts, adi=adi, thd=thd, mech=mech)
+        time = s.load_restart(path)
+        nb_stat = nb
+        # statistics times restart from the offset (module_input.f90:718-722)
+        timing = [offset] * (nb + 1) + [offset + k * dtime for k in range(1, nstat - nb)]
+    else:
+        box, v, T = initial_state(deck, adi, thd, args.n)
+        s = Solver(box, opts, adi=adi, thd=thd, mech=mech)
+        s.load_state(v, T)
+        del v, T
+        time = g("MONITORING%INITIAL_TIME")
+        nb_stat = 0
+        timing = [offset + k * dtime for k in range(nstat)]
+    timing = np.array(timing) / adi.time_ref  # nondimensional (module_tools.f90:180)
+    # the time state starts as the reference's dtmin = dt_4 = DTmax = 1
+    s.set_time_state(np.ones(6))
+    print(f"entering the main loop ({box.nx} x {box.ny} x {box.nz}, "
+          f"{'reacting' if opts.reaction else 'inert'})", flush=True)
+    ite, loop = 0, True
+    t0 = _time.perf_counter()
+    while loop:
+        ite += 1
+        if g("MONITORING%ITLIM") and ite >= g("MONITORING%ITMAX"):
+            loop = False
+        s.timestep()
+        ts = s.time_state()
+        dtmin, write_stat = ts[4], False
+        nxt = timing[nb_stat] if nb_stat < len(timing) else np.inf
+        if time + dtmin > nxt and time - dtmin < nxt:
+            dtmin = nxt - time
+            ts[4] = dtmin
+            s.set_time_state(ts)
+            write_stat = g("MONITORING%STAT")
+            if nb_stat + 1 >= nstat:
+                loop = False
+        dt_ptr = s.lib.izem_time_state(s.ctx) + 4 * 8
+        if opts.reaction:
+            s.reaction_selec(dt_ptr, 0.5)
+            s.rk3(dt_ptr)
+            time += dtmin
+            s.reaction_selec(dt_ptr, 0.5)
+        else:
+            s.rk3(dt_ptr)
+            time += dtmin
+        status = s.status()
+        if status & (1 | 4 | 16 | 32):
+            raise SystemExit(f"iteration {ite}: abort, status bits {status}")
+        if write_stat:
+            nb_stat += 1
+            ts = s.time_state()
+            with open(os.path.join(out, "times_stat.out"), "a") as f:
+                f.write(f"{nb_stat:8d} {time * adi.time_ref:18.8E} {dtmin * adi.time_ref:18.8E}"
+                        + "".join(f" {x / dtmin:18.8E}" for x in ts[:4]) + "\n")
+            s.save_restart(os.path.join(out, restart_name("restart", nb_stat)), time)
+        if ite % max(1, g("MONITORING%ITSHOW")) == 0 or not loop:
+            print(f" iteration {ite:8d}  time {time * adi.time_ref:.8e} s  "
+                  f"dt {dtmin * adi.time_ref:.8e} s  wall {_time.perf_counter() - t0:.1f} s"
+                  + (f"  (status {status})" if status else ""), flush=True)
+    s.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -134,6 +134,45 @@ class Solver:
             self.prim_inv_var(dom)
         self.sync()
 
+    def save_restart(self, path: str, time: float):
+        """save_restart_hdf5 (module_io.f90:51-104): the interior conserved
+        variables and the nondimensional time, in the reference's layout."""
+        from . import restart
+        self.sync()
+        restart.write(path, self.box, self.thd.names, self.v.cpu().numpy(), time)
+
+    def load_restart(self, path: str, ghosts: str = "reference") -> float:
+        """read_restart_hdf5 (module_io.f90:107-192): read the interior,
+        fill the periodic ghosts (comm_cons), invert energy to temperature
+        from T = 0 at machine precision (prim_inv_var_err_mach), then the
+        ghost slabs.  ghosts='reference' keeps the reference's
+        upd_extrapolation, which also overwrites periodic ghosts with the
+        edge values until the next ghost fill; 'periodic' keeps the periodic
+        images (the state the writing run had).  Returns the time."""
+        from . import restart
+        if ghosts not in ("reference", "periodic"):
+            raise ValueError(ghosts)
+        v, time = restart.read(path, self.box, self.thd.names)
+        self.v.copy_(torch.from_numpy(v))
+        for a in (self.T, self.W_i, self.cp, self.ha):
+            a.zero_()
+        _lib.check(self.lib.izem_comm_cons(self.ctx, _ptr(self.v)), "comm_cons")
+        self.prim_inv_var_mach(0)
+        if ghosts == "reference":
+            restart.extrapolate_faces(self.box, self.v)
+        for dom in [s * q for q in range(1, self.box.ndim + 1) for s in (-1, 1)]:
+            self.prim_inv_var_mach(dom)
+        self.sync()
+        return time
+
+    def prim_inv_var_mach(self, domain: int = 0) -> int:
+        it = C.c_int()
+        _lib.check(self.lib.izem_prim_inv_var_mach(self.ctx, domain, _ptr(self.v),
+                                                   _ptr(self.W_i), _ptr(self.T), _ptr(self.cp),
+                                                   _ptr(self.ha), C.byref(it)),
+                   "prim_inv_var_mach")
+        return it.value
+
     def sync(self):
         _lib.check(self.lib.izem_synchronize(self.ctx), "izem_synchronize")
 
@@ -183,9 +222,11 @@ class Solver:
                                      _ptr(self.W_i), _ptr(self.cp), _ptr(self.ha),
                                      _ptr(self.sensor), _ptr(self.v), _ptr(f)), "rhs")
 
-    def rk3(self, dt: torch.Tensor | None = None):
-        dtp = dt if dt is not None else self.dt
-        _lib.check(self.lib.izem_rk3(self.ctx, _ptr(dtp), *[_ptr(x) for x in self.d_i],
+    def rk3(self, dt: torch.Tensor | int | None = None):
+        """rk3 over the nondimensional step in dt: a device tensor, a device
+        address (e.g. of the time state's dtmin), or None for self.dt."""
+        dtp = C.c_void_p(dt) if isinstance(dt, int) else _ptr(dt if dt is not None else self.dt)
+        _lib.check(self.lib.izem_rk3(self.ctx, dtp, *[_ptr(x) for x in self.d_i],
                                      _ptr(self.T), _ptr(self.W_i), _ptr(self.cp), _ptr(self.ha),
                                      _ptr(self.sensor), _ptr(self.v)), "rk3")
 

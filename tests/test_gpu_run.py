@@ -1,0 +1,69 @@
+"""The deck driver (python -m phd_amd.run): a reacting 2-D run from a deck
+that lands on its statistics times and writes a restart at each, then a
+second run read from one of those restarts."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DECK = """# reacting hot spot, WENO5, every statistics time a restart
+[GENERAL]
+  DIM = 2
+  NRV = 9
+  INIT = 'HOTSPOT'
+[TIMESTEPPING]
+  CFL = 0.5
+[WENO]
+  ORDER_WENO = 5
+  PERCENT_WEIGHT = -1.0
+[REACTING]
+  REACTION = T
+[MONITORING]
+  ITSHOW = 5
+  ITLIM = T
+  ITMAX = 60
+  STAT = T
+  NSTAT = 3
+  TIME_OFFSET = 2.0e-7
+  DTIME = 2.0e-7
+{extra}
+"""
+
+
+def test_deck_run_and_restart(tmp_path, capsys):
+    from phd_amd import restart
+    from phd_amd.run import main
+    from phd_amd import chemdata as cd
+    deck = tmp_path / "hotspot.ini"
+    deck.write_text(DECK.format(extra=""))
+    assert main([str(deck), "--n", "16"]) == 0
+    lines = (tmp_path / "times_stat.out").read_text().splitlines()
+    assert len(lines) == 3
+    t = np.array([float(l.split()[1]) for l in lines])
+    assert np.allclose(t, [2e-7, 4e-7, 6e-7], rtol=1e-9, atol=0)
+    adi = cd.Adim()
+    for k in (1, 2, 3):
+        (nx, ny, nz), tk = restart.dims(str(tmp_path / f"restart_{k:05d}.h5"))
+        assert (nx, ny, nz) == (16, 16, 1)
+        assert tk * adi.time_ref == pytest.approx(2e-7 * k, rel=1e-9)
+    out = capsys.readouterr().out
+    assert "entering the main loop" in out
+    # resume from the second restart.  As in the reference, a restarted run's
+    # statistics times begin with NUMBER_RESTART + 1 copies of TIME_OFFSET
+    # (module_input.f90:718-722): with TIME_OFFSET at the restart time the
+    # first is met at once by a step of zero length (the chemistry's DVODE
+    # returns untouched for tout = t), the next one DTIME later
+    deck2 = tmp_path / "resume.ini"
+    deck2.write_text(DECK.format(extra="  READ_RESTART = T\n  NAME_RESTART = 'restart'\n"
+                                       "  NUMBER_RESTART = 2\n").replace(
+        "NSTAT = 3", "NSTAT = 4").replace("TIME_OFFSET = 2.0e-7", "TIME_OFFSET = 4.0e-7"))
+    os.remove(tmp_path / "times_stat.out")
+    assert main([str(deck2), "--n", "16"]) == 0
+    lines = (tmp_path / "times_stat.out").read_text().splitlines()
+    assert [int(l.split()[0]) for l in lines] == [3, 4]
+    t = np.array([float(l.split()[1]) for l in lines])
+    assert np.allclose(t, [4e-7, 6e-7], rtol=1e-9, atol=0)
