@@ -410,7 +410,7 @@ def main():
 
     from phd_amd import chemdata as cd
     from phd_amd import ics
-    from phd_amd.flops import EULER_FLOP_PER_FACE
+    from phd_amd.flops import EULER_FLOP_PER_FACE, EULER_FLOP_PER_FACE_MODE
     from phd_amd.solver import Options, Solver, profile_read, profile_start
 
     cpu = None
@@ -486,13 +486,16 @@ def main():
     e_ms, e_cnt = prof["euler_faces"]
     nfaces = (n + 1) * n * n
     e_avg_s = e_ms / e_cnt / 1e3
-    achieved = EULER_FLOP_PER_FACE * nfaces / e_avg_s / 1e12
+    flop_face = EULER_FLOP_PER_FACE_MODE[args.flux_mode]
+    achieved = flop_face * nfaces / e_avg_s / 1e12
     euler_roof = {"kernel": "k_euler_faces (x, y, z)", "bound": "fp64-valu",
                   "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                   "frac": achieved / PEAK_FP64_TFLOPS,
                   "traffic": read_traffic("euler_faces", n),
                   "avg_launch_ms": e_avg_s * 1e3, "launches": e_cnt,
-                  "flop_per_face": EULER_FLOP_PER_FACE, "faces_per_launch": nfaces}
+                  "flop_per_face": flop_face, "faces_per_launch": nfaces,
+                  "flux_mode": args.flux_mode,
+                  "flop_per_face_reference_order": EULER_FLOP_PER_FACE}
     c_ms, c_cnt = prof["chemistry"]
     chem_roof = None
     if c_cnt:

@@ -95,6 +95,20 @@ EULER_FACE_SECTIONS = {
 }
 EULER_FLOP_PER_FACE = sum(EULER_FACE_SECTIONS.values())
 
+# Flux mode 1 (izem_flux_mode): the projections through the structure of el
+# (k_euler_faces<DIR, true>) replace the dense rows and their assembly.  Per
+# stencil point: the shared scalars S of U and of F (-b1 U_0 + b2 u.m - b2 E
+# - b2 sum psi_l U_l: 27 and 32), sum psi_l U_l (18) and its flux (1), and
+# u_n U_0 - U_n with its flux (6): 84 minus the 25 the two S share = 59;
+# then per point the 14 fields' w and g from them: rows 0 and 2 (6 each),
+# row 1 (2), row 5 (4), the 9 species rows (5 each) and the two tangential
+# rows (5.5 each) = 74.
+EULER_FACE_SECTIONS_MODE1 = dict(EULER_FACE_SECTIONS)
+del EULER_FACE_SECTIONS_MODE1["left eigenvector rows"]
+EULER_FACE_SECTIONS_MODE1["characteristic projections"] = 6 * 59 + 6 * 74
+EULER_FLOP_PER_FACE_MODE = {0: EULER_FLOP_PER_FACE,
+                            1: sum(EULER_FACE_SECTIONS_MODE1.values())}
+
 
 # Algorithmic HBM bytes per cell (3-D, FP64, 9 species) of the memory-bound
 # flow kernels: every field a kernel must read or write, once (stencil
