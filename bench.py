@@ -441,7 +441,7 @@ def main():
         Ts, rs, Ys = autotune.sample_reacting_cells(s, 1 << 20)
         tuned = autotune.tune(s, Ts, rs, Ys, dt_half, key=f"reacting-hit-{n}", use_cache=False)
         log(f"chemistry kernels tuned: waves {tuned[0]}, blocks/CU {tuned[1]}, "
-            f"kinetics {'straight-line' if tuned[3] else 'tables'}, "
+            f"kinetics {autotune.KINETICS[tuned[3]]}, "
             f"{'stiffness' if tuned[5] else 'index'} order ({tuned[4]:.3e} cells/s)")
         del Ts, rs, Ys
     log("state resident on the GPU; warm-up (eager, then the captured graph)")
@@ -592,8 +592,8 @@ def main():
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
         "chem_tuning": (None if tuned is None else
                         {"waves_per_simd": tuned[0], "blocks_per_cu": tuned[1],
-                         "jac_age": tuned[2], "kinetics": "straight-line" if tuned[3] else
-                         "tables", "queue_order": "stiffness" if tuned[5] else "index",
+                         "jac_age": tuned[2], "kinetics": autotune.KINETICS[tuned[3]],
+                         "queue_order": "stiffness" if tuned[5] else "index",
                          "sample_cells_per_s": tuned[4]}),
         "chemistry": chem,
         "chem_surrogate": surr,

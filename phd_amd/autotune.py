@@ -2,15 +2,19 @@
 
 The persistent work-queue kernels (izem_psr_batch, izem_reaction_dvode) are
 compiled in several variants: table-driven kinetics at a register budget of
-1, 2 or 4 waves per SIMD, straight-line kinetics at 1 wave per SIMD, and
+1, 2 or 4 waves per SIMD, straight-line kinetics at 1 wave per SIMD with a
+finite-difference Jacobian or with its mass-fraction columns in closed form,
+and
 launched with a chosen number of resident 64-lane blocks per CU.  Which combination is fastest depends on
 the cells' stiffness (steps per cell), the device and the mechanism.  The
 tuner times each candidate with izem_psr_batch on a sample of the cells the
 caller will integrate, keeps the fastest and applies it with
 izem_chem_tune; with that variant it then times izem_psr_batch's stiffness
-order (izem_chem_order) against index order and keeps the faster.  Every candidate integrates the same BDF method with the
-same tolerances and Jacobian reuse ages (DVODE's MSBP = 20, MSBJ = 50), so the choice
-changes speed only; the two kinetics forms differ at round-off level.
+order (izem_chem_order) against index order and keeps the faster.  Every
+candidate integrates the same BDF method with the same tolerances and
+Jacobian reuse ages (DVODE's MSBP = 20, MSBJ = 50); the kinetics forms
+differ at round-off level and the Jacobian forms in the iteration matrix
+only, so the choice changes speed, and results within the tolerance.
 
 Results are cached per (device, mechanism hash, workload key) in a JSON
 file, by default ~/.cache/phd_amd/chem_tune.json.
@@ -28,8 +32,9 @@ import torch
 from . import _lib
 from .codegen_mech import mechanism_hash
 
-CANDIDATES = ([(1, b, 1) for b in (4, 2)] +
+CANDIDATES = ([(1, b, k) for k in (2, 1) for b in (4, 2)] +
               [(w, b, 0) for w in (4, 2, 1) for b in (4 * w, 2 * w)])
+KINETICS = {0: "tables", 1: "straight-line", 2: "straight-line, closed-form Jacobian"}
 JAC_AGE = 50  # DVODE MSBJ
 ORDER_MARGIN = 1.05  # the stiffness order must be at least 5 % faster to be kept
 

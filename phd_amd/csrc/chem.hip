@@ -43,7 +43,7 @@ struct IzChem {
   // work counters of the integrations since the last izem_chem_stats:
   // BDF steps, RHS evaluations, Jacobians, LU solves
   unsigned long long *stats;
-  int use_gen;    // run the straight-line kinetics (generated && not disabled)
+  int use_gen;    // 0 tables, 1 straight-line kinetics, 2 with the closed-form Jacobian
   // stiffness order of izem_psr_batch: on/off, per-cell bin and queue
   // permutation (capacity order_cap cells), bin counts then offsets
   int order;
@@ -433,6 +433,7 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
 
 // parameters chosen by phd_amd.autotune: register budget, persistent blocks
 // per CU, Jacobian reuse age, kinetics form (0 tables, 1 straight-line,
+// 2 straight-line with the closed-form mass-fraction Jacobian columns,
 // -1 straight-line when it matches the loaded mechanism and the budget).
 // The straight-line kinetics run at one wave per SIMD only: builds of them
 // at two and four waves per SIMD (history in private memory, several KB of
@@ -442,10 +443,11 @@ extern "C" int izem_chem_setup(izem_ctx *ctx, const double *mf, long nmech, doub
 extern "C" int izem_chem_tune(izem_ctx *ctx, int waves, int blocks_per_cu, int jac_age,
                               int kinetics) {
   if (!ctx || !ctx->chem || !(waves == 1 || waves == 2 || waves == 4) || blocks_per_cu < 1 ||
-      blocks_per_cu > 64 || jac_age < 1 || kinetics < -1 || kinetics > 1)
+      blocks_per_cu > 64 || jac_age < 1 || kinetics < -1 || kinetics > 2)
     return IZEM_E_ARG;
-  if (kinetics == 1 && (!ctx->chem->generated || waves != 1)) return IZEM_E_ARG;
-  ctx->chem->use_gen = (kinetics == 0 || waves != 1) ? 0 : ctx->chem->generated;
+  if (kinetics >= 1 && (!ctx->chem->generated || waves != 1)) return IZEM_E_ARG;
+  ctx->chem->use_gen = (kinetics == 0 || waves != 1) ? 0
+                       : (kinetics == 2 ? 2 : ctx->chem->generated);
   ctx->chem->waves = waves;
   ctx->chem->blocks_per_cu = blocks_per_cu;
   ctx->chem->jac_age = jac_age;
@@ -668,7 +670,9 @@ static void psr_launch(izem_ctx *ctx, dim3 grid, int n, double *T, const double 
   hipLaunchKernelGGL((k_psr_queue<K, W>), grid, dim3(64), 0, ctx->stream, n, T, rho, Y, dt,      \
                      ch->rtol, ch->atol, ch->jac_age, counter, order, nsteps, ctx->status,      \
                      ch->stats)
-  if (ch->use_gen) {
+  if (ch->use_gen == 2) {
+    IZ_PSR(KinGenAJ, 1);
+  } else if (ch->use_gen) {
     IZ_PSR(KinGen, 1);
   } else {
     if (ch->waves == 1) IZ_PSR(KinTables, 1);
@@ -776,7 +780,9 @@ static void reaction_queue(izem_ctx *ctx, const IzBox &b, dim3 grid, const doubl
                      b.i1, b.j0, b.j1, b.k0, b.k1, dt, scale, ctx->o.time_ref, ctx->o.T_ref,    \
                      rho_cgs, ch->rtol, ch->atol, ch->jac_age, T, v, ch->counter, order, list_n, \
                      ctx->ts, ctx->status, ch->stats)
-  if (ch->use_gen) {
+  if (ch->use_gen == 2) {
+    IZ_RQ(KinGenAJ, 1);
+  } else if (ch->use_gen) {
     IZ_RQ(KinGen, 1);
   } else {
     if (ch->waves == 1) IZ_RQ(KinTables, 1);

@@ -674,4 +674,34 @@ struct KinGen {
     for (int i = 0; i < N_; i++) P[i][0] = -hl0 * (ftmp[i] - f0[i]) * ri;
   }
 };
+
+// The straight-line kinetics with the mass-fraction columns of the
+// Jacobian in closed form (gen_jac, generated with the rates: mass action,
+// third-body sums, falloff factors differentiated) and the temperature
+// column by a finite difference as above: one evaluation of the kinetics
+// per Jacobian instead of GEN_KK + 1.  DVODE's user-Jacobian form (MF = 21,
+// vode.f:3596-3727) in place of the reference's MF = 22; the integration
+// keeps the same error control, so results stay within its tolerance.
+struct KinGenAJ : KinGen {
+  IZ_MHD inline void jac_c(double rho, const double *y, const double *f0, const Cache &c,
+                           double hl0, const double *w, double r0, double (*P)[N_]) const {
+    gen_jac(rho, y, c.kt, f0[0], P);
+    if (hl0 != -1.0) {
+#pragma unroll
+      for (int i = 0; i < N_; i++)
+#pragma unroll
+        for (int j = 1; j < N_; j++) P[i][j] = -hl0 * P[i][j];
+    }
+    double yl[N_], ftmp[N_], kt0[GEN_NKT];
+#pragma unroll
+    for (int n = 0; n < N_; n++) yl[n] = y[n];
+    const double r = d_fd_increment(0, yl[0], w[0], r0);
+    yl[0] = yl[0] + r;
+    gen_kT(yl[0], kt0);
+    gen_rhs(rho, yl, kt0, ftmp);
+    const double ri = 1.0 / r;
+#pragma unroll
+    for (int i = 0; i < N_; i++) P[i][0] = -hl0 * (ftmp[i] - f0[i]) * ri;
+  }
+};
 #endif

@@ -54,6 +54,8 @@ def host_bdf(tmp_path_factory, mech):
     L.host_setup.argtypes = [C.c_void_p, C.c_long]
     L.host_psr.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                            C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
+    L.host_rhs.argtypes = [C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+    L.host_jac.argtypes = [C.c_double, C.c_void_p, C.c_void_p]
     mf = mech_flat(mech)
     assert L.host_setup(mf.ctypes.data, mf.size) == 0
     return L

@@ -32,7 +32,9 @@ extern "C" void host_psr(int kinetics, int n, double *T, const double *rho, doub
     for (int k = 0; k < N_ - 1; k++) y[1 + k] = Y[c * (N_ - 1) + k];
     ZArr z;
     double J[N_][N_] = {};
-    BdfResult r = kinetics
+    BdfResult r = kinetics == 2
+        ? d_bdf_integrate(KinGenAJ(), z, J, rho[c], y, dt, rtol, atol, jac_age)
+        : kinetics
         ? d_bdf_integrate(KinGen(), z, J, rho[c], y, dt, rtol, atol, jac_age)
         : d_bdf_integrate(KinTables(), z, J, rho[c], y, dt, rtol, atol, jac_age);
     T[c] = y[0];
@@ -46,4 +48,13 @@ extern "C" void host_psr(int kinetics, int n, double *T, const double *rho, doub
 extern "C" void host_rhs(int kinetics, double rho, const double *z, double *zp) {
   if (kinetics) KinGen().rhs(rho, z, zp);
   else KinTables().rhs(rho, z, zp);
+}
+
+// the mass-fraction columns of the reactor Jacobian in closed form
+// (gen_jac) at z; J row-major 10 x 10, column 0 untouched
+extern "C" void host_jac(double rho, const double *z, double *J) {
+  double kt[GEN_NKT], f[N_];
+  gen_kT(z[0], kt);
+  gen_rhs(rho, z, kt, f);
+  gen_jac(rho, z, kt, f[0], (double (*)[N_])J);
 }

@@ -154,7 +154,7 @@ def test_element_conservation_of_rates(mech):
         assert np.max(np.abs(tot) / scale) <= 1e-12
 
 
-@pytest.mark.parametrize("kinetics,waves", [(0, 1), (0, 2), (0, 4), (1, 1)])
+@pytest.mark.parametrize("kinetics,waves", [(0, 1), (0, 2), (0, 4), (1, 1), (2, 1)])
 @pytest.mark.parametrize("rtol,atol", [(1e-6, 1e-12)])
 def test_psr_batch_vs_reference_dvode(mech, thd, rtol, atol, kinetics, waves):
     """Both kinetics forms (table-driven, straight-line generated) and two
@@ -274,8 +274,11 @@ def test_straight_line_kinetics_one_wave_only(mech):
     s = _solver(mech)
     for w in (2, 4):
         assert s.lib.izem_chem_tune(s.ctx, w, 4 * w, 50, 1) == -1
+        assert s.lib.izem_chem_tune(s.ctx, w, 4 * w, 50, 2) == -1
         assert s.lib.izem_chem_tune(s.ctx, w, 4 * w, 50, -1) == 0
     assert s.lib.izem_chem_tune(s.ctx, 1, 4, 50, 1) == 0
+    assert s.lib.izem_chem_tune(s.ctx, 1, 4, 50, 2) == 0
+    assert s.lib.izem_chem_tune(s.ctx, 1, 4, 50, 3) == -1
 
 
 def test_setup_self_check_is_clean(mech):
@@ -291,7 +294,7 @@ def test_setup_self_check_is_clean(mech):
     assert s.lib.izem_chem_tune(s.ctx, 1, 4, 50, 1) == 0
 
 
-@pytest.mark.parametrize("kinetics,waves", [(0, 1), (0, 2), (0, 4), (1, 1)])
+@pytest.mark.parametrize("kinetics,waves", [(0, 1), (0, 2), (0, 4), (1, 1), (2, 1)])
 def test_variants_follow_host_build(mech, thd, host_bdf, kinetics, waves):
     """Every kernel variant integrates the same source as a g++ build of it;
     the two differ only in the last bits of exp/log, so most cells take the
@@ -329,7 +332,7 @@ def test_variants_follow_host_build(mech, thd, host_bdf, kinetics, waves):
     assert abs(ng.mean() - nst_h.mean()) <= 0.03 * nst_h.mean()
 
 
-@pytest.mark.parametrize("kinetics,waves", [(0, 1), (0, 2), (0, 4), (1, 1)])
+@pytest.mark.parametrize("kinetics,waves", [(0, 1), (0, 2), (0, 4), (1, 1), (2, 1)])
 def test_reaction_queue_variants_follow_host_build(oracle, adi, thd, mech, host_bdf, kinetics,
                                                    waves):
     """The reacting-step kernel (k_reaction_queue, behind reaction_selec) in
@@ -502,7 +505,7 @@ def test_resetup_keeps_work_counters(mech, thd):
     assert s.status() == 0
 
 
-@pytest.mark.parametrize("kinetics", [0, 1])
+@pytest.mark.parametrize("kinetics", [0, 1, 2])
 def test_stiffness_order_flow_state(adi, thd, mech, kinetics):
     """izem_reaction_dvode with izem_chem_order on takes the cells of the
     reacting box stiffest first; the conserved state, temperature, DTmax and
