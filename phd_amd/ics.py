@@ -189,17 +189,37 @@ def config2_state(adi, thd, n: int = 256, seed: int = 2, L: float = 0.01):
     return box, v, Tn
 
 
+_BURNT = {}
+
+
+def burnt_table(thd, nphi: int = 48):
+    """Mass fractions (in thd's species order) of the adiabatic equilibrium
+    products of H2/air at 1 atm burnt from 300 K, on nphi equivalence ratios
+    log-spaced over [0.3, 3] (phd_amd.equilibrium)."""
+    key = (tuple(thd.names), nphi)
+    if key not in _BURNT:
+        from . import equilibrium as eq
+        mech = cd.read_mechanism()
+        order = [[s.upper() for s in mech.species].index(n.upper()) for n in thd.names]
+        lphi = np.linspace(math.log(0.3), math.log(3.0), nphi)
+        Yb = np.array([eq.burnt_mass_fractions(mech, math.exp(lp))[order] for lp in lphi])
+        _BURNT[key] = (lphi, Yb)
+    return _BURNT[key]
+
+
 def config3_cells(thd, ncell: int, seed: int = 3):
     """Independent reactor cells: T ~ U[900, 2000] K, p log-uniform in
     [1, 10] atm, phi log-normal (median 1, sigma 0.4) clipped to [0.3, 3],
-    0-20 % products blended in.  Returns (T_K, p_Pa, Y[ncell, 9])."""
+    and a mass fraction U[0, 0.2] of the equilibrium products of the same
+    premix blended in (its adiabatic burnt gas at 1 atm from 300 K, with the
+    H2O / OH / H / O / H2 / O2 of dissociation: burnt_table).  Returns
+    (T_K, p_Pa, Y[ncell, 9])."""
     rng = np.random.default_rng(seed)
     T = rng.uniform(900.0, 2000.0, ncell)
     p = P_ATM * np.exp(rng.uniform(0.0, math.log(10.0), ncell))
     phi = np.clip(np.exp(rng.normal(0.0, 0.4, ncell)), 0.3, 3.0)
     prod = rng.uniform(0.0, 0.2, ncell)
-    Y = np.empty((ncell, 9))
-    # vectorised premix blend (same recipe as chemdata.mass_fractions_premix)
+    # vectorised premix (same recipe as chemdata.mass_fractions_premix)
     tr = cd.read_tranfit()
     idx = {nm.upper(): i for i, nm in enumerate(thd.names)}
     X = np.zeros((ncell, 9))
@@ -208,11 +228,10 @@ def config3_cells(thd, ncell: int, seed: int = 3):
     X[:, idx["N2"]] = 0.79 / 0.21
     Ym = X * tr.W
     Ym /= Ym.sum(axis=1, keepdims=True)
-    Xp = np.zeros(9)
-    Xp[idx["H2O"]], Xp[idx["OH"]], Xp[idx["N2"]] = 0.9, 0.1, 0.79 / 0.21 * 0.5
-    Yp = Xp * tr.W
-    Yp /= Yp.sum()
-    Y[:] = (1.0 - prod)[:, None] * Ym + prod[:, None] * Yp[None, :]
+    lphi, Yb = burnt_table(thd)
+    Yp = np.stack([np.interp(np.log(phi), lphi, Yb[:, k]) for k in range(9)], axis=1)
+    Yp /= Yp.sum(axis=1, keepdims=True)
+    Y = (1.0 - prod)[:, None] * Ym + prod[:, None] * Yp
     return T, p, Y
 
 
