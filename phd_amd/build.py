@@ -29,6 +29,7 @@ SOURCES = {
     "chem.hip": ["-ffp-contract=off"],
     "step.hip": ["-ffp-contract=off"],
     "mlp.hip": ["-ffp-contract=fast"],
+    "images.hip": ["-ffp-contract=off"],
 }
 
 
@@ -60,6 +61,7 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
     build_io(verbose)
+    build_fortran(verbose)
     return LIB
 
 
@@ -85,6 +87,39 @@ def build_io(verbose: bool = False) -> str | None:
         raise RuntimeError("restart I/O library build failed")
     os.replace(tmp, IO_LIB)
     return IO_LIB
+
+
+FORTRAN_LIB = os.path.join(HERE, "libizem_fortran.so")
+FORTRAN_SOURCES = ("izem_hip_mod.f90", "izem_fdriver.f90")
+
+
+def fortran_compiler() -> str | None:
+    import shutil
+    return shutil.which("amdflang") or shutil.which("flang")
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """libizem_fortran.so: the ISO_C_BINDING module and the Fortran host
+    driver (fortran/), linked against libizem_hip.so; skipped with a message
+    when no Fortran compiler is installed."""
+    fc = fortran_compiler()
+    if fc is None:
+        print("[build] no Fortran compiler: libizem_fortran.so not built", flush=True)
+        return None
+    mod_dir = os.path.join(BUILD, "fortran")
+    os.makedirs(mod_dir, exist_ok=True)
+    tmp = FORTRAN_LIB + ".tmp"
+    cmd = [fc, "-O2", "-fPIC", "-shared", "-J", mod_dir, "-o", tmp]
+    cmd += [os.path.join(HERE, "fortran", f) for f in FORTRAN_SOURCES]
+    cmd += ["-L" + HERE, "-lizem_hip", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=mod_dir)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("Fortran host library build failed")
+    os.replace(tmp, FORTRAN_LIB)
+    return FORTRAN_LIB
 
 
 if __name__ == "__main__":

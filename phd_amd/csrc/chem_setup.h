@@ -11,14 +11,15 @@
 // The flat image holds, in order: kk, ii; wt[KMAX]; a[KMAX][2][7];
 // tmid[KMAX]; A, b, EoR [IMAX each]; nu[IMAX][6]; nunk[IMAX][6];
 // reversible[IMAX]; ntb[IMAX]; tb_k[IMAX][TBMAX]; tb_eff[IMAX][TBMAX];
-// fal[IMAX]; fpar[IMAX][8] (the content of the CHEMKIN linking file,
+// fal[IMAX]; fpar[IMAX][8]; kfal[IMAX]; rpar[IMAX][3] (the content of the CHEMKIN linking file,
 // chemkin/vode.f:391).  Returns false when the length or kk does not fit.
+#define IZ_MECH_IMAGE_LEN                                                                      \
+  (2L + CK_KMAX + CK_KMAX * 14L + CK_KMAX + 3L * CK_IMAX + 12L * CK_IMAX + 2L * CK_IMAX +        \
+   2L * CK_IMAX * CK_TBMAX + CK_IMAX + 8L * CK_IMAX + CK_IMAX + 3L * CK_IMAX)
+
 static inline bool iz_mech_from_flat(const double *mf, long nmech, int kk_expected, IzMech &m) {
   memset(&m, 0, sizeof m);
-  const long need = 2 + CK_KMAX + CK_KMAX * 14 + CK_KMAX + 3 * CK_IMAX + 12 * CK_IMAX +
-                    2 * CK_IMAX + 2 * CK_IMAX * CK_TBMAX + CK_IMAX + 8 * CK_IMAX + CK_IMAX +
-                    3 * CK_IMAX;
-  if (!mf || nmech != need) return false;
+  if (!mf || nmech != IZ_MECH_IMAGE_LEN) return false;
   long p = 0;
   auto take = [&](long n) { long s = p; p += n; return s; };
   long o;

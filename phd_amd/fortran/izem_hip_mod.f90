@@ -50,7 +50,12 @@ module izem_hip_mod
             izem_chem_tune, izem_chem_stats, izem_ckwyr, izem_psr_batch, izem_reaction_dvode, &
             izem_chem_order
   public :: izem_surr_features, izem_mlp_infer, izem_surr_apply, izem_surr_fallback
+  public :: izem_thd_image, izem_mech_image_ckwrk, izem_reaction_selec, izem_reac_selec
   public :: izem_check, hip_rk3, hip_timestep, hip_euler_llf, hip_viscflux, hip_reaction_selec
+  public :: hip_create, hip_chem_init, hip_dtmin
+
+  ! lengths of the flat table images (doubles)
+  integer(c_long), parameter, public :: IZEM_THD_IMAGE_LEN = 745, IZEM_MECH_IMAGE_LEN = 2850
 
   ! ------------------------------------------------------------ part 1
   interface
@@ -183,6 +188,28 @@ module izem_hip_mod
       real(c_double), value :: rtol, atol
       integer(c_int), value :: trim_box
     end function
+    ! table images from the host's own arrays: thd_type after thd_init,
+    ! CHEMKIN-II's work arrays after CKINIT and its /CKSTRT/ pointers
+    integer(c_long) function izem_thd_image(nspc, ld, W, Wc, Wc_i, HeatCap, Tpolymin, &
+                                            Tpolylim, Tpolymax, CofD, CofLam, CofEta, gam2, &
+                                            Tinf, p0, pinf, Dinf, laminf, muinf, image, cap) &
+        bind(C, name="izem_thd_image")
+      import :: c_int, c_long, c_double
+      integer(c_int), value :: nspc, ld
+      real(c_double), intent(in) :: W(*), Wc(*), Wc_i(*), HeatCap(*), Tpolymin(*), &
+                                    Tpolylim(*), Tpolymax(*), CofD(*), CofLam(*), CofEta(*)
+      real(c_double), value :: gam2, Tinf, p0, pinf, Dinf, laminf, muinf
+      real(c_double), intent(out) :: image(*)
+      integer(c_long), value :: cap
+    end function
+    integer(c_long) function izem_mech_image_ckwrk(ickwrk, rckwrk, ckstrt, image, cap) &
+        bind(C, name="izem_mech_image_ckwrk")
+      import :: c_int, c_long, c_double
+      integer(c_int), intent(in) :: ickwrk(*), ckstrt(*)
+      real(c_double), intent(in) :: rckwrk(*)
+      real(c_double), intent(out) :: image(*)
+      integer(c_long), value :: cap
+    end function
     integer(c_int) function izem_chem_tune(ctx, waves, blocks_per_cu, jac_age, kinetics) &
         bind(C, name="izem_chem_tune")
       import :: c_int, c_ptr
@@ -216,6 +243,19 @@ module izem_hip_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx, dt, T, W_i, cp, ha, v
       real(c_double), value :: scale
+    end function
+    ! reaction_selec (module_reaction.f90:47): the solver izem_reac_selec
+    ! chose (0 DVODE, 1 the attached surrogate)
+    integer(c_int) function izem_reaction_selec(ctx, dt, scale, T, W_i, cp, ha, v) &
+        bind(C, name="izem_reaction_selec")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, dt, T, W_i, cp, ha, v
+      real(c_double), value :: scale
+    end function
+    integer(c_int) function izem_reac_selec(ctx, solver) bind(C, name="izem_reac_selec")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: solver
     end function
     ! chemistry surrogate (no reference counterpart): apply a trained network
     ! to n cells, with the BDF fallback for the flagged ones
@@ -325,18 +365,72 @@ contains
     call izem_check(ctx, izem_viscflux(ctx, dx_i, dy_i, dz_i, T, W_i, ha, v, fl), 'viscflux')
   end subroutine
 
-  ! reaction_selec with the DVODE branch (module_reaction.f90:47): one
-  ! Strang half step of dt (device address) when scale = 0.5; DTmax [K]
-  ! returned in host memory
+  ! reaction_selec (module_reaction.f90:47), DVODE unless izem_reac_selec
+  ! chose the surrogate: one Strang half step of dt (device address) when
+  ! scale = 0.5; DTmax [K] returned in host memory
   subroutine hip_reaction_selec(ctx, dt, scale, T, W_i, cp, ha, v, DTmax)
     type(c_ptr), intent(in) :: ctx, dt, T, W_i, cp, ha, v
     real(c_double), intent(in) :: scale
     real(c_double), intent(out) :: DTmax
     real(c_double) :: ts(6)
-    call izem_check(ctx, izem_reaction_dvode(ctx, dt, scale, T, W_i, cp, ha, v), &
-                    'reaction_dvode')
-    call izem_check(ctx, izem_get_time_state(ctx, ts), 'reaction_dvode')
+    call izem_check(ctx, izem_reaction_selec(ctx, dt, scale, T, W_i, cp, ha, v), &
+                    'reaction_selec')
+    call izem_check(ctx, izem_get_time_state(ctx, ts), 'reaction_selec')
     DTmax = ts(6)
+  end subroutine
+
+  ! device address of dtmin (slot 5 of izem_time_state): the dt that
+  ! hip_rk3 and hip_reaction_selec take, kept on the device
+  type(c_ptr) function hip_dtmin(ctx)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_intptr_t) :: a
+    a = transfer(izem_time_state(ctx), a)
+    hip_dtmin = transfer(a + 4_c_intptr_t * c_sizeof(0.0_c_double), hip_dtmin)
+  end function
+
+  ! izem_create from the tables thd_init filled (thd/type_thd.f90:58-73,
+  ! declared with the species dimension Nspcmax_thd) and the reference
+  ! quantities of adi_type (module_tools.f90:453-562): gam2 = (gamma-1)/gamma
+  subroutine hip_create(ctx, grid, opts, nspc, W, Wc, Wc_i, HeatCap, Tpolymin, Tpolylim, &
+                        Tpolymax, CofD, CofLam, CofEta, gam2, T_ref, p_ref, D_ref, lbda_ref, &
+                        mu_ref)
+    type(c_ptr), intent(out) :: ctx
+    type(izem_grid), intent(in) :: grid
+    type(izem_opts), intent(in) :: opts
+    integer(c_int), intent(in) :: nspc
+    real(c_double), intent(in) :: W(:), Wc(:), Wc_i(:), HeatCap(:, :, :), Tpolymin(:), &
+                                  Tpolylim(:), Tpolymax(:), CofD(:, :, :), CofLam(:, :), &
+                                  CofEta(:, :)
+    real(c_double), intent(in) :: gam2, T_ref, p_ref, D_ref, lbda_ref, mu_ref
+    real(c_double), target :: image(IZEM_THD_IMAGE_LEN)
+    integer(c_long) :: n
+    if (size(HeatCap, 1) /= 7 .or. size(HeatCap, 2) /= 2 .or. size(CofD, 1) /= 4 .or. &
+        size(CofD, 2) /= size(W) .or. size(CofLam, 1) /= 4 .or. size(CofEta, 1) /= 4) then
+      write (*, '(a)') 'izem_hip: thd arrays are not shaped as thd_type declares them'
+      error stop 1
+    end if
+    n = izem_thd_image(nspc, int(size(W), c_int), W, Wc, Wc_i, HeatCap, Tpolymin, Tpolylim, &
+                       Tpolymax, CofD, CofLam, CofEta, gam2, T_ref, 101325.0_c_double, p_ref, &
+                       D_ref, lbda_ref, mu_ref, image, IZEM_THD_IMAGE_LEN)
+    if (n /= IZEM_THD_IMAGE_LEN) call izem_check(c_null_ptr, int(n, c_int), 'thd_image')
+    ctx = c_null_ptr
+    call izem_check(ctx, izem_create(ctx, grid, c_loc(image), n, opts), 'izem_create')
+  end subroutine
+
+  ! izem_chem_setup from CKINIT's work arrays (chemkin/vode.f:391) and the
+  ! /CKSTRT/ pointers (ckstrt: its 85 integers, e.g. an array equivalenced
+  ! to NMM)
+  subroutine hip_chem_init(ctx, ickwrk, rckwrk, ckstrt, rtol, atol)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: ickwrk(*), ckstrt(*)
+    real(c_double), intent(in) :: rckwrk(*)
+    real(c_double), intent(in) :: rtol, atol
+    real(c_double), target :: image(IZEM_MECH_IMAGE_LEN)
+    integer(c_long) :: n
+    n = izem_mech_image_ckwrk(ickwrk, rckwrk, ckstrt, image, IZEM_MECH_IMAGE_LEN)
+    if (n /= IZEM_MECH_IMAGE_LEN) call izem_check(ctx, int(n, c_int), 'mech_image_ckwrk')
+    call izem_check(ctx, izem_chem_setup(ctx, c_loc(image), n, rtol, atol, 1_c_int), &
+                    'izem_chem_setup')
   end subroutine
 
 end module izem_hip_mod

@@ -132,9 +132,11 @@ def main(argv=None):
         if write_stat:
             nb_stat += 1
             ts = s.time_state()
-            with open(os.path.join(out, "times_stat.out"), "a") as f:
+            # a zero-length step (a resumed run's first statistics time)
+            # writes the ratios as Infinity, as the reference's division does
+            with open(os.path.join(out, "times_stat.out"), "a") as f, np.errstate(divide="ignore"):
                 f.write(f"{nb_stat:8d} {time * adi.time_ref:18.8E} {dtmin * adi.time_ref:18.8E}"
-                        + "".join(f" {x / dtmin:18.8E}" for x in ts[:4]) + "\n")
+                        + "".join(f" {x / np.float64(dtmin):18.8E}" for x in ts[:4]) + "\n")
             s.save_restart(os.path.join(out, restart_name("restart", nb_stat)), time)
         if ite % max(1, g("MONITORING%ITSHOW")) == 0 or not loop:
             print(f" iteration {ite:8d}  time {time * adi.time_ref:.8e} s  "

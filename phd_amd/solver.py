@@ -85,6 +85,7 @@ class Solver:
         _lib.check(self.lib.izem_create(C.byref(ctx), C.byref(g), flat.ctypes.data, flat.size,
                                         C.byref(op)), "izem_create")
         self.ctx = ctx
+        self.grid_struct, self.opts_struct = g, op
         _lib.check(self.lib.izem_flux_mode(ctx, o.flux_mode), "izem_flux_mode")
         # run on torch's current stream so that library and torch work is ordered
         _lib.check(self.lib.izem_set_stream(ctx, C.c_void_p(

@@ -42,13 +42,16 @@ def dvode_errors(rc, T, rho, Y, dt, Tg, Yg, rtol=1e-6, atol=1e-12):
     return compare(zg, zd, zt, rtol, atol)
 
 
-def assert_like_dvode(e, factor=2.0, comp_factor=3.0):
+def assert_like_dvode(e, factor=2.0, comp_factor=3.0, floor=0.0):
+    """floor: component means below this many tolerance units pass (after
+    one substep of a barely reacting field the temperature error is
+    round-off, ~1e-7 units, and the ratio of two such means says nothing)."""
     (gm, dm), (ga, da), (gc, dc) = e["norm_max"], e["norm_mean"], e["comp_mean"]
     assert gm <= factor * dm, ("per-cell norm, max", gm, dm)
     assert ga <= factor * da, ("per-cell norm, mean", ga, da)
     # a component DVODE integrates exactly (an inert species) must stay
     # within one tolerance unit
-    bad = gc > np.where(dc > 0, comp_factor * dc, 1.0)
+    bad = (gc > np.where(dc > 0, comp_factor * dc, 1.0)) & (gc > floor)
     assert not bad.any(), ("component means", np.nonzero(bad)[0], gc[bad], dc[bad])
 
 

@@ -115,7 +115,7 @@ def test_config3_cells(thd):
 
 def _declared_symbols():
     hdr = open(os.path.join(ROOT, "include", "izem_hip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|double\s*\*)\s*(izem_\w+)\s*\(", hdr, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|long|double\s*\*)\s*(izem_\w+)\s*\(", hdr, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
