@@ -28,6 +28,10 @@ static inline long cidx(const ora_grid *g, int i, int j, int k) {
 
 /* domain_select: src/module_parallel.f90:1936-2009.  0 = interior,
  * -1/+1 x ghost slabs, -2/+2 y, -3/+3 z (faces only). */
+static inline int is_periodic(const ora_grid *g, int d) {
+  return g->bc[2 * d] == ORA_BC_PERIODIC && g->bc[2 * d + 1] == ORA_BC_PERIODIC;
+}
+
 static void domain_select(const ora_grid *g, int d, int *i0, int *i1, int *j0, int *j1,
                           int *k0, int *k1) {
   int nx = g->nx, ny = g->ny, nz = g->nz, ng = g->ng;
@@ -193,6 +197,74 @@ void ora_wenorec_nw(const double *vp0, const double *vm0, double *vminus, double
   *vplus = (-3.0 * vm[6] + 27.0 * vm[5] + 47.0 * vm[4] - 13.0 * vm[3] + 2.0 * vm[2]) * denom;
 }
 
+/* WENO7 (ng = 4): candidate-stencil coefficients and ideal weights
+ * (4, 18, 12, 1)/35 with their mirror for the '+' state
+ * (src/module_weno.f90:71-111), smoothness indicators (:154-172, 330-353,
+ * 451-471) and the JS weights (:369-413, 476-519).  The indicators are
+ * Balsara and Shu's r = 4 forms (J. Comput. Phys. 160 (2000) 405-452): for
+ * the four values a, b, c, d of a candidate stencil
+ *   beta = a (k0 a + k1 b + k2 c + k3 d) + b (k4 b + k5 c + k6 d)
+ *        + c (k7 c + k8 d) + d (k9 d),
+ * with one coefficient row per stencil position (the last two rows mirror
+ * the first two). */
+static const double CW7[5][4] = {
+    {25.0 / 12.0, -23.0 / 12.0, 13.0 / 12.0, -1.0 / 4.0},   /* r = -1 */
+    {1.0 / 4.0, 13.0 / 12.0, -5.0 / 12.0, 1.0 / 12.0},      /* r = 0 */
+    {-1.0 / 12.0, 7.0 / 12.0, 7.0 / 12.0, -1.0 / 12.0},     /* r = 1 */
+    {1.0 / 12.0, -5.0 / 12.0, 13.0 / 12.0, 1.0 / 4.0},      /* r = 2 */
+    {-1.0 / 4.0, 13.0 / 12.0, -23.0 / 12.0, 25.0 / 12.0}};  /* r = 3 */
+static const double B7[4][10] = {
+    {2107.0, -9402.0, 7042.0, -1854.0, 11003.0, -17246.0, 4642.0, 7043.0, -3882.0, 547.0},
+    {547.0, -2522.0, 1922.0, -494.0, 3443.0, -5966.0, 1602.0, 2843.0, -1642.0, 267.0},
+    {267.0, -1642.0, 1602.0, -494.0, 2843.0, -5966.0, 1922.0, 3443.0, -2522.0, 547.0},
+    {547.0, -3882.0, 4642.0, -1854.0, 7043.0, -17246.0, 7042.0, 11003.0, -9402.0, 2107.0}};
+
+static inline double beta7(const double *k, double a, double b, double c, double d) {
+  return a * (k[0] * a + k[1] * b + k[2] * c + k[3] * d) + b * (k[4] * b + k[5] * c + k[6] * d) +
+         c * (k[7] * c + k[8] * d) + d * (k[9] * d);
+}
+
+/* smoothness indicator of WENO7 candidate r on its four values q[0..3] */
+double ora_weno7_beta(int r, const double *q) { return beta7(B7[r], q[0], q[1], q[2], q[3]); }
+
+/* one side of the WENO7 reconstruction: q is 1-based, i the centre,
+ * row0 the coefficient row of candidate r = 0 (1 for '-', 0 for '+') */
+static double weno7_side(const double *q, int i, int row0, const double *dw) {
+  double vr[4], beta[4], alfa[4], sum = 0.0, out = 0.0;
+  for (int r = 0; r < 4; r++) {
+    vr[r] = 0.0;
+    for (int j = 0; j < 4; j++) vr[r] = vr[r] + CW7[r + row0][j] * q[i - r + j];
+  }
+  for (int r = 0; r < 4; r++)
+    beta[r] = beta7(B7[r], q[i - r], q[i - r + 1], q[i - r + 2], q[i - r + 3]);
+  for (int r = 0; r < 4; r++) {
+    alfa[r] = dw[r] / ((EPS_WENO + beta[r]) * (EPS_WENO + beta[r]));
+    sum = sum + alfa[r];
+  }
+  sum = 1.0 / sum;
+  for (int r = 0; r < 4; r++) out = out + alfa[r] * sum * vr[r];
+  return out;
+}
+
+/* wenorec, ng = 4 branch: vp, vm hold the 8 stencil values (0-based) */
+void ora_wenorec7(const double *vp0, const double *vm0, double *vminus, double *vplus) {
+  static const double d[4] = {4.0 / 35.0, 18.0 / 35.0, 12.0 / 35.0, 1.0 / 35.0};
+  static const double dt[4] = {1.0 / 35.0, 12.0 / 35.0, 18.0 / 35.0, 4.0 / 35.0};
+  *vminus = weno7_side(vp0 - 1, 4, 1, d);
+  *vplus = weno7_side(vm0 - 1, 5, 0, dt);
+}
+
+/* wenorec_nw, ng = 4: src/module_weno.f90:246-256 (the 7th-order linear
+ * upwind face value; the ng = 4 formula reconstructs at i+1/2) */
+void ora_wenorec7_nw(const double *vp0, const double *vm0, double *vminus, double *vplus) {
+  const double *vp = vp0 - 1, *vm = vm0 - 1;
+  double denom = 1.0 / 420.0;
+  *vminus = (-3.0 * vp[1] + 25.0 * vp[2] - 101.0 * vp[3] + 319.0 * vp[4] + 214.0 * vp[5] -
+             38.0 * vp[6] + 4.0 * vp[7]) * denom;
+  *vplus = (-3.0 * vm[8] + 25.0 * vm[7] - 101.0 * vm[6] + 319.0 * vm[5] + 214.0 * vm[4] -
+            38.0 * vm[3] + 4.0 * vm[2]) * denom;
+}
+
 /* --------------------------------------------------------- Euler fluxes */
 
 /* Left and right eigenvector matrices at an averaged state
@@ -280,6 +352,37 @@ void ora_eigvecs(int dir, const ora_thd *thd, const double *uc, const double *Yc
   for (int l = 0; l < NRV - 1; l++) er[NV - 1][l + NIV + 1] = wrk * psi_c[l];
 }
 
+/* ranks along a direction own (r n)/np .. ((r+1) n)/np - 1
+ * (src/module_parallel.f90:547-548); ghost positions belong to the nearest
+ * end rank */
+static inline int rank_lo(int r, int n, int np) { return (int)(((long)r * n) / np); }
+static inline int rank_hi(int r, int n, int np) { return (int)(((long)(r + 1) * n) / np) - 1; }
+static inline int rank_of(int p, int n, int np) {
+  int r = 0;
+  while (r + 1 < np && rank_lo(r + 1, n, np) <= p) r++;
+  return r;
+}
+
+/* any sensor flag among line positions p0..p1 that lie in [lo, hi] */
+static inline int flags_in(const uint8_t *sensor, long base, long stride, int p0, int p1, int lo,
+                           int hi) {
+  for (int p = p0 > lo ? p0 : lo; p <= p1 && p <= hi; p++)
+    if (sensor[base + (long)p * stride]) return 1;
+  return 0;
+}
+
+static inline double weno_face(int ng, int js, const double *gp, const double *gm) {
+  double gl, gr;
+  if (ng == 4) {
+    if (js) ora_wenorec7(gp, gm, &gl, &gr);
+    else ora_wenorec7_nw(gp, gm, &gl, &gr);
+  } else {
+    if (js) ora_wenorec(gp, gm, &gl, &gr);
+    else ora_wenorec_nw(gp, gm, &gl, &gr);
+  }
+  return gl + gr;
+}
+
 /* euler_LLF_x / _y / _z: src/module_weno.f90:537-1044, 1055-1584, 1595-2118.
  * dir 0/1/2.  x assigns fl, y and z accumulate (src/module_weno.f90:1002-1018
  * vs the += form of the y/z routines). */
@@ -297,11 +400,15 @@ int ora_euler_llf(int dir, const ora_grid *g, const ora_thd *thd, const double *
   const int t1 = (dir == 0) ? 2 : 1;            /* first tangential column */
   const int t2 = (dir == 2) ? 2 : 3;            /* second tangential column */
   const int L = n + 2 * ng;
+  const int S = 2 * ng; /* stencil of a face: 6 (WENO5) or 8 (WENO7) points */
+  if (ng != 3 && ng != 4) return -2;
   double *rho_i = malloc(sizeof(double) * L), *P = malloc(sizeof(double) * L),
          *gam = malloc(sizeof(double) * L), *cs = malloc(sizeof(double) * L),
          *vel = malloc(sizeof(double) * 3 * L), *ht = malloc(sizeof(double) * L),
          *et = malloc(sizeof(double) * L), *Ya = malloc(sizeof(double) * NRV * L),
-         *fhat = malloc(sizeof(double) * NV * (n + 1));
+         *fhat = malloc(sizeof(double) * NV * (n + 1)),
+         *fhat_alt = malloc(sizeof(double) * NV * (n + 1));
+  const int np_dir = g->nproc[dir] > 1 ? g->nproc[dir] : 1;
   int status = 0;
   for (int b = 0; b < nb; b++)
     for (int a = 0; a < na; a++) {
@@ -336,8 +443,8 @@ int ora_euler_llf(int dir, const ora_grid *g, const ora_thd *thd, const double *
       }
       for (int il = -1; il < n; il++) {
         int ir = il + 1, ql = il + ng, qr = ir + ng;
-        double fs[6][NV], Ls[6][3];
-        for (int st = 0; st < 6; st++) {
+        double fs[8][NV], Ls[8][3];
+        for (int st = 0; st < S; st++) {
           int is = il + st + 1 - ng, q = is + ng;
           double un = vel[cn - 1 + 3 * q];
           fs[st][0] = V(cn, is);
@@ -376,19 +483,26 @@ int ora_euler_llf(int dir, const ora_grid *g, const ora_thd *thd, const double *
         }
         double el[NV][NV], er[NV][NV];
         ora_eigvecs(dir, thd, uc, Yc, psi_c, ht_c, cs_c, gam_c, el, er);
-        /* characteristic LLF + WENO per field: src/module_weno.f90:955-989 */
-        int shk = 0;
-        for (int p = il - ng + 1; p <= il + ng; p++) {
-          long cc = C(p);
-          if (sensor[cc]) shk = 1;
-        }
-        double ghat[NV];
+        /* characteristic LLF + WENO per field: src/module_weno.f90:955-989.
+         * The flags a rank sees are its own interior cells' (the sensor is
+         * neither set on ghosts nor exchanged): the face takes the view of
+         * the rank of the cell to its right; a face between two ranks is
+         * also formed with the left rank's view (shk_alt) for that rank's
+         * last cell. */
+        const int rr = rank_of(il + 1, n, np_dir), rl = rank_of(il, n, np_dir);
+        const int shk = flags_in(sensor, base, stride, il - ng + 1, il + ng,
+                                 rank_lo(rr, n, np_dir), rank_hi(rr, n, np_dir));
+        const int split = il >= 0 && il + 1 < n && rl != rr;
+        const int shk_alt = split ? flags_in(sensor, base, stride, il - ng + 1, il + ng,
+                                             rank_lo(rl, n, np_dir), rank_hi(rl, n, np_dir))
+                                  : shk;
+        double ghat[NV], ghat_alt[NV];
         for (int m = 0; m < NV; m++) {
           int lam = (m == 0) ? 0 : (m == 2 ? 2 : 1);
           double eigenmax = -1.0;
-          for (int st = 0; st < 6; st++) eigenmax = fmax(Ls[st][lam], eigenmax);
-          double gp[6], gm[6];
-          for (int st = 0; st < 6; st++) {
+          for (int st = 0; st < S; st++) eigenmax = fmax(Ls[st][lam], eigenmax);
+          double gp[8], gm[8];
+          for (int st = 0; st < S; st++) {
             int is = il + st + 1 - ng;
             double wc = 0.0, gc = 0.0;
             for (int mm = 0; mm < NV; mm++) {
@@ -398,21 +512,24 @@ int ora_euler_llf(int dir, const ora_grid *g, const ora_thd *thd, const double *
             gp[st] = 0.5 * (gc + eigenmax * wc);
             gm[st] = gc - gp[st];
           }
-          double gl, gr;
-          if (shk && weno_avg) ora_wenorec(gp, gm, &gl, &gr);
-          else ora_wenorec_nw(gp, gm, &gl, &gr);
-          ghat[m] = gl + gr;
+          ghat[m] = weno_face(ng, shk && weno_avg, gp, gm);
+          ghat_alt[m] = (shk_alt == shk) ? ghat[m] : weno_face(ng, shk_alt && weno_avg, gp, gm);
         }
         for (int m = 0; m < NV; m++) {
-          double s = 0.0;
+          double s = 0.0, sa = 0.0;
           for (int mm = 0; mm < NV; mm++) s = s + er[m][mm] * ghat[mm];
+          for (int mm = 0; mm < NV; mm++) sa = sa + er[m][mm] * ghat_alt[mm];
           fhat[m * (n + 1) + (il + 1)] = s;
+          fhat_alt[m * (n + 1) + (il + 1)] = sa;
         }
       }
-      /* flux difference: src/module_weno.f90:998-1020 */
+      /* flux difference: src/module_weno.f90:998-1020 (a rank's last cell
+       * takes its right face from its own view) */
       for (int m = 0; m < NV; m++)
         for (int ic = 0; ic < n; ic++) {
-          double df = (fhat[m * (n + 1) + ic + 1] - fhat[m * (n + 1) + ic]) * d_i[ic + ng];
+          const double *fr = (ic + 1 < n && rank_of(ic + 1, n, np_dir) != rank_of(ic, n, np_dir))
+                                 ? fhat_alt : fhat;
+          double df = (fr[m * (n + 1) + ic + 1] - fhat[m * (n + 1) + ic]) * d_i[ic + ng];
           long cc = (long)m * nc + C(ic);
           if (dir == 0) fl[cc] = df * 1.0;
           else fl[cc] = fl[cc] + df * 1.0;
@@ -421,7 +538,7 @@ int ora_euler_llf(int dir, const ora_grid *g, const ora_thd *thd, const double *
 #undef C
     }
   free(rho_i); free(P); free(gam); free(cs); free(vel); free(ht); free(et); free(Ya);
-  free(fhat);
+  free(fhat); free(fhat_alt);
   return status;
 }
 
@@ -666,34 +783,73 @@ void ora_upd_prim_var_woT(int domain, const ora_grid *g, const ora_thd *thd, con
       }
 }
 
-/* comm_cons on one rank with periodic faces: src/module_parallel.f90:1715-1777
- * (self-exchange through the periodic Cartesian communicator; faces only,
- * edges and corners are not written). */
+/* comm_cons on one rank: src/module_parallel.f90:1715-1777.  Periodic
+ * directions exchange with themselves through the periodic Cartesian
+ * communicator (faces only, edges and corners are not written); a physical
+ * (non-periodic) face has no neighbour and is left to upd_boundaries. */
 void ora_comm_cons(const ora_grid *g, double *v) {
   int nx = g->nx, ny = g->ny, nz = g->nz, ng = g->ng;
   long nc = ncell(g);
   for (int l = 0; l < NV; l++) {
     double *a = v + l * nc;
-    for (int k = 0; k < nz; k++)
-      for (int j = 0; j < ny; j++)
-        for (int s = 1; s <= ng; s++) {
-          a[cidx(g, -s, j, k)] = a[cidx(g, nx - s, j, k)];
-          a[cidx(g, nx - 1 + s, j, k)] = a[cidx(g, s - 1, j, k)];
-        }
-    if (g->ndim >= 2)
+    if (is_periodic(g, 0))
+      for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+          for (int s = 1; s <= ng; s++) {
+            a[cidx(g, -s, j, k)] = a[cidx(g, nx - s, j, k)];
+            a[cidx(g, nx - 1 + s, j, k)] = a[cidx(g, s - 1, j, k)];
+          }
+    if (g->ndim >= 2 && is_periodic(g, 1))
       for (int k = 0; k < nz; k++)
         for (int i = 0; i < nx; i++)
           for (int s = 1; s <= ng; s++) {
             a[cidx(g, i, -s, k)] = a[cidx(g, i, ny - s, k)];
             a[cidx(g, i, ny - 1 + s, k)] = a[cidx(g, i, s - 1, k)];
           }
-    if (g->ndim == 3)
+    if (g->ndim == 3 && is_periodic(g, 2))
       for (int j = 0; j < ny; j++)
         for (int i = 0; i < nx; i++)
           for (int s = 1; s <= ng; s++) {
             a[cidx(g, i, j, -s)] = a[cidx(g, i, j, nz - s)];
             a[cidx(g, i, j, nz - 1 + s)] = a[cidx(g, i, j, s - 1)];
           }
+  }
+}
+
+/* upd_boundaries with bc_extrapolation (src/module_BCs.f90:41-141,
+ * 327-412): the ng ghost layers of an extrapolated face take the edge
+ * value of the interior, face slab only, faces in the order W, E, S, N, B,
+ * F. */
+void ora_upd_boundaries(const ora_grid *g, double *v) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz, ng = g->ng;
+  const long nc = ncell(g);
+  for (int f = 0; f < 2 * g->ndim; f++) {
+    if (g->bc[f] != ORA_BC_EXTRAPOLATION) continue;
+    const int d = f / 2, hi = f % 2;
+    for (int l = 0; l < NV; l++) {
+      double *a = v + l * nc;
+      if (d == 0)
+        for (int k = 0; k < nz; k++)
+          for (int j = 0; j < ny; j++)
+            for (int s = 1; s <= ng; s++) {
+              if (hi) a[cidx(g, nx - 1 + s, j, k)] = a[cidx(g, nx - 1, j, k)];
+              else a[cidx(g, -s, j, k)] = a[cidx(g, 0, j, k)];
+            }
+      else if (d == 1)
+        for (int k = 0; k < nz; k++)
+          for (int s = 1; s <= ng; s++)
+            for (int i = 0; i < nx; i++) {
+              if (hi) a[cidx(g, i, ny - 1 + s, k)] = a[cidx(g, i, ny - 1, k)];
+              else a[cidx(g, i, -s, k)] = a[cidx(g, i, 0, k)];
+            }
+      else
+        for (int s = 1; s <= ng; s++)
+          for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+              if (hi) a[cidx(g, i, j, nz - 1 + s)] = a[cidx(g, i, j, nz - 1)];
+              else a[cidx(g, i, j, -s)] = a[cidx(g, i, j, 0)];
+            }
+    }
   }
 }
 
@@ -719,11 +875,17 @@ void ora_sensor_adams_shariff(const ora_grid *g, double max_rel_weight, const do
   (void)nc;
   for (int d = 0; d < g->ndim; d++) {
     int n = (d == 0) ? nx : (d == 1 ? ny : nz);
+    int np = g->nproc[d] > 1 ? g->nproc[d] : 1;
     for (int k = 0; k < nz; k++)
       for (int j = 0; j < ny; j++)
         for (int i = 0; i < nx; i++) {
           int p = (d == 0) ? i : (d == 1 ? j : k);
-          if (p >= n - 1) continue;
+          /* the sweep of each rank stops one short of its last cell
+           * (ranks own (r n)/np .. ((r+1) n)/np - 1, module_parallel.f90:
+           * 547-548) */
+          int r = 0;
+          while (r + 1 < np && ((long)(r + 1) * n) / np <= p) r++;
+          if (p >= ((long)(r + 1) * n) / np - 1) continue;
           int i2 = i + (d == 0), j2 = j + (d == 1), k2 = k + (d == 2);
           long c = cidx(g, i, j, k), c2 = cidx(g, i2, j2, k2);
           double ri = 1.0 / v[c], ri2 = 1.0 / v[c2];
@@ -737,10 +899,10 @@ void ora_sensor_adams_shariff(const ora_grid *g, double max_rel_weight, const do
 
 /* shock_sensor_ducros: src/module_tools.f90:2665-2719, with the velocity
  * gradients of compute_duixj (:2521-2651: velocities on the interior and
- * the face ghost slabs, then d?_fixed1).  z uses alpha6 as x and y do (the
- * reference's z operator is the 8-point alph8, which reads k+-4 of an
- * ng = 3 halo).  Sticky: flags are only ever set. */
-static inline double deriv7(const double *a, long c, long stride, double dinv);
+ * the face ghost slabs, then d?_fixed1, derivatives as dgrid below).
+ * Sticky: flags are only ever set. */
+static inline double dgrid(const ora_grid *g, const double *a, long c, int d, int i, int j, int k,
+                           double dinv);
 
 void ora_sensor_ducros(const ora_grid *g, double treduc, const double *dx_i, const double *dy_i,
                        const double *dz_i, const double *v, uint8_t *sensor) {
@@ -758,15 +920,14 @@ void ora_sensor_ducros(const ora_grid *g, double treduc, const double *dx_i, con
           for (int q = 0; q < nd; q++) vel[q * nc + c] = v[(1 + q) * nc + c] * rho_i;
         }
   }
-  const long strd[3] = {1, NXg(g), NXg(g) * NYg(g)};
   for (int k = 0; k < g->nz; k++)
     for (int j = 0; j < g->ny; j++)
       for (int i = 0; i < g->nx; i++) {
         long c = cidx(g, i, j, k);
         const double dinv[3] = {dx_i[i + ng], dy_i[j + ng], dz_i[k + ng]};
-        double du[3][3]; /* du[a][b] = d u_a / d x_b: duixj(3 a + b + 1) */
+        double du[3][3] = {{0}}; /* du[a][b] = d u_a / d x_b: duixj(3 a + b + 1) */
         for (int a = 0; a < nd; a++)
-          for (int b = 0; b < nd; b++) du[a][b] = deriv7(vel + a * nc, c, strd[b], dinv[b]);
+          for (int b = 0; b < nd; b++) du[a][b] = dgrid(g, vel + a * nc, c, b, i, j, k, dinv[b]);
         double div, vort, wrk;
         if (nd == 2) {
           div = du[0][0] + du[1][1];
@@ -847,22 +1008,73 @@ void ora_transport_cell(const ora_thd *thd, double rho, double T, double W_i, co
   *mu = mus;
 }
 
-/* 6th-order central first derivative, periodic/MPI branch (alpha6):
- * src/module_deriv.f90:37-40, 212-469 (dx), 926-1090 (dx_fixed1),
- * 1644-1720 (dx_fixed2).  The z operators of the reference always use the
- * 8-point alph8 stencil (src/module_deriv.f90:753-763), which reads k+-4 out
- * of an ng = 3 halo; this restatement uses alpha6 in z as in x and y. */
-static const double ALPHA6[7] = {-1.0 / 60.0, 9.0 / 60.0, -45.0 / 60.0, 0.0,
-                                 45.0 / 60.0, -9.0 / 60.0, 1.0 / 60.0};
+/* First derivatives: src/module_deriv.f90:20-78 (weights), 212-469 (dx;
+ * d?_fixed1 at :926-1171 and d?_fixed2 at :1644-1720 have the same
+ * structure).  Central 6th order (ng = 3) or 8th order (ng = 4), summed over
+ * the whole stencil including its zero weight; next to a physical
+ * (non-periodic) boundary the ng points nearest to it take the closures:
+ * 4th-order one-sided, 4th-order upwind, 4th-order central and, for ng = 4,
+ * 6th-order central (the right-hand ones negated).  The right boundary's
+ * block is applied after the left one's.  The reference's z operators use
+ * the 8-point stencil whatever ng is, which reads k+-4 of an ng = 3 halo;
+ * with ng = 3 this restatement uses the 6th-order stencil in z as in x and
+ * y. */
+static const double D6[7] = {-1.0 / 60.0, 9.0 / 60.0, -45.0 / 60.0, 0.0,
+                             45.0 / 60.0, -9.0 / 60.0, 1.0 / 60.0};
+static const double D8[9] = {3.0 / 840.0, -32.0 / 840.0, 168.0 / 840.0, -672.0 / 840.0, 0.0,
+                             672.0 / 840.0, -168.0 / 840.0, 32.0 / 840.0, -3.0 / 840.0};
+static const double D4[5] = {1.0 / 12.0, -8.0 / 12.0, 0.0, 8.0 / 12.0, -1.0 / 12.0};
+static const double D4OL[5] = {-25.0 / 12.0, 48.0 / 12.0, -36.0 / 12.0, 16.0 / 12.0,
+                               -3.0 / 12.0};                      /* l = 0..4 */
+static const double D4UL[5] = {-3.0 / 12.0, -10.0 / 12.0, 18.0 / 12.0, -6.0 / 12.0,
+                               1.0 / 12.0};                       /* l = -1..3 */
+static const double D4UR[5] = {1.0 / 12.0, -6.0 / 12.0, 18.0 / 12.0, -10.0 / 12.0,
+                               -3.0 / 12.0};                      /* l = -3..1 */
+static const double D4OR[5] = {-3.0 / 12.0, 16.0 / 12.0, -36.0 / 12.0, 48.0 / 12.0,
+                               -25.0 / 12.0};                     /* l = -4..0 */
 
-static inline double deriv7(const double *a, long c, long stride, double dinv) {
+static inline double wsum(const double *w, int lo, int hi, const double *a, long c, long s) {
   double dd = 0.0;
-  for (int l = -3; l <= 3; l++) dd = dd + ALPHA6[l + 3] * a[c + l * stride];
-  return dd * dinv;
+  for (int l = lo; l <= hi; l++) dd = dd + w[l - lo] * a[c + l * s];
+  return dd;
+}
+
+static double deriv_at(const double *a, long c, long s, double dinv, int p, int n, int ng,
+                       int lphys, int rphys) {
+  if (rphys && p >= n - ng) {
+    if (p == n - 1) return -(wsum(D4OR, -4, 0, a, c, s) * dinv);
+    if (p == n - 2) return -(wsum(D4UR, -3, 1, a, c, s) * dinv);
+    if (p == n - 3) return wsum(D4, -2, 2, a, c, s) * dinv;
+    return wsum(D6, -3, 3, a, c, s) * dinv;           /* p == n - 4, ng = 4 */
+  }
+  if (lphys && p < ng) {
+    if (p == 0) return wsum(D4OL, 0, 4, a, c, s) * dinv;
+    if (p == 1) return wsum(D4UL, -1, 3, a, c, s) * dinv;
+    if (p == 2) return wsum(D4, -2, 2, a, c, s) * dinv;
+    return wsum(D6, -3, 3, a, c, s) * dinv;           /* p == 3, ng = 4 */
+  }
+  if (ng == 4) return wsum(D8, -4, 4, a, c, s) * dinv;
+  return wsum(D6, -3, 3, a, c, s) * dinv;
+}
+
+double ora_deriv_line(const double *a, long stride, double dinv, int p, int n, int ng,
+                      int lphys, int rphys) {
+  return deriv_at(a, (long)p * stride, stride, dinv, p, n, ng, lphys, rphys);
+}
+
+/* derivative along direction d of field a at interior cell (i, j, k) = c */
+static inline double dgrid(const ora_grid *g, const double *a, long c, int d, int i, int j, int k,
+                           double dinv) {
+  const int n = (d == 0) ? g->nx : (d == 1 ? g->ny : g->nz);
+  const int p = (d == 0) ? i : (d == 1 ? j : k);
+  const long s = (d == 0) ? 1 : (d == 1 ? NXg(g) : NXg(g) * NYg(g));
+  return deriv_at(a, c, s, dinv, p, n, g->ng, g->bc[2 * d] != ORA_BC_PERIODIC,
+                  g->bc[2 * d + 1] != ORA_BC_PERIODIC);
 }
 
 /* viscflux_xy / viscflux_xyz: src/module_viscflux.f90:721-1135, 1660-2189.
- * Periodic faces only (no wall, symmetry or injection branches). */
+ * Periodic or extrapolated faces (no wall, symmetry or injection branches);
+ * comm_deriv fills the derivative ghosts of periodic directions only. */
 int ora_viscflux(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const double *dx_i,
                  const double *dy_i, const double *dz_i, const double *T, const double *W_i,
                  const double *ha, const double *v, double *fl) {
@@ -873,7 +1085,6 @@ int ora_viscflux(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, cons
   const double twothirds = 2.0 / 3.0;
   const double rvd0 = adi->sqgmr / adi->Sc;
   const double rlam0 = adi->sqgmr * adi->ggmopr;
-  const long sx = 1, sy = NXg(g), sz = NXg(g) * NYg(g);
   double *vel = calloc((size_t)3 * nc, sizeof(double));
   double *Xa = calloc((size_t)NRV * nc, sizeof(double));
   double *fd = calloc((size_t)nderiv * nc, sizeof(double));
@@ -895,7 +1106,6 @@ int ora_viscflux(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, cons
   }
   /* first derivatives on the interior (d?_fixed1) */
   const double *dinv[3] = {dx_i, dy_i, dz_i};
-  const long strd[3] = {sx, sy, sz};
   for (int k = 0; k < g->nz; k++)
     for (int j = 0; j < g->ny; j++)
       for (int i = 0; i < g->nx; i++) {
@@ -904,12 +1114,12 @@ int ora_viscflux(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, cons
         int slot = 0;
         for (int q = 0; q < ndim; q++)          /* du/dx_j, dv/dx_j, dw/dx_j */
           for (int dd = 0; dd < ndim; dd++)
-            fd[(slot++) * nc + c] = deriv7(vel + q * nc, c, strd[dd], dinv[dd][idx3[dd]]);
+            fd[(slot++) * nc + c] = dgrid(g, vel + q * nc, c, dd, i, j, k, dinv[dd][idx3[dd]]);
         for (int dd = 0; dd < ndim; dd++)       /* dT/dx_j */
-          fd[(slot++) * nc + c] = deriv7(T, c, strd[dd], dinv[dd][idx3[dd]]);
+          fd[(slot++) * nc + c] = dgrid(g, T, c, dd, i, j, k, dinv[dd][idx3[dd]]);
         for (int l = 0; l < NRV; l++)           /* dX_l/dx_j */
           for (int dd = 0; dd < ndim; dd++)
-            fd[(slot++) * nc + c] = deriv7(Xa + l * nc, c, strd[dd], dinv[dd][idx3[dd]]);
+            fd[(slot++) * nc + c] = dgrid(g, Xa + l * nc, c, dd, i, j, k, dinv[dd][idx3[dd]]);
       }
   /* comm_deriv: periodic self-exchange of fd on the faces (:1923) */
   {
@@ -1016,35 +1226,35 @@ int ora_viscflux(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, cons
         double dxi = dx_i[i + ng], dyi = dy_i[j + ng], dzi = dz_i[k + ng];
         if (ndim == 3) {
           double wx, wy, wz;
-          wx = deriv7(sf + 0 * nc, c, sx, dxi); wy = deriv7(sf + 1 * nc, c, sy, dyi);
-          wz = deriv7(sf + 2 * nc, c, sz, dzi);
+          wx = dgrid(g, sf + 0 * nc, c, 0, i, j, k, dxi); wy = dgrid(g, sf + 1 * nc, c, 1, i, j, k, dyi);
+          wz = dgrid(g, sf + 2 * nc, c, 2, i, j, k, dzi);
           fl[1 * nc + c] = fl[1 * nc + c] + (wx + wy + wz);
-          wx = deriv7(sf + 1 * nc, c, sx, dxi); wy = deriv7(sf + 3 * nc, c, sy, dyi);
-          wz = deriv7(sf + 4 * nc, c, sz, dzi);
+          wx = dgrid(g, sf + 1 * nc, c, 0, i, j, k, dxi); wy = dgrid(g, sf + 3 * nc, c, 1, i, j, k, dyi);
+          wz = dgrid(g, sf + 4 * nc, c, 2, i, j, k, dzi);
           fl[2 * nc + c] = fl[2 * nc + c] + (wx + wy + wz);
-          wx = deriv7(sf + 2 * nc, c, sx, dxi); wy = deriv7(sf + 4 * nc, c, sy, dyi);
-          wz = deriv7(sf + 5 * nc, c, sz, dzi);
+          wx = dgrid(g, sf + 2 * nc, c, 0, i, j, k, dxi); wy = dgrid(g, sf + 4 * nc, c, 1, i, j, k, dyi);
+          wz = dgrid(g, sf + 5 * nc, c, 2, i, j, k, dzi);
           fl[3 * nc + c] = fl[3 * nc + c] + (wx + wy + wz);
-          wx = deriv7(sf + 6 * nc, c, sx, dxi); wy = deriv7(sf + 7 * nc, c, sy, dyi);
-          wz = deriv7(sf + 8 * nc, c, sz, dzi);
+          wx = dgrid(g, sf + 6 * nc, c, 0, i, j, k, dxi); wy = dgrid(g, sf + 7 * nc, c, 1, i, j, k, dyi);
+          wz = dgrid(g, sf + 8 * nc, c, 2, i, j, k, dzi);
           fl[4 * nc + c] = fl[4 * nc + c] + (wx + wy + wz);
           for (int l = 1; l <= NRV; l++) {
-            wx = deriv7(fd + (long)(nfluid + ndim * l - 3) * nc, c, sx, dxi);
-            wy = deriv7(fd + (long)(nfluid + ndim * l - 2) * nc, c, sy, dyi);
-            wz = deriv7(fd + (long)(nfluid + ndim * l - 1) * nc, c, sz, dzi);
+            wx = dgrid(g, fd + (long)(nfluid + ndim * l - 3) * nc, c, 0, i, j, k, dxi);
+            wy = dgrid(g, fd + (long)(nfluid + ndim * l - 2) * nc, c, 1, i, j, k, dyi);
+            wz = dgrid(g, fd + (long)(nfluid + ndim * l - 1) * nc, c, 2, i, j, k, dzi);
             fl[(NIV + l - 1) * nc + c] = fl[(NIV + l - 1) * nc + c] + (wx + wy + wz);
           }
         } else {
           double wx, wy;
-          wx = deriv7(sf + 0 * nc, c, sx, dxi); wy = deriv7(sf + 1 * nc, c, sy, dyi);
+          wx = dgrid(g, sf + 0 * nc, c, 0, i, j, k, dxi); wy = dgrid(g, sf + 1 * nc, c, 1, i, j, k, dyi);
           fl[1 * nc + c] = fl[1 * nc + c] + (wx + wy);
-          wx = deriv7(sf + 1 * nc, c, sx, dxi); wy = deriv7(sf + 2 * nc, c, sy, dyi);
+          wx = dgrid(g, sf + 1 * nc, c, 0, i, j, k, dxi); wy = dgrid(g, sf + 2 * nc, c, 1, i, j, k, dyi);
           fl[2 * nc + c] = fl[2 * nc + c] + (wx + wy);
-          wx = deriv7(sf + 3 * nc, c, sx, dxi); wy = deriv7(sf + 4 * nc, c, sy, dyi);
+          wx = dgrid(g, sf + 3 * nc, c, 0, i, j, k, dxi); wy = dgrid(g, sf + 4 * nc, c, 1, i, j, k, dyi);
           fl[4 * nc + c] = fl[4 * nc + c] + (wx + wy);
           for (int l = 1; l <= NRV; l++) {
-            wx = deriv7(fd + (long)(nfluid + ndim * l - 2) * nc, c, sx, dxi);
-            wy = deriv7(fd + (long)(nfluid + ndim * l - 1) * nc, c, sy, dyi);
+            wx = dgrid(g, fd + (long)(nfluid + ndim * l - 2) * nc, c, 0, i, j, k, dxi);
+            wy = dgrid(g, fd + (long)(nfluid + ndim * l - 1) * nc, c, 1, i, j, k, dyi);
             fl[(NIV + l - 1) * nc + c] = fl[(NIV + l - 1) * nc + c] + (wx + wy);
           }
         }
@@ -1055,8 +1265,8 @@ int ora_viscflux(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, cons
 
 /* ------------------------------------------------------------ SSP-RK3 */
 
-/* rk3: src/module_solver.f90:694-856 (periodic faces, no LES/EGLIB/NSCBC,
- * Adams-Shariff sensor).  f_out (optional, ncell*NV) receives the RHS of
+/* rk3: src/module_solver.f90:694-856 (periodic or extrapolated faces, no
+ * LES/EGLIB/NSCBC).  f_out (optional, ncell*NV) receives the RHS of
  * the first stage. */
 int ora_rk3(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const ora_opts *op,
             double dt, const double *dx_i, const double *dy_i, const double *dz_i, double *T,
@@ -1091,6 +1301,7 @@ int ora_rk3(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const ora
     s = ora_prim_inv_var(0, g, thd, v, W_i, T, cp, ha, NULL);
     if (s && !status) status = s;
     ora_comm_cons(g, v);
+    ora_upd_boundaries(g, v);
     s = ora_upd_prim_var_ghost(g, thd, op->forcesum, v, W_i, T, cp, ha);
     if (s && !status) status = s;
   }
@@ -1350,5 +1561,5 @@ void ora_reactor_rhs(const ora_mech *m, double rho, const double *z, double *zp)
 /* one line of the 6th-order central derivative (alpha6), for tests:
  * out[i] = d/dx a at a[i], i = 3..n-4 */
 void ora_deriv7_line(int n, const double *a, double dinv, double *out) {
-  for (int i = 3; i < n - 3; i++) out[i] = deriv7(a, i, 1, dinv);
+  for (int i = 3; i < n - 3; i++) out[i] = wsum(D6, -3, 3, a, i, 1) * dinv;
 }

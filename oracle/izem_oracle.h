@@ -24,12 +24,27 @@
 
 #include <stdint.h>
 
+/* species count: 9 (the H2/air mechanism) unless the build sets it (the
+ * 2-species O2/N2 shock-tube case is built with -DORA_NRV=2) */
+#ifndef ORA_NRV
 #define ORA_NRV 9
+#endif
 #define ORA_NIV 5
 #define ORA_NV (ORA_NIV + ORA_NRV)
 
+/* ng = 3 (WENO5) or 4 (WENO7): ng = (ORDER_WENO + 1) / 2,
+ * src/module_input.f90:712.  bc[face] for the faces W, E, S, N, B, F:
+ * ORA_BC_PERIODIC or ORA_BC_EXTRAPOLATION (src/module_BCs.f90:41-141,
+ * 327-412).  nproc[d]: MPI ranks along direction d in the run being
+ * restated (0 or 1: one); only the shock sensor depends on it, because its
+ * sweep stops one cell short of each rank's end (src/module_tools.f90:
+ * 2758-2770). */
+#define ORA_BC_PERIODIC 0
+#define ORA_BC_EXTRAPOLATION 1
 typedef struct {
   int nx, ny, nz, ng, ndim;
+  int bc[6];
+  int nproc[3];
 } ora_grid;
 
 /* thd_type subset used by the flow kernels (src/thd/type_thd.f90:10-91) */
@@ -95,9 +110,22 @@ double ora_wmix_i_scalar(const ora_thd *thd, const double *Y);
 double ora_cp_scalar(const ora_thd *thd, double temp, const double *Y);
 void ora_ha_scalar(const ora_thd *thd, double temp, double *ha);
 
-/* WENO */
+/* WENO: 5th order on 6 values (ng = 3), 7th order on 8 values (ng = 4) */
 void ora_wenorec(const double *vp, const double *vm, double *vminus, double *vplus);
 void ora_wenorec_nw(const double *vp, const double *vm, double *vminus, double *vplus);
+void ora_wenorec7(const double *vp, const double *vm, double *vminus, double *vplus);
+void ora_wenorec7_nw(const double *vp, const double *vm, double *vminus, double *vplus);
+double ora_weno7_beta(int r, const double *q);
+/* first derivative of a line of n interior values (a[0] is interior point
+ * 0, ghosts at negative indices and past n) at interior point p: central
+ * 6th (ng = 3) or 8th (ng = 4) order, with the one-sided closures of a
+ * physical boundary where lphys / rphys is set (src/module_deriv.f90:
+ * 212-469) */
+double ora_deriv_line(const double *a, long stride, double dinv, int p, int n, int ng,
+                      int lphys, int rphys);
+/* bc_extrapolation on every face whose bc is ORA_BC_EXTRAPOLATION, after
+ * ora_comm_cons (upd_boundaries, src/module_BCs.f90:41-141) */
+void ora_upd_boundaries(const ora_grid *g, double *v);
 
 void ora_eigvecs(int dir, const ora_thd *thd, const double *uc, const double *Yc,
                  const double *psi_c, double ht_c, double cs_c, double gam_c,

@@ -26,20 +26,32 @@ _dp = C.POINTER(C.c_double)
 _u8 = C.POINTER(C.c_uint8)
 
 
+BC_PERIODIC, BC_EXTRAPOLATION = 0, 1
+
+
 class Grid(C.Structure):
+    """ora_grid: bc per face W, E, S, N, B, F (zero: periodic) and the MPI
+    ranks per direction of the run being restated (zero: one)."""
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("ng", C.c_int),
-                ("ndim", C.c_int)]
+                ("ndim", C.c_int), ("bc", C.c_int * 6), ("nproc", C.c_int * 3)]
 
 
-class Thd(C.Structure):
-    _fields_ = [("W", C.c_double * NRV), ("Wc", C.c_double * NRV), ("Wc_i", C.c_double * NRV),
-                ("HeatCap", C.c_double * (NRV * 2 * 7)), ("Tpolymin", C.c_double * NRV),
-                ("Tpolylim", C.c_double * NRV), ("Tpolymax", C.c_double * NRV),
-                ("CofD", C.c_double * (NRV * NRV * 4)), ("CofLam", C.c_double * (NRV * 4)),
-                ("CofEta", C.c_double * (NRV * 4)), ("wilke_c1", C.c_double * (NRV * NRV)),
-                ("wilke_c2", C.c_double * (NRV * NRV)), ("gam2", C.c_double),
-                ("Tinf", C.c_double), ("p0", C.c_double), ("pinf", C.c_double),
-                ("Dinf", C.c_double), ("laminf", C.c_double), ("muinf", C.c_double)]
+def _thd_struct(nrv: int):
+    class Thd(C.Structure):
+        _fields_ = [("W", C.c_double * nrv), ("Wc", C.c_double * nrv),
+                    ("Wc_i", C.c_double * nrv), ("HeatCap", C.c_double * (nrv * 2 * 7)),
+                    ("Tpolymin", C.c_double * nrv), ("Tpolylim", C.c_double * nrv),
+                    ("Tpolymax", C.c_double * nrv), ("CofD", C.c_double * (nrv * nrv * 4)),
+                    ("CofLam", C.c_double * (nrv * 4)), ("CofEta", C.c_double * (nrv * 4)),
+                    ("wilke_c1", C.c_double * (nrv * nrv)),
+                    ("wilke_c2", C.c_double * (nrv * nrv)), ("gam2", C.c_double),
+                    ("Tinf", C.c_double), ("p0", C.c_double), ("pinf", C.c_double),
+                    ("Dinf", C.c_double), ("laminf", C.c_double), ("muinf", C.c_double)]
+    return Thd
+
+
+Thd = _thd_struct(NRV)
+_THD = {NRV: Thd, 2: _thd_struct(2)}
 
 
 class Adi(C.Structure):
@@ -73,8 +85,9 @@ def _fill(arr, values):
         arr[i] = x
 
 
-def make_thd(thd) -> Thd:
-    t = Thd()
+def make_thd(thd):
+    """The ora_thd struct of a chemdata.Thd (9 or 2 species)."""
+    t = _THD[len(thd.W)]()
     for name in ("W", "Wc", "Wc_i"):
         _fill(getattr(t, name), getattr(thd, name))
     _fill(t.HeatCap, thd.heatcap)
@@ -126,11 +139,12 @@ def _p(a):
 class Oracle:
     """The C restatement.  Arrays use the Fortran-order layout with ghosts."""
 
-    def __init__(self, lib_path: str | None = None):
-        lib_path = lib_path or os.path.join(HERE, "liboracle.so")
+    def __init__(self, lib_path: str | None = None, nrv: int = NRV):
+        name = "liboracle.so" if nrv == NRV else f"liboracle_nrv{nrv}.so"
+        lib_path = lib_path or os.path.join(HERE, name)
         if not os.path.exists(lib_path):
-            subprocess.run(["make", "-C", HERE, "liboracle.so"], check=True,
-                           stdout=subprocess.DEVNULL)
+            subprocess.run(["make", "-C", HERE, name], check=True, stdout=subprocess.DEVNULL)
+        self.nrv = nrv
         self.lib = C.CDLL(lib_path)
         L = self.lib
         L.ora_euler_llf.restype = C.c_int
@@ -148,6 +162,11 @@ class Oracle:
         L.ora_reactor_rhs.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
         L.ora_transport_cell.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, _dp,
                                          _dp, _dp, _dp]
+
+    def thd_struct(self, thd):
+        if len(thd.W) != self.nrv:
+            raise ValueError(f"{len(thd.W)} species for an oracle built for {self.nrv}")
+        return make_thd(thd)
 
 
 class RefChem:

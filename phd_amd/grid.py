@@ -6,7 +6,8 @@ grid.dat holds the keyword X-DIRECTION, ntx coordinates one per record, then
 Y-DIRECTION and nty, Z-DIRECTION and ntz, and END (blank records are skipped
 by the list-directed reads).  The metric of a direction is the derivative
 of its coordinates with respect to the index, taken with the reference's
-stencils: sixth-order central in the interior (ng = 3), fourth-order central
+stencils: sixth-order central in the interior (ng = 3; eighth order with
+sixth order at the fourth point from each end for ng = 4), fourth-order central
 at the third point from each end and fourth-order one-sided and upwind
 closures at the first two; it is inverted on 1..nt, and the ghost entries
 keep the value 1 that the reference initialises them to.  Nondimensional
@@ -22,6 +23,8 @@ from .ics import Box
 # stencil coefficients (module_deriv.f90:23-66), index offset -> weight
 _A6 = {-3: -1.0 / 60.0, -2: 9.0 / 60.0, -1: -45.0 / 60.0, 0: 0.0, 1: 45.0 / 60.0,
        2: -9.0 / 60.0, 3: 1.0 / 60.0}
+_A8 = {-4: 3.0 / 840.0, -3: -32.0 / 840.0, -2: 168.0 / 840.0, -1: -672.0 / 840.0, 0: 0.0,
+       1: 672.0 / 840.0, 2: -168.0 / 840.0, 3: 32.0 / 840.0, 4: -3.0 / 840.0}
 _A4 = {-2: 1.0 / 12.0, -1: -8.0 / 12.0, 0: 0.0, 1: 8.0 / 12.0, 2: -1.0 / 12.0}
 _A4OL = {0: -25.0 / 12.0, 1: 48.0 / 12.0, 2: -36.0 / 12.0, 3: 16.0 / 12.0, 4: -3.0 / 12.0}
 _A4UL = {-1: -3.0 / 12.0, 0: -10.0 / 12.0, 1: 18.0 / 12.0, 2: -6.0 / 12.0, 3: 1.0 / 12.0}
@@ -57,15 +60,20 @@ def _deriv(v: np.ndarray, m: int, coef: dict) -> float:
 
 def metric(t: np.ndarray, ng: int = 3) -> np.ndarray:
     """1/(dt/di) of one direction's nt coordinates, as n + 2 ng entries with
-    the ghost entries at 1 (dscalar with d_i = 1, then the inversion)."""
-    if ng != 3:
-        raise ValueError("the metric is implemented for ng = 3 (WENO5)")
+    the ghost entries at 1 (dscalar with d_i = 1, then the inversion).  ng = 3
+    (WENO5): sixth order inside; ng = 4 (WENO7): eighth order inside and
+    sixth order at the fourth point from each end."""
+    if ng not in (3, 4):
+        raise ValueError("the metric is defined for ng = 3 (WENO5) and ng = 4 (WENO7)")
     nt = t.size
     if nt < 2 * ng + 1:
         raise ValueError("not enough points for the metric stencils")
     dv = np.ones(nt)
     for m in range(ng, nt - ng):
-        dv[m] = _deriv(t, m, _A6) * 1.0
+        dv[m] = _deriv(t, m, _A6 if ng == 3 else _A8) * 1.0
+    if ng == 4:
+        dv[3] = _deriv(t, 3, _A6) * 1.0
+        dv[nt - 4] = _deriv(t, nt - 4, _A6) * 1.0
     dv[0] = _deriv(t, 0, _A4OL) * 1.0
     dv[1] = _deriv(t, 1, _A4UL) * 1.0
     dv[2] = _deriv(t, 2, _A4) * 1.0

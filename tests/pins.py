@@ -28,6 +28,14 @@ raises AssertionError when the build departs from what the check fixes:
   thermal limits of a uniform state in closed form (SI units).
 * Adams-Shariff sensor (module_tools.f90:2733-2814) at a real threshold:
   exactly the cells its definition flags on a density/pressure step.
+* WENO7 (module_weno.f90:71-172, 246-527): the linear 7th-order face value is
+  exact on degree-6 polynomials; with indicators far below eps the JS
+  weights are the ideal ones (exact on degree 6); the indicators are 240
+  times Jiang and Shu's integral definition for every candidate.
+* First derivatives next to physical boundaries (module_deriv.f90:212-469):
+  exact on degree 4 at every point, reading no ghost, with the ng points
+  nearest each end taking the 4th-order closures; central 6th / 8th order on
+  periodic lines.
 * The negative-temperature repair of prim_inv_var (module_thermodynamics.f90:
   120-157): the conserved variables of a cell whose Newton iterate turns
   negative become the mean of its x neighbours' (in sweep order for
@@ -894,9 +902,140 @@ def check_negT_repair_at_the_array_edge(oracle, adi, thd):
     assert st == -2
 
 
+
+
+# ----------------------------------------------------------- WENO7, closures
+def _cell_avg_poly(c, j):
+    """Average of sum_k c[k] x^k over the unit cell [j, j+1]."""
+    return sum(ck * ((j + 1) ** (k + 1) - j ** (k + 1)) / (k + 1) for k, ck in enumerate(c))
+
+
+def _poly(c, x):
+    return sum(ck * x ** k for k, ck in enumerate(c))
+
+
+def _weno7(oracle, vals, fn):
+    vp = np.ascontiguousarray(vals, dtype=np.float64)
+    lo, hi = C.c_double(), C.c_double()
+    getattr(oracle.lib, fn)(_dp(vp), _dp(vp), C.byref(lo), C.byref(hi))
+    return lo.value, hi.value
+
+
+def check_weno7_linear_exact(oracle, adi, thd):
+    """wenorec_nw, ng = 4 (module_weno.f90:246-256): from the averages of a
+    degree-6 polynomial over cells 0..7 (unit width) both face values at the
+    face between cells 3 and 4 (x = 4) equal the polynomial there: the '-'
+    value from cells 0..6, the '+' value from cells 1..7."""
+    rng = np.random.default_rng(11)
+    for _ in range(5):
+        c = rng.standard_normal(7)
+        q = np.array([_cell_avg_poly(c, j) for j in range(8)])
+        lo, hi = _weno7(oracle, q, "ora_wenorec7_nw")
+        ex = _poly(c, 4.0)
+        scale = np.max(np.abs(q))
+        assert abs(lo - ex) <= 1e-12 * scale and abs(hi - ex) <= 1e-12 * scale, (lo, hi, ex)
+
+
+def check_weno7_js_ideal(oracle, adi, thd):
+    """wenorec, ng = 4 (module_weno.f90:286-527): on data whose smoothness
+    indicators are far below eps = 1e-10 the JS weights are the ideal ones,
+    so the reconstruction is the 7th-order linear one: exact on degree-6
+    polynomials (pins the candidate coefficients and the ideal weights
+    (4, 18, 12, 1)/35 and their mirror); on cubics every candidate is exact
+    whatever its weight."""
+    rng = np.random.default_rng(12)
+    for deg, amp in ((6, 1e-14), (3, 1.0)):
+        for _ in range(4):
+            c = amp * rng.standard_normal(deg + 1)
+            q = np.array([_cell_avg_poly(c, j) for j in range(8)])
+            lo, hi = _weno7(oracle, q, "ora_wenorec7")
+            ex = _poly(c, 4.0)
+            # (indicators ~1e-15 against eps = 1e-10: weights ideal to ~1e-5)
+            tol = (1e-4 if deg == 6 else 1e-12) * np.max(np.abs(q))
+            assert abs(lo - ex) <= tol and abs(hi - ex) <= tol, (deg, lo, hi, ex)
+
+
+def _js_indicator(q4, cell):
+    """Jiang-Shu indicator of the cubic with averages q4 over [0,1]..[3,4]:
+    sum over l = 1..3 of the integral over [cell, cell+1] of (d^l P/dx^l)^2
+    (unit spacing)."""
+    A = np.array([[((j + 1) ** (k + 1) - j ** (k + 1)) / (k + 1) for k in range(4)]
+                  for j in range(4)])
+    P = np.polynomial.Polynomial(np.linalg.solve(A, q4))
+    tot = 0.0
+    for l in (1, 2, 3):
+        d = P.deriv(l)
+        sq = (d * d).integ()
+        tot += sq(cell + 1.0) - sq(cell)
+    return tot
+
+
+def check_weno7_betas(oracle, adi, thd):
+    """The smoothness indicators of the WENO7 candidates (module_weno.f90:
+    330-353) are one common multiple of Jiang and Shu's definition: for
+    candidate r (cells i-r .. i-r+3) the integral over cell i, i.e. stencil
+    position r (the reference's integer forms are 240 times it).  Pins every
+    coefficient of the four quadratic forms."""
+    oracle.lib.ora_weno7_beta.restype = C.c_double
+    rng = np.random.default_rng(13)
+    ratios = []
+    for _ in range(6):
+        q = rng.standard_normal(4)
+        for r in range(4):
+            b = oracle.lib.ora_weno7_beta(r, _dp(np.ascontiguousarray(q)))
+            ratios.append(b / _js_indicator(q, r))
+    ratios = np.array(ratios)
+    assert np.max(np.abs(ratios / 240.0 - 1.0)) < 1e-10, ratios
+
+
+def check_deriv_closures(oracle, adi, thd):
+    """First derivatives (module_deriv.f90:212-469) on a line of 20 points
+    with both ends physical: exact for degree-4 polynomials everywhere (the
+    4th-order one-sided, upwind and central closures, 6th/8th-order central
+    inside), reading no ghost (NaN there); on degree-5 data the closure
+    points are not exact while the inner points are (the closure points are
+    the ng nearest each end).  Periodic lines: central everywhere, exact for
+    degree 6 (ng = 3) and 8 (ng = 4)."""
+    L = oracle.lib
+    L.ora_deriv_line.restype = C.c_double
+    L.ora_deriv_line.argtypes = [C.c_void_p, C.c_long, C.c_double, C.c_int, C.c_int, C.c_int,
+                                 C.c_int, C.c_int]
+    n, h = 20, 0.37
+    rng = np.random.default_rng(14)
+    for ng in (3, 4):
+        x = (np.arange(-ng, n + ng) - 0.3) * h
+        for deg in (4, 5):
+            c = rng.standard_normal(deg + 1)
+            a = np.array([_poly(c, xx) for xx in x])
+            a[:ng] = np.nan
+            a[n + ng:] = np.nan
+            ex = np.array([sum(k * c[k] * xx ** (k - 1) for k in range(1, deg + 1))
+                           for xx in x[ng:ng + n]])
+            got = np.array([L.ora_deriv_line(a[ng:].ctypes.data, 1, 1.0 / h, p, n, ng, 1, 1)
+                            for p in range(n)])
+            err = np.abs(got - ex) / np.max(np.abs(ex))
+            assert np.all(np.isfinite(got)), (ng, deg)
+            if deg == 4:
+                assert err.max() < 1e-11, (ng, deg, err)
+            else:
+                inner = err[ng:n - ng]
+                edge = np.concatenate([err[:ng - 1], err[n - ng + 1:]])
+                assert inner.max() < 1e-11 and edge.min() > 1e-9, (ng, inner.max(), edge)
+        deg = 2 * ng
+        c = rng.standard_normal(deg + 1)
+        a = np.array([_poly(c, xx) for xx in x])
+        ex = np.array([sum(k * c[k] * xx ** (k - 1) for k in range(1, deg + 1))
+                       for xx in x[ng:ng + n]])
+        got = np.array([L.ora_deriv_line(a[ng:].ctypes.data, 1, 1.0 / h, p, n, ng, 0, 0)
+                        for p in range(n)])
+        assert np.max(np.abs(got - ex)) / np.max(np.abs(ex)) < 1e-10, ng
+
+
 ALL_CHECKS = [check_euler_convergence, check_euler_linear_upwind, check_euler_riemann_face,
               check_euler_axis_permutation, check_visc_stress, check_visc_heat,
-              check_visc_binary_fick, check_visc_species_sum, check_species_split_invariance, check_rk3_third_order,
-              check_timestep_closed_form, check_sensor_threshold, check_ducros_sensor,
-              check_negT_repair_converges, check_negT_repair_runs_to_itmax,
-              check_negT_repair_adjacent_pair, check_negT_repair_at_the_array_edge]
+              check_visc_binary_fick, check_visc_species_sum, check_species_split_invariance,
+              check_rk3_third_order, check_timestep_closed_form, check_sensor_threshold,
+              check_ducros_sensor, check_negT_repair_converges, check_negT_repair_runs_to_itmax,
+              check_negT_repair_adjacent_pair, check_negT_repair_at_the_array_edge,
+              check_weno7_linear_exact, check_weno7_js_ideal, check_weno7_betas,
+              check_deriv_closures]

@@ -87,6 +87,22 @@ MUTANTS = [
      "            correct_T = 1;", [P.check_negT_repair_converges]),
     ("negT_no_T_reset", "for (int l = 0; l < ipt; l++) T[rec[l]] = 0.5 * (T[rec[l] - 1] + T[rec[l] + 1]);",
      "for (int l = 0; l < ipt; l++) (void)rec[l];", [P.check_negT_repair_runs_to_itmax]),
+    ("weno7_indicator_coefficient", "{547.0, -2522.0, 1922.0, -494.0,",
+     "{547.0, -2522.0, 1922.0, -495.0,", [P.check_weno7_betas]),
+    ("weno7_mirror_indicator", "{267.0, -1642.0, 1602.0, -494.0, 2843.0,",
+     "{267.0, -1642.0, 1602.0, -494.0, 2834.0,", [P.check_weno7_betas]),
+    ("weno7_linear_coefficient", "319.0 * vp[4]", "318.0 * vp[4]", [P.check_weno7_linear_exact]),
+    ("weno7_ideal_weights", "{4.0 / 35.0, 18.0 / 35.0, 12.0 / 35.0, 1.0 / 35.0}",
+     "{4.0 / 35.0, 12.0 / 35.0, 18.0 / 35.0, 1.0 / 35.0}", [P.check_weno7_js_ideal]),
+    ("weno7_candidate_row", "{-1.0 / 12.0, 7.0 / 12.0, 7.0 / 12.0, -1.0 / 12.0},",
+     "{-1.0 / 12.0, 7.0 / 12.0, 7.0 / 12.0, 1.0 / 12.0},", [P.check_weno7_js_ideal]),
+    ("closure_upwind_weight", "{-3.0 / 12.0, -10.0 / 12.0, 18.0 / 12.0, -6.0 / 12.0,",
+     "{-3.0 / 12.0, -10.0 / 12.0, 18.0 / 12.0, -5.0 / 12.0,", [P.check_deriv_closures]),
+    ("closure_right_sign", "if (p == n - 2) return -(wsum(D4UR, -3, 1, a, c, s) * dinv);",
+     "if (p == n - 2) return wsum(D4UR, -3, 1, a, c, s) * dinv;", [P.check_deriv_closures]),
+    ("closure_sixth_order_point", "    return wsum(D6, -3, 3, a, c, s) * dinv;           /* p == 3, ng = 4 */",
+     "    return wsum(D8, -4, 4, a, c, s) * dinv;           /* p == 3, ng = 4 */",
+     [P.check_deriv_closures]),
     ("negT_wmix_after_repair", "  if (iters) *iters = it;\n  free(DF);",
      "  ora_wmix_i(domain, g, thd, v, W_i);\n  if (iters) *iters = it;\n  free(DF);",
      [P.check_negT_repair_converges]),
