@@ -71,12 +71,25 @@ typedef struct {
 #define IZEM_THD_IMAGE_LEN 745
 #define IZEM_MECH_IMAGE_LEN 2850
 
+/* face conditions (GRID%BCXMIN, BCXMAX, BCYMIN, BCYMAX, BCZMIN, BCZMAX;
+ * module_BCs.f90:41-141): IZEM_BC_PERIODIC, the default, or
+ * IZEM_BC_EXTRAPOLATION (bc_extrapolation, :327-412: the ng ghost layers
+ * take the edge value, and the first derivatives of the viscous fluxes and
+ * of the Ducros sensor take the one-sided closures of a face without a
+ * neighbour, module_deriv.f90:268-469).  A direction is periodic on both
+ * faces or on neither.  izem_comm_cons then fills extrapolated faces as
+ * comm_cons followed by upd_boundaries does. */
+#define IZEM_BC_PERIODIC 0
+#define IZEM_BC_EXTRAPOLATION 1
+
 /* thd: flat image of the thermodynamic/transport tables (thd_init,
  * thd/thd_tools_perfectgas.f90:12-248), see phd_amd.chemdata.Thd.flat(),
  * or izem_thd_image below */
 int izem_create(izem_ctx **out, const izem_grid *grid, const double *thd, long nthd,
                 const izem_opts *opts);
 int izem_destroy(izem_ctx *ctx);
+/* bc[6] in the order W, E, S, N, B, F (faces beyond ndim must be periodic) */
+int izem_set_bc(izem_ctx *ctx, const int bc[6]);
 int izem_synchronize(izem_ctx *ctx);
 /* run on a caller-owned stream (hipStream_t); NULL restores the context's */
 int izem_set_stream(izem_ctx *ctx, void *stream);

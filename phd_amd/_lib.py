@@ -22,7 +22,7 @@ EXPORTS = [
     "izem_mlp_gemm", "izem_mlp_gemm_splitk", "izem_mlp_colsum", "izem_mlp_adam", "izem_mlp_to_bf16",
     "izem_mlp_mse", "izem_mlp_infer", "izem_surr_features", "izem_surr_apply", "izem_surr_fallback",
     "izem_reac_selec", "izem_reaction_selec", "izem_surr_attach", "izem_reaction_surrogate",
-    "izem_surr_counts", "izem_thd_image", "izem_mech_image_ckwrk",
+    "izem_surr_counts", "izem_thd_image", "izem_mech_image_ckwrk", "izem_set_bc",
 ]
 
 
@@ -128,6 +128,7 @@ def load(path: str | None = None):
     lib.izem_thd_image.restype = C.c_long
     lib.izem_thd_image.argtypes = [C.c_int, C.c_int] + [vp] * 10 + [C.c_double] * 7 + [vp,
                                                                                       C.c_long]
+    lib.izem_set_bc.argtypes = [vp, vp]
     lib.izem_mech_image_ckwrk.restype = C.c_long
     lib.izem_mech_image_ckwrk.argtypes = [vp, vp, vp, vp, C.c_long]
     _lib = lib

@@ -867,9 +867,10 @@ __global__ void k_upd_prim_woT(IzDims d, IzBox b, const double *__restrict__ v,
   for (int l = 0; l < IZ_NRV; l++) ha[l * d.nc + c] = h[l];
 }
 
-// Periodic face-ghost fill of nf fields (comm_cons / comm_deriv on one rank,
-// module_parallel.f90:1715-1777).  grid.y = direction, threads cover the
-// 2*ng slabs of that direction.
+// Face-ghost fill of nf fields on one rank: comm_cons's periodic self-exchange
+// (module_parallel.f90:1715-1777) or, on a physical face, upd_boundaries'
+// bc_extrapolation (module_BCs.f90:327-412: the ng ghost layers take the
+// edge value).  Threads cover the 2*ng slabs of direction dir.
 __global__ void k_ghost_fill(IzDims d, double *__restrict__ a, int nf, int dir) {
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int n0, n1, n;  // transverse sizes and the direction length
@@ -884,8 +885,14 @@ __global__ void k_ghost_fill(IzDims d, double *__restrict__ a, int nf, int dir) 
   r /= (2 * d.ng);
   int p0 = (int)(r % n0), p1 = (int)(r / n0);
   int dst, src;
-  if (s < d.ng) { dst = -1 - s; src = n - 1 - s; }
-  else { int q = s - d.ng; dst = n + q; src = q; }
+  if (s < d.ng) {
+    dst = -1 - s;
+    src = ((d.phys >> (2 * dir)) & 1) ? 0 : n - 1 - s;
+  } else {
+    int q = s - d.ng;
+    dst = n + q;
+    src = ((d.phys >> (2 * dir + 1)) & 1) ? n - 1 : q;
+  }
   long cd, cs;
   if (dir == 0) { cd = iz_cidx(d, dst, p0, p1); cs = iz_cidx(d, src, p0, p1); }
   else if (dir == 1) { cd = iz_cidx(d, p0, dst, p1); cs = iz_cidx(d, p0, src, p1); }
@@ -951,6 +958,12 @@ __global__ void k_sensor_ducros(IzDims d, double treduc, const double *__restric
         const double rho_i = 1.0 / v[cc];
         return v[(long)(1 + a) * d.nc + cc] * rho_i;
       };
+      if (d.phys) {
+        const int pos = b == 0 ? i : (b == 1 ? j : k), n = b == 0 ? d.nx : (b == 1 ? d.ny : d.nz);
+        g[a][b] = d_deriv_line(u, dinv[b], pos, n, (d.phys >> (2 * b)) & 1,
+                               (d.phys >> (2 * b + 1)) & 1);
+        continue;
+      }
       double dd = 0.0;
       dd = dd + (-1.0 / 60.0) * u(-3);
       dd = dd + (9.0 / 60.0) * u(-2);
@@ -1081,6 +1094,7 @@ __device__ inline void d_transport(double rho, double T, double W_i, const doubl
 // cell itself, as in the reference.  Only the diffusion fluxes rho Y_k V_k
 // (into the species slots of fd) and the stresses and heat flux (sf) are
 // stored; the derivative slots of fd are not written.
+template <bool PHYS>
 __global__ void __attribute__((amdgpu_flat_work_group_size(128, 128), amdgpu_waves_per_eu(2)))
 k_visc_fused(IzDims d, IzDomSet ds, double sqgmr, double rvd0, double rlam0,
              const double *__restrict__ v, const double *__restrict__ T,
@@ -1114,13 +1128,13 @@ k_visc_fused(IzDims d, IzDomSet ds, double sqgmr, double rvd0, double rlam0,
   double rlam = -rlam0 * ct;
   if (d.ndim == 3) {
     const int nfluid = 12;
-    const double f1 = d_deriv7(prim, cd, sx, dxi), f2 = d_deriv7(prim, cd, sy, dyi),
-                 f3 = d_deriv7(prim, cd, sz, dzi);
-    const double f4 = d_deriv7(prim + nc, cd, sx, dxi), f5 = d_deriv7(prim + nc, cd, sy, dyi),
-                 f6 = d_deriv7(prim + nc, cd, sz, dzi);
-    const double f7 = d_deriv7(prim + 2 * nc, cd, sx, dxi),
-                 f8 = d_deriv7(prim + 2 * nc, cd, sy, dyi),
-                 f9 = d_deriv7(prim + 2 * nc, cd, sz, dzi);
+    const double f1 = d_deriv_dir<PHYS>(d, prim, cd, 0, ii, dxi), f2 = d_deriv_dir<PHYS>(d, prim, cd, 1, jj, dyi),
+                 f3 = d_deriv_dir<PHYS>(d, prim, cd, 2, kk, dzi);
+    const double f4 = d_deriv_dir<PHYS>(d, prim + nc, cd, 0, ii, dxi), f5 = d_deriv_dir<PHYS>(d, prim + nc, cd, 1, jj, dyi),
+                 f6 = d_deriv_dir<PHYS>(d, prim + nc, cd, 2, kk, dzi);
+    const double f7 = d_deriv_dir<PHYS>(d, prim + 2 * nc, cd, 0, ii, dxi),
+                 f8 = d_deriv_dir<PHYS>(d, prim + 2 * nc, cd, 1, jj, dyi),
+                 f9 = d_deriv_dir<PHYS>(d, prim + 2 * nc, cd, 2, kk, dzi);
     double sxx = twothirds * rmu * (f1 + f1 - (f5 + f9));
     double sxy = rmu * (f4 + f2);
     double sxz = rmu * (f7 + f3);
@@ -1131,9 +1145,9 @@ k_visc_fused(IzDims d, IzDomSet ds, double sqgmr, double rvd0, double rlam0,
 #pragma unroll
     for (int l = 0; l < IZ_NRV; l++) {
       const double *a = prim + (3 + l) * nc;
-      gx[l] = d_deriv7(a, cd, sx, dxi);
-      gy[l] = d_deriv7(a, cd, sy, dyi);
-      gz[l] = d_deriv7(a, cd, sz, dzi);
+      gx[l] = d_deriv_dir<PHYS>(d, a, cd, 0, ii, dxi);
+      gy[l] = d_deriv_dir<PHYS>(d, a, cd, 1, jj, dyi);
+      gz[l] = d_deriv_dir<PHYS>(d, a, cd, 2, kk, dzi);
     }
     double vdcx = 0.0, vdcy = 0.0, vdcz = 0.0;
 #pragma unroll
@@ -1161,8 +1175,8 @@ k_visc_fused(IzDims d, IzDomSet ds, double sqgmr, double rvd0, double rlam0,
       vdhy = vdhy + h * jy;
       vdhz = vdhz + h * jz;
     }
-    const double tx = d_deriv7(T, cd, sx, dxi), ty = d_deriv7(T, cd, sy, dyi),
-                 tz = d_deriv7(T, cd, sz, dzi);
+    const double tx = d_deriv_dir<PHYS>(d, T, cd, 0, ii, dxi), ty = d_deriv_dir<PHYS>(d, T, cd, 1, jj, dyi),
+                 tz = d_deriv_dir<PHYS>(d, T, cd, 2, kk, dzi);
     sf[0 * nc + c] = sxx; sf[1 * nc + c] = sxy; sf[2 * nc + c] = sxz;
     sf[3 * nc + c] = syy; sf[4 * nc + c] = syz; sf[5 * nc + c] = szz;
     sf[6 * nc + c] = uxs * sxx + vys * sxy + wzs * sxz + rlam * tx + vdhx;
@@ -1170,8 +1184,8 @@ k_visc_fused(IzDims d, IzDomSet ds, double sqgmr, double rvd0, double rlam0,
     sf[8 * nc + c] = uxs * sxz + vys * syz + wzs * szz + rlam * tz + vdhz;
   } else {
     const int nfluid = 6;
-    const double f1 = d_deriv7(prim, cd, sx, dxi), f2 = d_deriv7(prim, cd, sy, dyi);
-    const double f3 = d_deriv7(prim + nc, cd, sx, dxi), f4 = d_deriv7(prim + nc, cd, sy, dyi);
+    const double f1 = d_deriv_dir<PHYS>(d, prim, cd, 0, ii, dxi), f2 = d_deriv_dir<PHYS>(d, prim, cd, 1, jj, dyi);
+    const double f3 = d_deriv_dir<PHYS>(d, prim + nc, cd, 0, ii, dxi), f4 = d_deriv_dir<PHYS>(d, prim + nc, cd, 1, jj, dyi);
     double sxx = twothirds * rmu * (f1 + f1 - f4);
     double sxy = rmu * (f3 + f2);
     double syy = twothirds * rmu * (f4 + f4 - f1);
@@ -1179,8 +1193,8 @@ k_visc_fused(IzDims d, IzDomSet ds, double sqgmr, double rvd0, double rlam0,
 #pragma unroll
     for (int l = 0; l < IZ_NRV; l++) {
       const double *a = prim + (3 + l) * nc;
-      gx[l] = d_deriv7(a, cd, sx, dxi);
-      gy[l] = d_deriv7(a, cd, sy, dyi);
+      gx[l] = d_deriv_dir<PHYS>(d, a, cd, 0, ii, dxi);
+      gy[l] = d_deriv_dir<PHYS>(d, a, cd, 1, jj, dyi);
     }
     double vdcx = 0.0, vdcy = 0.0;
 #pragma unroll
@@ -1204,7 +1218,7 @@ k_visc_fused(IzDims d, IzDomSet ds, double sqgmr, double rvd0, double rlam0,
       vdhx = vdhx + h * jx;
       vdhy = vdhy + h * jy;
     }
-    const double tx = d_deriv7(T, cd, sx, dxi), ty = d_deriv7(T, cd, sy, dyi);
+    const double tx = d_deriv_dir<PHYS>(d, T, cd, 0, ii, dxi), ty = d_deriv_dir<PHYS>(d, T, cd, 1, jj, dyi);
     sf[0 * nc + c] = sxx; sf[1 * nc + c] = sxy; sf[2 * nc + c] = syy;
     sf[3 * nc + c] = uxs * sxx + vys * sxy + rlam * tx + vdhx;
     sf[4 * nc + c] = uxs * sxy + vys * syy + rlam * ty + vdhy;
@@ -1213,6 +1227,7 @@ k_visc_fused(IzDims d, IzDomSet ds, double sqgmr, double rvd0, double rlam0,
 
 // viscous divergence of one cell into out[0..13] (0 for the continuity and,
 // in 2-D, the z-momentum equations), :2087-2183
+template <bool PHYS>
 __device__ inline void d_visc_div(const IzDims &d, long c, int i, int j, int k,
                                   const double *__restrict__ sf, const double *__restrict__ fd,
                                   const double *__restrict__ dx_i, const double *__restrict__ dy_i,
@@ -1223,41 +1238,42 @@ __device__ inline void d_visc_div(const IzDims &d, long c, int i, int j, int k,
   if (d.ndim == 3) {
     const int nf = 12;
     double wx, wy, wz;
-    wx = d_deriv7(sf + 0 * nc, c, sx, dxi); wy = d_deriv7(sf + 1 * nc, c, sy, dyi);
-    wz = d_deriv7(sf + 2 * nc, c, sz, dzi); out[1] = wx + wy + wz;
-    wx = d_deriv7(sf + 1 * nc, c, sx, dxi); wy = d_deriv7(sf + 3 * nc, c, sy, dyi);
-    wz = d_deriv7(sf + 4 * nc, c, sz, dzi); out[2] = wx + wy + wz;
-    wx = d_deriv7(sf + 2 * nc, c, sx, dxi); wy = d_deriv7(sf + 4 * nc, c, sy, dyi);
-    wz = d_deriv7(sf + 5 * nc, c, sz, dzi); out[3] = wx + wy + wz;
-    wx = d_deriv7(sf + 6 * nc, c, sx, dxi); wy = d_deriv7(sf + 7 * nc, c, sy, dyi);
-    wz = d_deriv7(sf + 8 * nc, c, sz, dzi); out[4] = wx + wy + wz;
+    wx = d_deriv_dir<PHYS>(d, sf + 0 * nc, c, 0, i, dxi); wy = d_deriv_dir<PHYS>(d, sf + 1 * nc, c, 1, j, dyi);
+    wz = d_deriv_dir<PHYS>(d, sf + 2 * nc, c, 2, k, dzi); out[1] = wx + wy + wz;
+    wx = d_deriv_dir<PHYS>(d, sf + 1 * nc, c, 0, i, dxi); wy = d_deriv_dir<PHYS>(d, sf + 3 * nc, c, 1, j, dyi);
+    wz = d_deriv_dir<PHYS>(d, sf + 4 * nc, c, 2, k, dzi); out[2] = wx + wy + wz;
+    wx = d_deriv_dir<PHYS>(d, sf + 2 * nc, c, 0, i, dxi); wy = d_deriv_dir<PHYS>(d, sf + 4 * nc, c, 1, j, dyi);
+    wz = d_deriv_dir<PHYS>(d, sf + 5 * nc, c, 2, k, dzi); out[3] = wx + wy + wz;
+    wx = d_deriv_dir<PHYS>(d, sf + 6 * nc, c, 0, i, dxi); wy = d_deriv_dir<PHYS>(d, sf + 7 * nc, c, 1, j, dyi);
+    wz = d_deriv_dir<PHYS>(d, sf + 8 * nc, c, 2, k, dzi); out[4] = wx + wy + wz;
     has[1] = has[2] = has[3] = has[4] = true;
     for (int l = 1; l <= IZ_NRV; l++) {
-      wx = d_deriv7(fd + (long)(nf + 3 * l - 3) * nc, c, sx, dxi);
-      wy = d_deriv7(fd + (long)(nf + 3 * l - 2) * nc, c, sy, dyi);
-      wz = d_deriv7(fd + (long)(nf + 3 * l - 1) * nc, c, sz, dzi);
+      wx = d_deriv_dir<PHYS>(d, fd + (long)(nf + 3 * l - 3) * nc, c, 0, i, dxi);
+      wy = d_deriv_dir<PHYS>(d, fd + (long)(nf + 3 * l - 2) * nc, c, 1, j, dyi);
+      wz = d_deriv_dir<PHYS>(d, fd + (long)(nf + 3 * l - 1) * nc, c, 2, k, dzi);
       out[IZ_NIV + l - 1] = wx + wy + wz;
       has[IZ_NIV + l - 1] = true;
     }
   } else {
     const int nf = 6;
     double wx, wy;
-    wx = d_deriv7(sf + 0 * nc, c, sx, dxi); wy = d_deriv7(sf + 1 * nc, c, sy, dyi);
+    wx = d_deriv_dir<PHYS>(d, sf + 0 * nc, c, 0, i, dxi); wy = d_deriv_dir<PHYS>(d, sf + 1 * nc, c, 1, j, dyi);
     out[1] = wx + wy;
-    wx = d_deriv7(sf + 1 * nc, c, sx, dxi); wy = d_deriv7(sf + 2 * nc, c, sy, dyi);
+    wx = d_deriv_dir<PHYS>(d, sf + 1 * nc, c, 0, i, dxi); wy = d_deriv_dir<PHYS>(d, sf + 2 * nc, c, 1, j, dyi);
     out[2] = wx + wy;
-    wx = d_deriv7(sf + 3 * nc, c, sx, dxi); wy = d_deriv7(sf + 4 * nc, c, sy, dyi);
+    wx = d_deriv_dir<PHYS>(d, sf + 3 * nc, c, 0, i, dxi); wy = d_deriv_dir<PHYS>(d, sf + 4 * nc, c, 1, j, dyi);
     out[4] = wx + wy;
     has[1] = has[2] = has[4] = true;
     for (int l = 1; l <= IZ_NRV; l++) {
-      wx = d_deriv7(fd + (long)(nf + 2 * l - 2) * nc, c, sx, dxi);
-      wy = d_deriv7(fd + (long)(nf + 2 * l - 1) * nc, c, sy, dyi);
+      wx = d_deriv_dir<PHYS>(d, fd + (long)(nf + 2 * l - 2) * nc, c, 0, i, dxi);
+      wy = d_deriv_dir<PHYS>(d, fd + (long)(nf + 2 * l - 1) * nc, c, 1, j, dyi);
       out[IZ_NIV + l - 1] = wx + wy;
       has[IZ_NIV + l - 1] = true;
     }
   }
 }
 
+template <bool PHYS>
 __global__ void k_visc_apply(IzDims d, const double *__restrict__ sf,
                              const double *__restrict__ fd, const double *__restrict__ dx_i,
                              const double *__restrict__ dy_i, const double *__restrict__ dz_i,
@@ -1271,7 +1287,7 @@ __global__ void k_visc_apply(IzDims d, const double *__restrict__ sf,
   long c = iz_cidx(d, i, j, k);
   double out[IZ_NV];
   bool has[IZ_NV];
-  d_visc_div(d, c, i, j, k, sf, fd, dx_i, dy_i, dz_i, out, has);
+  d_visc_div<PHYS>(d, c, i, j, k, sf, fd, dx_i, dy_i, dz_i, out, has);
   for (int m = 0; m < IZ_NV; m++)
     if (has[m]) fl[m * d.nc + c] = fl[m * d.nc + c] + out[m];
 }
@@ -1283,6 +1299,7 @@ __global__ void k_visc_apply(IzDims d, const double *__restrict__ sf,
 #ifndef IZ_RHS_WAVES
 #define IZ_RHS_WAVES 2
 #endif
+template <bool PHYS>
 __global__ void __attribute__((amdgpu_flat_work_group_size(256, 256),
                                amdgpu_waves_per_eu(IZ_RHS_WAVES, IZ_RHS_WAVES)))
 k_rhs_update(IzDims d, int mode, const double *__restrict__ Fx, const double *__restrict__ Fy,
@@ -1326,7 +1343,7 @@ k_rhs_update(IzDims d, int mode, const double *__restrict__ Fx, const double *__
     if (vis) {
       double out[IZ_NV];
       bool has[IZ_NV];
-      d_visc_div(d, c, i, j, k, sf, fd, dx_i, dy_i, dz_i, out, has);
+      d_visc_div<PHYS>(d, c, i, j, k, sf, fd, dx_i, dy_i, dz_i, out, has);
 #pragma unroll
       for (int m = 0; m < IZ_NV; m++)
         if (has[m]) f[m] = f[m] + out[m];
@@ -1538,12 +1555,17 @@ static long box_count(const IzBox &b) {
   return (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
 }
 static long n_interior(const IzDims &d) { return (long)d.nx * d.ny * d.nz; }
-// the interior and the 2 * ndim ghost slabs, in domain_select's order
+// the interior and the 2 * ndim ghost slabs, in domain_select's order; the
+// slabs of a direction with physical faces are left out (the viscous fluxes
+// there are never read: the derivative closures next to a physical face use
+// interior points only)
 static IzDomSet domain_set(const IzDims &d) {
   IzDomSet s{};
   s.n = 0;
   s.first[0] = 0;
   for (int dom = -d.ndim; dom <= d.ndim; dom++) {
+    const int dir = (dom < 0 ? -dom : dom) - 1;
+    if (dom != 0 && ((d.phys >> (2 * dir)) & 3)) continue;
     s.box[s.n] = domain_box(d, dom);
     s.dom[s.n] = dom;
     s.first[s.n + 1] = s.first[s.n] + box_count(s.box[s.n]);
@@ -1674,6 +1696,29 @@ extern "C" int izem_destroy(izem_ctx *c) {
 extern "C" int izem_flux_mode(izem_ctx *c, int mode) {
   if (!c || mode < 0 || mode > 1) return IZEM_E_ARG;
   c->flux_mode = mode;
+  return IZEM_OK;
+}
+
+// GRID%BCXMIN .. BCZMAX (module_input.f90, bc(1:6)): 0 periodic, 1
+// extrapolation (bc_extrapolation + the one-sided derivative closures of a
+// face without a neighbour, module_BCs.f90:327-412, module_deriv.f90:268-469)
+extern "C" int izem_set_bc(izem_ctx *c, const int bc[6]) {
+  if (!c || !bc) return IZEM_E_ARG;
+  int phys = 0;
+  for (int f = 0; f < 6; f++) {
+    if (bc[f] != IZEM_BC_PERIODIC && bc[f] != IZEM_BC_EXTRAPOLATION) return IZEM_E_ARG;
+    if (bc[f] == IZEM_BC_EXTRAPOLATION) {
+      if (f / 2 >= c->d.ndim) return IZEM_E_ARG;
+      phys |= 1 << f;
+    }
+  }
+  // a direction is periodic on both faces or on neither
+  for (int dir = 0; dir < 3; dir++)
+    if (((phys >> (2 * dir)) & 3) == 1 || ((phys >> (2 * dir)) & 3) == 2) return IZEM_E_ARG;
+  if (phys != c->d.phys) {
+    c->d.phys = phys;
+    iz_graph_destroy(c);  // a captured step holds the old dimensions
+  }
   return IZEM_OK;
 }
 
@@ -1820,9 +1865,14 @@ static int visc_fluxes(izem_ctx *c, const double *dx_i, const double *dy_i, cons
   hipLaunchKernelGGL(k_visc_prim, dim3(iz_blocks(ncell, 256)), dim3(256), 0, c->stream, d, ds, v,
                      W_i, c->prim);
   const double rvd0 = c->o.sqgmr / c->o.Sc, rlam0 = c->o.sqgmr * c->o.ggmopr;
-  hipLaunchKernelGGL(k_visc_fused, dim3(iz_blocks(ncell, 128)), dim3(128), 0, c->stream, d, ds,
-                     c->o.sqgmr, rvd0, rlam0, v, T, W_i, ha, c->prim, dx_i, dy_i, dz_i, c->fd,
-                     c->sf);
+  if (d.phys)
+    hipLaunchKernelGGL(k_visc_fused<true>, dim3(iz_blocks(ncell, 128)), dim3(128), 0, c->stream,
+                       d, ds, c->o.sqgmr, rvd0, rlam0, v, T, W_i, ha, c->prim, dx_i, dy_i, dz_i,
+                       c->fd, c->sf);
+  else
+    hipLaunchKernelGGL(k_visc_fused<false>, dim3(iz_blocks(ncell, 128)), dim3(128), 0, c->stream,
+                       d, ds, c->o.sqgmr, rvd0, rlam0, v, T, W_i, ha, c->prim, dx_i, dy_i, dz_i,
+                       c->fd, c->sf);
   iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
@@ -1834,8 +1884,12 @@ extern "C" int izem_viscflux(izem_ctx *c, const double *dx_i, const double *dy_i
   if (!c || !dx_i || !dy_i || !dz_i || !T || !W_i || !ha || !v || !fl) return IZEM_E_ARG;
   int r = visc_fluxes(c, dx_i, dy_i, dz_i, T, W_i, ha, v);
   if (r) return r;
-  hipLaunchKernelGGL(k_visc_apply, dim3(iz_blocks(n_interior(c->d), 256)), dim3(256), 0,
-                     c->stream, c->d, c->sf, c->fd, dx_i, dy_i, dz_i, fl);
+  if (c->d.phys)
+    hipLaunchKernelGGL(k_visc_apply<true>, dim3(iz_blocks(n_interior(c->d), 256)), dim3(256), 0,
+                       c->stream, c->d, c->sf, c->fd, dx_i, dy_i, dz_i, fl);
+  else
+    hipLaunchKernelGGL(k_visc_apply<false>, dim3(iz_blocks(n_interior(c->d), 256)), dim3(256), 0,
+                       c->stream, c->d, c->sf, c->fd, dx_i, dy_i, dz_i, fl);
   IZ_CHECK(hipGetLastError());
   return IZEM_OK;
 }
@@ -1984,8 +2038,9 @@ extern "C" int izem_rhs(izem_ctx *c, const double *dx_i, const double *dy_i, con
   int r = stage_fluxes(c, dx_i, dy_i, dz_i, T, W_i, cp, ha, sensor, v);
   if (r) return r;
   const IzDims &d = c->d;
-  hipLaunchKernelGGL(k_rhs_update, dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0, c->stream,
-                     d, 0, c->F[0], c->F[1], c->F[2], c->o.vis, c->sf, c->fd, dx_i, dy_i, dz_i,
+  hipLaunchKernelGGL(d.phys ? k_rhs_update<true> : k_rhs_update<false>,
+                     dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0, c->stream, d, 0, c->F[0],
+                     c->F[1], c->F[2], c->o.vis, c->sf, c->fd, dx_i, dy_i, dz_i,
                      (const double *)nullptr, 0.0, 0.0, (const double *)nullptr,
                      (double *)nullptr, f, 0, T, 0.0, (unsigned long long *)nullptr);
   IZ_CHECK(hipGetLastError());
@@ -2009,10 +2064,11 @@ extern "C" int izem_rk3(izem_ctx *c, const double *dt, const double *dx_i, const
     if (r) return r;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->masks, 2 * IZ_NEG_MASKS);
     int pk = iz_prof_begin(c, IZP_RHS);
-    hipLaunchKernelGGL(k_rhs_update, dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0,
-                       c->stream, d, 1, c->F[0], c->F[1], c->F[2], c->o.vis, c->sf, c->fd, dx_i,
-                       dy_i, dz_i, dt, bk3[ik], ck3[ik], c->v0, v, (double *)nullptr,
-                       c->o.forcesum, T, 1.0e-5, c->masks + 3);
+    hipLaunchKernelGGL(d.phys ? k_rhs_update<true> : k_rhs_update<false>,
+                       dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0, c->stream, d, 1,
+                       c->F[0], c->F[1], c->F[2], c->o.vis, c->sf, c->fd, dx_i, dy_i, dz_i, dt,
+                       bk3[ik], ck3[ik], c->v0, v, (double *)nullptr, c->o.forcesum, T, 1.0e-5,
+                       c->masks + 3);
     iz_prof_end(c, pk);
     IzBox b0 = domain_box(d, 0);
     pk = iz_prof_begin(c, IZP_NEWTON);

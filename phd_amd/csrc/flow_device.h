@@ -211,3 +211,78 @@ __device__ inline double d_deriv7(const double *__restrict__ a, long c, long s, 
   dd = dd + (1.0 / 60.0) * a[c + 3 * s];
   return dd * dinv;
 }
+
+// First derivative at position p (0 .. n-1) of a line whose values at
+// offsets l from p are f(l), next to physical faces (lp: the face before
+// position 0, rp: the one after n-1): the three points nearest a physical
+// face take 4th-order closures, one-sided at the face point, upwind at the
+// next, central at the third, the right-hand ones negated
+// (module_deriv.f90:20-36, 268-469, ng = 3); elsewhere the 6th-order central
+// stencil.  Weights summed in increasing offset.
+template <typename Fn>
+__device__ inline double d_deriv_line(Fn f, double dinv, int p, int n, bool lp, bool rp) {
+  double dd = 0.0;
+  if (rp && p >= n - 3) {
+    if (p == n - 1) {
+      dd = dd + (-3.0 / 12.0) * f(-4);
+      dd = dd + (16.0 / 12.0) * f(-3);
+      dd = dd + (-36.0 / 12.0) * f(-2);
+      dd = dd + (48.0 / 12.0) * f(-1);
+      dd = dd + (-25.0 / 12.0) * f(0);
+      return -(dd * dinv);
+    }
+    if (p == n - 2) {
+      dd = dd + (1.0 / 12.0) * f(-3);
+      dd = dd + (-6.0 / 12.0) * f(-2);
+      dd = dd + (18.0 / 12.0) * f(-1);
+      dd = dd + (-10.0 / 12.0) * f(0);
+      dd = dd + (-3.0 / 12.0) * f(1);
+      return -(dd * dinv);
+    }
+    dd = dd + (1.0 / 12.0) * f(-2);
+    dd = dd + (-8.0 / 12.0) * f(-1);
+    dd = dd + (8.0 / 12.0) * f(1);
+    dd = dd + (-1.0 / 12.0) * f(2);
+    return dd * dinv;
+  }
+  if (lp && p < 3) {
+    if (p == 0) {
+      dd = dd + (-25.0 / 12.0) * f(0);
+      dd = dd + (48.0 / 12.0) * f(1);
+      dd = dd + (-36.0 / 12.0) * f(2);
+      dd = dd + (16.0 / 12.0) * f(3);
+      dd = dd + (-3.0 / 12.0) * f(4);
+    } else if (p == 1) {
+      dd = dd + (-3.0 / 12.0) * f(-1);
+      dd = dd + (-10.0 / 12.0) * f(0);
+      dd = dd + (18.0 / 12.0) * f(1);
+      dd = dd + (-6.0 / 12.0) * f(2);
+      dd = dd + (1.0 / 12.0) * f(3);
+    } else {
+      dd = dd + (1.0 / 12.0) * f(-2);
+      dd = dd + (-8.0 / 12.0) * f(-1);
+      dd = dd + (8.0 / 12.0) * f(1);
+      dd = dd + (-1.0 / 12.0) * f(2);
+    }
+    return dd * dinv;
+  }
+  dd = dd + (-1.0 / 60.0) * f(-3);
+  dd = dd + (9.0 / 60.0) * f(-2);
+  dd = dd + (-45.0 / 60.0) * f(-1);
+  dd = dd + (45.0 / 60.0) * f(1);
+  dd = dd + (-9.0 / 60.0) * f(2);
+  dd = dd + (1.0 / 60.0) * f(3);
+  return dd * dinv;
+}
+
+// derivative along direction dir of the array a at cell c, interior
+// position p along dir; PHYS = false: every face periodic (d_deriv7)
+template <bool PHYS>
+__device__ inline double d_deriv_dir(const IzDims &d, const double *__restrict__ a, long c, int dir,
+                                     int p, double dinv) {
+  const long s = dir == 0 ? 1 : (dir == 1 ? d.NX : d.NX * d.NY);
+  if (!PHYS) return d_deriv7(a, c, s, dinv);
+  const int n = dir == 0 ? d.nx : (dir == 1 ? d.ny : d.nz);
+  const bool lp = (d.phys >> (2 * dir)) & 1, rp = (d.phys >> (2 * dir + 1)) & 1;
+  return d_deriv_line([&](int l) { return a[c + l * s]; }, dinv, p, n, lp, rp);
+}

@@ -28,9 +28,12 @@ struct IzThd {
   double gam2, Tinf, p0, pinf, Dinf, laminf, muinf;
 };
 
+// phys: bit f set when face f (W, E, S, N, B, F) is a physical boundary
+// with extrapolated ghosts (bc_extrapolation) instead of a periodic one
 struct IzDims {
   int nx, ny, nz, ng, ndim;
   long NX, NY, NZ, nc;
+  int phys;
 };
 
 __host__ __device__ inline long iz_cidx(const IzDims &d, int i, int j, int k) {

@@ -109,6 +109,12 @@ class Solver:
             if o.chem_tune is not None:
                 self.chem_tune(*o.chem_tune)
 
+    def set_bc(self, bc) -> None:
+        """Face conditions W, E, S, N, B, F (GRID%BCXMIN .. BCZMAX): 0 periodic,
+        1 extrapolation (izem_set_bc)."""
+        a = (C.c_int * 6)(*[int(x) for x in bc])
+        _lib.check(self.lib.izem_set_bc(self.ctx, a), "izem_set_bc")
+
     def chem_tune(self, waves: int, blocks_per_cu: int, jac_age: int, kinetics: int):
         """Launch parameters of the chemistry kernels (izem_chem_tune)."""
         _lib.check(self.lib.izem_chem_tune(self.ctx, waves, blocks_per_cu, jac_age, kinetics),
