@@ -212,6 +212,18 @@ class Deck:
 SENSORS = {"ADAMSSHARIFF": 0, "DUCROS": 1}
 
 
+BC_KEYS = ("GRID%BCXMIN", "GRID%BCXMAX", "GRID%BCYMIN", "GRID%BCYMAX", "GRID%BCZMIN",
+           "GRID%BCZMAX")
+BCS = {"PERIODIC": 0, "EXTRAPOLATION": 1}
+
+
+def face_conditions(deck: Deck) -> tuple:
+    """The six face conditions W, E, S, N, B, F for izem_set_bc (0 periodic,
+    1 extrapolation); faces beyond GENERAL%DIM are periodic."""
+    dim = deck.get("GENERAL%DIM")
+    return tuple(BCS[deck.get(k).upper()] if f // 2 < dim else 0 for f, k in enumerate(BC_KEYS))
+
+
 def options(deck: Deck, **overrides):
     """The solver Options of a deck (module_input.f90:673-705), refusing
     what the hot path does not build."""
@@ -229,10 +241,9 @@ def options(deck: Deck, **overrides):
                 "REACTING%DVODESIMP", "WENO%OPT"):
         if g(key):
             refuse.append(f"{key} = T")
-    for key in ("GRID%BCXMIN", "GRID%BCXMAX", "GRID%BCYMIN", "GRID%BCYMAX", "GRID%BCZMIN",
-                "GRID%BCZMAX"):
-        if g(key).upper() != "PERIODIC":
-            refuse.append(f"{key} = {g(key)} (periodic boundaries only)")
+    for key in BC_KEYS:
+        if g(key).upper() not in BCS:
+            refuse.append(f"{key} = {g(key)} (periodic or extrapolated faces only)")
     for key in ("GRID%NUMPROCX", "GRID%NUMPROCY", "GRID%NUMPROCZ"):
         if g(key) != 1:
             refuse.append(f"{key} = {g(key)} (one GPU, no decomposition)")

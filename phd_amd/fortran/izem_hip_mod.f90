@@ -54,6 +54,10 @@ module izem_hip_mod
   public :: izem_check, hip_rk3, hip_timestep, hip_euler_llf, hip_viscflux, hip_reaction_selec
   public :: hip_create, hip_chem_init, hip_dtmin
 
+  ! face conditions of izem_set_bc (GRID%BCXMIN .. BCZMAX)
+  integer(c_int), parameter, public :: IZEM_BC_PERIODIC = 0, IZEM_BC_EXTRAPOLATION = 1
+  public :: izem_set_bc
+
   ! lengths of the flat table images (doubles)
   integer(c_long), parameter, public :: IZEM_THD_IMAGE_LEN = 745, IZEM_MECH_IMAGE_LEN = 2850
 
@@ -66,6 +70,11 @@ module izem_hip_mod
       type(c_ptr), value :: thd
       integer(c_long), value :: nthd
       type(izem_opts), intent(in) :: opts
+    end function
+    integer(c_int) function izem_set_bc(ctx, bc) bind(C, name="izem_set_bc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: bc(6)
     end function
     integer(c_int) function izem_destroy(ctx) bind(C, name="izem_destroy")
       import :: c_int, c_ptr

@@ -61,7 +61,7 @@ def main(argv=None):
     ap.add_argument("--flux-mode", type=int, default=1)
     args = ap.parse_args(argv)
     from . import chemdata as cd
-    from .deck import Deck, options
+    from .deck import Deck, face_conditions, options
     from .ics import Box
     from .restart import restart_name
     from .solver import Solver
@@ -83,6 +83,7 @@ def main(argv=None):
         dim, side = INITS[_init(deck)]
         box = Box(nx, ny, nz, 3, dim, (side, side, side))
         s = Solver(box, opts, adi=adi, thd=thd, mech=mech)
+        s.set_bc(face_conditions(deck))
         time = s.load_restart(path)
         nb_stat = nb
         # statistics times restart from the offset (module_input.f90:718-722)
@@ -90,6 +91,7 @@ def main(argv=None):
     else:
         box, v, T = initial_state(deck, adi, thd, args.n)
         s = Solver(box, opts, adi=adi, thd=thd, mech=mech)
+        s.set_bc(face_conditions(deck))
         s.load_state(v, T)
         del v, T
         time = g("MONITORING%INITIAL_TIME")

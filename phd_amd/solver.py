@@ -133,10 +133,13 @@ class Solver:
 
     # ---------------------------------------------------------------- state
     def load_state(self, v: np.ndarray, T: np.ndarray):
-        """Upload conserved variables and temperature (ghosts filled) and
-        initialise W_i, cp, ha with a Newton pass (prim_inv_var)."""
+        """Upload conserved variables and temperature, fill the face ghosts
+        as the face conditions say (izem_comm_cons: periodic copies, or the
+        edge values on extrapolated faces) and initialise W_i, cp, ha with a
+        Newton pass (prim_inv_var)."""
         self.v.copy_(torch.from_numpy(np.ascontiguousarray(v)))
         self.T.copy_(torch.from_numpy(np.ascontiguousarray(T)))
+        _lib.check(self.lib.izem_comm_cons(self.ctx, _ptr(self.v)), "comm_cons")
         for dom in [0] + [s * q for q in range(1, self.box.ndim + 1) for s in (-1, 1)]:
             self.prim_inv_var(dom)
         self.sync()

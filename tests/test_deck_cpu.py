@@ -78,8 +78,13 @@ def test_options_refuse_what_is_not_built():
     sod = Deck.read(os.path.join(GOLD, "sod.inp"))
     with pytest.raises(DeckError) as e:
         options(sod)
-    for what in ("ORDER_WENO = 7", "NRV = 2", "BCXMIN = EXTRAPOLATION", "NUMPROCX = 2"):
+    for what in ("ORDER_WENO = 7", "NRV = 2", "NUMPROCX = 2"):
         assert what in str(e.value)
+    assert "BCXMIN" not in str(e.value)      # extrapolated faces are built
+    from phd_amd.deck import face_conditions
+    assert face_conditions(sod) == (1, 1, 0, 0, 0, 0)
+    with pytest.raises(DeckError, match="BCYMAX"):
+        options(Deck.parse("[GRID]\n  BCYMAX = 'NOREFLECTION'\n"))
     ok = Deck.parse("[GENERAL]\n  DIM = 3\n  NRV = 9\n[WENO]\n  ORDER_WENO = 5\n"
                     "  PERCENT_WEIGHT = -1\n  SHOCK_SENSOR_TYPE = 'DUCROS'\n"
                     "  DUCROS_TREDUC = 0.5\n[REACTING]\n  REACTION = T\n"
