@@ -69,6 +69,47 @@ def load(path: str | None = None):
     if not os.path.exists(path):
         raise IzemError(f"{path} not built: run `python -m phd_amd.build` (hipcc, gfx950)")
     lib = C.CDLL(path)
+    _declare(lib)
+    _lib = lib
+    return lib
+
+
+class _Lenient:
+    """Attribute access to a library build that lacks some entry points (a
+    flow-only variant): declarations of absent ones go nowhere."""
+
+    class _Absent:
+        def __setattr__(self, name, value):
+            pass
+
+    def __init__(self, lib):
+        object.__setattr__(self, "_l", lib)
+
+    def __getattr__(self, name):
+        try:
+            return getattr(self._l, name)
+        except AttributeError:
+            return _Lenient._Absent()
+
+
+_variants: dict = {}
+
+
+def load_variant(nrv: int, ng: int):
+    """A flow-only build for nrv species and ng ghost layers (ng = 4: WENO7),
+    built by `python -m phd_amd.build` (phd_amd/build.py FLOW_VARIANTS)."""
+    from .build import flow_variant_path
+    path = flow_variant_path(nrv, ng)
+    if path not in _variants:
+        if not os.path.exists(path):
+            raise IzemError(f"{path} not built: run `python -m phd_amd.build` (hipcc, gfx950)")
+        lib = C.CDLL(path)
+        _declare(_Lenient(lib))
+        _variants[path] = lib
+    return _variants[path]
+
+
+def _declare(lib):
     vp = C.c_void_p
     for name in EXPORTS:
         getattr(lib, name).restype = C.c_int
@@ -131,8 +172,6 @@ def load(path: str | None = None):
     lib.izem_set_bc.argtypes = [vp, vp]
     lib.izem_mech_image_ckwrk.restype = C.c_long
     lib.izem_mech_image_ckwrk.argtypes = [vp, vp, vp, vp, C.c_long]
-    _lib = lib
-    return lib
 
 
 def check(rc: int, what: str = ""):

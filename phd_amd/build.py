@@ -60,9 +60,48 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    for nrv, ng in FLOW_VARIANTS:
+        build_flow_variant(nrv, ng, verbose)
     build_io(verbose)
     build_fortran(verbose)
     return LIB
+
+
+# flow-only builds for other (species count, ghost depth) pairs: the flow
+# kernels of flow.hip compiled with IZ_NRV / IZ_NG set, without the
+# chemistry, the surrogate and the captured step.  (9, 4): WENO7 for the
+# H2/air species; (2, 4): the reference's O2/N2 shock tube.
+FLOW_VARIANTS = ((9, 4), (2, 4))
+
+
+def flow_variant_path(nrv: int, ng: int) -> str:
+    return os.path.join(HERE, f"libizem_flow_nrv{nrv}_ng{ng}.so")
+
+
+def build_flow_variant(nrv: int, ng: int, verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    defs = [f"-DIZ_NRV={nrv}", f"-DIZ_NG={ng}"]
+    objs = []
+    for src in ("flow.hip", "flow_only.hip"):
+        obj = os.path.join(BUILD, src.replace(".hip", f"_nrv{nrv}_ng{ng}.o"))
+        cmd = [hipcc(), "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + \
+            ["-ffp-contract=off"] + defs
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc failed on {src} ({' '.join(defs)})")
+        objs.append(obj)
+    out = flow_variant_path(nrv, ng)
+    tmp = out + ".tmp"
+    r = subprocess.run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs,
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("link failed")
+    os.replace(tmp, out)
+    return out
 
 
 HDF5 = os.environ.get("IZEM_HDF5_PREFIX", "/opt/conda")

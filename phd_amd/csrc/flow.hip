@@ -33,8 +33,11 @@ __constant__ IzThd c_thd;
 #define IZ_EU_BLOCK 256
 #define IZ_EU_TX 32
 #define IZ_EU_TF (IZ_EU_BLOCK / IZ_EU_TX)
-#define IZ_EU_ROWS (IZ_EU_TF + 5)
-#define IZ_EU_NQ 17  // staged per cell: 14 conserved variables, u_n, P, c
+#define IZ_EU_ROWS (IZ_EU_TF + IZ_S - 1)
+#define IZ_EU_NQ (IZ_NV + 3)  // staged per cell: the conserved variables, u_n, P, c
+#define IZ_QU IZ_NV           // staged slots of u_n, P, c
+#define IZ_QP (IZ_NV + 1)
+#define IZ_QC (IZ_NV + 2)
 #ifndef IZ_EULER_WAVES
 #define IZ_EULER_WAVES 2
 #endif
@@ -56,13 +59,13 @@ __device__ inline void eu_stage_cell(const double *__restrict__ v, const double 
   double gam = c_thd.gam2 * W_i[c] / cp[c];
   gam = 1.0 / (1.0 - gam);
   const double cs = sqrt(gam * P * rho_i);
-  dst[14 * fs] = un;
-  dst[15 * fs] = P;
-  dst[16 * fs] = cs;
+  dst[IZ_QU * fs] = un;
+  dst[IZ_QP * fs] = P;
+  dst[IZ_QC * fs] = cs;
 }
 
-// F^ of one face from its six staged stencil cells: Q(q, st) is staged
-// quantity q of stencil point st (st = 2 is i_l, st = 3 is i_r); cL, cR are
+// F^ of one face from its IZ_S staged stencil cells: Q(q, st) is staged
+// quantity q of stencil point st (st = ng - 1 is i_l, st = ng is i_r); cL, cR are
 // the HBM indices of i_l, i_r (for h_k); F^ goes to F[m * nfaces + fidx].
 // FAST = false keeps the reference's dense projections (w = el . U, g =
 // el . F over all 14 columns, in its summation order: bit parity with the
@@ -84,8 +87,8 @@ __device__ inline void eu_face(const double *tile, int off, bool js,
   constexpr int t2 = (DIR == 2) ? 2 : 3;
   double lmax0 = -1.0, lmax1 = -1.0, lmax2 = -1.0;
 #pragma unroll
-  for (int st = 0; st < 6; st++) {
-    const double u = Q(14, st), cs = Q(16, st);
+  for (int st = 0; st < IZ_S; st++) {
+    const double u = Q(IZ_QU, st), cs = Q(IZ_QC, st);
     lmax0 = fmax(fabs(u - cs), lmax0);
     lmax1 = fmax(fabs(u), lmax1);
     lmax2 = fmax(fabs(u + cs), lmax2);
@@ -93,23 +96,24 @@ __device__ inline void eu_face(const double *tile, int off, bool js,
   // Roe average of i_l and i_r (module_weno.f90:654-667, 726-735)
   double uc[3], Yc[IZ_NRV], ht_c, et_c, r, r1;
   {
-    const double riL = 1.0 / Q(0, 2), riR = 1.0 / Q(0, 3);
+    constexpr int SL = IZ_NG - 1, SR = IZ_NG;  // stencil points of i_l, i_r
+    const double riL = 1.0 / Q(0, SL), riR = 1.0 / Q(0, SR);
     double uL[3], uR[3];
 #pragma unroll
-    for (int q = 0; q < 3; q++) { uL[q] = Q(1 + q, 2) * riL; uR[q] = Q(1 + q, 3) * riR; }
+    for (int q = 0; q < 3; q++) { uL[q] = Q(1 + q, SL) * riL; uR[q] = Q(1 + q, SR) * riR; }
     double htL = 0.0, htR = 0.0;
-    r = sqrt(Q(0, 3) * riL);
+    r = sqrt(Q(0, SR) * riL);
     r1 = 1.0 / (r + 1.0);
 #pragma unroll
     for (int l = 0; l < IZ_NRV; l++) {
-      const double yL = Q(IZ_NIV + l, 2) * riL, yR = Q(IZ_NIV + l, 3) * riR;
+      const double yL = Q(IZ_NIV + l, SL) * riL, yR = Q(IZ_NIV + l, SR) * riR;
       htL = htL + ha[l * nc + cL] * yL;
       htR = htR + ha[l * nc + cR] * yR;
       Yc[l] = (r * (yR) + yL) * r1;
     }
     htL = htL + 0.5 * (uL[0] * uL[0] + uL[1] * uL[1] + uL[2] * uL[2]);
     htR = htR + 0.5 * (uR[0] * uR[0] + uR[1] * uR[1] + uR[2] * uR[2]);
-    const double etL = Q(4, 2) * riL, etR = Q(4, 3) * riR;
+    const double etL = Q(4, SL) * riL, etR = Q(4, SR) * riR;
 #pragma unroll
     for (int q = 0; q < 3; q++) uc[q] = (r * uR[q] + uL[q]) * r1;
     ht_c = (r * htR + htL) * r1;
@@ -144,11 +148,11 @@ __device__ inline void eu_face(const double *tile, int off, bool js,
 
   // FAST: per stencil point, S of U and of F, sum_l psi_l U_l and its F
   // counterpart, and u_n U_0 - U_n with its F counterpart
-  double sw[6], sg[6], pw[6], pg[6], dw[6], dg[6];
+  double sw[IZ_S], sg[IZ_S], pw[IZ_S], pg[IZ_S], dw[IZ_S], dg[IZ_S];
   if constexpr (FAST) {
 #pragma unroll
-    for (int st = 0; st < 6; st++) {
-      const double u = Q(14, st), p = Q(15, st);
+    for (int st = 0; st < IZ_S; st++) {
+      const double u = Q(IZ_QU, st), p = Q(IZ_QP, st);
       const double u0 = Q(0, st), un = Q(cn, st), u4 = Q(4, st);
       double aw = -b1 * u0 - b2 * u4, ag = -b1 * un - b2 * (u * (u4 + p));
 #pragma unroll
@@ -232,10 +236,10 @@ __device__ inline void eu_face(const double *tile, int off, bool js,
     } else {
       lam = lmax1;  // m = 3, 4: two nonzeros, handled below
     }
-    double gp[6], gm[6];
+    double gp[IZ_S], gm[IZ_S];
 #pragma unroll
-    for (int st = 0; st < 6; st++) {
-      const double u = Qm(14, st), p = Qm(15, st);
+    for (int st = 0; st < IZ_S; st++) {
+      const double u = Qm(IZ_QU, st), p = Qm(IZ_QP, st);
       double wc, gc;
       if (FAST && m != 3 && m != 4) {
         const double hc = 0.5 * cs_c_i;
@@ -289,8 +293,7 @@ __device__ inline void eu_face(const double *tile, int off, bool js,
       gm[st] = gc - gp[st];
     }
     double gl, gr;
-    if (js) d_wenorec(gp, gm, gl, gr);
-    else d_wenorec_nw(gp, gm, gl, gr);
+    d_weno_face(js, gp, gm, gl, gr);
     ghat[m] = gl + gr;
   }
 
@@ -333,8 +336,11 @@ __device__ inline void eu_face(const double *tile, int off, bool js,
 }
 
 // LDS bytes of one block of k_euler_faces<DIR>; tile0 = cells staged per
-// wave for x (69 + 5 per row boundary that 64 consecutive faces can cross)
-static inline int eu_tile0(const IzDims &d) { return 69 + 5 * (63 / (d.nx + 1) + 1); }
+// wave for x (64 + IZ_S - 1, and IZ_S - 1 more per row boundary that 64
+// consecutive faces can cross)
+static inline int eu_tile0(const IzDims &d) {
+  return (64 + IZ_S - 1) + (IZ_S - 1) * (63 / (d.nx + 1) + 1);
+}
 static inline size_t eu_lds_bytes(int dir, int tile0) {
   if (dir == 0) return (size_t)4 * tile0 * (IZ_EU_NQ * sizeof(double) + 1);
   return (size_t)IZ_EU_ROWS * IZ_EU_TX * (IZ_EU_NQ * sizeof(double) + 1);
@@ -362,13 +368,13 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
     // context creation); only HBM cell indices are 64-bit
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned f0 = (blockIdx.x * 4u + wave) * 64u;  // first face of this wave
-    const unsigned nf = d.nx + 1, NXc = d.nx + 6;          // faces, cells per row
+    const unsigned nf = d.nx + 1, NXc = d.nx + 2 * IZ_NG;  // faces, cells per row
     const unsigned nrows = (unsigned)d.ny * d.nz, nfc = (unsigned)nfaces;
     // cell-major tile (slot * IZ_EU_NQ + q): every stencil access is a
     // compile-time offset from the lane's first stencil cell
     double *tl = lds + wave * tile0 * IZ_EU_NQ;
     uint8_t *tsn = (uint8_t *)(lds + 4 * tile0 * IZ_EU_NQ) + wave * tile0;
-    // cells in row-flattened order r * NXc + (x + 3); face f's stencil
+    // cells in row-flattened order r * NXc + (x + ng); face f's stencil
     // starts at S(f) = r * NXc + x + 1 for its left cell x
     unsigned S0 = 0;
     if (f0 < nfc) S0 = (f0 / nf) * NXc + (f0 % nf);
@@ -376,7 +382,7 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
       const unsigned cf = S0 + t;
       const unsigned r = cf / NXc;
       if (f0 < nfc && r < nrows) {
-        const int xc = (int)(cf - r * NXc) - 3;
+        const int xc = (int)(cf - r * NXc) - IZ_NG;
         const long c = iz_cidx(d, xc, (int)(r % d.ny), (int)(r / d.ny));
         eu_stage_cell(v, T, W_i, cp, d.nc, c, 1, tl + t * IZ_EU_NQ, 1, neg);
         tsn[t] = sensor[c];
@@ -391,7 +397,7 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
       const long cL = iz_cidx(d, x, (int)(r % d.ny), (int)(r / d.ny));
       int shk = 0;
 #pragma unroll
-      for (int st = 0; st < 6; st++) shk |= tsn[o + st];
+      for (int st = 0; st < IZ_S; st++) shk |= tsn[o + st];
       eu_face<0, 1, IZ_EU_NQ, FAST>(lds, (wave * tile0 + o) * IZ_EU_NQ, shk && weno_avg, ha, d.nc,
                               cL, cL + 1, F, nfaces, (long)f);
     }
@@ -407,13 +413,13 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
     const int tf = (int)(L % ntf);
     const int tx = (int)((L / ntf) % ntx);
     const int oth = (int)(L / ((long)ntf * ntx));  // k for y faces, j for z faces
-    const int row0 = tf * IZ_EU_TF - 3;              // first staged cell along DIR
+    const int row0 = tf * IZ_EU_TF - IZ_NG;          // first staged cell along DIR
     constexpr int FS = IZ_EU_ROWS * IZ_EU_TX;        // field stride in LDS
     uint8_t *tsn = (uint8_t *)(lds + (long)IZ_EU_NQ * FS);
     for (int t = threadIdx.x; t < FS; t += IZ_EU_BLOCK) {
       const int rr = t / IZ_EU_TX, cx = tx * IZ_EU_TX + t % IZ_EU_TX;
       const int cdir = row0 + rr;
-      if (blk_ok && cx < d.nx && cdir <= nd + 2) {
+      if (blk_ok && cx < d.nx && cdir <= nd + IZ_NG - 1) {
         const long c = DIR == 1 ? iz_cidx(d, cx, cdir, oth) : iz_cidx(d, cx, oth, cdir);
         eu_stage_cell(v, T, W_i, cp, d.nc, c, 1 + DIR, lds + t, FS, neg);
         tsn[t] = sensor[c];
@@ -427,7 +433,7 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
       // stencil point st of this face is staged row fl + st
       int shk = 0;
 #pragma unroll
-      for (int st = 0; st < 6; st++) shk |= tsn[(fl + st) * IZ_EU_TX + xl];
+      for (int st = 0; st < IZ_S; st++) shk |= tsn[(fl + st) * IZ_EU_TX + xl];
       long cL, cR, fidx;
       if (DIR == 1) {
         cL = iz_cidx(d, x, f, oth);
@@ -964,14 +970,7 @@ __global__ void k_sensor_ducros(IzDims d, double treduc, const double *__restric
                                (d.phys >> (2 * b + 1)) & 1);
         continue;
       }
-      double dd = 0.0;
-      dd = dd + (-1.0 / 60.0) * u(-3);
-      dd = dd + (9.0 / 60.0) * u(-2);
-      dd = dd + (-45.0 / 60.0) * u(-1);
-      dd = dd + (45.0 / 60.0) * u(1);
-      dd = dd + (-9.0 / 60.0) * u(2);
-      dd = dd + (1.0 / 60.0) * u(3);
-      g[a][b] = dd * dinv[b];
+      g[a][b] = d_deriv_central(u, dinv[b]);
     }
   double div, vort, wrk;
   if (d.ndim == 2) {
@@ -1578,8 +1577,10 @@ extern "C" int izem_create(izem_ctx **out, const izem_grid *g, const double *thd
                            const izem_opts *opts) {
   if (!out || !g || !thd || !opts) return IZEM_E_ARG;
   if (nthd != (long)(sizeof(IzThd) / sizeof(double))) return IZEM_E_ARG;
-  if (g->ng != 3 || g->ndim < 2 || g->ndim > 3 || g->nx < 7 || g->ny < 7) return IZEM_E_ARG;
-  if (g->ndim == 3 && g->nz < 7) return IZEM_E_ARG;
+  if (g->ng != IZ_NG || g->ndim < 2 || g->ndim > 3 || g->nx < 2 * IZ_NG + 1 ||
+      g->ny < 2 * IZ_NG + 1)
+    return IZEM_E_ARG;
+  if (g->ndim == 3 && g->nz < 2 * IZ_NG + 1) return IZEM_E_ARG;
   if (g->ndim == 2 && g->nz != 1) return IZEM_E_ARG;
   // face indices of a direction are 32-bit in the Euler kernels
   if ((long)(g->nx + 1) * (g->ny + 1) * (g->nz + 1) >= (1L << 31)) return IZEM_E_ARG;

@@ -199,6 +199,81 @@ __device__ inline void d_wenorec_nw(const double *vp0, const double *vm0, double
   vplus = (-3.0 * vm[6] + 27.0 * vm[5] + 47.0 * vm[4] - 13.0 * vm[3] + 2.0 * vm[2]) * denom;
 }
 
+// WENO7 (ng = 4): the candidate stencils' coefficients and the ideal weights
+// (4, 18, 12, 1)/35 mirrored for the '+' state (module_weno.f90:71-111), and
+// Balsara and Shu's r = 4 smoothness indicators (:154-172, 330-353): for the
+// four values a..d of a candidate,
+//   beta = a (k0 a + k1 b + k2 c + k3 d) + b (k4 b + k5 c + k6 d)
+//        + c (k7 c + k8 d) + d (k9 d).
+__device__ inline double d_beta7(int r, double a, double b, double c, double d) {
+  const double K[4][10] = {
+      {2107.0, -9402.0, 7042.0, -1854.0, 11003.0, -17246.0, 4642.0, 7043.0, -3882.0, 547.0},
+      {547.0, -2522.0, 1922.0, -494.0, 3443.0, -5966.0, 1602.0, 2843.0, -1642.0, 267.0},
+      {267.0, -1642.0, 1602.0, -494.0, 2843.0, -5966.0, 1922.0, 3443.0, -2522.0, 547.0},
+      {547.0, -3882.0, 4642.0, -1854.0, 7043.0, -17246.0, 7042.0, 11003.0, -9402.0, 2107.0}};
+  const double *k = K[r];
+  return a * (k[0] * a + k[1] * b + k[2] * c + k[3] * d) + b * (k[4] * b + k[5] * c + k[6] * d) +
+         c * (k[7] * c + k[8] * d) + d * (k[9] * d);
+}
+
+// one side of the WENO7 reconstruction: q 1-based, i the centre, row0 the
+// coefficient row of candidate 0 (1 for the '-' state, 0 for '+')
+__device__ inline double d_weno7_side(const double *q, int i, int row0, const double *dw) {
+  const double c[5][4] = {{25.0 / 12.0, -23.0 / 12.0, 13.0 / 12.0, -1.0 / 4.0},
+                          {1.0 / 4.0, 13.0 / 12.0, -5.0 / 12.0, 1.0 / 12.0},
+                          {-1.0 / 12.0, 7.0 / 12.0, 7.0 / 12.0, -1.0 / 12.0},
+                          {1.0 / 12.0, -5.0 / 12.0, 13.0 / 12.0, 1.0 / 4.0},
+                          {-1.0 / 4.0, 13.0 / 12.0, -23.0 / 12.0, 25.0 / 12.0}};
+  double vr[4], alfa[4], sum = 0.0, out = 0.0;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    vr[r] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) vr[r] = vr[r] + c[r + row0][j] * q[i - r + j];
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const double beta = d_beta7(r, q[i - r], q[i - r + 1], q[i - r + 2], q[i - r + 3]);
+    alfa[r] = dw[r] / ((IZ_EPS_WENO + beta) * (IZ_EPS_WENO + beta));
+    sum = sum + alfa[r];
+  }
+  sum = 1.0 / sum;
+#pragma unroll
+  for (int r = 0; r < 4; r++) out = out + alfa[r] * sum * vr[r];
+  return out;
+}
+
+__device__ inline void d_wenorec7(const double *vp0, const double *vm0, double &vminus,
+                                  double &vplus) {
+  const double d[4] = {4.0 / 35.0, 18.0 / 35.0, 12.0 / 35.0, 1.0 / 35.0};
+  const double dt[4] = {1.0 / 35.0, 12.0 / 35.0, 18.0 / 35.0, 4.0 / 35.0};
+  vminus = d_weno7_side(vp0 - 1, 4, 1, d);
+  vplus = d_weno7_side(vm0 - 1, 5, 0, dt);
+}
+
+// wenorec_nw, ng = 4 (module_weno.f90:246-256): the 7th-order linear face value
+__device__ inline void d_wenorec7_nw(const double *vp0, const double *vm0, double &vminus,
+                                     double &vplus) {
+  const double *vp = vp0 - 1, *vm = vm0 - 1;
+  const double denom = 1.0 / 420.0;
+  vminus = (-3.0 * vp[1] + 25.0 * vp[2] - 101.0 * vp[3] + 319.0 * vp[4] + 214.0 * vp[5] -
+            38.0 * vp[6] + 4.0 * vp[7]) * denom;
+  vplus = (-3.0 * vm[8] + 25.0 * vm[7] - 101.0 * vm[6] + 319.0 * vm[5] + 214.0 * vm[4] -
+           38.0 * vm[3] + 4.0 * vm[2]) * denom;
+}
+
+// the reconstruction of the build's order on IZ_S stencil values
+__device__ inline void d_weno_face(bool js, const double *gp, const double *gm, double &gl,
+                                   double &gr) {
+#if IZ_NG == 4
+  if (js) d_wenorec7(gp, gm, gl, gr);
+  else d_wenorec7_nw(gp, gm, gl, gr);
+#else
+  if (js) d_wenorec(gp, gm, gl, gr);
+  else d_wenorec_nw(gp, gm, gl, gr);
+#endif
+}
+
 // 6th-order central first derivative (alpha6), periodic branch of
 // dx/dy/dz, d?_fixed1, d?_fixed2: module_deriv.f90:37-40
 __device__ inline double d_deriv7(const double *__restrict__ a, long c, long s, double dinv) {
@@ -216,13 +291,52 @@ __device__ inline double d_deriv7(const double *__restrict__ a, long c, long s, 
 // offsets l from p are f(l), next to physical faces (lp: the face before
 // position 0, rp: the one after n-1): the three points nearest a physical
 // face take 4th-order closures, one-sided at the face point, upwind at the
-// next, central at the third, the right-hand ones negated
-// (module_deriv.f90:20-36, 268-469, ng = 3); elsewhere the 6th-order central
-// stencil.  Weights summed in increasing offset.
+// next, central at the third, the right-hand ones negated, and for ng = 4
+// 6th-order central at the fourth (module_deriv.f90:20-78, 268-469);
+// elsewhere the central stencil of the build's order.  Weights summed in
+// increasing offset.
+// central first derivative of the build's order at offsets f(l): 6th order
+// (alpha6) for ng = 3, 8th order (alph8) for ng = 4 (module_deriv.f90:37-78)
+template <typename Fn>
+__device__ inline double d_deriv_central(Fn f, double dinv) {
+  double dd = 0.0;
+#if IZ_NG == 4
+  dd = dd + (3.0 / 840.0) * f(-4);
+  dd = dd + (-32.0 / 840.0) * f(-3);
+  dd = dd + (168.0 / 840.0) * f(-2);
+  dd = dd + (-672.0 / 840.0) * f(-1);
+  dd = dd + (672.0 / 840.0) * f(1);
+  dd = dd + (-168.0 / 840.0) * f(2);
+  dd = dd + (32.0 / 840.0) * f(3);
+  dd = dd + (-3.0 / 840.0) * f(4);
+#else
+  dd = dd + (-1.0 / 60.0) * f(-3);
+  dd = dd + (9.0 / 60.0) * f(-2);
+  dd = dd + (-45.0 / 60.0) * f(-1);
+  dd = dd + (45.0 / 60.0) * f(1);
+  dd = dd + (-9.0 / 60.0) * f(2);
+  dd = dd + (1.0 / 60.0) * f(3);
+#endif
+  return dd * dinv;
+}
+
+template <typename Fn>
+__device__ inline double d_deriv6(Fn f, double dinv) {
+  double dd = 0.0;
+  dd = dd + (-1.0 / 60.0) * f(-3);
+  dd = dd + (9.0 / 60.0) * f(-2);
+  dd = dd + (-45.0 / 60.0) * f(-1);
+  dd = dd + (45.0 / 60.0) * f(1);
+  dd = dd + (-9.0 / 60.0) * f(2);
+  dd = dd + (1.0 / 60.0) * f(3);
+  return dd * dinv;
+}
+
 template <typename Fn>
 __device__ inline double d_deriv_line(Fn f, double dinv, int p, int n, bool lp, bool rp) {
   double dd = 0.0;
-  if (rp && p >= n - 3) {
+  if (rp && p >= n - IZ_NG) {
+    if (IZ_NG == 4 && p == n - 4) return d_deriv6(f, dinv);
     if (p == n - 1) {
       dd = dd + (-3.0 / 12.0) * f(-4);
       dd = dd + (16.0 / 12.0) * f(-3);
@@ -245,7 +359,8 @@ __device__ inline double d_deriv_line(Fn f, double dinv, int p, int n, bool lp, 
     dd = dd + (-1.0 / 12.0) * f(2);
     return dd * dinv;
   }
-  if (lp && p < 3) {
+  if (lp && p < IZ_NG) {
+    if (IZ_NG == 4 && p == 3) return d_deriv6(f, dinv);
     if (p == 0) {
       dd = dd + (-25.0 / 12.0) * f(0);
       dd = dd + (48.0 / 12.0) * f(1);
@@ -266,13 +381,7 @@ __device__ inline double d_deriv_line(Fn f, double dinv, int p, int n, bool lp, 
     }
     return dd * dinv;
   }
-  dd = dd + (-1.0 / 60.0) * f(-3);
-  dd = dd + (9.0 / 60.0) * f(-2);
-  dd = dd + (-45.0 / 60.0) * f(-1);
-  dd = dd + (45.0 / 60.0) * f(1);
-  dd = dd + (-9.0 / 60.0) * f(2);
-  dd = dd + (1.0 / 60.0) * f(3);
-  return dd * dinv;
+  return d_deriv_central(f, dinv);
 }
 
 // derivative along direction dir of the array a at cell c, interior
@@ -281,7 +390,11 @@ template <bool PHYS>
 __device__ inline double d_deriv_dir(const IzDims &d, const double *__restrict__ a, long c, int dir,
                                      int p, double dinv) {
   const long s = dir == 0 ? 1 : (dir == 1 ? d.NX : d.NX * d.NY);
+#if IZ_NG == 3
   if (!PHYS) return d_deriv7(a, c, s, dinv);
+#else
+  if (!PHYS) return d_deriv_central([&](int l) { return a[c + l * s]; }, dinv);
+#endif
   const int n = dir == 0 ? d.nx : (dir == 1 ? d.ny : d.nz);
   const bool lp = (d.phys >> (2 * dir)) & 1, rp = (d.phys >> (2 * dir + 1)) & 1;
   return d_deriv_line([&](int l) { return a[c + l * s]; }, dinv, p, n, lp, rp);

@@ -8,7 +8,17 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// species count (9 for the H2/air mechanism) and ghost layers (3 for WENO5,
+// 4 for WENO7: ng = (ORDER_WENO + 1) / 2, module_input.f90:712) are fixed
+// when a library is built; the default build is 9 species and WENO5, and
+// flow-only variants are built for other pairs (phd_amd/build.py)
+#ifndef IZ_NRV
 #define IZ_NRV 9
+#endif
+#ifndef IZ_NG
+#define IZ_NG 3
+#endif
+#define IZ_S (2 * IZ_NG)  // points of a face's WENO stencil
 #define IZ_NIV 5
 #define IZ_NV (IZ_NIV + IZ_NRV)
 

@@ -63,10 +63,12 @@ def _ptr(t: torch.Tensor):
 class Solver:
     def __init__(self, box: Box, opts: Options | None = None, device: str = "cuda:0",
                  adi: cd.Adim | None = None, thd: cd.Thd | None = None,
-                 mech: cd.Mechanism | None = None):
+                 mech: cd.Mechanism | None = None, lib=None):
+        """lib: another build of the library (_lib.load_variant for other
+        species counts or WENO7), matching thd's species and box.ng."""
         if not torch.cuda.is_available():
             raise _lib.IzemError("the IZEM solver needs a ROCm GPU")
-        self.lib = _lib.load()
+        self.lib = lib or _lib.load()
         self.box = box
         self.opts = opts or Options()
         self.dev = torch.device(device)
@@ -92,11 +94,12 @@ class Solver:
             torch.cuda.current_stream(self.dev).cuda_stream)), "izem_set_stream")
         shp = box.shape
         f64 = dict(dtype=torch.float64, device=self.dev)
-        self.v = torch.zeros((cd.NV,) + shp, **f64)
+        nrv = len(self.thd.W)
+        self.v = torch.zeros((cd.NIV + nrv,) + shp, **f64)
         self.T = torch.zeros(shp, **f64)
         self.W_i = torch.zeros(shp, **f64)
         self.cp = torch.zeros(shp, **f64)
-        self.ha = torch.zeros((cd.NRV,) + shp, **f64)
+        self.ha = torch.zeros((nrv,) + shp, **f64)
         self.sensor = torch.zeros(shp, dtype=torch.uint8, device=self.dev)
         dxi = box.inv_spacing(self.adi)
         self.d_i = [torch.tensor(x, **f64) for x in dxi]

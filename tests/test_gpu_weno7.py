@@ -1,0 +1,195 @@
+"""WENO7 (ng = 4) on the GPU: the flow-only library variants built with
+IZ_NG = 4 (9 species; 2 species for the reference's O2/N2 shock tube)
+against the oracle, whose WENO7 path reproduces the reference's own Sod
+trajectory (tests/test_sod_reference_cpu.py).
+
+* The 3-D turbulence recipe at 16^3 (9 species, periodic) and the 2-D hot
+  spot at 24^2 re-padded to ng = 4: the right-hand side and one SSP-RK3 step
+  with a sensor threshold of 0.005 (JS and linear faces both occur) to 1e-12,
+  and 20 main-loop iterations in the parity setting (every face JS): bit for
+  bit without the viscous fluxes, within 1e-10 with them (8th-order viscous
+  stencils).
+* The reference's Sod deck on the 2-species variant (extrapolated x faces
+  with the closures and their 6th-order points, periodic y), from the state
+  the oracle initialised, through the main loop to the first statistics
+  time (647 iterations): against the oracle run as one rank (a GPU has no
+  ranks: the two-rank sensor views of the reference's own run are the
+  oracle's to restate, tests/test_sod_reference_cpu.py).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+import pytest
+
+from tests.flowcase import OracleRun, P, small_state
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+# Over many steps with viscous fluxes: the transport's exp and log differ
+# between the GPU and the host library in the last bit (7e-17 in one
+# right-hand side), and WENO7's smoothness indicators (integer forms up to
+# 17246 against eps = 1e-10) let such differences move the nonlinear weights
+# far more than WENO5's do; without the viscous fluxes the two runs stay
+# bit for bit equal.
+MULTISTEP_TOL = 1e-10
+
+
+def _relerr(a, b, box):
+    I = (slice(None),) + box.interior()
+    a, b = a[I], b[I]
+    out = []
+    for m in range(a.shape[0]):
+        s = np.max(np.abs(b[m]))
+        d = np.max(np.abs(a[m] - b[m]))
+        out.append(0.0 if s == 0 and d == 0 else d / (s if s > 0 else 1.0))
+    return np.array(out)
+
+
+def _repad(box, v, T, ng=4):
+    """The same interior with ng periodic ghost layers."""
+    I = box.interior()
+    b4 = dataclasses.replace(box, ng=ng)
+    pad = [(ng, ng)] * 3
+    v4 = np.stack([np.pad(v[(l,) + I], pad, mode="wrap") for l in range(v.shape[0])])
+    T4 = np.pad(T[I], pad, mode="wrap")
+    return b4, v4, T4
+
+
+def _upload(s, run):
+    import torch
+    for name in ("v", "T", "W_i", "cp", "ha", "sensor"):
+        getattr(s, name).copy_(torch.from_numpy(np.ascontiguousarray(getattr(run, name))))
+    torch.cuda.synchronize()
+
+
+def _otimestep(run, opts, dtmin=1.0):
+    ts = np.zeros(4)
+    d = C.c_double(dtmin)
+    run.o.lib.ora_timestep(C.byref(run.g), C.byref(run.t), C.byref(run.a), C.byref(opts),
+                           P(run.d_i[0]), P(run.d_i[1]), P(run.d_i[2]), P(run.cp), P(run.W_i),
+                           P(run.T), P(run.v), C.c_double(0.0), P(ts), C.byref(d))
+    return d.value
+
+
+def _ork3(run, opts, dt, f_out=None):
+    st = run.o.lib.ora_rk3(C.byref(run.g), C.byref(run.t), C.byref(run.a), C.byref(opts),
+                           C.c_double(dt), P(run.d_i[0]), P(run.d_i[1]), P(run.d_i[2]),
+                           P(run.T), P(run.W_i), P(run.cp), P(run.ha),
+                           run.sensor.ctypes.data_as(C.POINTER(C.c_uint8)), P(run.v),
+                           P(f_out) if f_out is not None else None)
+    assert st == 0
+
+
+@pytest.mark.parametrize("kind,n,mrw,steps,vis", [("3d", 16, 0.005, 1, 1), ("3d", 16, -1.0, 20, 0),
+                                                   ("3d", 16, -1.0, 20, 1), ("2d", 24, -1.0, 20, 0),
+                                                   ("2d", 24, -1.0, 20, 1)])
+def test_weno7_rhs_rk3_and_steps(oracle, adi, thd, kind, n, mrw, steps, vis):
+    import torch
+    from phd_amd import _lib
+    from phd_amd.solver import Options, Solver
+    box3, v3, T3 = small_state(adi, thd, kind, n)
+    box, v, T = _repad(box3, v3, T3)
+    print(f"case {kind} n={n} ng=4", flush=True)
+    run = OracleRun(oracle, adi, thd, box, v, T)
+    run.prim_all()
+    # a real sensor threshold (JS faces where it fired, linear ones
+    # elsewhere) for the right-hand side and one step: over more steps a
+    # cell within rounding of the threshold may flip between the builds, so
+    # the multi-step runs take the parity setting (every face JS)
+    opts = run.opts(vis=vis, mrw=mrw)
+    s = Solver(box, Options(reaction=0, max_rel_weight=mrw, vis=vis), adi=adi, thd=thd,
+               lib=_lib.load_variant(9, 4))
+    _upload(s, run)
+    f_gpu = torch.zeros((14,) + box.shape, dtype=torch.float64, device="cuda:0")
+    s.rhs(f_gpu)
+    s.sync()
+    dt = _otimestep(run, opts)
+    s.timestep()
+    assert abs(s.time_state()[4] - dt) <= 1e-14 * dt
+    f1 = np.zeros((14,) + box.shape)
+    sens0 = run.sensor.copy()
+    _ork3(run, opts, dt, f1)
+    e_rhs = _relerr(f_gpu.cpu().numpy(), f1, box).max()
+    s.rk3(torch.tensor([dt], dtype=torch.float64, device="cuda:0"))
+    s.sync()
+    assert s.status() == 0
+    e_v = _relerr(s.v.cpu().numpy(), run.v, box).max()
+    flags = int(run.sensor.sum())
+    print(f"  rhs {e_rhs:.2e}, rk3 state {e_v:.2e}, {flags} flagged cells of {box.nx * box.ny * box.nz}")
+    assert 0 < flags < box.nx * box.ny * box.nz and sens0.sum() == 0
+    if steps == 1:
+        s.close()
+        return
+    assert np.array_equal(s.sensor.cpu().numpy(), run.sensor)
+    assert e_rhs <= TOL and e_v <= TOL
+    dtmin = dt
+    ts = s.time_state()
+    for it in range(steps - 1):
+        dtmin = _otimestep(run, opts, dtmin)
+        _ork3(run, opts, dtmin)
+        s.timestep()
+        s.rk3(s.lib.izem_time_state(s.ctx) + 4 * 8)
+    s.sync()
+    e = _relerr(s.v.cpu().numpy(), run.v, box).max()
+    print(f"  {steps} steps (viscous {vis}): state {e:.2e}")
+    assert s.status() == 0
+    if vis:
+        assert e <= MULTISTEP_TOL
+    else:
+        assert np.array_equal(s.v.cpu().numpy(), run.v)       # bit for bit
+    assert np.array_equal(s.sensor.cpu().numpy(), run.sensor)
+    s.close()
+
+
+def test_weno7_sod_deck_on_the_gpu(adi):
+    """The reference's Sod deck to its first statistics time on the
+    2-species WENO7 variant, against the oracle run as one rank."""
+    import torch
+    from phd_amd import _lib
+    from phd_amd.solver import Options, Solver
+    from tests import sodcase as sc
+    run = sc.SodRun(adi)
+    run.g.nproc[0] = 1
+    s = Solver(run.box, Options(weno_avg=1, max_rel_weight=sc.PERCENT_WEIGHT, vis=1, vis_dt=0,
+                                reaction=0, forcesum=1, CFL=sc.CFL, CFLmin=sc.CFLMIN, Fo=sc.FO,
+                                CPR=sc.CPR),
+               adi=adi, thd=run.thd, lib=_lib.load_variant(2, 4))
+    s.set_bc((1, 1, 0, 0, 0, 0))
+    _upload(s, run)
+    s.set_time_state([1.0, 1.0, 1.0, 1.0, 1.0, 1.0])
+    dt_ptr = s.lib.izem_time_state(s.ctx) + 4 * 8
+    time, nb, worst_dt, checks = 0.0, 0, 0.0, []
+    while nb < 1:
+        s.timestep()
+        ts = s.time_state()
+        dtg = ts[4]
+        nxt = run.times[0]
+        land = time + dtg > nxt and time - dtg < nxt
+        if land:
+            ts[4] = nxt - time
+            s.set_time_state(ts)
+        s.rk3(dt_ptr)
+        wrote = run.step()
+        worst_dt = max(worst_dt, abs(ts[4] - run.dtmin) / run.dtmin)
+        time += ts[4]
+        assert wrote == land
+        if land:
+            nb += 1
+        if run.ite in (1, 100) or land:
+            s.sync()
+            vg = s.v.cpu().numpy()
+            r = _relerr(vg, run.v, run.box)
+            # rho v, rho w are rounding noise (3e-13) in this 1-D problem
+            e = max(r[0], r[1], r[4], r[5], r[6])
+            I = (slice(2, 4),) + run.box.interior()
+            assert np.max(np.abs(vg[I] - run.v[I])) <= 1e-14
+            checks.append((run.ite, e))
+            print(f"  iteration {run.ite}: state {e:.2e}", flush=True)
+    assert s.status() == 0
+    assert run.ite == 647 and worst_dt <= TOL
+    assert max(e for _, e in checks) <= MULTISTEP_TOL
+    assert np.array_equal(s.sensor.cpu().numpy(), run.sensor)
+    s.close()

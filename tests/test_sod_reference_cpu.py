@@ -6,7 +6,7 @@ From ini_sod's initial state the oracle built for 2 species runs the
 reference's main loop (WENO7, Adams-Shariff sensor at 0.15 with JS weights
 where it fired, extrapolated x faces with the one-sided derivative closures,
 periodic y, viscous fluxes, CFL 0.5) until the first and second statistics
-times (648 and 713 iterations) and is compared with the reference's
+times (647 and 712 iterations) and is compared with the reference's
 restart_00001.h5 and restart_00002.h5 and with its times_stat.out.  The run
 being restated used 2 MPI ranks along x; two consequences for the sensor are
 restated too: each rank's sweep stops one cell short of its own last cell,
