@@ -90,6 +90,19 @@ int izem_create(izem_ctx **out, const izem_grid *grid, const double *thd, long n
 int izem_destroy(izem_ctx *ctx);
 /* bc[6] in the order W, E, S, N, B, F (faces beyond ndim must be periodic) */
 int izem_set_bc(izem_ctx *ctx, const int bc[6]);
+/* Reproduce on one GPU a run decomposed into nproc[0] x nproc[1] x nproc[2]
+ * ranks (GRID%NUMPROCX/Y/Z, module_input.f90:361-363, 498-500; rank r of
+ * np along a line of n cells owns cells r n / np .. (r + 1) n / np - 1,
+ * module_parallel.f90:547-548).  The results of the reference's decomposed run depend on the
+ * decomposition through the shock sensor alone: each rank's
+ * shock_sensor_modified_adams_shariff sweep stops one short of its last
+ * cell (module_tools.f90:2733-2814), flags are neither set on ghosts nor
+ * exchanged, so a face sees the flags of the rank that forms it, and the
+ * face between two ranks is formed by both, each with its own view
+ * (module_weno.f90:955-1020 per rank).  {1, 1, 1} (the default) is the
+ * undecomposed run; directions beyond ndim take 1, and a direction takes at
+ * most one rank per cell. */
+int izem_set_decomposition(izem_ctx *ctx, const int nproc[3]);
 int izem_synchronize(izem_ctx *ctx);
 /* run on a caller-owned stream (hipStream_t); NULL restores the context's */
 int izem_set_stream(izem_ctx *ctx, void *stream);

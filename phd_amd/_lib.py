@@ -23,6 +23,7 @@ EXPORTS = [
     "izem_mlp_mse", "izem_mlp_infer", "izem_surr_features", "izem_surr_apply", "izem_surr_fallback",
     "izem_reac_selec", "izem_reaction_selec", "izem_surr_attach", "izem_reaction_surrogate",
     "izem_surr_counts", "izem_thd_image", "izem_mech_image_ckwrk", "izem_set_bc",
+    "izem_set_decomposition",
 ]
 
 
@@ -170,6 +171,7 @@ def _declare(lib):
     lib.izem_thd_image.argtypes = [C.c_int, C.c_int] + [vp] * 10 + [C.c_double] * 7 + [vp,
                                                                                       C.c_long]
     lib.izem_set_bc.argtypes = [vp, vp]
+    lib.izem_set_decomposition.argtypes = [vp, vp]
     lib.izem_mech_image_ckwrk.restype = C.c_long
     lib.izem_mech_image_ckwrk.argtypes = [vp, vp, vp, vp, C.c_long]
 

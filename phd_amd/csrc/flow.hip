@@ -76,7 +76,9 @@ __device__ inline void eu_stage_cell(const double *__restrict__ v, const double 
 // -(1 + xi) / b2 S - sum_l psi_l U_l, and species row 6 + s is U_{5+s} + Y_s S.
 // The characteristic projections then cost about a tenth of the dense ones;
 // results differ from the dense order by rounding only.
-template <int DIR, int QS, int SS, bool FAST>
+// SITE gives every calling kernel an instantiation of its own: with a single
+// caller each is inlined as before, where a shared one is left out of line.
+template <int DIR, int QS, int SS, bool FAST, int SITE>
 __device__ inline void eu_face(const double *tile, int off, bool js,
                                const double *__restrict__ ha, long nc, long cL, long cR,
                                double *__restrict__ F, long nfaces, long fidx) {
@@ -353,7 +355,26 @@ static inline long eu_ntiles(const IzDims &d, int dir) {
   return ntf * ntx * (dir == 1 ? d.nz : d.ny);
 }
 
-template <int DIR, bool FAST>
+// The flags a face sees in a decomposed run: those of its stencil cells
+// (left cell il, points il - ng + 1 .. il + ng) that the viewing rank rv
+// owns -- the sensor is neither set on ghosts nor exchanged between ranks
+// (module_tools.f90:2733-2814 per rank, then module_weno.f90:955-989).
+// tsn[st * stride] is stencil point st's flag.
+__device__ inline int eu_flags_view(const uint8_t *tsn, int stride, int il, int n, int np,
+                                    int rv) {
+  const int lo = iz_rank_lo(rv, n, np), hi = iz_rank_lo(rv + 1, n, np) - 1;
+  int shk = 0;
+#pragma unroll
+  for (int st = 0; st < IZ_S; st++) {
+    const int p = il - IZ_NG + 1 + st;
+    if (p >= lo && p <= hi) shk |= tsn[st * stride];
+  }
+  return shk;
+}
+
+// RANKS: the run reproduces a decomposed one (d.np[DIR] > 1): a face takes
+// the flag view of the rank of its right cell
+template <int DIR, bool FAST, bool RANKS>
 __global__ void __attribute__((amdgpu_flat_work_group_size(IZ_EU_BLOCK, IZ_EU_BLOCK),
                                amdgpu_waves_per_eu(IZ_EULER_WAVES, IZ_EULER_WAVES)))
 k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
@@ -396,9 +417,13 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
       const int o = (int)(r * NXc + x + 1 - S0);
       const long cL = iz_cidx(d, x, (int)(r % d.ny), (int)(r / d.ny));
       int shk = 0;
+      if constexpr (RANKS) {
+        shk = eu_flags_view(tsn + o, 1, x, d.nx, d.np[0], iz_rank_of(x + 1, d.nx, d.np[0]));
+      } else {
 #pragma unroll
-      for (int st = 0; st < IZ_S; st++) shk |= tsn[o + st];
-      eu_face<0, 1, IZ_EU_NQ, FAST>(lds, (wave * tile0 + o) * IZ_EU_NQ, shk && weno_avg, ha, d.nc,
+        for (int st = 0; st < IZ_S; st++) shk |= tsn[o + st];
+      }
+      eu_face<0, 1, IZ_EU_NQ, FAST, RANKS>(lds, (wave * tile0 + o) * IZ_EU_NQ, shk && weno_avg, ha, d.nc,
                               cL, cL + 1, F, nfaces, (long)f);
     }
   } else {
@@ -432,8 +457,13 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
     if (blk_ok && x < d.nx && f <= nd - 1) {
       // stencil point st of this face is staged row fl + st
       int shk = 0;
+      if constexpr (RANKS) {
+        shk = eu_flags_view(tsn + fl * IZ_EU_TX + xl, IZ_EU_TX, f, nd, d.np[DIR],
+                            iz_rank_of(f + 1, nd, d.np[DIR]));
+      } else {
 #pragma unroll
-      for (int st = 0; st < IZ_S; st++) shk |= tsn[(fl + st) * IZ_EU_TX + xl];
+        for (int st = 0; st < IZ_S; st++) shk |= tsn[(fl + st) * IZ_EU_TX + xl];
+      }
       long cL, cR, fidx;
       if (DIR == 1) {
         cL = iz_cidx(d, x, f, oth);
@@ -444,18 +474,69 @@ k_euler_faces(IzDims d, const double *__restrict__ v, const double *__restrict__
         cR = cL + d.NX * d.NY;
         fidx = x + (long)d.nx * (oth + (long)d.ny * (f + 1));
       }
-      eu_face<DIR, FS, IZ_EU_TX, FAST>(lds, fl * IZ_EU_TX + xl, shk && weno_avg, ha, d.nc, cL, cR, F,
+      eu_face<DIR, FS, IZ_EU_TX, FAST, RANKS>(lds, fl * IZ_EU_TX + xl, shk && weno_avg, ha, d.nc, cL, cR, F,
                                  nfaces, fidx);
     }
   }
   if (neg) atomicOr(status, IZ_ST_NEG_PRESSURE);
 }
 
-// fl = (F^_r - F^_l) * d_i  (x assigns, y/z accumulate): module_weno.f90:998-1020
+// The face between two ranks along DIR formed a second time, with the left
+// rank's flag view: each rank of a decomposed run forms its boundary faces
+// itself, and the left rank's last cell takes its right face from its own
+// view (module_weno.f90:955-1020 per rank).  One thread per seam face
+// t = (r) + (np - 1) line, r the left rank; F^ goes to Fs[m * nseam + t].
+// Each thread stages its IZ_S stencil cells in its own LDS run, quantity
+// q of stencil point st at q * IZ_S + st.
+#define IZ_SEAM_BLOCK 32
+template <int DIR, bool FAST>
+__global__ void __launch_bounds__(IZ_SEAM_BLOCK)
+k_euler_seams(IzDims d, const double *__restrict__ v, const double *__restrict__ T,
+              const double *__restrict__ W_i, const double *__restrict__ cp,
+              const double *__restrict__ ha, const uint8_t *__restrict__ sensor, int weno_avg,
+              double *__restrict__ Fs, long nseam, int *__restrict__ status) {
+  extern __shared__ double lds[];
+  const long t = (long)blockIdx.x * IZ_SEAM_BLOCK + threadIdx.x;
+  int neg = 0;
+  if (t < nseam) {
+    const int np = d.np[DIR];
+    const int n = DIR == 0 ? d.nx : (DIR == 1 ? d.ny : d.nz);
+    const int r = (int)(t % (np - 1));
+    const long line = t / (np - 1);
+    // x lines are (j, k), y lines (i, k), z lines (i, j)
+    const int a = DIR == 0 ? (int)(line % d.ny) : (int)(line % d.nx);
+    const int b = DIR == 0 ? (int)(line / d.ny) : (int)(line / d.nx);
+    auto cell = [&](int p) {
+      return DIR == 0 ? iz_cidx(d, p, a, b) : (DIR == 1 ? iz_cidx(d, a, p, b) : iz_cidx(d, a, b, p));
+    };
+    const int il = iz_rank_lo(r + 1, n, np) - 1;  // the left rank's last cell
+    const int off = threadIdx.x * IZ_S * IZ_EU_NQ;
+    uint8_t fl[IZ_S];
+#pragma unroll
+    for (int st = 0; st < IZ_S; st++) {
+      const long c = cell(il - IZ_NG + 1 + st);
+      eu_stage_cell(v, T, W_i, cp, d.nc, c, 1 + DIR, lds + off + st, IZ_S, neg);
+      fl[st] = sensor[c];
+    }
+    const int shk = eu_flags_view(fl, 1, il, n, np, r);
+    eu_face<DIR, IZ_S, 1, FAST, 2>(lds, off, shk && weno_avg, ha, d.nc, cell(il), cell(il + 1),
+                                    Fs, nseam, t);
+  }
+  if (neg) atomicOr(status, IZ_ST_NEG_PRESSURE);
+}
+
+// seam face arrays of the three directions (d.np > 1)
+struct IzSeams {
+  const double *F[3];
+  long n[3];
+};
+
+// fl = (F^_r - F^_l) * d_i  (x assigns, y/z accumulate): module_weno.f90:998-1020;
+// a rank's last cell takes its right face from the seam array
 template <int DIR>
 __global__ void k_euler_apply(IzDims d, const double *__restrict__ F, long nfaces,
                               const double *__restrict__ d_i, int accumulate,
-                              double *__restrict__ fl) {
+                              double *__restrict__ fl, const double *__restrict__ Fs, long nseam) {
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long ncin = (long)d.nx * d.ny * d.nz;
   if (tid >= ncin) return;
@@ -468,9 +549,16 @@ __global__ void k_euler_apply(IzDims d, const double *__restrict__ F, long nface
   else if (DIR == 1) { fl_ = i + (long)d.nx * (j + (long)(d.ny + 1) * k); fr = fl_ + d.nx; di = d_i[j + d.ng]; }
   else { fl_ = i + (long)d.nx * (j + (long)d.ny * k); fr = fl_ + (long)d.nx * d.ny; di = d_i[k + d.ng]; }
   long c = iz_cidx(d, i, j, k);
+  long sr = -1;
+  if (d.np[DIR] > 1) {
+    if (DIR == 0) sr = iz_seam(i, d.nx, d.np[0], j + (long)d.ny * k);
+    else if (DIR == 1) sr = iz_seam(j, d.ny, d.np[1], i + (long)d.nx * k);
+    else sr = iz_seam(k, d.nz, d.np[2], i + (long)d.nx * j);
+  }
 #pragma unroll
   for (int m = 0; m < IZ_NV; m++) {
-    double df = (F[m * nfaces + fr] - F[m * nfaces + fl_]) * di;
+    const double Fr = sr >= 0 ? Fs[m * nseam + sr] : F[m * nfaces + fr];
+    double df = (Fr - F[m * nfaces + fl_]) * di;
     if (accumulate) fl[m * d.nc + c] = fl[m * d.nc + c] + df * 1.0;
     else fl[m * d.nc + c] = df * 1.0;
   }
@@ -923,7 +1011,8 @@ __global__ void k_sensor(IzDims d, double max_rel_weight, const double *__restri
   for (int dd = 0; dd < d.ndim; dd++) {
     int p = (dd == 0) ? i : (dd == 1 ? j : k);
     int n = (dd == 0) ? d.nx : (dd == 1 ? d.ny : d.nz);
-    if (p >= n - 1) continue;
+    // the sweep of each rank stops one short of its last cell
+    if (p >= iz_rank_lo(iz_rank_of(p, n, d.np[dd]) + 1, n, d.np[dd]) - 1) continue;
     long c2 = c + ((dd == 0) ? 1 : (dd == 1 ? d.NX : d.NX * d.NY));
     double ri2 = 1.0 / v[c2];
     double P2 = v[c2] * T[c2] * W_i[c2];
@@ -1298,11 +1387,13 @@ __global__ void k_visc_apply(IzDims d, const double *__restrict__ sf,
 #ifndef IZ_RHS_WAVES
 #define IZ_RHS_WAVES 2
 #endif
-template <bool PHYS>
+// SEAMS: the run reproduces a decomposed one (d.np > 1); a rank's last cell
+// takes its right face from the seam arrays
+template <bool PHYS, bool SEAMS>
 __global__ void __attribute__((amdgpu_flat_work_group_size(256, 256),
                                amdgpu_waves_per_eu(IZ_RHS_WAVES, IZ_RHS_WAVES)))
 k_rhs_update(IzDims d, int mode, const double *__restrict__ Fx, const double *__restrict__ Fy,
-             const double *__restrict__ Fz, int vis, const double *__restrict__ sf,
+             const double *__restrict__ Fz, IzSeams sm, int vis, const double *__restrict__ sf,
              const double *__restrict__ fd, const double *__restrict__ dx_i,
              const double *__restrict__ dy_i, const double *__restrict__ dz_i,
              const double *__restrict__ dtp, double bk, double ck,
@@ -1326,17 +1417,39 @@ k_rhs_update(IzDims d, int mode, const double *__restrict__ Fx, const double *__
     const long fzl = i + (long)d.nx * (j + (long)d.ny * k);
     const double dxi = dx_i[i + d.ng], dyi = dy_i[j + d.ng], dzi = dz_i[k + d.ng];
     double f[IZ_NV];
+    if constexpr (SEAMS) {
+      const long sx = iz_seam(i, d.nx, d.np[0], j + (long)d.ny * k);
+      const long sy = iz_seam(j, d.ny, d.np[1], i + (long)d.nx * k);
+      const long sz = iz_seam(k, d.nz, d.np[2], i + (long)d.nx * j);
 #pragma unroll
-    for (int m = 0; m < IZ_NV; m++) {
-      double df = (Fx[m * nfx + fxl + 1] - Fx[m * nfx + fxl]) * dxi;
-      f[m] = df * 1.0;
-      if (d.ndim >= 2) {
-        df = (Fy[m * nfy + fyl + d.nx] - Fy[m * nfy + fyl]) * dyi;
-        f[m] = f[m] + df * 1.0;
+      for (int m = 0; m < IZ_NV; m++) {
+        double Fr = sx >= 0 ? sm.F[0][m * sm.n[0] + sx] : Fx[m * nfx + fxl + 1];
+        double df = (Fr - Fx[m * nfx + fxl]) * dxi;
+        f[m] = df * 1.0;
+        if (d.ndim >= 2) {
+          Fr = sy >= 0 ? sm.F[1][m * sm.n[1] + sy] : Fy[m * nfy + fyl + d.nx];
+          df = (Fr - Fy[m * nfy + fyl]) * dyi;
+          f[m] = f[m] + df * 1.0;
+        }
+        if (d.ndim == 3) {
+          Fr = sz >= 0 ? sm.F[2][m * sm.n[2] + sz] : Fz[m * nfz + fzl + (long)d.nx * d.ny];
+          df = (Fr - Fz[m * nfz + fzl]) * dzi;
+          f[m] = f[m] + df * 1.0;
+        }
       }
-      if (d.ndim == 3) {
-        df = (Fz[m * nfz + fzl + (long)d.nx * d.ny] - Fz[m * nfz + fzl]) * dzi;
-        f[m] = f[m] + df * 1.0;
+    } else {
+#pragma unroll
+      for (int m = 0; m < IZ_NV; m++) {
+        double df = (Fx[m * nfx + fxl + 1] - Fx[m * nfx + fxl]) * dxi;
+        f[m] = df * 1.0;
+        if (d.ndim >= 2) {
+          df = (Fy[m * nfy + fyl + d.nx] - Fy[m * nfy + fyl]) * dyi;
+          f[m] = f[m] + df * 1.0;
+        }
+        if (d.ndim == 3) {
+          df = (Fz[m * nfz + fzl + (long)d.nx * d.ny] - Fz[m * nfz + fzl]) * dzi;
+          f[m] = f[m] + df * 1.0;
+        }
       }
     }
     if (vis) {
@@ -1506,33 +1619,12 @@ __global__ void k_wenorec_batch(int n, int js, const double *__restrict__ vp,
 }
 
 // explicit instantiations used by the API
-template __global__ void k_euler_faces<0, false>(IzDims, const double *, const double *,
-                                                 const double *, const double *, const double *,
-                                                 const uint8_t *, int, double *, long, int *, int,
-                                                 long);
-template __global__ void k_euler_faces<0, true>(IzDims, const double *, const double *,
-                                                 const double *, const double *, const double *,
-                                                 const uint8_t *, int, double *, long, int *, int,
-                                                 long);
-template __global__ void k_euler_faces<1, false>(IzDims, const double *, const double *,
-                                                 const double *, const double *, const double *,
-                                                 const uint8_t *, int, double *, long, int *, int,
-                                                 long);
-template __global__ void k_euler_faces<1, true>(IzDims, const double *, const double *,
-                                                 const double *, const double *, const double *,
-                                                 const uint8_t *, int, double *, long, int *, int,
-                                                 long);
-template __global__ void k_euler_faces<2, false>(IzDims, const double *, const double *,
-                                                 const double *, const double *, const double *,
-                                                 const uint8_t *, int, double *, long, int *, int,
-                                                 long);
-template __global__ void k_euler_faces<2, true>(IzDims, const double *, const double *,
-                                                 const double *, const double *, const double *,
-                                                 const uint8_t *, int, double *, long, int *, int,
-                                                 long);
-template __global__ void k_euler_apply<0>(IzDims, const double *, long, const double *, int, double *);
-template __global__ void k_euler_apply<1>(IzDims, const double *, long, const double *, int, double *);
-template __global__ void k_euler_apply<2>(IzDims, const double *, long, const double *, int, double *);
+template __global__ void k_euler_apply<0>(IzDims, const double *, long, const double *, int, double *,
+                                              const double *, long);
+template __global__ void k_euler_apply<1>(IzDims, const double *, long, const double *, int, double *,
+                                              const double *, long);
+template __global__ void k_euler_apply<2>(IzDims, const double *, long, const double *, int, double *,
+                                              const double *, long);
 
 // ===================================================================
 // C ABI (include/izem_hip.h)
@@ -1596,6 +1688,7 @@ extern "C" int izem_create(izem_ctx **out, const izem_grid *g, const double *thd
   d.nx = g->nx; d.ny = g->ny; d.nz = g->nz; d.ng = g->ng; d.ndim = g->ndim;
   d.NX = d.nx + 2 * d.ng; d.NY = d.ny + 2 * d.ng; d.NZ = d.nz + 2 * d.ng;
   d.nc = d.NX * d.NY * d.NZ;
+  d.np[0] = d.np[1] = d.np[2] = 1;
   c->o = *opts;
   if (hipStreamCreate(&c->own_stream) != hipSuccess) { free(c); return IZEM_E_HIP; }
   c->stream = c->own_stream;
@@ -1679,6 +1772,7 @@ extern "C" int izem_destroy(izem_ctx *c) {
   iz_graph_destroy(c);
   iz_chem_destroy(c);
   for (int q = 0; q < 3; q++) if (c->F[q]) (void)hipFree(c->F[q]);
+  for (int q = 0; q < 3; q++) if (c->Fs[q]) (void)hipFree(c->Fs[q]);
   if (c->prim) (void)hipFree(c->prim);
   if (c->fd) (void)hipFree(c->fd);
   if (c->sf) (void)hipFree(c->sf);
@@ -1720,6 +1814,43 @@ extern "C" int izem_set_bc(izem_ctx *c, const int bc[6]) {
     c->d.phys = phys;
     iz_graph_destroy(c);  // a captured step holds the old dimensions
   }
+  return IZEM_OK;
+}
+
+extern "C" int izem_set_decomposition(izem_ctx *c, const int nproc[3]) {
+  if (!c || !nproc) return IZEM_E_ARG;
+  const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
+  for (int q = 0; q < 3; q++) {
+    if (nproc[q] < 1 || nproc[q] > n[q]) return IZEM_E_ARG;
+    if (q >= c->d.ndim && nproc[q] != 1) return IZEM_E_ARG;
+  }
+  if (nproc[0] == c->d.np[0] && nproc[1] == c->d.np[1] && nproc[2] == c->d.np[2])
+    return IZEM_OK;
+  IZ_CHECK(hipStreamSynchronize(c->stream));
+  iz_graph_destroy(c);  // a captured step holds the old launches
+  const long lines[3] = {(long)c->d.ny * c->d.nz, (long)c->d.nx * c->d.nz,
+                         (long)c->d.nx * c->d.ny};
+  for (int q = 0; q < 3; q++) {
+    if (c->Fs[q]) (void)hipFree(c->Fs[q]);
+    c->Fs[q] = nullptr;
+    c->nseam[q] = 0;
+    c->d.np[q] = 1;
+  }
+  for (int q = 0; q < 3; q++) {
+    if (nproc[q] == 1) continue;
+    const long ns = (long)(nproc[q] - 1) * lines[q];
+    if (hipMalloc(&c->Fs[q], sizeof(double) * IZ_NV * ns) != hipSuccess) {
+      for (int p = 0; p < 3; p++) {
+        if (c->Fs[p]) (void)hipFree(c->Fs[p]);
+        c->Fs[p] = nullptr;
+        c->nseam[p] = 0;
+        c->d.np[p] = 1;
+      }
+      return IZEM_E_NOMEM;
+    }
+    c->nseam[q] = ns;
+  }
+  for (int q = 0; q < 3; q++) c->d.np[q] = nproc[q];
   return IZEM_OK;
 }
 
@@ -1772,6 +1903,23 @@ extern "C" int izem_wenorec(izem_ctx *c, int n, int js, const double *vp, const 
   return IZEM_OK;
 }
 
+// k_rhs_update for the context's faces and decomposition
+typedef void (*IzRhsKernel)(IzDims, int, const double *, const double *, const double *, IzSeams,
+                            int, const double *, const double *, const double *, const double *,
+                            const double *, const double *, double, double, const double *,
+                            double *, double *, int, const double *, double,
+                            unsigned long long *);
+static IzRhsKernel rhs_update_kernel(const IzDims &d) {
+  const bool seams = d.np[0] > 1 || d.np[1] > 1 || d.np[2] > 1;
+  if (seams) return d.phys ? k_rhs_update<true, true> : k_rhs_update<false, true>;
+  return d.phys ? k_rhs_update<true, false> : k_rhs_update<false, false>;
+}
+static IzSeams seams_of(const izem_ctx *c) {
+  IzSeams s;
+  for (int q = 0; q < 3; q++) { s.F[q] = c->Fs[q]; s.n[q] = c->nseam[q]; }
+  return s;
+}
+
 static int euler_dir(izem_ctx *c, int dir, const double *d_i, const double *T,
                      const double *W_i, const double *cp, const double *ha,
                      const uint8_t *sensor, const double *v, double *fl, bool apply) {
@@ -1786,19 +1934,43 @@ static int euler_dir(izem_ctx *c, int dir, const double *d_i, const double *T,
   // multiple of the 8 XCDs (blocks past the last tile only pass the barrier)
   const unsigned nb = dir == 0 ? iz_blocks(nf, IZ_EU_BLOCK) : (unsigned)(((ntiles + 7) / 8) * 8);
   int pk = iz_prof_begin(c, IZP_EULER);
-#define IZ_EU(D, FAST)                                                                     \
-  hipLaunchKernelGGL((k_euler_faces<D, FAST>), dim3(nb), dim3(IZ_EU_BLOCK), lds, c->stream, \
-                     c->d, v, T, W_i, cp, ha, sensor, wa, c->F[D], nf, c->status, tile0, ntiles)
+#define IZ_EU(D, FAST, RANKS)                                                          \
+  hipLaunchKernelGGL((k_euler_faces<D, FAST, RANKS>), dim3(nb), dim3(IZ_EU_BLOCK), lds,   \
+                     c->stream, c->d, v, T, W_i, cp, ha, sensor, wa, c->F[D], nf, c->status, \
+                     tile0, ntiles)
+#define IZ_EU_DIR(FAST, RANKS)            \
+  if (dir == 0) IZ_EU(0, FAST, RANKS);    \
+  else if (dir == 1) IZ_EU(1, FAST, RANKS); \
+  else IZ_EU(2, FAST, RANKS)
+  const bool ranks = c->d.np[dir] > 1;
   if (c->flux_mode) {
-    if (dir == 0) IZ_EU(0, true);
-    else if (dir == 1) IZ_EU(1, true);
-    else IZ_EU(2, true);
+    if (ranks) { IZ_EU_DIR(true, true); }
+    else { IZ_EU_DIR(true, false); }
   } else {
-    if (dir == 0) IZ_EU(0, false);
-    else if (dir == 1) IZ_EU(1, false);
-    else IZ_EU(2, false);
+    if (ranks) { IZ_EU_DIR(false, true); }
+    else { IZ_EU_DIR(false, false); }
   }
+#undef IZ_EU_DIR
 #undef IZ_EU
+  if (c->d.np[dir] > 1) {
+    // the faces between ranks again, with the left rank's view
+    const long ns = c->nseam[dir];
+    const unsigned nbs = iz_blocks(ns, IZ_SEAM_BLOCK);
+    const size_t lds_s = (size_t)IZ_SEAM_BLOCK * IZ_S * IZ_EU_NQ * sizeof(double);
+#define IZ_SEAM(D, FAST)                                                                    \
+  hipLaunchKernelGGL((k_euler_seams<D, FAST>), dim3(nbs), dim3(IZ_SEAM_BLOCK), lds_s, c->stream, \
+                     c->d, v, T, W_i, cp, ha, sensor, wa, c->Fs[D], ns, c->status)
+    if (c->flux_mode) {
+      if (dir == 0) IZ_SEAM(0, true);
+      else if (dir == 1) IZ_SEAM(1, true);
+      else IZ_SEAM(2, true);
+    } else {
+      if (dir == 0) IZ_SEAM(0, false);
+      else if (dir == 1) IZ_SEAM(1, false);
+      else IZ_SEAM(2, false);
+    }
+#undef IZ_SEAM
+  }
   iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
   if (apply) {
@@ -1807,13 +1979,13 @@ static int euler_dir(izem_ctx *c, int dir, const double *d_i, const double *T,
     int acc = dir > 0;
     if (dir == 0)
       hipLaunchKernelGGL(k_euler_apply<0>, dim3(nbc), dim3(256), 0, c->stream, c->d, c->F[0], nf,
-                         d_i, acc, fl);
+                         d_i, acc, fl, c->Fs[0], c->nseam[0]);
     else if (dir == 1)
       hipLaunchKernelGGL(k_euler_apply<1>, dim3(nbc), dim3(256), 0, c->stream, c->d, c->F[1], nf,
-                         d_i, acc, fl);
+                         d_i, acc, fl, c->Fs[1], c->nseam[1]);
     else
       hipLaunchKernelGGL(k_euler_apply<2>, dim3(nbc), dim3(256), 0, c->stream, c->d, c->F[2], nf,
-                         d_i, acc, fl);
+                         d_i, acc, fl, c->Fs[2], c->nseam[2]);
     IZ_CHECK(hipGetLastError());
   }
   return IZEM_OK;
@@ -2039,9 +2211,9 @@ extern "C" int izem_rhs(izem_ctx *c, const double *dx_i, const double *dy_i, con
   int r = stage_fluxes(c, dx_i, dy_i, dz_i, T, W_i, cp, ha, sensor, v);
   if (r) return r;
   const IzDims &d = c->d;
-  hipLaunchKernelGGL(d.phys ? k_rhs_update<true> : k_rhs_update<false>,
-                     dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0, c->stream, d, 0, c->F[0],
-                     c->F[1], c->F[2], c->o.vis, c->sf, c->fd, dx_i, dy_i, dz_i,
+  hipLaunchKernelGGL(rhs_update_kernel(d), dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0,
+                     c->stream, d, 0, c->F[0], c->F[1], c->F[2], seams_of(c), c->o.vis, c->sf,
+                     c->fd, dx_i, dy_i, dz_i,
                      (const double *)nullptr, 0.0, 0.0, (const double *)nullptr,
                      (double *)nullptr, f, 0, T, 0.0, (unsigned long long *)nullptr);
   IZ_CHECK(hipGetLastError());
@@ -2065,9 +2237,9 @@ extern "C" int izem_rk3(izem_ctx *c, const double *dt, const double *dx_i, const
     if (r) return r;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->masks, 2 * IZ_NEG_MASKS);
     int pk = iz_prof_begin(c, IZP_RHS);
-    hipLaunchKernelGGL(d.phys ? k_rhs_update<true> : k_rhs_update<false>,
-                       dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0, c->stream, d, 1,
-                       c->F[0], c->F[1], c->F[2], c->o.vis, c->sf, c->fd, dx_i, dy_i, dz_i, dt,
+    hipLaunchKernelGGL(rhs_update_kernel(d), dim3(iz_blocks(n_interior(d), 256)), dim3(256), 0,
+                       c->stream, d, 1, c->F[0], c->F[1], c->F[2], seams_of(c), c->o.vis, c->sf,
+                       c->fd, dx_i, dy_i, dz_i, dt,
                        bk3[ik], ck3[ik], c->v0, v, (double *)nullptr, c->o.forcesum, T, 1.0e-5,
                        c->masks + 3);
     iz_prof_end(c, pk);

@@ -40,11 +40,34 @@ struct IzThd {
 
 // phys: bit f set when face f (W, E, S, N, B, F) is a physical boundary
 // with extrapolated ghosts (bc_extrapolation) instead of a periodic one
+// np: ranks of the decomposed run whose shock-sensor views are reproduced,
+// per direction (1: the whole line is one rank)
 struct IzDims {
   int nx, ny, nz, ng, ndim;
   long NX, NY, NZ, nc;
   int phys;
+  int np[3];
 };
+
+// rank r of np along a line of n cells owns cells (r n)/np .. ((r+1) n)/np - 1
+// (sx = coords*ntx/dims + 1, ex = (coords+1)*ntx/dims: module_parallel.f90:
+// 547-548, zero-based here); positions past either end go to the end ranks
+__host__ __device__ inline int iz_rank_lo(int r, int n, int np) {
+  return (int)(((long)r * n) / np);
+}
+__host__ __device__ inline int iz_rank_of(int p, int n, int np) {
+  int r = 0;
+  while (r + 1 < np && iz_rank_lo(r + 1, n, np) <= p) r++;
+  return r;
+}
+// index of the seam face to the right of cell p when p is the last cell of a
+// rank that is not the last one ((r) + (np - 1) line), else -1
+__host__ __device__ inline long iz_seam(int p, int n, int np, long line) {
+  if (np <= 1) return -1;
+  const int r = iz_rank_of(p, n, np);
+  if (r + 1 < np && p == iz_rank_lo(r + 1, n, np) - 1) return r + (long)(np - 1) * line;
+  return -1;
+}
 
 __host__ __device__ inline long iz_cidx(const IzDims &d, int i, int j, int k) {
   return (long)(i + d.ng) + d.NX * ((long)(j + d.ng) + d.NY * (long)(k + d.ng));

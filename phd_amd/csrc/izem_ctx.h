@@ -51,6 +51,10 @@ struct izem_ctx {
   // Euler face arrays (14 x faces per direction)
   double *F[3];
   long nfaces[3];
+  // faces between two ranks formed with the left rank's view (d.np > 1):
+  // 14 x nseam per direction
+  double *Fs[3];
+  long nseam[3];
   // viscous scratch: velocities + mole fractions, gradients / diffusion
   // fluxes, stresses + heat flux
   double *prim, *fd, *sf;

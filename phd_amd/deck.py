@@ -224,16 +224,41 @@ def face_conditions(deck: Deck) -> tuple:
     return tuple(BCS[deck.get(k).upper()] if f // 2 < dim else 0 for f, k in enumerate(BC_KEYS))
 
 
+def ghost_layers(deck: Deck) -> int:
+    """ng = (ORDER_WENO + 1) / 2 (module_input.f90:712): 3 for WENO5, 4 for
+    WENO7."""
+    return (deck.get("WENO%ORDER_WENO") + 1) // 2
+
+
+def library_key(deck: Deck) -> tuple[int, int]:
+    """(species, ghost layers) of the library a deck runs on: the full
+    library (9 species, WENO5, chemistry) or a flow-only build
+    (phd_amd/build.py FLOW_VARIANTS)."""
+    return deck.get("GENERAL%NRV"), ghost_layers(deck)
+
+
+def decomposition(deck: Deck) -> tuple[int, int, int]:
+    """GRID%NUMPROCX/Y/Z: the ranks of the reference's decomposed run, which
+    the GPU run reproduces on one device (izem_set_decomposition)."""
+    return tuple(deck.get(k) for k in ("GRID%NUMPROCX", "GRID%NUMPROCY", "GRID%NUMPROCZ"))
+
+
 def options(deck: Deck, **overrides):
     """The solver Options of a deck (module_input.f90:673-705), refusing
     what the hot path does not build."""
+    from .build import FLOW_VARIANTS
     from .solver import Options
     g = deck.get
     refuse = []
-    if g("WENO%ORDER_WENO") != 5:
-        refuse.append(f"ORDER_WENO = {g('WENO%ORDER_WENO')} (WENO5 only)")
-    if g("GENERAL%NRV") != 9:
-        refuse.append(f"NRV = {g('GENERAL%NRV')} (the 9-species H2/air mechanism only)")
+    if g("WENO%ORDER_WENO") not in (5, 7):
+        refuse.append(f"ORDER_WENO = {g('WENO%ORDER_WENO')} (WENO5 or WENO7)")
+    key = library_key(deck)
+    if key != (9, 3) and key not in FLOW_VARIANTS:
+        refuse.append(f"NRV = {key[0]} with ORDER_WENO = {g('WENO%ORDER_WENO')} (built: 9 "
+                      f"species WENO5, and flow-only {', '.join(f'{n} species ng {q}' for n, q in FLOW_VARIANTS)})")
+    elif key != (9, 3) and g("REACTING%REACTION"):
+        refuse.append(f"REACTION = T with NRV = {key[0]}, ORDER_WENO = "
+                      f"{g('WENO%ORDER_WENO')} (chemistry: the 9-species WENO5 library)")
     for key in ("GENERAL%NPV", "GENERAL%NVV"):
         if g(key) != 0:
             refuse.append(f"{key} = {g(key)}")
@@ -244,9 +269,9 @@ def options(deck: Deck, **overrides):
     for key in BC_KEYS:
         if g(key).upper() not in BCS:
             refuse.append(f"{key} = {g(key)} (periodic or extrapolated faces only)")
-    for key in ("GRID%NUMPROCX", "GRID%NUMPROCY", "GRID%NUMPROCZ"):
-        if g(key) != 1:
-            refuse.append(f"{key} = {g(key)} (one GPU, no decomposition)")
+    for q, key in enumerate(("GRID%NUMPROCX", "GRID%NUMPROCY", "GRID%NUMPROCZ")):
+        if g(key) < 1 or (q >= g("GENERAL%DIM") and g(key) != 1):
+            refuse.append(f"{key} = {g(key)}")
     sensor = g("WENO%SHOCK_SENSOR_TYPE").upper()
     if sensor not in SENSORS:
         refuse.append(f"SHOCK_SENSOR_TYPE = {sensor}")

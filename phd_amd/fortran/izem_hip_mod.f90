@@ -57,6 +57,8 @@ module izem_hip_mod
   ! face conditions of izem_set_bc (GRID%BCXMIN .. BCZMAX)
   integer(c_int), parameter, public :: IZEM_BC_PERIODIC = 0, IZEM_BC_EXTRAPOLATION = 1
   public :: izem_set_bc
+  ! ranks of a decomposed run to reproduce (GRID%NUMPROCX/Y/Z)
+  public :: izem_set_decomposition
 
   ! lengths of the flat table images (doubles)
   integer(c_long), parameter, public :: IZEM_THD_IMAGE_LEN = 745, IZEM_MECH_IMAGE_LEN = 2850
@@ -75,6 +77,12 @@ module izem_hip_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), intent(in) :: bc(6)
+    end function
+    integer(c_int) function izem_set_decomposition(ctx, nproc) &
+        bind(C, name="izem_set_decomposition")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: nproc(3)
     end function
     integer(c_int) function izem_destroy(ctx) bind(C, name="izem_destroy")
       import :: c_int, c_ptr

@@ -32,6 +32,21 @@ _A4UR = {-3: 1.0 / 12.0, -2: -6.0 / 12.0, -1: 18.0 / 12.0, 0: -10.0 / 12.0, 1: -
 _A4OR = {-4: -3.0 / 12.0, -3: 16.0 / 12.0, -2: -36.0 / 12.0, -1: 48.0 / 12.0, 0: -25.0 / 12.0}
 
 
+def count_points(path: str) -> tuple[int, int, int]:
+    """Points per direction of a grid.dat, counted as the reference counts
+    them (module_parallel.f90:271-306): the records between X-DIRECTION and
+    Y-DIRECTION, Y-DIRECTION and Z-DIRECTION, Z-DIRECTION and END (a
+    list-directed read skips blank records)."""
+    words = [ln.split()[0] for ln in open(path).read().splitlines() if ln.split()]
+    try:
+        p = [words.index(w) for w in ("X-DIRECTION", "Y-DIRECTION", "Z-DIRECTION", "END")]
+    except ValueError:
+        raise ValueError(f"{path}: missing direction in grid.dat") from None
+    if not p[0] < p[1] < p[2] < p[3]:
+        raise ValueError(f"{path}: directions out of order")
+    return p[1] - p[0] - 1, p[2] - p[1] - 1, p[3] - p[2] - 1
+
+
 def read_grid(path: str, ntx: int, nty: int, ntz: int):
     """Coordinates (xt, yt, zt) of a grid.dat (module_tools.f90:276-297)."""
     recs = [ln.split() for ln in open(path).read().splitlines()]

@@ -281,3 +281,52 @@ def config5_state(adi, thd, n: int = 384, seed: int = 5, L: float = 0.008,
     del pert
     v, Tn = conserved_from_primitive(box, adi, thd, T, P_ATM, np.ascontiguousarray(Y), u)
     return box, v, Tn
+
+
+def sod_state(adi, thd, box: Box, xt: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """ini_sod (devel-izem/src/module_ICs.f90:2715-2881) for an air-only
+    (O2, N2) or H2/air (H2, O2, N2) species set: cells with x < 0.5 m at
+    p_ref, rho_ref, the others at 0.1 p_ref, 0.125 rho_ref, both air (mole
+    fractions 0.233 O2, 0.767 N2 turned into mass fractions by X_to_Y), at
+    rest; the temperature guess is p / (rho W_i), and the right state's
+    density is recomputed from it, as the reference does.  xt: the cell x
+    coordinates [m] of the grid file.  Returns the conserved v (interior
+    only; load_conserved fills the ghosts) and the interior temperature
+    guess T."""
+    from .thermo_host import ha_scalar, wmix_i_scalar, x_to_y
+    nrv = len(thd.W)
+    if nrv == 2:
+        X = [0.233, 0.767]
+    elif nrv == 3:
+        X = [0.0, 0.233, 0.767]
+    else:
+        raise ValueError(f"ini_sod initialises 2 or 3 species, not {nrv}")
+    Ya = x_to_y(thd, X)
+    Ws_i = wmix_i_scalar(thd, Ya)
+    states = []
+    for right, (P, rho) in enumerate(((adi.p_ref / adi.p_ref, adi.rho_ref / adi.rho_ref),
+                                      (0.1 * adi.p_ref / adi.p_ref,
+                                       0.125 * adi.rho_ref / adi.rho_ref))):
+        Temp = P / (rho * Ws_i)
+        has = ha_scalar(thd, Temp)
+        hm = 0.0
+        for l in range(nrv):
+            hm = hm + has[l] * Ya[l]
+        if right:
+            rho = P / (Temp * Ws_i)
+        ux = vy = wz = 0.0
+        e = rho * hm - P + 0.5 * rho * (ux * ux + vy * vy + wz * wz)
+        states.append((rho, rho * ux, rho * vy, rho * wz, e, [rho * y for y in Ya], Temp))
+    v = np.zeros((5 + nrv,) + box.shape)
+    T = np.zeros(box.shape)
+    I = box.interior()
+    x = np.asarray(xt, dtype=np.float64) * (1.0 / adi.L_ref)   # nondimensional, as read
+    for i in range(box.nx):
+        st = states[0] if x[i] * adi.L_ref < 0.5 else states[1]
+        sl = (I[0], I[1], box.ng + i)
+        for q in range(5):
+            v[(q,) + sl] = st[q]
+        for l in range(nrv):
+            v[(5 + l,) + sl] = st[5][l]
+        T[sl] = st[6]
+    return v, T

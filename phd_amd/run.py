@@ -13,11 +13,19 @@ limits are appended to times_stat.out and restart_<k>.h5 is written
 (STAT = T); the run ends after ITMAX iterations (ITLIM = T) or at the last
 statistics time.  ITSHOW sets how often the progress line is printed.
 
-Initial states (GENERAL%INIT; the reference's own initialisations are
-not rebuilt): HOTSPOT (2-D periodic 1 cm box, stoichiometric H2/air, 900 K
-with a 1500 K Gaussian hot spot, u_rms 10 m/s), HIT (3-D periodic 1 cm box,
-phi = 0.8, 1000 K +- 50 K, u_rms 20 m/s) and SHEARLAYER2 (3-D periodic double
-H2/N2 - air shear layer, 8 mm box).
+Initial states (GENERAL%INIT): SOD, the reference's ini_sod
+(module_ICs.f90:2715-2881) on the grid of the deck directory's grid.dat, and
+three states of this project's own: HOTSPOT (2-D periodic 1 cm box,
+stoichiometric H2/air, 900 K with a 1500 K Gaussian hot spot, u_rms 10 m/s),
+HIT (3-D periodic 1 cm box, phi = 0.8, 1000 K +- 50 K, u_rms 20 m/s) and
+SHEARLAYER2 (3-D periodic double H2/N2 - air shear layer, 8 mm box).
+
+The species set is the deck directory's tranfit.out when there is one (as
+the reference reads its run directory's), the bundled 9-species H2/air set
+otherwise; GENERAL%NRV and WENO%ORDER_WENO pick the library (the full one
+for 9 species and WENO5, a flow-only build otherwise), and a deck
+decomposed over GRID%NUMPROCX/Y/Z ranks is reproduced on the one GPU
+(izem_set_decomposition).
 """
 from __future__ import annotations
 
@@ -28,19 +36,38 @@ import time as _time
 
 import numpy as np
 
-# GENERAL%INIT -> (dimension, box side [m]); a restart file holds no grid,
-# so a run read from one takes the side of its deck's INIT
-INITS = {"HOTSPOT": (2, 0.01), "HIT": (3, 0.01), "SHEARLAYER2": (3, 0.008)}
+# GENERAL%INIT -> (dimension, box side [m]; None: the grid of grid.dat); a
+# restart file holds no grid, so a run read from one takes the grid of its
+# deck's INIT
+INITS = {"HOTSPOT": (2, 0.01), "HIT": (3, 0.01), "SHEARLAYER2": (3, 0.008), "SOD": (2, None)}
 
 
-def initial_state(deck, adi, thd, n: int):
+def initial_state(deck, adi, thd, n: int, grid_path: str | None = None):
     from . import ics
     init = _init(deck)
+    if init == "SOD":
+        from . import grid
+        box = _grid_box(deck, adi, grid_path)
+        xt, _, _ = grid.read_grid(grid_path, box.nx, box.ny, box.nz)
+        v, T = ics.sod_state(adi, thd, box, xt)
+        return box, v, T
     if init == "HOTSPOT":
         return ics.config1_state(adi, thd, n=n)
     if init == "HIT":
         return ics.config2_state(adi, thd, n=n, L=INITS[init][1])
     return ics.config5_state(adi, thd, n=n, L=INITS[init][1])
+
+
+def _grid_box(deck, adi, grid_path: str):
+    """The box of a grid.dat (points counted as module_parallel.f90:271-306
+    counts them), with the deck's ghost layers."""
+    from . import grid
+    from .deck import ghost_layers
+    if not grid_path or not os.path.exists(grid_path):
+        raise SystemExit(f"GENERAL%INIT = {_init(deck)} reads grid.dat: {grid_path} is missing")
+    nx, ny, nz = grid.count_points(grid_path)
+    return grid.box_from_grid(grid_path, nx, ny, nz, deck.get("GENERAL%DIM"), adi,
+                              ng=ghost_layers(deck))
 
 
 def _init(deck) -> str:
@@ -60,43 +87,72 @@ def main(argv=None):
     ap.add_argument("--dir", default=None, help="output directory (default: the deck's)")
     ap.add_argument("--flux-mode", type=int, default=1)
     args = ap.parse_args(argv)
+    from . import _lib
     from . import chemdata as cd
-    from .deck import Deck, face_conditions, options
+    from .deck import Deck, decomposition, face_conditions, ghost_layers, library_key, options
     from .ics import Box
     from .restart import restart_name
     from .solver import Solver
 
     deck = Deck.read(args.deck)
     opts = options(deck, flux_mode=args.flux_mode)
-    out = args.dir or os.path.dirname(os.path.abspath(args.deck))
+    here = os.path.dirname(os.path.abspath(args.deck))
+    out = args.dir or here
     os.makedirs(out, exist_ok=True)
     adi = cd.Adim()
-    thd = cd.build_thd(adi)
-    mech = cd.read_mechanism() if opts.reaction else None
+    tranfit = os.path.join(here, "tranfit.out")
+    thd = cd.build_thd(adi, tranfit_path=tranfit if os.path.exists(tranfit) else None)
     g = deck.get
+    if len(thd.W) != g("GENERAL%NRV"):
+        raise SystemExit(f"GENERAL%NRV = {g('GENERAL%NRV')} but the species set has "
+                         f"{len(thd.W)} species")
+    key = library_key(deck)
+    lib = None if key == (9, 3) else _lib.load_variant(*key)
+    mech = cd.read_mechanism() if opts.reaction else None
+    grid_path = os.path.join(here, "grid.dat")
     nstat, dtime, offset = g("MONITORING%NSTAT"), g("MONITORING%DTIME"), g("MONITORING%TIME_OFFSET")
+
+    def solver(box):
+        s = Solver(box, opts, adi=adi, thd=thd, mech=mech, lib=lib)
+        s.set_bc(face_conditions(deck))
+        s.set_decomposition(decomposition(deck))
+        return s
+
     if g("MONITORING%READ_RESTART"):
         nb = g("MONITORING%NUMBER_RESTART")
         path = os.path.join(out, restart_name(g("MONITORING%NAME_RESTART"), nb))
         from . import restart
         (nx, ny, nz), _ = restart.dims(path)
         dim, side = INITS[_init(deck)]
-        box = Box(nx, ny, nz, 3, dim, (side, side, side))
-        s = Solver(box, opts, adi=adi, thd=thd, mech=mech)
-        s.set_bc(face_conditions(deck))
+        if side is None:
+            box = _grid_box(deck, adi, grid_path)
+            if (box.nx, box.ny, box.nz) != (nx, ny, nz):
+                raise SystemExit(f"{path} holds {nx} x {ny} x {nz} cells, grid.dat "
+                                 f"{box.nx} x {box.ny} x {box.nz}")
+        else:
+            box = Box(nx, ny, nz, ghost_layers(deck), dim, (side, side, side))
+        s = solver(box)
         time = s.load_restart(path)
         nb_stat = nb
-        # statistics times restart from the offset (module_input.f90:718-722)
-        timing = [offset] * (nb + 1) + [offset + k * dtime for k in range(1, nstat - nb)]
+        # statistics times restart from the offset, then DTIME is added one
+        # at a time (module_input.f90:718-721)
+        timing = [offset] * (nb + 1)
+        while len(timing) < nstat:
+            timing.append(timing[-1] + dtime)
     else:
-        box, v, T = initial_state(deck, adi, thd, args.n)
-        s = Solver(box, opts, adi=adi, thd=thd, mech=mech)
-        s.set_bc(face_conditions(deck))
-        s.load_state(v, T)
+        box, v, T = initial_state(deck, adi, thd, args.n, grid_path)
+        s = solver(box)
+        if _init(deck) == "SOD":
+            s.load_conserved(v, T)
+        else:
+            s.load_state(v, T)
         del v, T
         time = g("MONITORING%INITIAL_TIME")
         nb_stat = 0
-        timing = [offset + k * dtime for k in range(nstat)]
+        # TIME_OFFSET, then DTIME added one at a time (module_input.f90:722-726)
+        timing = [offset]
+        for _ in range(1, nstat):
+            timing.append(timing[-1] + dtime)
     timing = np.array(timing) / adi.time_ref  # nondimensional (module_tools.f90:180)
     # the time state starts as the reference's dtmin = dt_4 = DTmax = 1
     s.set_time_state(np.ones(6))

@@ -118,6 +118,13 @@ class Solver:
         a = (C.c_int * 6)(*[int(x) for x in bc])
         _lib.check(self.lib.izem_set_bc(self.ctx, a), "izem_set_bc")
 
+    def set_decomposition(self, nproc) -> None:
+        """Reproduce a run decomposed into nproc[0] x nproc[1] x nproc[2] ranks
+        (GRID%NUMPROCX/Y/Z): the per-rank shock-sensor sweep and flag views
+        of that run (izem_set_decomposition)."""
+        a = (C.c_int * 3)(*[int(x) for x in nproc])
+        _lib.check(self.lib.izem_set_decomposition(self.ctx, a), "izem_set_decomposition")
+
     def chem_tune(self, waves: int, blocks_per_cu: int, jac_age: int, kinetics: int):
         """Launch parameters of the chemistry kernels (izem_chem_tune)."""
         _lib.check(self.lib.izem_chem_tune(self.ctx, waves, blocks_per_cu, jac_age, kinetics),
@@ -163,12 +170,32 @@ class Solver:
         edge values until the next ghost fill; 'periodic' keeps the periodic
         images (the state the writing run had).  Returns the time."""
         from . import restart
+        v, time = restart.read(path, self.box, self.thd.names)
+        self.load_conserved(v, None, ghosts)
+        return time
+
+    def load_conserved(self, v: np.ndarray, T: np.ndarray | None = None,
+                       ghosts: str = "reference") -> None:
+        """The closing steps of the reference's initialisations and of its
+        restart reader (ini_sod, module_ICs.f90:2872-2877; read_restart_hdf5,
+        module_io.f90:107-192): the interior conserved variables v, the
+        temperature guess T on the interior (zero when None; ghosts start at
+        zero), comm_cons, prim_inv_var_err_mach on the interior,
+        upd_extrapolation on every face (ghosts='reference'; 'periodic' keeps
+        the periodic images), then the ghost slabs."""
+        from . import restart
         if ghosts not in ("reference", "periodic"):
             raise ValueError(ghosts)
-        v, time = restart.read(path, self.box, self.thd.names)
-        self.v.copy_(torch.from_numpy(v))
-        for a in (self.T, self.W_i, self.cp, self.ha):
+        self.v.copy_(torch.from_numpy(np.ascontiguousarray(v)))
+        for a in (self.W_i, self.cp, self.ha):
             a.zero_()
+        if T is None:
+            self.T.zero_()
+        else:
+            Tg = np.zeros(self.box.shape)
+            I = self.box.interior()
+            Tg[I] = np.asarray(T)[I]
+            self.T.copy_(torch.from_numpy(Tg))
         _lib.check(self.lib.izem_comm_cons(self.ctx, _ptr(self.v)), "comm_cons")
         self.prim_inv_var_mach(0)
         if ghosts == "reference":
@@ -176,7 +203,6 @@ class Solver:
         for dom in [s * q for q in range(1, self.box.ndim + 1) for s in (-1, 1)]:
             self.prim_inv_var_mach(dom)
         self.sync()
-        return time
 
     def prim_inv_var_mach(self, domain: int = 0) -> int:
         it = C.c_int()

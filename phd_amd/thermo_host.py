@@ -32,3 +32,61 @@ def ha_scalar_vec(thd, Tn: np.ndarray) -> np.ndarray:
         h = np.where(TK < tmin, hlo, np.where(TK >= tmax, hhi, h))
         out[s] = thd.Wc_i[s] / thd.Tinf * h
     return out
+
+
+def x_to_y(thd, X) -> np.ndarray:
+    """X_to_Y (module_thermodynamics.f90:45-68): mole to mass fractions, in
+    the reference's summation order."""
+    wrk = 0.0
+    for l in range(len(thd.W)):
+        wrk = wrk + float(X[l]) * float(thd.Wc[l])
+    wrk = 1.0 / wrk
+    return np.array([float(X[l]) * float(thd.Wc[l]) * wrk for l in range(len(thd.W))])
+
+
+def wmix_i_scalar(thd, Y) -> float:
+    """Wmix_i_scalar (module_thermodynamics.f90:739-756): sum_l Y_l Wc_i_l."""
+    w = 0.0
+    for l in range(len(thd.W)):
+        w = w + float(Y[l]) * float(thd.Wc_i[l])
+    return w
+
+
+def ha_scalar(thd, temp: float) -> np.ndarray:
+    """ha_scalar (module_thermodynamics.f90:873-978) at one nondimensional
+    temperature, in the reference's operation order (powers by repeated
+    products, 1/3 as a product), so that a state built from it equals the
+    reference's to the last bit."""
+    Tinf_i, onethird = 1.0 / thd.Tinf, 1.0 / 3.0
+
+    def h(c, t1):
+        t2 = t1 * t1
+        t3 = t2 * t1
+        t4 = t3 * t1
+        t5 = t4 * t1
+        return (c[0] * t1 + c[1] * t2 * 0.50 + c[2] * t3 * onethird + c[3] * t4 * 0.25
+                + c[4] * t5 * 0.20 + c[5])
+
+    def dh(c, t1):
+        t2 = t1 * t1
+        t3 = t2 * t1
+        t4 = t3 * t1
+        return c[0] + c[1] * t1 + c[2] * t2 + c[3] * t3 + c[4] * t4
+
+    out = np.empty(len(thd.W))
+    t1 = temp * thd.Tinf
+    for s in range(len(thd.W)):
+        lo = [float(x) for x in thd.heatcap[s, 0]]
+        hi = [float(x) for x in thd.heatcap[s, 1]]
+        tmin, tlim, tmax = float(thd.tpolymin[s]), float(thd.tpolylim[s]), float(thd.tpolymax[s])
+        fac = float(thd.Wc_i[s]) * Tinf_i
+        if t1 < tmin:
+            val = fac * (dh(lo, tmin) * (temp * thd.Tinf - tmin) + h(lo, tmin))
+        elif t1 < tlim:
+            val = fac * h(lo, t1)
+        elif t1 < tmax:
+            val = fac * h(hi, t1)
+        else:
+            val = fac * (dh(hi, tmax) * (temp * thd.Tinf - tmax) + h(hi, tmax))
+        out[s] = 0.0 + val
+    return out

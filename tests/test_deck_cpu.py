@@ -74,15 +74,26 @@ def test_syntax_rules():
 
 
 def test_options_refuse_what_is_not_built():
-    from phd_amd.deck import Deck, DeckError, options
+    from phd_amd.deck import (Deck, DeckError, decomposition, face_conditions, library_key,
+                              options)
+    # the reference's Sod deck runs on the 2-species WENO7 flow-only library,
+    # reproducing its 2-rank decomposition along x
     sod = Deck.read(os.path.join(GOLD, "sod.inp"))
-    with pytest.raises(DeckError) as e:
-        options(sod)
-    for what in ("ORDER_WENO = 7", "NRV = 2", "NUMPROCX = 2"):
-        assert what in str(e.value)
-    assert "BCXMIN" not in str(e.value)      # extrapolated faces are built
-    from phd_amd.deck import face_conditions
+    o = options(sod)
+    assert (o.vis, o.vis_dt, o.reaction, o.CFL, o.max_rel_weight) == (1, 0, 0, 0.5, 0.15)
+    assert library_key(sod) == (2, 4) and decomposition(sod) == (2, 1, 1)
     assert face_conditions(sod) == (1, 1, 0, 0, 0, 0)
+    # WENO5 with 2 species has no library; chemistry needs the full one
+    with pytest.raises(DeckError, match="NRV = 2 with ORDER_WENO = 5"):
+        options(Deck.parse("[GENERAL]\n  NRV = 2\n[WENO]\n  ORDER_WENO = 5\n"))
+    with pytest.raises(DeckError, match="REACTION = T with NRV = 9, ORDER_WENO = 7"):
+        options(Deck.parse("[GENERAL]\n  NRV = 9\n[WENO]\n  ORDER_WENO = 7\n"
+                           "[REACTING]\n  REACTION = T\n"))
+    with pytest.raises(DeckError, match="ORDER_WENO = 3"):
+        options(Deck.parse("[GENERAL]\n  NRV = 9\n[WENO]\n  ORDER_WENO = 3\n"))
+    with pytest.raises(DeckError, match="NUMPROCZ = 2"):
+        options(Deck.parse("[GENERAL]\n  DIM = 2\n  NRV = 9\n[WENO]\n  ORDER_WENO = 5\n"
+                           "[GRID]\n  NUMPROCZ = 2\n"))
     with pytest.raises(DeckError, match="BCYMAX"):
         options(Deck.parse("[GRID]\n  BCYMAX = 'NOREFLECTION'\n"))
     ok = Deck.parse("[GENERAL]\n  DIM = 3\n  NRV = 9\n[WENO]\n  ORDER_WENO = 5\n"

@@ -67,3 +67,53 @@ def test_deck_run_and_restart(tmp_path, capsys):
     assert [int(l.split()[0]) for l in lines] == [3, 4]
     t = np.array([float(l.split()[1]) for l in lines])
     assert np.allclose(t, [4e-7, 6e-7], rtol=1e-9, atol=0)
+
+
+def test_reference_sod_deck_reproduces_reference_restarts(tmp_path, capsys):
+    """The reference's own Sod deck (devel-izem/testcases/sod: sod.inp,
+    grid.dat, tranfit.out; NSTAT cut from 30 to 2) through the run driver:
+    ini_sod on the grid file, the 2-species WENO7 library, extrapolated x
+    faces, the deck's 2 ranks along x reproduced on the GPU.  Its restart
+    files 1 and 2 (iterations 647 and 712) against the reference's own, to
+    the bars the oracle meets (tests/test_sod_reference_cpu.py), and its
+    times_stat.out against the reference's."""
+    import shutil
+    from phd_amd import chemdata as cd
+    from phd_amd import restart
+    from phd_amd.run import main
+    from tests import sodcase as sc
+    from tests.test_sod_reference_cpu import ABS_TRANSVERSE, BARS
+    if not os.path.exists(restart.IO_PATH):
+        pytest.skip("libizem_io.so not built (needs the HDF5 C library)")
+    deck = open(os.path.join(sc.GOLDEN, "sod.inp")).read()
+    assert deck.count("NSTAT = 30") == 1
+    (tmp_path / "sod.inp").write_text(deck.replace("NSTAT = 30", "NSTAT = 2"))
+    shutil.copy(sc.GRID, tmp_path / "grid.dat")
+    shutil.copy(sc.TRANFIT, tmp_path / "tranfit.out")
+    assert main([str(tmp_path / "sod.inp")]) == 0
+    out = capsys.readouterr().out
+    assert "entering the main loop (400 x 11 x 1, inert)" in out
+    assert " iteration      712 " in out and " iteration      713 " not in out
+    adi = cd.Adim()
+    thd = sc.sod_thd(adi)
+    box = sc.sod_box(adi)
+    stats = sc.reference_stats()
+    rows = [[float(x) for x in ln.split()] for ln in
+            (tmp_path / "times_stat.out").read_text().splitlines()]
+    assert [int(r[0]) for r in rows] == [1, 2]
+    I = (slice(None),) + box.interior()
+    for k in (1, 2):
+        ours, t = restart.read(str(tmp_path / f"restart_{k:05d}.h5"), box, thd.names)
+        ref, t_ref = sc.read_reference(adi, k, thd)
+        assert t == t_ref
+        rel = {}
+        for l in range(ours.shape[0]):
+            d = np.max(np.abs(ours[I][l] - ref[I][l]))
+            rel[l] = d / np.max(np.abs(ref[I][l])) if l in BARS else d
+        print(f"statistics {k}: " + ", ".join(f"var {l} {e:.1e}" for l, e in rel.items()))
+        for l, bar in BARS.items():
+            assert rel[l] <= bar, (k, l, rel[l])
+        assert rel[2] <= ABS_TRANSVERSE and rel[3] <= ABS_TRANSVERSE
+        # time, dtmin and the four limits / dtmin (9 significant digits)
+        for q in range(1, 7):
+            assert abs(rows[k - 1][q] - stats[k][q - 1]) <= 1e-8 * abs(stats[k][q - 1]), (k, q)

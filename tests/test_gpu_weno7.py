@@ -12,9 +12,12 @@ trajectory (tests/test_sod_reference_cpu.py).
 * The reference's Sod deck on the 2-species variant (extrapolated x faces
   with the closures and their 6th-order points, periodic y), from the state
   the oracle initialised, through the main loop to the first statistics
-  time (647 iterations): against the oracle run as one rank (a GPU has no
-  ranks: the two-rank sensor views of the reference's own run are the
-  oracle's to restate, tests/test_sod_reference_cpu.py).
+  time (647 iterations): against the oracle run undecomposed and run as the
+  reference's two ranks along x (izem_set_decomposition: each rank's sensor
+  sweep and flag view, the face between the ranks formed by both), whose
+  trajectory is the reference's own (tests/test_sod_reference_cpu.py).
+* Decompositions of the 2-D and 3-D cases (2 to 3 ranks per direction):
+  the right-hand side against the oracle's decomposed one.
 """
 from __future__ import annotations
 
@@ -144,20 +147,24 @@ def test_weno7_rhs_rk3_and_steps(oracle, adi, thd, kind, n, mrw, steps, vis):
     s.close()
 
 
-def test_weno7_sod_deck_on_the_gpu(adi):
+@pytest.mark.parametrize("nproc", [1, 2])
+def test_weno7_sod_deck_on_the_gpu(adi, nproc):
     """The reference's Sod deck to its first statistics time on the
-    2-species WENO7 variant, against the oracle run as one rank."""
+    2-species WENO7 variant, against the oracle run with nproc ranks along
+    x."""
     import torch
     from phd_amd import _lib
     from phd_amd.solver import Options, Solver
     from tests import sodcase as sc
+    print(f"sod deck, {nproc} rank(s) along x", flush=True)
     run = sc.SodRun(adi)
-    run.g.nproc[0] = 1
+    run.g.nproc[0] = nproc
     s = Solver(run.box, Options(weno_avg=1, max_rel_weight=sc.PERCENT_WEIGHT, vis=1, vis_dt=0,
                                 reaction=0, forcesum=1, CFL=sc.CFL, CFLmin=sc.CFLMIN, Fo=sc.FO,
                                 CPR=sc.CPR),
                adi=adi, thd=run.thd, lib=_lib.load_variant(2, 4))
     s.set_bc((1, 1, 0, 0, 0, 0))
+    s.set_decomposition((nproc, 1, 1))
     _upload(s, run)
     s.set_time_state([1.0, 1.0, 1.0, 1.0, 1.0, 1.0])
     dt_ptr = s.lib.izem_time_state(s.ctx) + 4 * 8

@@ -91,3 +91,39 @@ def test_sod_trajectory_sees_rank_views(adi, tmp_path):
     run.run_to_stat(1)
     _, rel = _errors(run, adi, 1)
     assert rel[0] > 100 * BARS[0], rel
+
+
+def test_product_sod_state_equals_oracle_state(adi):
+    """phd_amd.ics.sod_state (the run driver's ini_sod) builds the oracle's
+    initial state to the last bit: the same X_to_Y, Wmix_i and ha_scalar
+    operation order, the right state's density recomputed from its
+    temperature."""
+    from oracle.oracle import Oracle
+    from phd_amd import grid, ics
+    o = Oracle(nrv=2)
+    thd, box = sc.sod_thd(adi), sc.sod_box(adi)
+    v0, T0 = sc.initial_state(o, adi, thd, box)
+    assert grid.count_points(sc.GRID) == (sc.NX, sc.NY, sc.NZ)
+    xt, _, _ = grid.read_grid(sc.GRID, sc.NX, sc.NY, sc.NZ)
+    v, T = ics.sod_state(adi, thd, box, xt)
+    assert np.array_equal(v, v0) and np.array_equal(T, T0)
+    I = box.interior()
+    rho = v[(0,) + I][0, 0]
+    assert rho[0] == 1.0 and abs(rho[-1] - 0.125) < 1e-15 and (rho == 1.0).sum() == sc.NX // 2
+
+
+def test_host_ha_scalar_equals_oracle(adi):
+    """phd_amd.thermo_host.ha_scalar against the oracle's ha_scalar, bit for
+    bit, below Tpolymin, in both NASA ranges and above Tpolymax."""
+    import ctypes as C
+    from oracle.oracle import Oracle
+    from phd_amd.thermo_host import ha_scalar
+    o = Oracle(nrv=2)
+    thd = sc.sod_thd(adi)
+    t = o.thd_struct(thd)
+    for TK in (150.0, 299.0, 300.0, 800.0, 999.9, 1000.0, 2500.0, 4999.0, 5000.0, 7000.0):
+        Tn = TK / adi.T_ref
+        ref = np.zeros(2)
+        o.lib.ora_ha_scalar(C.byref(t), C.c_double(Tn), ref.ctypes.data_as(C.POINTER(C.c_double)))
+        assert np.array_equal(ha_scalar(thd, Tn), ref), TK
+
