@@ -43,6 +43,14 @@ static void domain_select(const ora_grid *g, int d, int *i0, int *i1, int *j0, i
     case 2: *j0 = ny; *j1 = ny + ng - 1; break;
     case -3: *k0 = -ng; *k1 = -1; break;
     case 3: *k0 = nz; *k1 = nz + ng - 1; break;
+    /* the Dirichlet rows: the first interior layer of a face
+     * (module_parallel.f90:1946-1957, 1983-1994) */
+    case -10: *i1 = 0; break;
+    case 10: *i0 = nx - 1; break;
+    case -20: *j1 = 0; break;
+    case 20: *j0 = ny - 1; break;
+    case -30: *k1 = 0; break;
+    case 30: *k0 = nz - 1; break;
     default: break;
   }
 }
@@ -824,6 +832,26 @@ void ora_upd_boundaries(const ora_grid *g, double *v) {
   const int nx = g->nx, ny = g->ny, nz = g->nz, ng = g->ng;
   const long nc = ncell(g);
   for (int f = 0; f < 2 * g->ndim; f++) {
+    if (g->bc[f] == ORA_BC_POSTSHOCK_SLIPWALL && f == 2) {
+      /* bc_postshock_slipwall (src/module_BCs.f90:1065-1172), face S only:
+       * on x(i) <= x0 (the first nfix cells) the post-shock state on the
+       * ghost rows and on the first interior row; beyond, the ghost rows
+       * mirror the rows above the first one (rho v odd) */
+      for (int k = 0; k < nz; k++)
+        for (int i = 0; i < nx; i++) {
+          if (i < g->nfix) {
+            for (int j = -ng; j <= 0; j++)
+              for (int l = 0; l < NV; l++) v[l * nc + cidx(g, i, j, k)] = g->wall[l];
+          } else {
+            for (int s = 1; s <= ng; s++)
+              for (int l = 0; l < NV; l++) {
+                const double a = v[l * nc + cidx(g, i, s, k)];
+                v[l * nc + cidx(g, i, -s, k)] = (l == 2) ? -a : a;
+              }
+          }
+        }
+      continue;
+    }
     if (g->bc[f] != ORA_BC_EXTRAPOLATION) continue;
     const int d = f / 2, hi = f % 2;
     for (int l = 0; l < NV; l++) {
@@ -851,6 +879,23 @@ void ora_upd_boundaries(const ora_grid *g, double *v) {
             }
     }
   }
+}
+
+/* the primitives of the Dirichlet row of each face whose condition writes
+ * the first interior row (upd_prim_var_dirichlet after the face's
+ * condition, src/module_BCs.f90:93-96, 131, 297-314): force_sumY and
+ * prim_inv_var on that row */
+int ora_upd_dirichlet(const ora_grid *g, const ora_thd *thd, int forcesum, double *v,
+                      double *W_i, double *T, double *cp, double *ha) {
+  static const int row[6] = {-10, 10, -20, 20, -30, 30};
+  int st = 0;
+  for (int f = 0; f < 2 * g->ndim; f++) {
+    if (g->bc[f] != ORA_BC_POSTSHOCK_SLIPWALL) continue;
+    if (forcesum) ora_force_sumY(row[f], g, v);
+    int s = ora_prim_inv_var(row[f], g, thd, v, W_i, T, cp, ha, NULL);
+    if (s && !st) st = s;
+  }
+  return st;
 }
 
 /* upd_prim_var_ghost: src/module_BCs.f90:254-284 (faces -1,1,-2,2,-3,3) */
@@ -1302,6 +1347,8 @@ int ora_rk3(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const ora
     if (s && !status) status = s;
     ora_comm_cons(g, v);
     ora_upd_boundaries(g, v);
+    s = ora_upd_dirichlet(g, thd, op->forcesum, v, W_i, T, cp, ha);
+    if (s && !status) status = s;
     s = ora_upd_prim_var_ghost(g, thd, op->forcesum, v, W_i, T, cp, ha);
     if (s && !status) status = s;
   }

@@ -34,17 +34,22 @@
 
 /* ng = 3 (WENO5) or 4 (WENO7): ng = (ORDER_WENO + 1) / 2,
  * src/module_input.f90:712.  bc[face] for the faces W, E, S, N, B, F:
- * ORA_BC_PERIODIC or ORA_BC_EXTRAPOLATION (src/module_BCs.f90:41-141,
- * 327-412).  nproc[d]: MPI ranks along direction d in the run being
+ * ORA_BC_PERIODIC, ORA_BC_EXTRAPOLATION (src/module_BCs.f90:41-141,
+ * 327-412) or, on face S, ORA_BC_POSTSHOCK_SLIPWALL (:1065-1172).  nproc[d]: MPI ranks along direction d in the run being
  * restated (0 or 1: one); only the shock sensor depends on it, because its
  * sweep stops one cell short of each rank's end (src/module_tools.f90:
  * 2758-2770). */
 #define ORA_BC_PERIODIC 0
 #define ORA_BC_EXTRAPOLATION 1
+#define ORA_BC_POSTSHOCK_SLIPWALL 2
 typedef struct {
   int nx, ny, nz, ng, ndim;
   int bc[6];
   int nproc[3];
+  /* ORA_BC_POSTSHOCK_SLIPWALL: the conserved post-shock state (NV values)
+   * and the cells of the face that take it */
+  int nfix;
+  const double *wall;
 } ora_grid;
 
 /* thd_type subset used by the flow kernels (src/thd/type_thd.f90:10-91) */
@@ -123,9 +128,12 @@ double ora_weno7_beta(int r, const double *q);
  * 212-469) */
 double ora_deriv_line(const double *a, long stride, double dinv, int p, int n, int ng,
                       int lphys, int rphys);
-/* bc_extrapolation on every face whose bc is ORA_BC_EXTRAPOLATION, after
- * ora_comm_cons (upd_boundaries, src/module_BCs.f90:41-141) */
+/* the conditions of the non-periodic faces, after ora_comm_cons
+ * (upd_boundaries, src/module_BCs.f90:41-141); ora_upd_dirichlet then
+ * updates the primitives of the rows a condition wrote (:131) */
 void ora_upd_boundaries(const ora_grid *g, double *v);
+int ora_upd_dirichlet(const ora_grid *g, const ora_thd *thd, int forcesum, double *v,
+                      double *W_i, double *T, double *cp, double *ha);
 
 void ora_eigvecs(int dir, const ora_thd *thd, const double *uc, const double *Yc,
                  const double *psi_c, double ht_c, double cs_c, double gam_c,

@@ -26,14 +26,18 @@ _dp = C.POINTER(C.c_double)
 _u8 = C.POINTER(C.c_uint8)
 
 
-BC_PERIODIC, BC_EXTRAPOLATION = 0, 1
+BC_PERIODIC, BC_EXTRAPOLATION, BC_POSTSHOCK_SLIPWALL = 0, 1, 2
 
 
 class Grid(C.Structure):
-    """ora_grid: bc per face W, E, S, N, B, F (zero: periodic) and the MPI
-    ranks per direction of the run being restated (zero: one)."""
+    """ora_grid: bc per face W, E, S, N, B, F (zero: periodic), the MPI
+    ranks per direction of the run being restated (zero: one), and for a
+    post-shock slip wall the cells of the face that take the post-shock
+    state and that state (a pointer to NV doubles, kept alive by the
+    caller)."""
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("ng", C.c_int),
-                ("ndim", C.c_int), ("bc", C.c_int * 6), ("nproc", C.c_int * 3)]
+                ("ndim", C.c_int), ("bc", C.c_int * 6), ("nproc", C.c_int * 3),
+                ("nfix", C.c_int), ("wall", C.c_void_p)]
 
 
 def _thd_struct(nrv: int):
@@ -151,6 +155,7 @@ class Oracle:
         L.ora_prim_inv_var.restype = C.c_int
         L.ora_rk3.restype = C.c_int
         L.ora_upd_prim_var_ghost.restype = C.c_int
+        L.ora_upd_dirichlet.restype = C.c_int
         L.ora_wmix_i_scalar.restype = C.c_double
         L.ora_cp_scalar.restype = C.c_double
         L.ora_ckcvbs.restype = C.c_double

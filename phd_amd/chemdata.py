@@ -127,7 +127,9 @@ def read_therm(path: str | None = None) -> tuple[dict, tuple[float, float, float
             row = lines[i + q]
             n = 5 if q < 3 else 4
             c += [float(row[15 * p:15 * p + 15].replace("D", "E")) for p in range(n)]
-        out[name.upper()] = NasaEntry(name, tlo, thi, tmid, c[0:7], c[7:14])
+        # the first entry of a species is the one read (the reference's reader
+        # stops at the first match, thd_tools_perfectgas.f90:409-440)
+        out.setdefault(name.upper(), NasaEntry(name, tlo, thi, tmid, c[0:7], c[7:14]))
         i += 4
     return out, defaults
 

@@ -798,7 +798,7 @@ static int reaction_tail(izem_ctx *ctx, double *T, double *W_i, double *cp, doub
                          double *v) {
   int r = iz_upd_prim_woT(ctx, 0, v, T, W_i, cp, ha);
   if (r) return r;
-  r = iz_comm_cons(ctx, v, IZ_NV);
+  r = iz_comm_cons(ctx, v, IZ_NV, 1);
   if (r) return r;
   return iz_ghost_newton(ctx, T, W_i, cp, ha, v);
 }

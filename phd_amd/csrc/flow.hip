@@ -965,7 +965,13 @@ __global__ void k_upd_prim_woT(IzDims d, IzBox b, const double *__restrict__ v,
 // (module_parallel.f90:1715-1777) or, on a physical face, upd_boundaries'
 // bc_extrapolation (module_BCs.f90:327-412: the ng ghost layers take the
 // edge value).  Threads cover the 2*ng slabs of direction dir.
-__global__ void k_ghost_fill(IzDims d, double *__restrict__ a, int nf, int dir) {
+// wall (the conserved variables, nf = IZ_NV, with face S a post-shock slip
+// wall): bc_postshock_slipwall (module_BCs.f90:1065-1172) on that face: the
+// first nfix cells take the post-shock state on the ghost rows and on the
+// first interior row, the others mirror rows 1..ng (rho v odd).
+// bcs = 0: the periodic exchange alone (comm_cons), physical faces untouched.
+__global__ void k_ghost_fill(IzDims d, double *__restrict__ a, int nf, int dir,
+                             const double *__restrict__ wall, int bcs) {
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int n0, n1, n;  // transverse sizes and the direction length
   if (dir == 0) { n0 = d.ny; n1 = d.nz; n = d.nx; }
@@ -978,6 +984,17 @@ __global__ void k_ghost_fill(IzDims d, double *__restrict__ a, int nf, int dir) 
   int s = (int)(r % (2 * d.ng));
   r /= (2 * d.ng);
   int p0 = (int)(r % n0), p1 = (int)(r / n0);
+  if (!bcs && ((d.phys >> (2 * dir + (s < d.ng ? 0 : 1))) & 1)) return;
+  if (wall && dir == 1 && s < d.ng) {
+    if (p0 < d.nfix) {
+      a[f * d.nc + iz_cidx(d, p0, -1 - s, p1)] = wall[f];
+      if (s == 0) a[f * d.nc + iz_cidx(d, p0, 0, p1)] = wall[f];
+    } else {
+      const double val = a[f * d.nc + iz_cidx(d, p0, 1 + s, p1)];
+      a[f * d.nc + iz_cidx(d, p0, -1 - s, p1)] = (f == 2) ? -val : val;
+    }
+    return;
+  }
   int dst, src;
   if (s < d.ng) {
     dst = -1 - s;
@@ -1638,10 +1655,21 @@ static IzBox domain_box(const IzDims &d, int dom) {
     case 2: b.j0 = d.ny; b.j1 = d.ny + d.ng - 1; break;
     case -3: b.k0 = -d.ng; b.k1 = -1; break;
     case 3: b.k0 = d.nz; b.k1 = d.nz + d.ng - 1; break;
+    // the Dirichlet rows: a face's first interior layer
+    // (module_parallel.f90:1946-1957, 1983-1994)
+    case -10: b.i1 = 0; break;
+    case 10: b.i0 = d.nx - 1; break;
+    case -20: b.j1 = 0; break;
+    case 20: b.j0 = d.ny - 1; break;
+    case -30: b.k1 = 0; break;
+    case 30: b.k0 = d.nz - 1; break;
     default: break;
   }
   return b;
 }
+// probe-mask and iteration-count slot of a domain (slot 7 serves the
+// Dirichlet rows, one per call)
+static int mask_slot(int dom) { return (dom <= -10 || dom >= 10) ? 7 : dom + 3; }
 static long box_count(const IzBox &b) {
   return (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
 }
@@ -1773,6 +1801,7 @@ extern "C" int izem_destroy(izem_ctx *c) {
   iz_chem_destroy(c);
   for (int q = 0; q < 3; q++) if (c->F[q]) (void)hipFree(c->F[q]);
   for (int q = 0; q < 3; q++) if (c->Fs[q]) (void)hipFree(c->Fs[q]);
+  if (c->wall_state) (void)hipFree(c->wall_state);
   if (c->prim) (void)hipFree(c->prim);
   if (c->fd) (void)hipFree(c->fd);
   if (c->sf) (void)hipFree(c->sf);
@@ -1799,10 +1828,17 @@ extern "C" int izem_flux_mode(izem_ctx *c, int mode) {
 // face without a neighbour, module_BCs.f90:327-412, module_deriv.f90:268-469)
 extern "C" int izem_set_bc(izem_ctx *c, const int bc[6]) {
   if (!c || !bc) return IZEM_E_ARG;
-  int phys = 0;
+  int phys = 0, wall = 0;
   for (int f = 0; f < 6; f++) {
-    if (bc[f] != IZEM_BC_PERIODIC && bc[f] != IZEM_BC_EXTRAPOLATION) return IZEM_E_ARG;
-    if (bc[f] == IZEM_BC_EXTRAPOLATION) {
+    if (bc[f] != IZEM_BC_PERIODIC && bc[f] != IZEM_BC_EXTRAPOLATION &&
+        bc[f] != IZEM_BC_POSTSHOCK_SLIPWALL)
+      return IZEM_E_ARG;
+    // the post-shock slip wall is declared for face S only (module_BCs.f90:1163-1168)
+    if (bc[f] == IZEM_BC_POSTSHOCK_SLIPWALL) {
+      if (f != 2) return IZEM_E_ARG;
+      wall = 1;
+    }
+    if (bc[f] != IZEM_BC_PERIODIC) {
       if (f / 2 >= c->d.ndim) return IZEM_E_ARG;
       phys |= 1 << f;
     }
@@ -1810,10 +1846,27 @@ extern "C" int izem_set_bc(izem_ctx *c, const int bc[6]) {
   // a direction is periodic on both faces or on neither
   for (int dir = 0; dir < 3; dir++)
     if (((phys >> (2 * dir)) & 3) == 1 || ((phys >> (2 * dir)) & 3) == 2) return IZEM_E_ARG;
-  if (phys != c->d.phys) {
+  if (phys != c->d.phys || wall != c->d.wall) {
     c->d.phys = phys;
+    c->d.wall = wall;
     iz_graph_destroy(c);  // a captured step holds the old dimensions
   }
+  return IZEM_OK;
+}
+
+extern "C" int izem_set_wall_state(izem_ctx *c, const double *state, int nfix) {
+  if (!c || !state || nfix < 0 || nfix > c->d.nx) return IZEM_E_ARG;
+  if (!c->wall_state &&
+      hipMalloc(&c->wall_state, sizeof(double) * IZ_NV) != hipSuccess) {
+    c->wall_state = nullptr;
+    return IZEM_E_NOMEM;
+  }
+  IZ_CHECK(hipMemcpyAsync(c->wall_state, state, sizeof(double) * IZ_NV, hipMemcpyHostToDevice,
+                          c->stream));
+  IZ_CHECK(hipStreamSynchronize(c->stream));
+  if (nfix != c->d.nfix) iz_graph_destroy(c);
+  c->d.nfix = nfix;
+  c->wall_set = 1;
   return IZEM_OK;
 }
 
@@ -2007,14 +2060,16 @@ extern "C" int izem_euler_llf_z(izem_ctx *c, const double *dz_i, const double *T
   return euler_dir(c, 2, dz_i, T, W_i, cp, ha, sensor, v, fl, true);
 }
 
-int iz_comm_cons(izem_ctx *c, double *a, int nf) {
+int iz_comm_cons(izem_ctx *c, double *a, int nf, int bcs) {
   const IzDims &d = c->d;
+  if (bcs && d.wall && (!c->wall_set || nf != IZ_NV)) return IZEM_E_STATE;
+  const double *wall = (bcs && d.wall) ? c->wall_state : nullptr;
   int pk = iz_prof_begin(c, IZP_OTHER);
   for (int dir = 0; dir < d.ndim; dir++) {
     long n0 = (dir == 0) ? d.ny : d.nx, n1 = (dir == 2) ? d.ny : d.nz;
     long tot = n0 * n1 * 2 * d.ng * nf;
     hipLaunchKernelGGL(k_ghost_fill, dim3(iz_blocks(tot, 256)), dim3(256), 0, c->stream, d, a, nf,
-                       dir);
+                       dir, wall, bcs);
   }
   iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
@@ -2023,7 +2078,12 @@ int iz_comm_cons(izem_ctx *c, double *a, int nf) {
 
 extern "C" int izem_comm_cons(izem_ctx *c, double *v) {
   if (!c || !v) return IZEM_E_ARG;
-  return iz_comm_cons(c, v, IZ_NV);
+  return iz_comm_cons(c, v, IZ_NV, 1);
+}
+
+extern "C" int izem_comm_cons_exchange(izem_ctx *c, double *v) {
+  if (!c || !v) return IZEM_E_ARG;
+  return iz_comm_cons(c, v, IZ_NV, 0);
 }
 
 // viscous fluxes into the context scratch (prim, fd, sf)
@@ -2067,7 +2127,10 @@ extern "C" int izem_viscflux(izem_ctx *c, const double *dx_i, const double *dy_i
   return IZEM_OK;
 }
 
-static int dom_ok(const IzDims &d, int dom) { return dom >= -d.ndim && dom <= d.ndim; }
+static int dom_ok(const IzDims &d, int dom) {
+  if (dom % 10 == 0 && dom != 0) dom /= 10;
+  return dom >= -d.ndim && dom <= d.ndim;
+}
 
 extern "C" int izem_force_sumY(izem_ctx *c, int dom, double *v) {
   if (!c || !v || !dom_ok(c->d, dom)) return IZEM_E_ARG;
@@ -2093,17 +2156,17 @@ static int newton_domains(izem_ctx *c, const int *doms, int nd, int forcesum, do
   for (int q = 0; q < nd; q++) {
     IzBox b = domain_box(d, doms[q]);
     hipLaunchKernelGGL(k_force_probe, dim3(iz_blocks_capped(c, box_count(b), 256, 16)), dim3(256), 0,
-                       c->stream, d, b, forcesum, v, T, eps, c->masks + (doms[q] + 3));
+                       c->stream, d, b, forcesum, v, T, eps, c->masks + mask_slot(doms[q]));
   }
   for (int q = 0; q < nd; q++) {
     IzBox b = domain_box(d, doms[q]);
     hipLaunchKernelGGL(k_newton_apply, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0,
-                       c->stream, d, b, v, T, W_i, cp, ha, c->masks + (doms[q] + 3), itmax,
-                       !mach, c->status, c->iters + (doms[q] + 3));
+                       c->stream, d, b, v, T, W_i, cp, ha, c->masks + mask_slot(doms[q]), itmax,
+                       !mach, c->status, c->iters + mask_slot(doms[q]));
     if (!mach)
       hipLaunchKernelGGL(k_newton_repair, dim3(1), dim3(1024), 0, c->stream, d, b, v, T, W_i,
-                         cp, ha, c->masks + (doms[q] + 3), itmax, eps, c->negT_rec, c->status,
-                         c->iters + (doms[q] + 3));
+                         cp, ha, c->masks + mask_slot(doms[q]), itmax, eps, c->negT_rec,
+                         c->status, c->iters + mask_slot(doms[q]));
   }
   iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
@@ -2116,7 +2179,7 @@ extern "C" int izem_prim_inv_var(izem_ctx *c, int dom, double *v, double *W_i, d
   int r = newton_domains(c, &dom, 1, 0, v, W_i, T, cp, ha);
   if (r) return r;
   if (iters) {
-    IZ_CHECK(hipMemcpyAsync(iters, c->iters + dom + 3, sizeof(int), hipMemcpyDeviceToHost,
+    IZ_CHECK(hipMemcpyAsync(iters, c->iters + mask_slot(dom), sizeof(int), hipMemcpyDeviceToHost,
                             c->stream));
     IZ_CHECK(hipStreamSynchronize(c->stream));
   }
@@ -2129,7 +2192,7 @@ extern "C" int izem_prim_inv_var_mach(izem_ctx *c, int dom, double *v, double *W
   int r = newton_domains(c, &dom, 1, 0, v, W_i, T, cp, ha, 1);
   if (r) return r;
   if (iters) {
-    IZ_CHECK(hipMemcpyAsync(iters, c->iters + dom + 3, sizeof(int), hipMemcpyDeviceToHost,
+    IZ_CHECK(hipMemcpyAsync(iters, c->iters + mask_slot(dom), sizeof(int), hipMemcpyDeviceToHost,
                             c->stream));
     IZ_CHECK(hipStreamSynchronize(c->stream));
   }
@@ -2153,8 +2216,16 @@ int iz_upd_prim_woT(izem_ctx *c, int dom, const double *v, const double *T, doub
   return IZEM_OK;
 }
 
+// upd_prim_var_dirichlet of a post-shock slip wall (its condition wrote the
+// first interior row: force_sumY and prim_inv_var there, module_BCs.f90:
+// 93-96, 131, 297-314), then upd_prim_var_ghost on the ghost slabs
 int iz_ghost_newton(izem_ctx *c, double *T, double *W_i, double *cp, double *ha, double *v) {
   static const int faces[6] = {-1, 1, -2, 2, -3, 3};
+  if (c->d.wall) {
+    static const int row = -20;
+    int r = newton_domains(c, &row, 1, c->o.forcesum, v, W_i, T, cp, ha);
+    if (r) return r;
+  }
   return newton_domains(c, faces, 2 * c->d.ndim, c->o.forcesum, v, W_i, T, cp, ha);
 }
 
@@ -2252,7 +2323,7 @@ extern "C" int izem_rk3(izem_ctx *c, const double *dt, const double *dx_i, const
                        ha, c->masks + 3, 200, 1.0e-5, c->negT_rec, c->status, c->iters + 3);
     iz_prof_end(c, pk);
     IZ_CHECK(hipGetLastError());
-    r = iz_comm_cons(c, v, IZ_NV);
+    r = iz_comm_cons(c, v, IZ_NV, 1);
     if (r) return r;
     r = iz_ghost_newton(c, T, W_i, cp, ha, v);
     if (r) return r;

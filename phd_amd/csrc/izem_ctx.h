@@ -55,6 +55,10 @@ struct izem_ctx {
   // 14 x nseam per direction
   double *Fs[3];
   long nseam[3];
+  // the post-shock state of a slip-wall face (IZ_NV doubles) and whether
+  // it was given
+  double *wall_state;
+  int wall_set;
   // viscous scratch: velocities + mole fractions, gradients / diffusion
   // fluxes, stresses + heat flux
   double *prim, *fd, *sf;
@@ -99,7 +103,9 @@ static inline unsigned iz_blocks_capped(const izem_ctx *c, long n, int bs, int p
 
 // internal entry points across translation units
 int iz_ghost_newton(izem_ctx *ctx, double *T, double *W_i, double *cp, double *ha, double *v);
-int iz_comm_cons(izem_ctx *ctx, double *a, int nf);
+// bcs = 1: the periodic exchange and the non-periodic faces' conditions
+// (comm_cons + upd_boundaries); 0: the exchange alone
+int iz_comm_cons(izem_ctx *ctx, double *a, int nf, int bcs);
 int iz_upd_prim_woT(izem_ctx *ctx, int domain, const double *v, const double *T, double *W_i,
                     double *cp, double *ha);
 void iz_chem_destroy(izem_ctx *ctx);

@@ -214,12 +214,13 @@ SENSORS = {"ADAMSSHARIFF": 0, "DUCROS": 1}
 
 BC_KEYS = ("GRID%BCXMIN", "GRID%BCXMAX", "GRID%BCYMIN", "GRID%BCYMAX", "GRID%BCZMIN",
            "GRID%BCZMAX")
-BCS = {"PERIODIC": 0, "EXTRAPOLATION": 1}
+BCS = {"PERIODIC": 0, "EXTRAPOLATION": 1, "POSTSHOCK_SLIPWALL": 2}
 
 
 def face_conditions(deck: Deck) -> tuple:
     """The six face conditions W, E, S, N, B, F for izem_set_bc (0 periodic,
-    1 extrapolation); faces beyond GENERAL%DIM are periodic."""
+    1 extrapolation, 2 post-shock slip wall); faces beyond GENERAL%DIM are
+    periodic."""
     dim = deck.get("GENERAL%DIM")
     return tuple(BCS[deck.get(k).upper()] if f // 2 < dim else 0 for f, k in enumerate(BC_KEYS))
 
@@ -267,8 +268,15 @@ def options(deck: Deck, **overrides):
         if g(key):
             refuse.append(f"{key} = T")
     for key in BC_KEYS:
-        if g(key).upper() not in BCS:
-            refuse.append(f"{key} = {g(key)} (periodic or extrapolated faces only)")
+        bc = g(key).upper()
+        if bc not in BCS:
+            refuse.append(f"{key} = {g(key)} (periodic, extrapolated or, on BCYMIN, "
+                          f"post-shock slip-wall faces only)")
+        elif bc == "POSTSHOCK_SLIPWALL" and key != "GRID%BCYMIN":
+            refuse.append(f"{key} = {g(key)} (declared for BCYMIN only, module_BCs.f90:1163-1168)")
+        elif bc == "POSTSHOCK_SLIPWALL" and g("GENERAL%INIT").upper() != "DMR":
+            refuse.append(f"{key} = {g(key)} (its state is the double Mach reflection's: "
+                          f"INIT = 'DMR')")
     for q, key in enumerate(("GRID%NUMPROCX", "GRID%NUMPROCY", "GRID%NUMPROCZ")):
         if g(key) < 1 or (q >= g("GENERAL%DIM") and g(key) != 1):
             refuse.append(f"{key} = {g(key)}")

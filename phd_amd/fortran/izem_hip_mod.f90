@@ -45,7 +45,8 @@ module izem_hip_mod
             izem_time_state, izem_get_time_state, izem_set_time_state, izem_profile, &
             izem_profile_read, izem_wenorec, izem_euler_llf_x, izem_euler_llf_y, &
             izem_euler_llf_z, izem_viscflux, izem_force_sumY, izem_prim_inv_var, &
-            izem_upd_prim_var_woT, izem_comm_cons, izem_upd_prim_var_ghost, &
+            izem_upd_prim_var_woT, izem_comm_cons, izem_comm_cons_exchange, &
+            izem_upd_prim_var_ghost, &
             izem_shock_sensor, izem_rk3, izem_rhs, izem_timestep, izem_chem_setup, &
             izem_chem_tune, izem_chem_stats, izem_ckwyr, izem_psr_batch, izem_reaction_dvode, &
             izem_chem_order
@@ -55,8 +56,9 @@ module izem_hip_mod
   public :: hip_create, hip_chem_init, hip_dtmin
 
   ! face conditions of izem_set_bc (GRID%BCXMIN .. BCZMAX)
-  integer(c_int), parameter, public :: IZEM_BC_PERIODIC = 0, IZEM_BC_EXTRAPOLATION = 1
-  public :: izem_set_bc
+  integer(c_int), parameter, public :: IZEM_BC_PERIODIC = 0, IZEM_BC_EXTRAPOLATION = 1, &
+                                       IZEM_BC_POSTSHOCK_SLIPWALL = 2
+  public :: izem_set_bc, izem_set_wall_state
   ! ranks of a decomposed run to reproduce (GRID%NUMPROCX/Y/Z)
   public :: izem_set_decomposition
 
@@ -77,6 +79,13 @@ module izem_hip_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), intent(in) :: bc(6)
+    end function
+    integer(c_int) function izem_set_wall_state(ctx, state, nfix) &
+        bind(C, name="izem_set_wall_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: state(*)
+      integer(c_int), value :: nfix
     end function
     integer(c_int) function izem_set_decomposition(ctx, nproc) &
         bind(C, name="izem_set_decomposition")
@@ -169,6 +178,11 @@ module izem_hip_mod
       integer(c_int), value :: domain
     end function
     integer(c_int) function izem_comm_cons(ctx, v) bind(C, name="izem_comm_cons")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, v
+    end function
+    integer(c_int) function izem_comm_cons_exchange(ctx, v) &
+        bind(C, name="izem_comm_cons_exchange")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, v
     end function

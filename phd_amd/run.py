@@ -14,15 +14,18 @@ limits are appended to times_stat.out and restart_<k>.h5 is written
 statistics time.  ITSHOW sets how often the progress line is printed.
 
 Initial states (GENERAL%INIT): SOD, the reference's ini_sod
-(module_ICs.f90:2715-2881) on the grid of the deck directory's grid.dat, and
-three states of this project's own: HOTSPOT (2-D periodic 1 cm box,
+(module_ICs.f90:2715-2881), and DMR, its double Mach reflection ini_DMR
+(:265-412, with the post-shock slip wall of BCYMIN = POSTSHOCK_SLIPWALL),
+both on the grid of the deck directory's grid.dat, and three states of this
+project's own: HOTSPOT (2-D periodic 1 cm box,
 stoichiometric H2/air, 900 K with a 1500 K Gaussian hot spot, u_rms 10 m/s),
 HIT (3-D periodic 1 cm box, phi = 0.8, 1000 K +- 50 K, u_rms 20 m/s) and
 SHEARLAYER2 (3-D periodic double H2/N2 - air shear layer, 8 mm box).
 
-The species set is the deck directory's tranfit.out when there is one (as
-the reference reads its run directory's), the bundled 9-species H2/air set
-otherwise; GENERAL%NRV and WENO%ORDER_WENO pick the library (the full one
+The species set is the deck directory's, as the reference reads its run
+directory: its therm.dat (else the bundled one), with the transport fits of
+its tranfit.out, or fitted from its chem.inp and tran.dat as TRANFIT fits
+them (phd_amd.tranfit); the bundled 9-species H2/air set otherwise; GENERAL%NRV and WENO%ORDER_WENO pick the library (the full one
 for 9 species and WENO5, a flow-only build otherwise), and a deck
 decomposed over GRID%NUMPROCX/Y/Z ranks is reproduced on the one GPU
 (izem_set_decomposition).
@@ -39,23 +42,48 @@ import numpy as np
 # GENERAL%INIT -> (dimension, box side [m]; None: the grid of grid.dat); a
 # restart file holds no grid, so a run read from one takes the grid of its
 # deck's INIT
-INITS = {"HOTSPOT": (2, 0.01), "HIT": (3, 0.01), "SHEARLAYER2": (3, 0.008), "SOD": (2, None)}
+INITS = {"HOTSPOT": (2, 0.01), "HIT": (3, 0.01), "SHEARLAYER2": (3, 0.008), "SOD": (2, None),
+         "DMR": (2, None)}
 
 
 def initial_state(deck, adi, thd, n: int, grid_path: str | None = None):
     from . import ics
     init = _init(deck)
-    if init == "SOD":
+    if init in ("SOD", "DMR"):
         from . import grid
         box = _grid_box(deck, adi, grid_path)
-        xt, _, _ = grid.read_grid(grid_path, box.nx, box.ny, box.nz)
-        v, T = ics.sod_state(adi, thd, box, xt)
+        xt, yt, _ = grid.read_grid(grid_path, box.nx, box.ny, box.nz)
+        if init == "SOD":
+            v, T = ics.sod_state(adi, thd, box, xt)
+        else:
+            from .shock import dmr_initial_state
+            v, T = dmr_initial_state(adi, thd, box, xt, yt)
         return box, v, T
     if init == "HOTSPOT":
         return ics.config1_state(adi, thd, n=n)
     if init == "HIT":
         return ics.config2_state(adi, thd, n=n, L=INITS[init][1])
     return ics.config5_state(adi, thd, n=n, L=INITS[init][1])
+
+
+def species_set(here: str, adi):
+    """The thermodynamic and transport tables of a run directory (thd_init
+    reads the run directory's files): therm.dat there or the bundled one;
+    the transport fits of tranfit.out there, or fitted from chem.inp and
+    tran.dat there; the bundled 9-species set when the directory has
+    neither."""
+    from . import chemdata as cd
+    therm = os.path.join(here, "therm.dat")
+    therm = therm if os.path.exists(therm) else None
+    tranfit = os.path.join(here, "tranfit.out")
+    if os.path.exists(tranfit):
+        return cd.build_thd(adi, therm_path=therm, tranfit_path=tranfit)
+    inp, tran = os.path.join(here, "chem.inp"), os.path.join(here, "tran.dat")
+    if os.path.exists(inp) and os.path.exists(tran):
+        from . import tranfit as tf
+        mech = cd.read_mechanism(inp, therm)
+        return cd.build_thd(adi, therm_path=therm, fits=tf.fit_transport(mech, tran, therm))
+    return cd.build_thd(adi, therm_path=therm)
 
 
 def _grid_box(deck, adi, grid_path: str):
@@ -100,8 +128,7 @@ def main(argv=None):
     out = args.dir or here
     os.makedirs(out, exist_ok=True)
     adi = cd.Adim()
-    tranfit = os.path.join(here, "tranfit.out")
-    thd = cd.build_thd(adi, tranfit_path=tranfit if os.path.exists(tranfit) else None)
+    thd = species_set(here, adi)
     g = deck.get
     if len(thd.W) != g("GENERAL%NRV"):
         raise SystemExit(f"GENERAL%NRV = {g('GENERAL%NRV')} but the species set has "
@@ -114,7 +141,13 @@ def main(argv=None):
 
     def solver(box):
         s = Solver(box, opts, adi=adi, thd=thd, mech=mech, lib=lib)
-        s.set_bc(face_conditions(deck))
+        faces = face_conditions(deck)
+        s.set_bc(faces)
+        if 2 in faces:
+            # the post-shock slip wall of the double Mach reflection
+            from . import grid, shock
+            xt, _, _ = grid.read_grid(grid_path, box.nx, box.ny, box.nz)
+            s.set_wall_state(shock.dmr_wall_state(adi, thd), shock.dmr_wall_cells(adi, xt))
         s.set_decomposition(decomposition(deck))
         return s
 
@@ -142,7 +175,7 @@ def main(argv=None):
     else:
         box, v, T = initial_state(deck, adi, thd, args.n, grid_path)
         s = solver(box)
-        if _init(deck) == "SOD":
+        if _init(deck) in ("SOD", "DMR"):
             s.load_conserved(v, T)
         else:
             s.load_state(v, T)

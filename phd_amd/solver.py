@@ -114,9 +114,19 @@ class Solver:
 
     def set_bc(self, bc) -> None:
         """Face conditions W, E, S, N, B, F (GRID%BCXMIN .. BCZMAX): 0 periodic,
-        1 extrapolation (izem_set_bc)."""
+        1 extrapolation, 2 post-shock slip wall on S (izem_set_bc)."""
         a = (C.c_int * 6)(*[int(x) for x in bc])
         _lib.check(self.lib.izem_set_bc(self.ctx, a), "izem_set_bc")
+
+    def set_wall_state(self, state, nfix: int) -> None:
+        """The post-shock state (5 + nspc conserved values) of a
+        post-shock slip-wall face and the number of its cells that take it
+        (izem_set_wall_state)."""
+        a = np.ascontiguousarray(state, dtype=np.float64)
+        if a.size != 5 + len(self.thd.W):
+            raise ValueError(f"the wall state has {a.size} values, not {5 + len(self.thd.W)}")
+        _lib.check(self.lib.izem_set_wall_state(self.ctx, a.ctypes.data, int(nfix)),
+                   "izem_set_wall_state")
 
     def set_decomposition(self, nproc) -> None:
         """Reproduce a run decomposed into nproc[0] x nproc[1] x nproc[2] ranks
@@ -196,10 +206,16 @@ class Solver:
             I = self.box.interior()
             Tg[I] = np.asarray(T)[I]
             self.T.copy_(torch.from_numpy(Tg))
-        _lib.check(self.lib.izem_comm_cons(self.ctx, _ptr(self.v)), "comm_cons")
-        self.prim_inv_var_mach(0)
         if ghosts == "reference":
+            # the exchange alone: the faces are extrapolated after the
+            # inversion (a post-shock wall's condition, which writes the
+            # first interior row, is not applied here)
+            _lib.check(self.lib.izem_comm_cons_exchange(self.ctx, _ptr(self.v)), "comm_cons")
+            self.prim_inv_var_mach(0)
             restart.extrapolate_faces(self.box, self.v)
+        else:
+            _lib.check(self.lib.izem_comm_cons(self.ctx, _ptr(self.v)), "comm_cons")
+            self.prim_inv_var_mach(0)
         for dom in [s * q for q in range(1, self.box.ndim + 1) for s in (-1, 1)]:
             self.prim_inv_var_mach(dom)
         self.sync()

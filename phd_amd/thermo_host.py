@@ -90,3 +90,37 @@ def ha_scalar(thd, temp: float) -> np.ndarray:
             val = fac * (dh(hi, tmax) * (temp * thd.Tinf - tmax) + h(hi, tmax))
         out[s] = 0.0 + val
     return out
+
+
+def cp_scalar(thd, temp: float, Y) -> float:
+    """cp_scalar (module_thermodynamics.f90:767-862) at one nondimensional
+    temperature, in the reference's operation order; cp is linear outside
+    [Tpolymin, Tpolymax]."""
+    cp = 0.0
+    for s in range(len(thd.W)):
+        lo = [float(x) for x in thd.heatcap[s, 0]]
+        hi = [float(x) for x in thd.heatcap[s, 1]]
+        tmin, tlim, tmax = float(thd.tpolymin[s]), float(thd.tpolylim[s]), float(thd.tpolymax[s])
+        fac = float(Y[s]) * thd.gam2 * float(thd.Wc_i[s])
+        t1 = temp * thd.Tinf
+
+        def poly(c, t1):
+            t2 = t1 * t1
+            t3 = t2 * t1
+            t4 = t3 * t1
+            return c[0] + c[1] * t1 + c[2] * t2 + c[3] * t3 + c[4] * t4
+
+        def dpoly(c, t1):
+            t2 = t1 * t1
+            t3 = t2 * t1
+            return c[1] + c[2] * t1 * 2.0 + c[3] * t2 * 3.0 + c[4] * t3 * 4.0
+
+        if t1 < tmin:
+            cp = cp + fac * (dpoly(lo, tmin) * (temp * thd.Tinf - tmin) + poly(lo, tmin))
+        elif t1 < tlim:
+            cp = cp + fac * poly(lo, t1)
+        elif t1 < tmax:
+            cp = cp + fac * poly(hi, t1)
+        else:
+            cp = cp + fac * (dpoly(hi, tmax) * (temp * thd.Tinf - tmax) + poly(hi, tmax))
+    return cp
