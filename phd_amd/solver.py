@@ -88,6 +88,7 @@ class Solver:
                                         C.byref(op)), "izem_create")
         self.ctx = ctx
         self.grid_struct, self.opts_struct = g, op
+        self.bc = (0,) * 6
         _lib.check(self.lib.izem_flux_mode(ctx, o.flux_mode), "izem_flux_mode")
         # run on torch's current stream so that library and torch work is ordered
         _lib.check(self.lib.izem_set_stream(ctx, C.c_void_p(
@@ -117,6 +118,7 @@ class Solver:
         1 extrapolation, 2 post-shock slip wall on S (izem_set_bc)."""
         a = (C.c_int * 6)(*[int(x) for x in bc])
         _lib.check(self.lib.izem_set_bc(self.ctx, a), "izem_set_bc")
+        self.bc = tuple(int(x) for x in bc)
 
     def set_wall_state(self, state, nfix: int) -> None:
         """The post-shock state (5 + nspc conserved values) of a
@@ -181,18 +183,21 @@ class Solver:
         images (the state the writing run had).  Returns the time."""
         from . import restart
         v, time = restart.read(path, self.box, self.thd.names)
-        self.load_conserved(v, None, ghosts)
+        self.load_conserved(v, None, ghosts, boundaries=True)
         return time
 
     def load_conserved(self, v: np.ndarray, T: np.ndarray | None = None,
-                       ghosts: str = "reference") -> None:
+                       ghosts: str = "reference", boundaries: bool = False) -> None:
         """The closing steps of the reference's initialisations and of its
         restart reader (ini_sod, module_ICs.f90:2872-2877; read_restart_hdf5,
         module_io.f90:107-192): the interior conserved variables v, the
         temperature guess T on the interior (zero when None; ghosts start at
         zero), comm_cons, prim_inv_var_err_mach on the interior,
         upd_extrapolation on every face (ghosts='reference'; 'periodic' keeps
-        the periodic images), then the ghost slabs."""
+        the periodic images), then the ghost slabs.  boundaries: the
+        restart reader's upd_boundaries after the extrapolation (module_io.f90:
+        183-186), which only a post-shock wall changes: its face condition
+        and the primitives of the row it writes."""
         from . import restart
         if ghosts not in ("reference", "periodic"):
             raise ValueError(ghosts)
@@ -213,6 +218,13 @@ class Solver:
             _lib.check(self.lib.izem_comm_cons_exchange(self.ctx, _ptr(self.v)), "comm_cons")
             self.prim_inv_var_mach(0)
             restart.extrapolate_faces(self.box, self.v)
+            if boundaries and 2 in self.bc:
+                _lib.check(self.lib.izem_comm_cons(self.ctx, _ptr(self.v)), "upd_boundaries")
+                if self.opts_struct.forcesum:
+                    _lib.check(self.lib.izem_force_sumY(self.ctx, -20, _ptr(self.v)), "force_sumY")
+                _lib.check(self.lib.izem_prim_inv_var(self.ctx, -20, _ptr(self.v), _ptr(self.W_i),
+                                                      _ptr(self.T), _ptr(self.cp), _ptr(self.ha),
+                                                      None), "prim_inv_var")
         else:
             _lib.check(self.lib.izem_comm_cons(self.ctx, _ptr(self.v)), "comm_cons")
             self.prim_inv_var_mach(0)

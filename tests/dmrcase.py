@@ -78,7 +78,8 @@ class DmrRun:
     """The oracle restatement of the run (liboracle_nrv2.so) on the grid of
     grid_path."""
 
-    def __init__(self, adi, grid_path: str, nstat: int, oracle=None):
+    def __init__(self, adi, grid_path: str, nstat: int, oracle=None, restart=None,
+                 time_offset: float = TIME_OFFSET):
         from oracle.oracle import (BC_EXTRAPOLATION, BC_POSTSHOCK_SLIPWALL, Grid, Opts, Oracle,
                                    make_adi)
         from phd_amd import grid, shock
@@ -101,16 +102,27 @@ class DmrRun:
         self.opts = Opts(1, PERCENT_WEIGHT, 0, 0, 0, 1, CFL, CFLMIN, FO, CPR, 0, 0.1)
         self.d_i = [np.ascontiguousarray(m) for m in box.metrics]
         shp = box.shape
-        self.v, self.T = shock.dmr_initial_state(adi, self.thd, box, xt, yt)
         self.W_i = np.zeros(shp)
         self.cp = np.zeros(shp)
         self.ha = np.zeros((2,) + shp)
         self.sensor = np.zeros(shp, np.uint8)
-        self._init_primitives()
-        self.times = stat_times(adi, nstat)
         self.nstat = nstat
-        self.time, self.ite, self.nb_stat = 0.0, 0, 0
+        self.ite = 0
         self.dtmin, self.DTmax = 1.0, 1.0
+        if restart is None:
+            self.v, self.T = shock.dmr_initial_state(adi, self.thd, box, xt, yt)
+            self._init_primitives()
+            self.times = stat_times(adi, nstat)
+            self.time, self.nb_stat = 0.0, 0
+        else:
+            path, nb = restart
+            self._read_restart(path)
+            # statistics times of a resumed run (module_input.f90:718-721)
+            t = [time_offset] * (nb + 1)
+            while len(t) < nstat:
+                t.append(t[-1] + DTIME)
+            self.times = np.array(t) / adi.time_ref
+            self.nb_stat = nb
 
     def _p(self, a):
         return a.ctypes.data_as(C.POINTER(C.c_double))
@@ -135,6 +147,31 @@ class DmrRun:
         for f in range(2 * NDIM):
             gx.bc[f] = BC_EXTRAPOLATION
         L.ora_upd_boundaries(C.byref(gx), self._p(self.v))
+        for dom in (-1, 1, -2, 2):
+            self._prim_mach(dom)
+
+    def _read_restart(self, path: str):
+        """read_restart_hdf5 (module_io.f90:107-192): the interior from the
+        file, comm_cons, prim_inv_var_err_mach on the interior from T = 0,
+        upd_extrapolation, upd_boundaries (the wall's condition and its
+        row's primitives), then the ghost slabs."""
+        from oracle.oracle import BC_EXTRAPOLATION, Grid
+        from phd_amd import restart
+        L = self.o.lib
+        self.v, self.time = restart.read(path, self.box, self.thd.names)
+        self.v = np.ascontiguousarray(self.v)
+        self.T = np.zeros(self.box.shape)
+        L.ora_comm_cons(C.byref(self.g), self._p(self.v))
+        self._prim_mach(0)
+        gx = Grid(self.g.nx, self.g.ny, self.g.nz, NG, NDIM)
+        for f in range(2 * NDIM):
+            gx.bc[f] = BC_EXTRAPOLATION
+        L.ora_upd_boundaries(C.byref(gx), self._p(self.v))
+        L.ora_upd_boundaries(C.byref(self.g), self._p(self.v))
+        st = L.ora_upd_dirichlet(C.byref(self.g), C.byref(self.t), 1, self._p(self.v),
+                                 self._p(self.W_i), self._p(self.T), self._p(self.cp),
+                                 self._p(self.ha))
+        assert st in (0, 1), st
         for dom in (-1, 1, -2, 2):
             self._prim_mach(dom)
 

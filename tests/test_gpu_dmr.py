@@ -78,6 +78,39 @@ def test_dmr_coarse_deck_matches_oracle(tmp_path, adi, capsys):
     assert worst <= TOL
 
 
+def test_dmr_coarse_resumed_from_restart(tmp_path, adi, capsys):
+    """A run resumed from restart file 3 (READ_RESTART): the reader's
+    upd_boundaries puts the wall's state back on its first cells and first
+    row before the ghost slabs, as read_restart_hdf5 does; 3 more
+    statistics times against the oracle resumed from the same file."""
+    _io_or_skip()
+    from phd_amd.run import main
+    nstat = 6
+    dc.write_deck(str(tmp_path), nstat, coarse=(15, 12, 40))
+    assert main([str(tmp_path / "dmr.inp"), "--flux-mode", "0"]) == 0
+    deck = (tmp_path / "dmr.inp").read_text()
+    # statistics times restart from TIME_OFFSET (module_input.f90:718-721):
+    # put it at the restart time, so that the first is met by a step of
+    # (about) zero length
+    assert deck.count("TIME_OFFSET = 1.0e-10") == 1
+    deck = deck.replace("TIME_OFFSET = 1.0e-10", "TIME_OFFSET = 4.01e-08").replace(
+        "[MONITORING]", "[MONITORING]\n        READ_RESTART = T\n"
+        "        NAME_RESTART = 'restart'\n        NUMBER_RESTART = 3")
+    os.makedirs(tmp_path / "resume")
+    for f in ("grid.dat", "therm.dat", "chem.inp", "tran.dat", "restart_00003.h5"):
+        os.link(tmp_path / f, tmp_path / "resume" / f)
+    (tmp_path / "resume" / "dmr.inp").write_text(deck)
+    assert main([str(tmp_path / "resume" / "dmr.inp"), "--flux-mode", "0"]) == 0
+    capsys.readouterr()
+    run = dc.DmrRun(adi, str(tmp_path / "grid.dat"), nstat,
+                    restart=(str(tmp_path / "restart_00003.h5"), 3), time_offset=4.01e-08)
+    worst = 0.0
+    for k in range(4, nstat + 1):
+        run.run_to_stat(k)
+        worst = max(worst, _compare(tmp_path / "resume", adi, run, k))
+    assert worst <= TOL
+
+
 def test_dmr_reference_deck(tmp_path, adi, capsys):
     _io_or_skip()
     from phd_amd.run import main
