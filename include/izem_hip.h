@@ -91,6 +91,16 @@ typedef struct {
  * prim_inv_var on that row, :93-96, 131, 297-314) before the ghost slabs.
  * Derivatives next to the face take the one-sided closures. */
 #define IZEM_BC_POSTSHOCK_SLIPWALL 2
+/* bc_nscbc_out (module_BCs.f90:2331-2516), face E only: a characteristic
+ * subsonic outflow.  Per row, the fourth-order one-sided x derivatives of
+ * rho, u, v, w, p and Y over the last five cells give the wave amplitudes
+ * L1..L5; the incoming one is relaxed toward the far-field pressure,
+ * L1 = 0.28 (1 - M^2) c / Lx (p - pinf); the last cell's state advanced by
+ * -dt times the resulting changes is written to it and to its ghosts, and
+ * its primitives recomputed (force_sumY, prim_inv_var).  Lx, pinf and 1/dx
+ * of the last cell come from izem_set_nscbc; the step dt from the call
+ * (izem_rk3's, the chemistry's scale x dt, izem_upd_boundaries'). */
+#define IZEM_BC_NSCBC_OUT 3
 
 /* thd: flat image of the thermodynamic/transport tables (thd_init,
  * thd/thd_tools_perfectgas.f90:12-248), see phd_amd.chemdata.Thd.flat(),
@@ -104,6 +114,11 @@ int izem_set_bc(izem_ctx *ctx, const int bc[6]);
  * conserved values (host memory) and the number nfix of its cells that take
  * it (phd_amd.shock.dmr_wall_state, dmr_wall_cells) */
 int izem_set_wall_state(izem_ctx *ctx, const double *state, int nfix);
+/* the characteristic outflow's constants (nondimensional): the domain
+ * length Lx = x(ntx) - x(1) (module_tools.f90:319, 489), the far-field
+ * pressure (94232.25 Pa in the reference, module_BCs.f90:2389) and 1/dx of
+ * the last cell */
+int izem_set_nscbc(izem_ctx *ctx, double Lx, double pinf, double dxi_last);
 /* Reproduce on one GPU a run decomposed into nproc[0] x nproc[1] x nproc[2]
  * ranks (GRID%NUMPROCX/Y/Z, module_input.f90:361-363, 498-500; rank r of
  * np along a line of n cells owns cells r n / np .. (r + 1) n / np - 1,
@@ -182,15 +197,22 @@ int izem_upd_prim_var_woT(izem_ctx *ctx, int domain, const double *v, const doub
                           double *W_i, double *cp, double *ha);
 /* comm_cons on one rank (module_parallel.f90:1715-1777: the periodic wrap)
  * followed by upd_boundaries' conditions of the non-periodic faces
- * (module_BCs.f90:41-141), as the reference's time step calls them */
+ * (module_BCs.f90:41-141), as the reference's time step calls them; a
+ * context with a characteristic outflow face returns IZEM_E_STATE (its
+ * condition needs the primitives and dt: izem_upd_boundaries) */
 int izem_comm_cons(izem_ctx *ctx, double *v);
+/* the same with the primitives and the step dt (device pointer) that the
+ * characteristic outflow reads: comm_cons + upd_boundaries */
+int izem_upd_boundaries(izem_ctx *ctx, const double *dt, const double *T, const double *W_i,
+                        const double *cp, double *v);
 /* comm_cons alone: the periodic wrap, non-periodic faces untouched (the
  * reference's initialisations call it before their own face treatment,
  * module_ICs.f90:404, 2872) */
 int izem_comm_cons_exchange(izem_ctx *ctx, double *v);
 /* upd_prim_var_ghost (module_BCs.f90:254), after the primitives of the
- * first interior row a post-shock slip wall wrote (upd_prim_var_dirichlet,
- * module_BCs.f90:297-314) */
+ * first interior rows that a post-shock slip wall or a characteristic
+ * outflow wrote (upd_prim_var_dirichlet, module_BCs.f90:297-314; their
+ * force_sumY came with the face condition) */
 int izem_upd_prim_var_ghost(izem_ctx *ctx, double *T, double *W_i, double *cp, double *ha,
                             double *v);
 /* shock_sensor_modified_adams_shariff (module_tools.f90:2733), sticky */

@@ -881,6 +881,128 @@ void ora_upd_boundaries(const ora_grid *g, double *v) {
   }
 }
 
+/* bc_nscbc_out: src/module_BCs.f90:2331-2516 (face E).  Per row: the
+ * one-sided fourth-order derivatives (alph4or) of rho, u, v, w, P and Y
+ * over cells ex-4..ex, the LODI amplitudes of a subsonic outflow with L1
+ * relaxed toward pinf, and the state of cell ex advanced by -dt d(.)
+ * written to ex..ex+ng. */
+void ora_bc_nscbc_out(const ora_grid *g, const ora_thd *thd, double dt, const double *dx_i,
+                      const double *T, const double *W_i, const double *cp, double *v) {
+  const long nc = ncell(g);
+  const int ex = g->nx - 1;
+  static const double alph4or[5] = {-3.0 / 12.0, 16.0 / 12.0, -36.0 / 12.0, 48.0 / 12.0,
+                                    -25.0 / 12.0};
+  const double sigma1 = 0.28, Pinf = g->nscbc_pinf, Lx = g->nscbc_Lx, dtime = dt;
+  const double dxi = dx_i[ex + g->ng];
+  for (int k = 0; k < g->nz; k++)
+    for (int j = 0; j < g->ny; j++) {
+      double Machm = 0.0;
+      long c = cidx(g, ex, j, k);
+      double rho = v[c], rho_i = 1.0 / rho;
+      double ux = v[nc + c] * rho_i, vy = v[2 * nc + c] * rho_i, wz = v[3 * nc + c] * rho_i;
+      double Pres = v[c] * T[c] * W_i[c];
+      double gam = thd->gam2 * W_i[c] / cp[c];
+      gam = 1.0 / (1.0 - gam);
+      double cs = sqrt(gam * Pres * rho_i);
+      double Mach = sqrt(ux * ux + vy * vy + wz * wz) / cs;
+      Machm = fmax(Machm, Mach);
+      double drho = 0.0, dux = 0.0, dvy = 0.0, dwz = 0.0, dPres = 0.0, dYa[NRV], Ya[NRV];
+      for (int l = 0; l < NRV; l++) dYa[l] = 0.0;
+      for (int m = -4; m <= 0; m++) {
+        long cm = cidx(g, ex + m, j, k);
+        rho = v[cm];
+        rho_i = 1.0 / rho;
+        ux = v[nc + cm] * rho_i;
+        vy = v[2 * nc + cm] * rho_i;
+        wz = v[3 * nc + cm] * rho_i;
+        Pres = v[cm] * T[cm] * W_i[cm];
+        for (int l = 0; l < NRV; l++) Ya[l] = v[(NIV + l) * nc + cm] * rho_i;
+        drho = drho + alph4or[m + 4] * rho;
+        dux = dux + alph4or[m + 4] * ux;
+        dvy = dvy + alph4or[m + 4] * vy;
+        dwz = dwz + alph4or[m + 4] * wz;
+        dPres = dPres + alph4or[m + 4] * Pres;
+        for (int l = 0; l < NRV; l++) dYa[l] = dYa[l] + alph4or[m + 4] * Ya[l];
+      }
+      drho = -drho * dxi;
+      dux = -dux * dxi;
+      dvy = -dvy * dxi;
+      dwz = -dwz * dxi;
+      dPres = -dPres * dxi;
+      for (int l = 0; l < NRV; l++) dYa[l] = -dYa[l] * dxi;
+      double L1 = 0.0, L2 = 0.0, L3 = 0.0, L4 = 0.0, L5 = 0.0, L5_Ya[NRV];
+      for (int l = 0; l < NRV; l++) L5_Ya[l] = 0.0;
+      gam = thd->gam2 * W_i[c] / cp[c];
+      gam = 1.0 / (1.0 - gam);
+      cs = sqrt(gam * Pres * rho_i);
+      if (ux - cs < 0.0) L1 = sigma1 * (1.0 - Machm * Machm) * cs / Lx * (Pres - Pinf);
+      else L1 = (ux - cs) * (dPres - rho * cs * dux);
+      if (ux + cs > 0.0) L5 = (ux + cs) * (dPres + rho * cs * dux);
+      else L5 = (ux + cs) * (2.0 * dPres - L1 / (ux - cs));
+      if (ux > 0.0) {
+        L2 = ux * (cs * cs * drho - dPres);
+        L3 = ux * dvy;
+        L4 = ux * dwz;
+        for (int l = 0; l < NRV; l++) L5_Ya[l] = ux * dYa[l];
+      } else {
+        L2 = ux * cs * cs * drho - 0.5 * ux * (L5 / (ux + cs) + L1 / (ux - cs));
+      }
+      double d2 = 0.5 * (L5 + L1);
+      double d1 = (L2 + d2) / (cs * cs);
+      double d3 = 0.5 * (L5 - L1) / (cs * rho);
+      double d4 = L3, d5 = L4, d5_Ya[NRV];
+      for (int l = 0; l < NRV; l++) d5_Ya[l] = L5_Ya[l];
+      double has[NRV];
+      ora_ha_scalar(thd, T[c], has);
+      double hm = 0.0;
+      for (int l = 0; l < NRV; l++) hm = hm + has[l] * Ya[l];
+      for (int i = ex; i <= ex + g->ng; i++) {
+        long ci = cidx(g, i, j, k);
+        v[ci] = rho - d1 * dtime;
+        v[nc + ci] = (rho * ux) - (ux * d1 + rho * d3) * dtime;
+        v[2 * nc + ci] = (rho * vy) - (vy * d1 + rho * d4) * dtime;
+        v[3 * nc + ci] = (rho * wz) - (wz * d1 + rho * d5) * dtime;
+        v[4 * nc + ci] = (rho * hm - Pres + 0.5 * rho * (ux * ux + vy * vy + wz * wz)) -
+                         (0.5 * (ux * ux + vy * vy + wz * wz) * d1 +
+                          rho * (ux * d3 + vy * d4 + wz * d5) + d2 / (gam - 1.0)) * dtime;
+        for (int l = 0; l < NRV; l++)
+          v[(NIV + l) * nc + ci] = (rho * Ya[l]) - (Ya[l] * d1 + rho * d5_Ya[l]) * dtime;
+      }
+    }
+}
+
+/* upd_boundaries in full: src/module_BCs.f90:41-141, the faces in the order
+ * W, E, S, N, B, F, each followed, when its condition writes the first
+ * interior row (the post-shock wall, the characteristic outflow), by
+ * upd_prim_var_dirichlet on that row (force_sumY and prim_inv_var,
+ * :131, 297-314).  dt: the step (for the outflow). */
+int ora_boundaries(const ora_grid *g, const ora_thd *thd, int forcesum, double dt,
+                   const double *dx_i, double *v, double *W_i, double *T, double *cp,
+                   double *ha) {
+  static const int row[6] = {-10, 10, -20, 20, -30, 30};
+  int st = 0;
+  for (int f = 0; f < 2 * g->ndim; f++) {
+    if (g->bc[f] == ORA_BC_PERIODIC) continue;
+    int dirichlet = 0;
+    if (g->bc[f] == ORA_BC_NSCBC_OUT && f == 1) {
+      ora_bc_nscbc_out(g, thd, dt, dx_i, T, W_i, cp, v);
+      dirichlet = 1;
+    } else {
+      ora_grid one = *g;
+      for (int q = 0; q < 6; q++)
+        if (q != f) one.bc[q] = ORA_BC_PERIODIC;
+      ora_upd_boundaries(&one, v);
+      dirichlet = g->bc[f] == ORA_BC_POSTSHOCK_SLIPWALL;
+    }
+    if (dirichlet) {
+      if (forcesum) ora_force_sumY(row[f], g, v);
+      int s = ora_prim_inv_var(row[f], g, thd, v, W_i, T, cp, ha, NULL);
+      if (s && !st) st = s;
+    }
+  }
+  return st;
+}
+
 /* the primitives of the Dirichlet row of each face whose condition writes
  * the first interior row (upd_prim_var_dirichlet after the face's
  * condition, src/module_BCs.f90:93-96, 131, 297-314): force_sumY and
@@ -1346,8 +1468,7 @@ int ora_rk3(const ora_grid *g, const ora_thd *thd, const ora_adi *adi, const ora
     s = ora_prim_inv_var(0, g, thd, v, W_i, T, cp, ha, NULL);
     if (s && !status) status = s;
     ora_comm_cons(g, v);
-    ora_upd_boundaries(g, v);
-    s = ora_upd_dirichlet(g, thd, op->forcesum, v, W_i, T, cp, ha);
+    s = ora_boundaries(g, thd, op->forcesum, dt, dx_i, v, W_i, T, cp, ha);
     if (s && !status) status = s;
     s = ora_upd_prim_var_ghost(g, thd, op->forcesum, v, W_i, T, cp, ha);
     if (s && !status) status = s;

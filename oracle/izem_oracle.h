@@ -42,6 +42,7 @@
 #define ORA_BC_PERIODIC 0
 #define ORA_BC_EXTRAPOLATION 1
 #define ORA_BC_POSTSHOCK_SLIPWALL 2
+#define ORA_BC_NSCBC_OUT 3
 typedef struct {
   int nx, ny, nz, ng, ndim;
   int bc[6];
@@ -50,6 +51,8 @@ typedef struct {
    * and the cells of the face that take it */
   int nfix;
   const double *wall;
+  /* ORA_BC_NSCBC_OUT: the domain length and the far-field pressure */
+  double nscbc_Lx, nscbc_pinf;
 } ora_grid;
 
 /* thd_type subset used by the flow kernels (src/thd/type_thd.f90:10-91) */
@@ -134,6 +137,11 @@ double ora_deriv_line(const double *a, long stride, double dinv, int p, int n, i
 void ora_upd_boundaries(const ora_grid *g, double *v);
 int ora_upd_dirichlet(const ora_grid *g, const ora_thd *thd, int forcesum, double *v,
                       double *W_i, double *T, double *cp, double *ha);
+void ora_bc_nscbc_out(const ora_grid *g, const ora_thd *thd, double dt, const double *dx_i,
+                      const double *T, const double *W_i, const double *cp, double *v);
+int ora_boundaries(const ora_grid *g, const ora_thd *thd, int forcesum, double dt,
+                   const double *dx_i, double *v, double *W_i, double *T, double *cp,
+                   double *ha);
 
 void ora_eigvecs(int dir, const ora_thd *thd, const double *uc, const double *Yc,
                  const double *psi_c, double ht_c, double cs_c, double gam_c,

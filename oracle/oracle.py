@@ -26,7 +26,7 @@ _dp = C.POINTER(C.c_double)
 _u8 = C.POINTER(C.c_uint8)
 
 
-BC_PERIODIC, BC_EXTRAPOLATION, BC_POSTSHOCK_SLIPWALL = 0, 1, 2
+BC_PERIODIC, BC_EXTRAPOLATION, BC_POSTSHOCK_SLIPWALL, BC_NSCBC_OUT = 0, 1, 2, 3
 
 
 class Grid(C.Structure):
@@ -37,7 +37,8 @@ class Grid(C.Structure):
     caller)."""
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("ng", C.c_int),
                 ("ndim", C.c_int), ("bc", C.c_int * 6), ("nproc", C.c_int * 3),
-                ("nfix", C.c_int), ("wall", C.c_void_p)]
+                ("nfix", C.c_int), ("wall", C.c_void_p), ("nscbc_Lx", C.c_double),
+                ("nscbc_pinf", C.c_double)]
 
 
 def _thd_struct(nrv: int):
@@ -156,6 +157,7 @@ class Oracle:
         L.ora_rk3.restype = C.c_int
         L.ora_upd_prim_var_ghost.restype = C.c_int
         L.ora_upd_dirichlet.restype = C.c_int
+        L.ora_boundaries.restype = C.c_int
         L.ora_wmix_i_scalar.restype = C.c_double
         L.ora_cp_scalar.restype = C.c_double
         L.ora_ckcvbs.restype = C.c_double

@@ -24,6 +24,7 @@ EXPORTS = [
     "izem_reac_selec", "izem_reaction_selec", "izem_surr_attach", "izem_reaction_surrogate",
     "izem_surr_counts", "izem_thd_image", "izem_mech_image_ckwrk", "izem_set_bc",
     "izem_set_decomposition", "izem_set_wall_state", "izem_comm_cons_exchange",
+    "izem_set_nscbc", "izem_upd_boundaries",
 ]
 
 
@@ -174,6 +175,8 @@ def _declare(lib):
     lib.izem_set_decomposition.argtypes = [vp, vp]
     lib.izem_set_wall_state.argtypes = [vp, vp, C.c_int]
     lib.izem_comm_cons_exchange.argtypes = [vp, vp]
+    lib.izem_set_nscbc.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+    lib.izem_upd_boundaries.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.izem_mech_image_ckwrk.restype = C.c_long
     lib.izem_mech_image_ckwrk.argtypes = [vp, vp, vp, vp, C.c_long]
 

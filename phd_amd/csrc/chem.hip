@@ -794,11 +794,11 @@ static void reaction_queue(izem_ctx *ctx, const IzBox &b, dim3 grid, const doubl
 
 // the tail of reaction_selec (module_reaction.f90:79-83): W_i, cp, ha
 // from the new T and Y, ghost fill, ghost-slab primitives
-static int reaction_tail(izem_ctx *ctx, double *T, double *W_i, double *cp, double *ha,
-                         double *v) {
+static int reaction_tail(izem_ctx *ctx, const double *dt, double scale, double *T, double *W_i,
+                         double *cp, double *ha, double *v) {
   int r = iz_upd_prim_woT(ctx, 0, v, T, W_i, cp, ha);
   if (r) return r;
-  r = iz_comm_cons(ctx, v, IZ_NV, 1);
+  r = iz_comm_cons(ctx, v, IZ_NV, 1, T, W_i, cp, dt, scale);
   if (r) return r;
   return iz_ghost_newton(ctx, T, W_i, cp, ha, v);
 }
@@ -836,7 +836,7 @@ extern "C" int izem_reaction_dvode(izem_ctx *ctx, const double *dt, double scale
     iz_prof_end(ctx, pk);
     IZ_CHECK(hipGetLastError());
   }
-  return reaction_tail(ctx, T, W_i, cp, ha, v);
+  return reaction_tail(ctx, dt, scale, T, W_i, cp, ha, v);
 }
 
 // work counters of the chemistry integrations since the previous call
@@ -1250,7 +1250,7 @@ extern "C" int izem_reaction_surrogate(izem_ctx *ctx, const double *dt, double s
     hipLaunchKernelGGL(k_surr_count, dim3(1), dim3(1), 0, ctx->stream, u->counts, n, u->nfall);
     IZ_CHECK(hipGetLastError());
   }
-  return reaction_tail(ctx, T, W_i, cp, ha, v);
+  return reaction_tail(ctx, dt, scale, T, W_i, cp, ha, v);
 }
 
 extern "C" int izem_reac_selec(izem_ctx *ctx, int solver) {

@@ -969,9 +969,10 @@ __global__ void k_upd_prim_woT(IzDims d, IzBox b, const double *__restrict__ v,
 // wall): bc_postshock_slipwall (module_BCs.f90:1065-1172) on that face: the
 // first nfix cells take the post-shock state on the ghost rows and on the
 // first interior row, the others mirror rows 1..ng (rho v odd).
-// bcs = 0: the periodic exchange alone (comm_cons), physical faces untouched.
+// mode bit 1: the periodic faces (comm_cons); bit 2: the physical faces'
+// conditions (upd_boundaries), a characteristic outflow face excepted.
 __global__ void k_ghost_fill(IzDims d, double *__restrict__ a, int nf, int dir,
-                             const double *__restrict__ wall, int bcs) {
+                             const double *__restrict__ wall, int mode) {
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int n0, n1, n;  // transverse sizes and the direction length
   if (dir == 0) { n0 = d.ny; n1 = d.nz; n = d.nx; }
@@ -984,7 +985,10 @@ __global__ void k_ghost_fill(IzDims d, double *__restrict__ a, int nf, int dir,
   int s = (int)(r % (2 * d.ng));
   r /= (2 * d.ng);
   int p0 = (int)(r % n0), p1 = (int)(r / n0);
-  if (!bcs && ((d.phys >> (2 * dir + (s < d.ng ? 0 : 1))) & 1)) return;
+  const int face = 2 * dir + (s < d.ng ? 0 : 1);
+  const int phys = (d.phys >> face) & 1;
+  if (!(mode & (phys ? 2 : 1))) return;
+  if (face == 1 && d.nscbc) return;
   if (wall && dir == 1 && s < d.ng) {
     if (p0 < d.nfix) {
       a[f * d.nc + iz_cidx(d, p0, -1 - s, p1)] = wall[f];
@@ -1009,6 +1013,124 @@ __global__ void k_ghost_fill(IzDims d, double *__restrict__ a, int nf, int dir,
   else if (dir == 1) { cd = iz_cidx(d, p0, dst, p1); cs = iz_cidx(d, p0, src, p1); }
   else { cd = iz_cidx(d, p0, p1, dst); cs = iz_cidx(d, p0, p1, src); }
   a[f * d.nc + cd] = a[f * d.nc + cs];
+}
+
+// bc_nscbc_out (module_BCs.f90:2331-2516), face E: per row, the one-sided
+// fourth-order x derivatives of rho, u, v, w, p and Y over the last five
+// cells, the characteristic amplitudes L1..L5 of a subsonic outflow with
+// L1 relaxed toward pinf (sigma (1 - M^2) c / Lx, sigma = 0.28), and the
+// last cell's state advanced by -dt d(.) written to it and to the ghosts.
+// One thread per (j, k) row; the row's reads all precede its writes.
+__global__ void k_nscbc_out(IzDims d, double *__restrict__ v, const double *__restrict__ T,
+                            const double *__restrict__ W_i, const double *__restrict__ cp,
+                            const double *__restrict__ dtp, double scale, double Lx,
+                            double pinf, double dxi) {
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long)d.ny * d.nz) return;
+  const int j = (int)(tid % d.ny), k = (int)(tid / d.ny);
+  const int ex = d.nx - 1;
+  const long nc = d.nc;
+  // alph4or(-4..0)
+  const double a4[5] = {-3.0 / 12.0, 16.0 / 12.0, -36.0 / 12.0, 48.0 / 12.0, -25.0 / 12.0};
+  const double sigma1 = 0.28;
+  const double dtime = *dtp * scale;
+  const long ce = iz_cidx(d, ex, j, k);
+  double rho = v[ce], rho_i = 1.0 / rho;
+  double ux = v[nc + ce] * rho_i, vy = v[2 * nc + ce] * rho_i, wz = v[3 * nc + ce] * rho_i;
+  double Pres = v[ce] * T[ce] * W_i[ce];
+  double gam = c_thd.gam2 * W_i[ce] / cp[ce];
+  gam = 1.0 / (1.0 - gam);
+  double cs = sqrt(gam * Pres * rho_i);
+  const double Mach = sqrt(ux * ux + vy * vy + wz * wz) / cs;
+  const double Machm = fmax(0.0, Mach);
+  double drho = 0.0, dux = 0.0, dvy = 0.0, dwz = 0.0, dPres = 0.0, dYa[IZ_NRV], Ya[IZ_NRV];
+#pragma unroll
+  for (int l = 0; l < IZ_NRV; l++) dYa[l] = 0.0;
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const long c = ce - 4 + m;
+    rho = v[c];
+    rho_i = 1.0 / rho;
+    ux = v[nc + c] * rho_i;
+    vy = v[2 * nc + c] * rho_i;
+    wz = v[3 * nc + c] * rho_i;
+    Pres = v[c] * T[c] * W_i[c];
+#pragma unroll
+    for (int l = 0; l < IZ_NRV; l++) Ya[l] = v[(IZ_NIV + l) * nc + c] * rho_i;
+    drho = drho + a4[m] * rho;
+    dux = dux + a4[m] * ux;
+    dvy = dvy + a4[m] * vy;
+    dwz = dwz + a4[m] * wz;
+    dPres = dPres + a4[m] * Pres;
+#pragma unroll
+    for (int l = 0; l < IZ_NRV; l++) dYa[l] = dYa[l] + a4[m] * Ya[l];
+  }
+  drho = -drho * dxi;
+  dux = -dux * dxi;
+  dvy = -dvy * dxi;
+  dwz = -dwz * dxi;
+  dPres = -dPres * dxi;
+#pragma unroll
+  for (int l = 0; l < IZ_NRV; l++) dYa[l] = -dYa[l] * dxi;
+  double L1 = 0.0, L2 = 0.0, L3 = 0.0, L4 = 0.0, L5 = 0.0, L5Y[IZ_NRV];
+#pragma unroll
+  for (int l = 0; l < IZ_NRV; l++) L5Y[l] = 0.0;
+  gam = c_thd.gam2 * W_i[ce] / cp[ce];
+  gam = 1.0 / (1.0 - gam);
+  cs = sqrt(gam * Pres * rho_i);
+  if (ux - cs < 0.0) L1 = sigma1 * (1.0 - Machm * Machm) * cs / Lx * (Pres - pinf);
+  else L1 = (ux - cs) * (dPres - rho * cs * dux);
+  if (ux + cs > 0.0) L5 = (ux + cs) * (dPres + rho * cs * dux);
+  else L5 = (ux + cs) * (2.0 * dPres - L1 / (ux - cs));
+  if (ux > 0.0) {
+    L2 = ux * (cs * cs * drho - dPres);
+    L3 = ux * dvy;
+    L4 = ux * dwz;
+#pragma unroll
+    for (int l = 0; l < IZ_NRV; l++) L5Y[l] = ux * dYa[l];
+  } else {
+    L2 = ux * cs * cs * drho - 0.5 * ux * (L5 / (ux + cs) + L1 / (ux - cs));
+  }
+  const double d2 = 0.5 * (L5 + L1);
+  const double d1 = (L2 + d2) / (cs * cs);
+  const double d3 = 0.5 * (L5 - L1) / (cs * rho);
+  const double d4 = L3, d5 = L4;
+  double has[IZ_NRV];
+  d_ha_scalar(T[ce], has);
+  double hm = 0.0;
+#pragma unroll
+  for (int l = 0; l < IZ_NRV; l++) hm = hm + has[l] * Ya[l];
+  const double q2 = ux * ux + vy * vy + wz * wz;
+  double out[IZ_NV];
+  out[0] = rho - d1 * dtime;
+  out[1] = (rho * ux) - (ux * d1 + rho * d3) * dtime;
+  out[2] = (rho * vy) - (vy * d1 + rho * d4) * dtime;
+  out[3] = (rho * wz) - (wz * d1 + rho * d5) * dtime;
+  out[4] = (rho * hm - Pres + 0.5 * rho * q2) -
+           (0.5 * q2 * d1 + rho * (ux * d3 + vy * d4 + wz * d5) + d2 / (gam - 1.0)) * dtime;
+#pragma unroll
+  for (int l = 0; l < IZ_NRV; l++)
+    out[IZ_NIV + l] = (rho * Ya[l]) - (Ya[l] * d1 + rho * L5Y[l]) * dtime;
+  for (int i = ex; i <= ex + d.ng; i++) {
+    const long c = iz_cidx(d, i, j, k);
+#pragma unroll
+    for (int l = 0; l < IZ_NV; l++) v[l * nc + c] = out[l];
+  }
+}
+
+// force_sumY on one Dirichlet row, right after its face's condition (the
+// reference's order: module_BCs.f90:131 before the next face)
+__global__ void k_force_row(IzDims d, IzBox b, double *__restrict__ v) {
+  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(b, tid, i, j, k)) return;
+  const long c = iz_cidx(d, i, j, k);
+  double vc[IZ_NV];
+#pragma unroll
+  for (int m = 0; m < IZ_NV; m++) vc[m] = v[m * d.nc + c];
+  d_force_sumY(vc);
+#pragma unroll
+  for (int m = IZ_NIV; m < IZ_NV; m++) v[m * d.nc + c] = vc[m];
 }
 
 // shock_sensor_modified_adams_shariff (sticky): module_tools.f90:2733-2814
@@ -1828,15 +1950,20 @@ extern "C" int izem_flux_mode(izem_ctx *c, int mode) {
 // face without a neighbour, module_BCs.f90:327-412, module_deriv.f90:268-469)
 extern "C" int izem_set_bc(izem_ctx *c, const int bc[6]) {
   if (!c || !bc) return IZEM_E_ARG;
-  int phys = 0, wall = 0;
+  int phys = 0, wall = 0, nscbc = 0;
   for (int f = 0; f < 6; f++) {
     if (bc[f] != IZEM_BC_PERIODIC && bc[f] != IZEM_BC_EXTRAPOLATION &&
-        bc[f] != IZEM_BC_POSTSHOCK_SLIPWALL)
+        bc[f] != IZEM_BC_POSTSHOCK_SLIPWALL && bc[f] != IZEM_BC_NSCBC_OUT)
       return IZEM_E_ARG;
-    // the post-shock slip wall is declared for face S only (module_BCs.f90:1163-1168)
+    // the post-shock slip wall is declared for face S only (module_BCs.f90:1163-1168),
+    // the characteristic outflow for face E only (:2511-2514)
     if (bc[f] == IZEM_BC_POSTSHOCK_SLIPWALL) {
       if (f != 2) return IZEM_E_ARG;
       wall = 1;
+    }
+    if (bc[f] == IZEM_BC_NSCBC_OUT) {
+      if (f != 1) return IZEM_E_ARG;
+      nscbc = 1;
     }
     if (bc[f] != IZEM_BC_PERIODIC) {
       if (f / 2 >= c->d.ndim) return IZEM_E_ARG;
@@ -1846,11 +1973,22 @@ extern "C" int izem_set_bc(izem_ctx *c, const int bc[6]) {
   // a direction is periodic on both faces or on neither
   for (int dir = 0; dir < 3; dir++)
     if (((phys >> (2 * dir)) & 3) == 1 || ((phys >> (2 * dir)) & 3) == 2) return IZEM_E_ARG;
-  if (phys != c->d.phys || wall != c->d.wall) {
+  if (phys != c->d.phys || wall != c->d.wall || nscbc != c->d.nscbc) {
     c->d.phys = phys;
     c->d.wall = wall;
+    c->d.nscbc = nscbc;
     iz_graph_destroy(c);  // a captured step holds the old dimensions
   }
+  return IZEM_OK;
+}
+
+extern "C" int izem_set_nscbc(izem_ctx *c, double Lx, double pinf, double dxi_last) {
+  if (!c || !(Lx > 0.0) || !(pinf > 0.0) || !(dxi_last > 0.0)) return IZEM_E_ARG;
+  c->nscbc_Lx = Lx;
+  c->nscbc_pinf = pinf;
+  c->nscbc_dxi = dxi_last;
+  c->nscbc_set = 1;
+  iz_graph_destroy(c);
   return IZEM_OK;
 }
 
@@ -2060,16 +2198,46 @@ extern "C" int izem_euler_llf_z(izem_ctx *c, const double *dz_i, const double *T
   return euler_dir(c, 2, dz_i, T, W_i, cp, ha, sensor, v, fl, true);
 }
 
-int iz_comm_cons(izem_ctx *c, double *a, int nf, int bcs) {
+// comm_cons (the periodic exchange of every direction), then, with bcs,
+// upd_boundaries' face conditions in the order W, E, S, N, B, F: a face
+// that writes its first interior row (a Dirichlet face) has that row's
+// force_sumY right after its condition, before the next face reads it
+// (module_BCs.f90:71-137); the row's primitives follow in iz_ghost_newton.
+int iz_comm_cons(izem_ctx *c, double *a, int nf, int bcs, const double *T, const double *W_i,
+                 const double *cp, const double *dtp, double scale) {
   const IzDims &d = c->d;
-  if (bcs && d.wall && (!c->wall_set || nf != IZ_NV)) return IZEM_E_STATE;
+  if (bcs && (d.wall || d.nscbc) && nf != IZ_NV) return IZEM_E_STATE;
+  if (bcs && d.wall && !c->wall_set) return IZEM_E_STATE;
+  if (bcs && d.nscbc && (!c->nscbc_set || !T || !W_i || !cp || !dtp)) return IZEM_E_STATE;
   const double *wall = (bcs && d.wall) ? c->wall_state : nullptr;
   int pk = iz_prof_begin(c, IZP_OTHER);
-  for (int dir = 0; dir < d.ndim; dir++) {
+  auto fill = [&](int dir, int mode) {
     long n0 = (dir == 0) ? d.ny : d.nx, n1 = (dir == 2) ? d.ny : d.nz;
     long tot = n0 * n1 * 2 * d.ng * nf;
-    hipLaunchKernelGGL(k_ghost_fill, dim3(iz_blocks(tot, 256)), dim3(256), 0, c->stream, d, a, nf,
-                       dir, wall, bcs);
+    hipLaunchKernelGGL(k_ghost_fill, dim3(iz_blocks(tot, 256)), dim3(256), 0, c->stream, d, a,
+                       nf, dir, wall, mode);
+  };
+  for (int dir = 0; dir < d.ndim; dir++) fill(dir, 1);
+  if (bcs) {
+    for (int dir = 0; dir < d.ndim; dir++) {
+      if (!((d.phys >> (2 * dir)) & 3)) continue;
+      fill(dir, 2);
+      if (dir == 0 && d.nscbc) {
+        hipLaunchKernelGGL(k_nscbc_out, dim3(iz_blocks((long)d.ny * d.nz, 64)), dim3(64), 0,
+                           c->stream, d, a, T, W_i, cp, dtp, scale, c->nscbc_Lx, c->nscbc_pinf,
+                           c->nscbc_dxi);
+        if (c->o.forcesum) {
+          IzBox b = domain_box(d, 10);
+          hipLaunchKernelGGL(k_force_row, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0,
+                             c->stream, d, b, a);
+        }
+      }
+      if (dir == 1 && d.wall && c->o.forcesum) {
+        IzBox b = domain_box(d, -20);
+        hipLaunchKernelGGL(k_force_row, dim3(iz_blocks(box_count(b), 256)), dim3(256), 0,
+                           c->stream, d, b, a);
+      }
+    }
   }
   iz_prof_end(c, pk);
   IZ_CHECK(hipGetLastError());
@@ -2084,6 +2252,12 @@ extern "C" int izem_comm_cons(izem_ctx *c, double *v) {
 extern "C" int izem_comm_cons_exchange(izem_ctx *c, double *v) {
   if (!c || !v) return IZEM_E_ARG;
   return iz_comm_cons(c, v, IZ_NV, 0);
+}
+
+extern "C" int izem_upd_boundaries(izem_ctx *c, const double *dt, const double *T,
+                                   const double *W_i, const double *cp, double *v) {
+  if (!c || !dt || !T || !W_i || !cp || !v) return IZEM_E_ARG;
+  return iz_comm_cons(c, v, IZ_NV, 1, T, W_i, cp, dt, 1.0);
 }
 
 // viscous fluxes into the context scratch (prim, fd, sf)
@@ -2221,9 +2395,15 @@ int iz_upd_prim_woT(izem_ctx *c, int dom, const double *v, const double *T, doub
 // 93-96, 131, 297-314), then upd_prim_var_ghost on the ghost slabs
 int iz_ghost_newton(izem_ctx *c, double *T, double *W_i, double *cp, double *ha, double *v) {
   static const int faces[6] = {-1, 1, -2, 2, -3, 3};
+  // the Dirichlet rows in face order (their force_sumY came with the fill)
+  if (c->d.nscbc) {
+    static const int row = 10;
+    int r = newton_domains(c, &row, 1, 0, v, W_i, T, cp, ha);
+    if (r) return r;
+  }
   if (c->d.wall) {
     static const int row = -20;
-    int r = newton_domains(c, &row, 1, c->o.forcesum, v, W_i, T, cp, ha);
+    int r = newton_domains(c, &row, 1, 0, v, W_i, T, cp, ha);
     if (r) return r;
   }
   return newton_domains(c, faces, 2 * c->d.ndim, c->o.forcesum, v, W_i, T, cp, ha);
@@ -2323,7 +2503,7 @@ extern "C" int izem_rk3(izem_ctx *c, const double *dt, const double *dx_i, const
                        ha, c->masks + 3, 200, 1.0e-5, c->negT_rec, c->status, c->iters + 3);
     iz_prof_end(c, pk);
     IZ_CHECK(hipGetLastError());
-    r = iz_comm_cons(c, v, IZ_NV, 1);
+    r = iz_comm_cons(c, v, IZ_NV, 1, T, W_i, cp, dt, 1.0);
     if (r) return r;
     r = iz_ghost_newton(c, T, W_i, cp, ha, v);
     if (r) return r;

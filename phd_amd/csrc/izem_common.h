@@ -43,13 +43,15 @@ struct IzThd {
 // np: ranks of the decomposed run whose shock-sensor views are reproduced,
 // per direction (1: the whole line is one rank)
 // wall: face S is a post-shock slip wall (its first nfix cells take a
-// post-shock state, the others mirror)
+// post-shock state, the others mirror); nscbc: face E is a subsonic
+// characteristic outflow
 struct IzDims {
   int nx, ny, nz, ng, ndim;
   long NX, NY, NZ, nc;
   int phys;
   int np[3];
   int wall, nfix;
+  int nscbc;
 };
 
 // rank r of np along a line of n cells owns cells (r n)/np .. ((r+1) n)/np - 1

@@ -59,6 +59,10 @@ struct izem_ctx {
   // it was given
   double *wall_state;
   int wall_set;
+  // the characteristic outflow's relaxation: domain length Lx, far-field
+  // pressure and 1/dx of the last cell (nondimensional), and whether given
+  double nscbc_Lx, nscbc_pinf, nscbc_dxi;
+  int nscbc_set;
   // viscous scratch: velocities + mole fractions, gradients / diffusion
   // fluxes, stresses + heat flux
   double *prim, *fd, *sf;
@@ -104,8 +108,12 @@ static inline unsigned iz_blocks_capped(const izem_ctx *c, long n, int bs, int p
 // internal entry points across translation units
 int iz_ghost_newton(izem_ctx *ctx, double *T, double *W_i, double *cp, double *ha, double *v);
 // bcs = 1: the periodic exchange and the non-periodic faces' conditions
-// (comm_cons + upd_boundaries); 0: the exchange alone
-int iz_comm_cons(izem_ctx *ctx, double *a, int nf, int bcs);
+// (comm_cons + upd_boundaries); 0: the exchange alone.  The characteristic
+// outflow needs the primitives T, W_i, cp and the step dt (scale x *dtp);
+// without them (nullptr) a context with such a face returns IZEM_E_STATE.
+int iz_comm_cons(izem_ctx *ctx, double *a, int nf, int bcs, const double *T = nullptr,
+                 const double *W_i = nullptr, const double *cp = nullptr,
+                 const double *dtp = nullptr, double scale = 1.0);
 int iz_upd_prim_woT(izem_ctx *ctx, int domain, const double *v, const double *T, double *W_i,
                     double *cp, double *ha);
 void iz_chem_destroy(izem_ctx *ctx);

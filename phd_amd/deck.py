@@ -214,13 +214,13 @@ SENSORS = {"ADAMSSHARIFF": 0, "DUCROS": 1}
 
 BC_KEYS = ("GRID%BCXMIN", "GRID%BCXMAX", "GRID%BCYMIN", "GRID%BCYMAX", "GRID%BCZMIN",
            "GRID%BCZMAX")
-BCS = {"PERIODIC": 0, "EXTRAPOLATION": 1, "POSTSHOCK_SLIPWALL": 2}
+BCS = {"PERIODIC": 0, "EXTRAPOLATION": 1, "POSTSHOCK_SLIPWALL": 2, "NSCBC_OUT": 3}
 
 
 def face_conditions(deck: Deck) -> tuple:
     """The six face conditions W, E, S, N, B, F for izem_set_bc (0 periodic,
-    1 extrapolation, 2 post-shock slip wall); faces beyond GENERAL%DIM are
-    periodic."""
+    1 extrapolation, 2 post-shock slip wall, 3 characteristic outflow);
+    faces beyond GENERAL%DIM are periodic."""
     dim = deck.get("GENERAL%DIM")
     return tuple(BCS[deck.get(k).upper()] if f // 2 < dim else 0 for f, k in enumerate(BC_KEYS))
 
@@ -270,8 +270,10 @@ def options(deck: Deck, **overrides):
     for key in BC_KEYS:
         bc = g(key).upper()
         if bc not in BCS:
-            refuse.append(f"{key} = {g(key)} (periodic, extrapolated or, on BCYMIN, "
-                          f"post-shock slip-wall faces only)")
+            refuse.append(f"{key} = {g(key)} (periodic, extrapolated, post-shock slip-wall "
+                          f"on BCYMIN or characteristic-outflow on BCXMAX faces only)")
+        elif bc == "NSCBC_OUT" and key != "GRID%BCXMAX":
+            refuse.append(f"{key} = {g(key)} (declared for BCXMAX only, module_BCs.f90:2511-2514)")
         elif bc == "POSTSHOCK_SLIPWALL" and key != "GRID%BCYMIN":
             refuse.append(f"{key} = {g(key)} (declared for BCYMIN only, module_BCs.f90:1163-1168)")
         elif bc == "POSTSHOCK_SLIPWALL" and g("GENERAL%INIT").upper() != "DMR":

@@ -57,8 +57,8 @@ module izem_hip_mod
 
   ! face conditions of izem_set_bc (GRID%BCXMIN .. BCZMAX)
   integer(c_int), parameter, public :: IZEM_BC_PERIODIC = 0, IZEM_BC_EXTRAPOLATION = 1, &
-                                       IZEM_BC_POSTSHOCK_SLIPWALL = 2
-  public :: izem_set_bc, izem_set_wall_state
+                                       IZEM_BC_POSTSHOCK_SLIPWALL = 2, IZEM_BC_NSCBC_OUT = 3
+  public :: izem_set_bc, izem_set_wall_state, izem_set_nscbc, izem_upd_boundaries
   ! ranks of a decomposed run to reproduce (GRID%NUMPROCX/Y/Z)
   public :: izem_set_decomposition
 
@@ -86,6 +86,17 @@ module izem_hip_mod
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: state(*)
       integer(c_int), value :: nfix
+    end function
+    integer(c_int) function izem_set_nscbc(ctx, Lx, pinf, dxi_last) &
+        bind(C, name="izem_set_nscbc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: Lx, pinf, dxi_last
+    end function
+    integer(c_int) function izem_upd_boundaries(ctx, dt, T, W_i, cp, v) &
+        bind(C, name="izem_upd_boundaries")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, dt, T, W_i, cp, v
     end function
     integer(c_int) function izem_set_decomposition(ctx, nproc) &
         bind(C, name="izem_set_decomposition")

@@ -66,6 +66,19 @@ def initial_state(deck, adi, thd, n: int, grid_path: str | None = None):
     return ics.config5_state(adi, thd, n=n, L=INITS[init][1])
 
 
+NSCBC_PINF_PA = 94232.25
+
+
+def _x_extent(box, grid_path: str) -> float:
+    """x(ntx) - x(1) [m]: of grid.dat when the case reads one, of the
+    uniform box (cell spacing L / nx) otherwise."""
+    if box.metrics is not None and grid_path and os.path.exists(grid_path):
+        from . import grid
+        xt, _, _ = grid.read_grid(grid_path, box.nx, box.ny, box.nz)
+        return float(xt[-1] - xt[0])
+    return box.L[0] * (box.nx - 1) / box.nx
+
+
 def species_set(here: str, adi):
     """The thermodynamic and transport tables of a run directory (thd_init
     reads the run directory's files): therm.dat there or the bundled one;
@@ -148,6 +161,11 @@ def main(argv=None):
             from . import grid, shock
             xt, _, _ = grid.read_grid(grid_path, box.nx, box.ny, box.nz)
             s.set_wall_state(shock.dmr_wall_state(adi, thd), shock.dmr_wall_cells(adi, xt))
+        if 3 in faces:
+            # the characteristic outflow: Lx = x(ntx) - x(1) (module_tools.f90:
+            # 319, 489), the reference's far-field pressure (module_BCs.f90:2389)
+            s.set_nscbc(_x_extent(box, grid_path) / adi.L_ref, NSCBC_PINF_PA / adi.p_ref,
+                        box.inv_spacing(adi)[0][box.nx - 1 + box.ng])
         s.set_decomposition(decomposition(deck))
         return s
 

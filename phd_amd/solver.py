@@ -115,7 +115,8 @@ class Solver:
 
     def set_bc(self, bc) -> None:
         """Face conditions W, E, S, N, B, F (GRID%BCXMIN .. BCZMAX): 0 periodic,
-        1 extrapolation, 2 post-shock slip wall on S (izem_set_bc)."""
+        1 extrapolation, 2 post-shock slip wall on S, 3 characteristic outflow
+        on E (izem_set_bc)."""
         a = (C.c_int * 6)(*[int(x) for x in bc])
         _lib.check(self.lib.izem_set_bc(self.ctx, a), "izem_set_bc")
         self.bc = tuple(int(x) for x in bc)
@@ -129,6 +130,12 @@ class Solver:
             raise ValueError(f"the wall state has {a.size} values, not {5 + len(self.thd.W)}")
         _lib.check(self.lib.izem_set_wall_state(self.ctx, a.ctypes.data, int(nfix)),
                    "izem_set_wall_state")
+
+    def set_nscbc(self, Lx: float, pinf: float, dxi_last: float) -> None:
+        """The characteristic outflow's domain length, far-field pressure and
+        1/dx of the last cell, nondimensional (izem_set_nscbc)."""
+        _lib.check(self.lib.izem_set_nscbc(self.ctx, float(Lx), float(pinf), float(dxi_last)),
+                   "izem_set_nscbc")
 
     def set_decomposition(self, nproc) -> None:
         """Reproduce a run decomposed into nproc[0] x nproc[1] x nproc[2] ranks
@@ -218,13 +225,23 @@ class Solver:
             _lib.check(self.lib.izem_comm_cons_exchange(self.ctx, _ptr(self.v)), "comm_cons")
             self.prim_inv_var_mach(0)
             restart.extrapolate_faces(self.box, self.v)
-            if boundaries and 2 in self.bc:
-                _lib.check(self.lib.izem_comm_cons(self.ctx, _ptr(self.v)), "upd_boundaries")
-                if self.opts_struct.forcesum:
-                    _lib.check(self.lib.izem_force_sumY(self.ctx, -20, _ptr(self.v)), "force_sumY")
-                _lib.check(self.lib.izem_prim_inv_var(self.ctx, -20, _ptr(self.v), _ptr(self.W_i),
-                                                      _ptr(self.T), _ptr(self.cp), _ptr(self.ha),
-                                                      None), "prim_inv_var")
+            if boundaries and (2 in self.bc or 3 in self.bc):
+                # the reader's timestep (DTmax = 1) and upd_boundaries with its
+                # dtmin (module_io.f90:184-186): the Dirichlet faces' conditions
+                # (their rows' force_sumY come with them), then their rows'
+                # primitives
+                self.set_time_state(np.ones(6))
+                self.timestep()
+                dt = self.lib.izem_time_state(self.ctx) + 4 * 8
+                _lib.check(self.lib.izem_upd_boundaries(self.ctx, dt, _ptr(self.T), _ptr(self.W_i),
+                                                        _ptr(self.cp), _ptr(self.v)),
+                           "upd_boundaries")
+                for dom, code in ((10, 3), (-20, 2)):
+                    if code in self.bc:
+                        _lib.check(self.lib.izem_prim_inv_var(self.ctx, dom, _ptr(self.v),
+                                                              _ptr(self.W_i), _ptr(self.T),
+                                                              _ptr(self.cp), _ptr(self.ha), None),
+                                   "prim_inv_var")
         else:
             _lib.check(self.lib.izem_comm_cons(self.ctx, _ptr(self.v)), "comm_cons")
             self.prim_inv_var_mach(0)
