@@ -1,6 +1,6 @@
 """GPU parity of the flow path against the oracle, through the C ABI.
 
-Bar: max|d f_m| / max|f_m| <= 1e-12 per equation (north-star tolerance).
+Bar: max|d f_m| / max|f_m| <= 1e-12 per equation.
 In flux mode 0 the Euler face kernels keep the reference's operation order
 and their fluxes are bit-identical to the oracle's (asserted); the RHS and
 RK3 step differ only by libm exp/log in the transport properties.  Flux mode
