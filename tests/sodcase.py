@@ -105,7 +105,7 @@ def initial_state(oracle, adi, thd, box):
 class SodRun:
     """The oracle restatement of the run (liboracle_nrv2.so)."""
 
-    def __init__(self, adi, oracle=None):
+    def __init__(self, adi, oracle=None, restart=None):
         from oracle.oracle import BC_EXTRAPOLATION, Grid, Opts, Oracle, make_adi
         self.o = oracle or Oracle(nrv=2)
         self.adi = adi
@@ -119,16 +119,35 @@ class SodRun:
         self.opts = Opts(1, PERCENT_WEIGHT, 1, 0, 0, 1, CFL, CFLMIN, FO, CPR, 0, 0.1)
         self.d_i = [np.ascontiguousarray(m) for m in box.metrics]
         shp = box.shape
-        self.v, self.T = initial_state(self.o, adi, self.thd, box)
         self.W_i = np.zeros(shp)
         self.cp = np.zeros(shp)
         self.ha = np.zeros((2,) + shp)
         self.sensor = np.zeros(shp, np.uint8)
-        self._init_primitives()
-        self.times = stat_times(adi)
-        self.time, self.ite, self.nb_stat = 0.0, 0, 0
+        self.ite = 0
         self.dtmin, self.DTmax = 1.0, 1.0
         self.dt_hist = []
+        if restart is None:
+            self.v, self.T = initial_state(self.o, adi, self.thd, box)
+            self._init_primitives()
+            self.times = stat_times(adi)
+            self.time, self.nb_stat = 0.0, 0
+        else:
+            # read_restart_hdf5 (module_io.f90:107-192): the interior from the
+            # file, T from zero, the same closing steps as ini_sod (the
+            # reader's upd_boundaries repeats the extrapolation), an empty
+            # sensor; statistics times from NUMBER_RESTART + 1 copies of
+            # TIME_OFFSET (module_input.f90:718-721)
+            from phd_amd import restart as rs
+            path, nb = restart
+            self.v, self.time = rs.read(path, box, [n.strip().upper() for n in self.thd.names])
+            self.v = np.ascontiguousarray(self.v)
+            self.T = np.zeros(shp)
+            self._init_primitives()
+            t = [TIME_OFFSET] * (nb + 1)
+            while len(t) < NSTAT:
+                t.append(t[-1] + DTIME)
+            self.times = np.array(t) / adi.time_ref
+            self.nb_stat = nb
 
     def _p(self, a):
         return a.ctypes.data_as(C.POINTER(C.c_double))
@@ -168,7 +187,7 @@ class SodRun:
                        self._p(dt_4), C.byref(d))
         dtmin = d.value
         write = False
-        nxt = self.times[self.nb_stat] if self.nb_stat < NSTAT else np.inf
+        nxt = self.times[self.nb_stat] if self.nb_stat < len(self.times) else np.inf
         if self.time + dtmin > nxt and self.time - dtmin < nxt:
             dtmin = nxt - self.time
             write = True

@@ -117,3 +117,54 @@ def test_reference_sod_deck_reproduces_reference_restarts(tmp_path, capsys):
         # time, dtmin and the four limits / dtmin (9 significant digits)
         for q in range(1, 7):
             assert abs(rows[k - 1][q] - stats[k][q - 1]) <= 1e-8 * abs(stats[k][q - 1]), (k, q)
+
+
+def test_reference_sod_deck_resumed_from_reference_restart(tmp_path, capsys):
+    """The reference's Sod deck resumed (READ_RESTART) from the reference's
+    own restart_00001.h5, against the oracle resumed from the same file.  A
+    resumed run's statistics times start with NUMBER_RESTART + 1 copies of
+    TIME_OFFSET (module_input.f90:718-721): the first is met at once by a
+    step of zero length and written as statistics 2, statistics 3 falls at
+    the original statistics 2's time.  Against the oracle in the reference's
+    operation order within 1e-10: the temperature re-derived from the file
+    (prim_inv_var_err_mach from T = 0) agrees to 1e-14, not bit for bit
+    (tests/test_gpu_restart.py), and 66 WENO7 iterations lift that to 2e-11.
+    It is not the writing run's trajectory:
+    the sticky shock sensor restarts empty and T is re-derived from the
+    file, so at the original statistics 2's time it lies 1e-3 from the
+    reference's restart_00002.h5 (the oracle's uninterrupted run: 5e-8)."""
+    import shutil
+    from phd_amd import chemdata as cd
+    from phd_amd import restart
+    from phd_amd.run import main
+    from tests import sodcase as sc
+    if not os.path.exists(restart.IO_PATH):
+        pytest.skip("libizem_io.so not built (needs the HDF5 C library)")
+    deck = open(os.path.join(sc.GOLDEN, "sod.inp")).read()
+    deck = deck.replace("NSTAT = 30", "NSTAT = 3").replace(
+        "[MONITORING]", "[MONITORING]\n        READ_RESTART = T\n"
+        "        NAME_RESTART = 'restart'\n        NUMBER_RESTART = 1")
+    (tmp_path / "sod.inp").write_text(deck)
+    shutil.copy(sc.GRID, tmp_path / "grid.dat")
+    shutil.copy(sc.TRANFIT, tmp_path / "tranfit.out")
+    shutil.copy(sc.restart_path(1), tmp_path / "restart_00001.h5")
+    assert main([str(tmp_path / "sod.inp"), "--flux-mode", "0"]) == 0
+    capsys.readouterr()
+    rows = [[float(x) for x in ln.split()] for ln in
+            (tmp_path / "times_stat.out").read_text().splitlines()]
+    assert [int(r[0]) for r in rows] == [2, 3]
+    adi = cd.Adim()
+    run = sc.SodRun(adi, restart=(sc.restart_path(1), 1))
+    run.run_to_stat(3)
+    box = run.box
+    ours, t = restart.read(str(tmp_path / "restart_00003.h5"), box, run.thd.names)
+    assert t == run.time
+    I = (slice(None),) + box.interior()
+    e = max(np.max(np.abs(ours[I][l] - run.v[I][l])) / np.max(np.abs(run.v[I][l]))
+            for l in range(ours.shape[0]) if np.max(np.abs(run.v[I][l])) > 0)
+    ref, _ = sc.read_reference(adi, 2, run.thd)
+    d = np.max(np.abs(ours[I][0] - ref[I][0])) / np.max(np.abs(ref[I][0]))
+    print(f"resumed: {run.ite} iterations, against the oracle {e:.2e}, "
+          f"against the reference's uninterrupted run {d:.1e}")
+    assert e <= 1e-10
+    assert 1e-5 < d < 1e-2
