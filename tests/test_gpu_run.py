@@ -168,3 +168,48 @@ def test_reference_sod_deck_resumed_from_reference_restart(tmp_path, capsys):
           f"against the reference's uninterrupted run {d:.1e}")
     assert e <= 1e-10
     assert 1e-5 < d < 1e-2
+
+
+def test_reference_sod_deck_to_late_statistics(tmp_path, capsys):
+    """The reference's Sod deck through the driver to statistics time 29
+    (iteration 2500 or so: shock, contact and expansion well apart), its
+    restart files 10, 20 and 29 against the reference's own, to the bars
+    of statistics 1 and 2 (tests/test_sod_reference_cpu.py): the
+    trajectory does not drift from the reference's."""
+    import shutil
+    from phd_amd import chemdata as cd
+    from phd_amd import restart
+    from phd_amd.run import main
+    from tests import sodcase as sc
+    from tests.test_sod_reference_cpu import ABS_TRANSVERSE, BARS
+    if not os.path.exists(restart.IO_PATH):
+        pytest.skip("libizem_io.so not built (needs the HDF5 C library)")
+    deck = open(os.path.join(sc.GOLDEN, "sod.inp")).read()
+    assert deck.count("NSTAT = 30") == 1 and deck.count("ITSHOW = 1\n") == 1
+    (tmp_path / "sod.inp").write_text(deck.replace("NSTAT = 30", "NSTAT = 29")
+                                      .replace("ITSHOW = 1\n", "ITSHOW = 500\n"))
+    shutil.copy(sc.GRID, tmp_path / "grid.dat")
+    shutil.copy(sc.TRANFIT, tmp_path / "tranfit.out")
+    assert main([str(tmp_path / "sod.inp")]) == 0
+    capsys.readouterr()
+    adi = cd.Adim()
+    thd = sc.sod_thd(adi)
+    box = sc.sod_box(adi)
+    I = (slice(None),) + box.interior()
+    for k in (10, 20, 29):
+        ours, t = restart.read(str(tmp_path / f"restart_{k:05d}.h5"), box, thd.names)
+        ref, t_ref = sc.read_reference(adi, k, thd)
+        assert abs(t / t_ref - 1.0) < 1e-13
+        rel = {}
+        for l in range(ours.shape[0]):
+            d = np.max(np.abs(ours[I][l] - ref[I][l]))
+            rel[l] = d / np.max(np.abs(ref[I][l])) if l in BARS else d
+        print(f"statistics {k}: " + ", ".join(f"var {l} {e:.1e}" for l, e in rel.items()),
+              flush=True)
+        for l, bar in BARS.items():
+            assert rel[l] <= bar, (k, l, rel[l])
+        # rho v and rho w are rounding noise in both codes; the reference's
+        # grows to 7.3e-13 by statistics 29
+        for l in (2, 3):
+            noise = np.max(np.abs(ref[I][l]))
+            assert rel[l] <= max(ABS_TRANSVERSE, 3.0 * noise), (k, l, rel[l], noise)
